@@ -1,0 +1,54 @@
+/* orc_abi.h — the extern "C" surface shared by the two CPU checkers of this repository:
+ *   oracle/liboracle.so            our CPU restatement of the reference r²HPDHG path (prefix orc_)
+ *   oracle/_ref/libps26119_ref.so  the reference's own sources + oracle/ref_driver.cpp (prefix ref_)
+ * Both are TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg load them, and only as the checker. The product never links or calls either.
+ *
+ * Model layout is the reference contract (include/ps26119/model.h of the reference): CSC A,
+ * infinity = HUGE_VAL, sense +1 min / -1 max, objective value cᵀx + offset.
+ */
+#ifndef PS26119_ORC_ABI_H
+#define PS26119_ORC_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int m, n, sense;
+  double offset;
+  const double *c, *col_lower, *col_upper, *row_lower, *row_upper;
+  const int *col_start, *row_index;
+  const double* value;
+} orc_lp;
+
+typedef struct {
+  int algorithm;  /* 3 = r2hpdhg (the only one the restatement implements) */
+  int precision;  /* 0 fp64, 1 mixed */
+  int presolve;   /* 0/1 */
+  int threads;    /* reference pool size; results never depend on it */
+  int check_every;
+  double tolerance, time_limit;
+  long long iteration_limit;
+  const double *warm_x, *warm_y;
+  double warm_primal_weight;
+  int n_params;
+  const char* const* param_names;
+  const double* param_values;
+} orc_opts;
+
+typedef struct {
+  int status;
+  long long iterations, iterations_to_fast;
+  double objective, dual_objective, primal_residual, dual_residual, gap;
+  double certified_bound, primal_weight, seconds, setup_seconds;
+  double check_primal, check_dual, check_gap;
+  int check; /* 1 PASS, 0 FAIL, -1 none */
+  char engine[32];
+  char message[512];
+} orc_result;
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+"""Seeded LP generators with a known optimum, for the workloads this backend is measured on.
+
+Every generator returns an :class:`Lp` in the reference's model contract (CSC matrix,
+``inf`` for missing bounds, ``sense`` +1 min / -1 max) together with the optimal objective
+value known by construction (a KKT point is planted). All draws come from
+``numpy.random.default_rng(seed)``, so a (generator, sizes, seed) triple always yields the
+same bits.
+
+random_lp            m rows, n = 2m columns, exactly k distinct uniformly drawn columns per
+                     row, values N(0,1); columns in [0, 10]; 40% of x* at a bound, the rest
+                     U(1, 9); 70% equality rows b = A x*, 30% rows <= with slack U(0, 1);
+                     y* N(0,1) on equality rows, 0 on slack rows; z* U(0.1, 1) signed by the
+                     active bound, 0 on interior columns; c = Aᵀy* + z*.
+refinery_lp          T periods of a 49 x 60 block with yield/balance coefficients
+                     log-uniform in [1e-3, 1e3] and +-1 inventory links between periods t
+                     and t+1 (block staircase); capacities U(10, 1000), demands U(1, 100)
+                     times a daily sinusoid, costs U(-5, 20) (see the function docstring).
+scenario_set         perturbs b, c and upper bounds of a base LP by lognormal(0, sigma).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+INF = np.inf
+
+
+@dataclass
+class Lp:
+    m: int
+    n: int
+    c: np.ndarray
+    col_lower: np.ndarray
+    col_upper: np.ndarray
+    row_lower: np.ndarray
+    row_upper: np.ndarray
+    col_start: np.ndarray  # int32, n+1
+    row_index: np.ndarray  # int32, nnz
+    value: np.ndarray      # float64, nnz
+    sense: int = 1
+    offset: float = 0.0
+    optimum: float = float("nan")  # known optimal objective (c·x* + offset), NaN if unknown
+    name: str = ""
+    x_star: np.ndarray | None = field(default=None, repr=False)
+    y_star: np.ndarray | None = field(default=None, repr=False)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col_start[-1])
+
+    def csr(self):
+        """(row_ptr, col, val) of A with columns sorted inside each row."""
+        cols = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.col_start))
+        order = np.lexsort((cols, self.row_index))
+        row_ptr = np.zeros(self.m + 1, dtype=np.int64)
+        np.cumsum(np.bincount(self.row_index, minlength=self.m), out=row_ptr[1:])
+        return row_ptr, cols[order].astype(np.int32), self.value[order]
+
+    def activity(self, x: np.ndarray) -> np.ndarray:
+        cols = np.repeat(np.arange(self.n), np.diff(self.col_start))
+        return np.bincount(self.row_index, weights=self.value * x[cols], minlength=self.m)
+
+
+def _csc_from_triplets(m: int, n: int, rows: np.ndarray, cols: np.ndarray, vals: np.ndarray):
+    key = cols.astype(np.int64) * m + rows.astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    col_start = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(cols, minlength=n), out=col_start[1:])
+    return (col_start.astype(np.int32), rows[order].astype(np.int32), vals[order].astype(np.float64))
+
+
+def random_lp(m: int, n: int | None = None, per_row: int = 6, seed: int = 1) -> Lp:
+    """The random LP with a planted optimum (see the module docstring)."""
+    rng = np.random.default_rng(seed)
+    n = 2 * m if n is None else n
+    cols = rng.integers(0, n, size=(m, per_row))
+    while True:  # exactly `per_row` DISTINCT columns per row: redraw rows with a repeat
+        s = np.sort(cols, axis=1)
+        dup = (s[:, 1:] == s[:, :-1]).any(axis=1)
+        if not dup.any():
+            break
+        cols[dup] = rng.integers(0, n, size=(int(dup.sum()), per_row))
+    vals = rng.standard_normal((m, per_row))
+    at_bound = rng.random(n) < 0.4
+    upper = rng.random(n) < 0.5
+    x = rng.uniform(1.0, 9.0, n)
+    x[at_bound & ~upper] = 0.0
+    x[at_bound & upper] = 10.0
+    ax = np.zeros(m)
+    for k in range(per_row):  # fixed left-to-right order per row
+        ax = ax + vals[:, k] * x[cols[:, k]]
+    eq = rng.random(m) < 0.7
+    slack = rng.uniform(0.0, 1.0, m)
+    rl = np.where(eq, ax, -INF)
+    ru = np.where(eq, ax, ax + slack)
+    y = np.where(eq, rng.standard_normal(m), 0.0)
+    z = rng.uniform(0.1, 1.0, n)
+    z[~at_bound] = 0.0
+    z[at_bound & upper] *= -1.0
+    rows = np.repeat(np.arange(m, dtype=np.int64), per_row)
+    col_start, row_index, value = _csc_from_triplets(m, n, rows, cols.ravel(), vals.ravel())
+    colidx = np.repeat(np.arange(n), np.diff(col_start))
+    aty = np.bincount(colidx, weights=value * y[row_index], minlength=n)
+    c = aty + z
+    opt = float(np.dot(c, x))
+    return Lp(m=m, n=n, c=c, col_lower=np.zeros(n), col_upper=np.full(n, 10.0), row_lower=rl, row_upper=ru,
+              col_start=col_start, row_index=row_index, value=value, optimum=opt,
+              name=f"random-{m}x{n}-k{per_row}-s{seed}", x_star=x, y_star=y)
+
+
+def scenario_set(base: Lp, k: int, sigma: float = 0.05, seed: int = 7):
+    """k scenarios of `base`: b (finite row bounds), c and finite column upper bounds each
+    multiplied by an independent lognormal(0, sigma) factor. Returns a list of dicts with
+    keys obj, col_upper, row_lower, row_upper."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        fb = rng.lognormal(0.0, sigma, base.m)
+        fc = rng.lognormal(0.0, sigma, base.n)
+        fu = rng.lognormal(0.0, sigma, base.n)
+        rl = np.where(np.isfinite(base.row_lower), base.row_lower * fb, base.row_lower)
+        ru = np.where(np.isfinite(base.row_upper), base.row_upper * fb, base.row_upper)
+        cu = np.where(np.isfinite(base.col_upper), base.col_upper * fu, base.col_upper)
+        out.append({"obj": base.c * fc, "col_upper": cu, "row_lower": rl, "row_upper": ru})
+    return out
+
+
+def read_lpm(path: str) -> Lp:
+    """Reader for the reference's `.lpm` text format, version 1 (sections as in its
+    tools/lpm.py specification: header keys, then OBJ, COL_LOWER, COL_UPPER, ROW_LOWER,
+    ROW_UPPER, COL_START, ROW_INDEX, VALUE; INTEGER and name sections are skipped)."""
+    with open(path) as f:
+        lines = [ln.rstrip("\n") for ln in f]
+    hdr = {}
+    sections = {}
+    cur = None
+    skip_names = False
+    for ln in lines:
+        s = ln.strip()
+        if not s or s.startswith("#"):
+            continue
+        head = s.split()[0]
+        if head in ("ROW_NAMES", "COL_NAMES"):
+            skip_names = True
+            cur = None
+            continue
+        if head == "END":
+            break
+        if skip_names:
+            continue
+        if head in ("LPM", "NAME", "SENSE", "ROWS", "COLS", "NNZ", "OFFSET"):
+            hdr[head] = s[len(head):].strip()
+            cur = None
+            continue
+        if head in ("OBJ", "COL_LOWER", "COL_UPPER", "ROW_LOWER", "ROW_UPPER", "COL_START", "ROW_INDEX", "VALUE",
+                    "INTEGER"):
+            cur = head
+            sections[cur] = []
+            rest = s[len(head):].split()
+            sections[cur].extend(rest)
+            continue
+        if cur is not None:
+            sections[cur].extend(s.split())
+    m, n = int(hdr["ROWS"]), int(hdr["COLS"])
+
+    def f(key):
+        return np.array([float(t) for t in sections.get(key, [])], dtype=np.float64)
+
+    def i(key):
+        return np.array([int(t) for t in sections.get(key, [])], dtype=np.int32)
+
+    return Lp(m=m, n=n, c=f("OBJ"), col_lower=f("COL_LOWER"), col_upper=f("COL_UPPER"),
+              row_lower=f("ROW_LOWER"), row_upper=f("ROW_UPPER"), col_start=i("COL_START"),
+              row_index=i("ROW_INDEX"), value=f("VALUE"), sense=-1 if hdr.get("SENSE", "MIN") == "MAX" else 1,
+              offset=float(hdr.get("OFFSET", "0") or 0), name=hdr.get("NAME", ""))
+
+
+def from_dense(c, A, rl, ru, cl, cu, sense: int = 1, name: str = "") -> Lp:
+    """Small models from dense rows (test fixtures)."""
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    rows, cols = np.nonzero(A.T)  # column-major order: cols of A are rows of A.T
+    col_start = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(np.bincount(rows, minlength=n), out=col_start[1:])
+    return Lp(m=m, n=n, c=np.asarray(c, float), col_lower=np.asarray(cl, float), col_upper=np.asarray(cu, float),
+              row_lower=np.asarray(rl, float), row_upper=np.asarray(ru, float), col_start=col_start,
+              row_index=cols.astype(np.int32), value=A.T[rows, cols].astype(np.float64), sense=sense, name=name)
