@@ -1,0 +1,83 @@
+/* ps26119_amd.h — extensions of the C API that the reference does not have (it has no C entry
+ * points for these). Same model arguments as ps26119_solve_lp. No function throws.
+ */
+#ifndef PS26119_AMD_H
+#define PS26119_AMD_H
+
+#include "ps26119/ps26119.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Named engine knobs (the reference's Options::engine_params, options.h:70-82 there), e.g.
+ * "restart_sufficient", "pid_kp", ..., plus the MI355X ones: "hip_cpu_order" (1 = every
+ * reduction in the reference CPU order, bit-identical trajectory), "hip_timing" (1 = per-kernel
+ * HIP event timing into ps26119_amd_stats), "hip_device". presolve: 1 (default) / 0. */
+typedef struct {
+  int presolve;
+  int check_every;            /* iterations between checks (0 = 64) */
+  int n_params;
+  const char* const* param_names;
+  const double* param_values;
+} ps26119_amd_options;
+
+typedef struct {
+  double setup_seconds;       /* scaling, upload, ‖Ã‖₂ */
+  long long iterations_to_fast;
+  double seconds_to_fast;
+  double primal_weight;
+  double check_primal, check_dual, check_gap;
+  long long kernel_launches[3]; /* hip_timing: fused primal, fused dual, check pass */
+  double kernel_ms[3];
+  double device_bytes;          /* device memory of the solve */
+} ps26119_amd_stats;
+
+void ps26119_amd_default_options(ps26119_amd_options* o);
+
+/* ps26119_solve_lp_ex with the extension options and statistics (either may be NULL). */
+int ps26119_amd_solve_lp(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                         const double* col_lower, const double* col_upper, const double* row_lower,
+                         const double* row_upper, const int* col_start, const int* row_index,
+                         const double* value, const ps26119_options* opt, const ps26119_amd_options* ext,
+                         ps26119_result* result, ps26119_amd_stats* stats, double* x, double* y, double* z,
+                         double* dual_ray, double* primal_ray);
+
+/* Number of visible GPUs the backend can use (0 when none). */
+int ps26119_amd_device_count(void);
+
+/* ---- host stages of the path, callable without a GPU (verification / tests) ---- */
+
+/* Scaling of the model with the default options: row_scale (m), col_scale (n),
+ * scalars = {bound_scale, obj_scale, log10_range, geometric_mean_applied}, the scaled values of
+ * Ã in CSR order and of Ãᵀ in CSR order (nnz each; may be NULL). */
+int ps26119_amd_scale(int num_rows, int num_cols, int sense, const double* c, const double* col_lower,
+                      const double* col_upper, const double* row_lower, const double* row_upper,
+                      const int* col_start, const int* row_index, const double* value, double* row_scale,
+                      double* col_scale, double* scalars, double* a_vals, double* at_vals);
+
+/* Rounding-proof bound from y (reference src/core/safe_bound.h semantics). */
+double ps26119_amd_certified_bound(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                                   const double* col_lower, const double* col_upper, const double* row_lower,
+                                   const double* row_upper, const int* col_start, const int* row_index,
+                                   const double* value, const double* y, int use_implied_bounds);
+
+/* In-process check: out = {max_primal_violation, max_dual_violation, relative_gap,
+ * primal_objective, dual_objective, passed}. */
+int ps26119_amd_check_solution(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                               const double* col_lower, const double* col_upper, const double* row_lower,
+                               const double* row_upper, const int* col_start, const int* row_index,
+                               const double* value, const double* x, const double* y, double* out);
+
+/* Presolve summary: out = {outcome (0 reduced, 1 infeasible, 2 unchanged), removed_rows,
+ * removed_cols, reduced_rows, reduced_cols, reduced_nnz, reduced_offset}. */
+int ps26119_amd_presolve_info(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                              const double* col_lower, const double* col_upper, const double* row_lower,
+                              const double* row_upper, const int* col_start, const int* row_index,
+                              const double* value, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PS26119_AMD_H */

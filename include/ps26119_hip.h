@@ -1,0 +1,164 @@
+/* ps26119_hip.h — the thin C-ABI layer between the C++ host code of the MI355X r²HPDHG backend
+ * and its hand-written CDNA4 (gfx950) HIP kernels. Built as libps26119_hip.so. Plain C types
+ * only: no HIP, no C++ and no torch types cross this boundary.
+ *
+ * It replaces the reference's CUDA backend object (reference src/gpu/cuda_backend.cu:423-670,
+ * behind the virtual interface src/pdhg/backend.h:36-69). The reference compiles its backend
+ * into the solver library, so there is no reference C-ABI for this layer; the calls below are
+ * what the reference's Backend methods map to, plus fused entry points (psh_chunk, psh_check)
+ * that the MI355X driver uses to keep the whole iteration on the device.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative PSH_E_* code; psh_error(ctx) gives the
+ *    text of the last error on that context (thread-compatible: one context per solve).
+ *  - Vectors live on the device in fixed slots (PSH_X ... PSH_AX), in the working precision
+ *    (fp64, or fp32 after psh_set_fp32(ctx, 1)); host copies are always fp64 arrays.
+ *  - Reductions accumulate in fp64. mode PSH_RED_TREE = fixed-shape tree (deterministic, any
+ *    size); PSH_RED_CPU = the reference CPU order (64 fixed chunks for dot / norms, exactly
+ *    src/la/parallel.h:169-188; single sequential loops for the KKT sums,
+ *    src/pdhg/cpu_backend.cpp:240-274), bit-identical to the CPU backend.
+ */
+#ifndef PS26119_HIP_H
+#define PS26119_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PSH_OK = 0,
+  PSH_E_NO_DEVICE = -1,   /* no usable GPU (reported like the reference's "backend unavailable") */
+  PSH_E_OOM = -2,         /* device allocation failed (reference: std::bad_alloc → "out of memory") */
+  PSH_E_ARG = -3,         /* bad argument / size mismatch */
+  PSH_E_HIP = -4,         /* HIP runtime error (reference: std::runtime_error → NumericalError) */
+  PSH_E_STATE = -5        /* call out of order (e.g. before psh_load) */
+};
+
+enum { PSH_RED_TREE = 0, PSH_RED_CPU = 1 };
+
+/* Vector slots. Primal (size n): X, X0, XH, XB, DX, ATY. Dual (size m): Y, Y0, YH, YB, DY, AX. */
+enum { PSH_X = 0, PSH_X0, PSH_XH, PSH_XB, PSH_DX, PSH_ATY, PSH_Y, PSH_Y0, PSH_YH, PSH_YB, PSH_DY, PSH_AX, PSH_NSLOT };
+
+typedef struct psh_ctx psh_ctx;
+
+/* The scaled problem (reference src/pdhg/scaling.h:48-67) plus the original data the KKT
+ * check needs. CSR with int32 row pointers and column indices (nnz < 2^31).
+ *   A  (m×n): a_ptr, a_col, a_val (scaled Ã), a_val_orig (A, same structure);
+ *   At (n×m): at_ptr, at_col, at_val (scaled Ãᵀ, rows sorted);
+ *   At_orig (n×m): at_ptr, ato_col (may equal at_col), ato_val (the model's CSC values).
+ * Vectors: c, l, u (scaled, n); rl, ru (scaled, m); row_scale (m), col_scale (n);
+ * original bounds cl_o, cu_o (n), rl_o, ru_o (m); original objective in min form c_min = sense·c. */
+typedef struct {
+  int m, n;
+  long long nnz;
+  const int *a_ptr, *a_col;
+  const double *a_val, *a_val_orig;
+  const int *at_ptr, *at_col, *ato_col;
+  const double *at_val, *ato_val;
+  const double *c, *l, *u, *rl, *ru;
+  const double *row_scale, *col_scale;
+  double bound_scale, obj_scale;
+  const double *cl_o, *cu_o, *rl_o, *ru_o, *c_min;
+} psh_problem;
+
+/* KKT statistics of an iterate on the ORIGINAL problem (reference src/pdhg/termination.h:35-60). */
+typedef struct {
+  double primal_residual, dual_residual, primal_obj, dual_obj, b_norm, c_norm, primal_max_rel, dual_max_rel;
+} psh_kkt;
+
+/* Ray-test scalars (reference src/pdhg/engine.cpp:82-155) for a scaled direction (Δx, Δy). */
+typedef struct {
+  double rmax, dual_obj, dual_viol, dual_mag;     /* dual ray r from Δy */
+  double dmax, primal_cd, primal_viol, primal_cmag; /* primal ray d from Δx */
+} psh_ray;
+
+/* One chunk of fused r²HPDHG iterations (reference src/pdhg/r2hpdhg.cpp:66-85). */
+typedef struct {
+  int iterations;      /* K */
+  long long inner0;    /* Halpern index k of the first iteration (w = (k+1)/(k+2)) */
+  double tau, sigma, rho, eta, omega;
+  int want_hat_last;   /* write x̂, ŷ, ȳ on the last iteration (a check follows) */
+  int mode;            /* PSH_RED_* for the epoch-start residual r⁰ */
+} psh_chunk_args;
+
+/* What a check computes on the device, in one pass and one host round trip. */
+typedef struct {
+  double omega, eta;   /* for r = sqrt(ω‖Δx‖² + ‖Δy‖²/ω + 2η Δyᵀ Ã Δx) */
+  int want_ray;        /* ray-test scalars of (Δx, Δy) = T(z) − z */
+  int want_drift;      /* ray-test scalars of the drift (x̂ − x0, ŷ − y0) */
+  int want_pid;        /* ‖x̂ − x0‖, ‖ŷ − y0‖ */
+  int mode;            /* PSH_RED_* for the KKT sums and the residual / PID norms */
+} psh_check_args;
+
+typedef struct {
+  double px, py, cross;      /* residual sums at this check */
+  double r0_px, r0_py, r0_cross; /* residual sums after the first step of the last epoch start (if any) */
+  int have_r0;
+  psh_kkt kkt;
+  psh_ray ray, drift;
+  double dx_norm, dy_norm;   /* ‖x̂ − x0‖, ‖ŷ − y0‖ (want_pid) */
+  int finite;                /* all of the above finite */
+} psh_stats;
+
+/* ---- device and context */
+int psh_device_count(void);
+int psh_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+int psh_create(int device, psh_ctx** out);
+void psh_destroy(psh_ctx* ctx);
+const char* psh_error(const psh_ctx* ctx);
+int psh_sync(psh_ctx* ctx);
+size_t psh_device_bytes(const psh_ctx* ctx); /* device memory held by the context */
+
+/* ---- problem and precision */
+int psh_load(psh_ctx* ctx, const psh_problem* p);
+int psh_set_fp32(psh_ctx* ctx, int on); /* working precision: converts every slot (reference set_precision) */
+int psh_is_fp32(const psh_ctx* ctx);
+
+/* ---- the reference Backend operations (src/pdhg/backend.h:46-65), on slots */
+int psh_upload(psh_ctx* ctx, int slot, const double* host);
+int psh_download(psh_ctx* ctx, int slot, double* host);
+int psh_fill(psh_ctx* ctx, int slot, double a);
+int psh_copy(psh_ctx* ctx, int dst, int src);
+int psh_axpby(psh_ctx* ctx, double a, int x, double b, int y);
+int psh_spmv(psh_ctx* ctx, int x, int out);   /* out ← Ã x   */
+int psh_spmv_t(psh_ctx* ctx, int y, int out); /* out ← Ãᵀ y  */
+int psh_dot(psh_ctx* ctx, int a, int b, int mode, double* out);
+int psh_diff_norm2(psh_ctx* ctx, int a, int b, int mode, double* out);
+int psh_primal_step(psh_ctx* ctx, int x, int aty, double tau, int xhat);
+int psh_dual_step(psh_ctx* ctx, int y, int ax, double sigma, int yhat);
+int psh_r2h_primal(psh_ctx* ctx, int x, int x0, int aty, double tau, double w, double rho, int xhat, int xbar);
+int psh_r2h_dual(psh_ctx* ctx, int y, int y0, int ax, double sigma, double w, double rho, int yhat, int ybar);
+int psh_kkt_of(psh_ctx* ctx, int x_scaled, int y_scaled, int mode, psh_kkt* out);
+/* Ray-test scalars of the direction in slots (dxs, dys). */
+int psh_ray_of(psh_ctx* ctx, int dx_slot, int dy_slot, psh_ray* out);
+/* The ray vectors of the last check / psh_ray_of in the original space: dual ray r (m) and
+ * primal ray d (n) (reference src/pdhg/engine.cpp:90-97, 132-139); drift = 1 for the rays of
+ * the drift direction. Either output may be NULL. */
+int psh_ray_vectors(psh_ctx* ctx, int drift, double* dual_ray, double* primal_ray);
+
+/* ---- fused fast path */
+int psh_chunk(psh_ctx* ctx, const psh_chunk_args* a);
+int psh_check(psh_ctx* ctx, const psh_check_args* a, psh_stats* out);
+int psh_restart(psh_ctx* ctx); /* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ */
+
+/* ---- ‖Ã‖₂ by Lanczos on ÃᵀÃ (reference src/la/csr.cpp:86-132): the host runs the
+ * tridiagonal eigenvalue test, the device runs the products and the dots.
+ * psh_lanczos_begin uploads the unit start vector v (n). psh_lanczos_step computes
+ * w = ÃᵀÃ v, α = wᵀv, w ← w − (α v + β_prev v_prev), β = ‖w‖ (sum of squares returned
+ * before the square root, as the host takes it). psh_lanczos_advance sets v_prev ← v, v ← w/β. */
+int psh_lanczos_begin(psh_ctx* ctx, const double* v0);
+int psh_lanczos_step(psh_ctx* ctx, double beta_prev, int mode, double* alpha, double* beta_sq);
+int psh_lanczos_advance(psh_ctx* ctx, double beta);
+
+/* ---- per-kernel timing (HIP events on the context's stream), for the benchmark. */
+int psh_timing(psh_ctx* ctx, int on);
+/* kernel classes: 0 fused primal (Ãᵀy + projection/reflection/Halpern), 1 fused dual (Ãx̄ + ...),
+ * 2 check pass; returns launches and total milliseconds. */
+int psh_timing_get(psh_ctx* ctx, int kernel_class, long long* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

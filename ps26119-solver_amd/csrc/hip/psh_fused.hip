@@ -1,0 +1,665 @@
+// psh_fused.hip — the fused r²HPDHG iteration and the one-pass check.
+//
+// Per iteration two kernels (reference loop src/pdhg/r2hpdhg.cpp:68-74):
+//   k_iter_primal  tiles of Ãᵀ: aty_j = Σ ã_ij y_i (fp64, CSR order), then in the epilogue
+//                  x̂ = proj(x − τ(c − aty)), x̄ = 2x̂ − x, x ← w(2ρx̂ + (1−2ρ)x) + (1−w)x0
+//                  (cpu_backend.cpp:140-157); aty never goes to memory.
+//   k_iter_dual    tiles of Ã: ax_i = Σ ã_ij x̄_j, then ŷ, ȳ and the Halpern update of y
+//                  (cpu_backend.cpp:159-178); ax never goes to memory.
+// x̂, ŷ and ȳ are stored only on the iterations whose values are read afterwards (the first
+// step of an epoch, whose residual gives r⁰, and the last step before a check); the values
+// themselves are the reference's.
+//
+// A check runs four kernels over the vectors and the original matrix (KKT of the unscaled
+// point, termination.h:35-60; fixed-point residual, r2hpdhg.cpp:50-59; ray-test scalars,
+// engine.cpp:82-155; PID distances, r2hpdhg.cpp:127) that write per-element terms, then the
+// deterministic reducer, then ONE copy of ~44 doubles to the host.
+#include <cmath>
+#include <type_traits>
+
+#include "psh_internal.h"
+
+using namespace psh;
+
+namespace {
+
+// ------------------------------------------------------------------ iteration kernels
+template <class T>
+struct PrimalArgs {
+  T* x;
+  const T* x0;
+  const T *c, *l, *u;
+  T* xh;
+  T* xb;
+  T tau, tw, a, b;
+  int write_hat;
+};
+
+template <class T>
+struct EpiPrimal {
+  PrimalArgs<T> p;
+  __device__ void operator()(int j, bool has, double s) {
+    if (!has) return;
+    const T g = static_cast<T>(s);
+    const T xj = p.x[j];
+    const T h = clampv<T>(xj - p.tau * (p.c[j] - g), p.l[j], p.u[j]);
+    if (p.write_hat) p.xh[j] = h;
+    p.xb[j] = 2 * h - xj;
+    p.x[j] = p.tw * (p.a * h + p.b * xj) + (1 - p.tw) * p.x0[j];
+  }
+};
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_iter_primal(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
+                                                        const int* __restrict__ col, const T* __restrict__ val,
+                                                        const T* __restrict__ y, PrimalArgs<T> p) {
+  EpiPrimal<T> e{p};
+  tiled_rows<T, T>(tiles, ntiles, ptr, col, val, y, e);
+}
+
+template <class T>
+struct DualArgs {
+  T* y;
+  const T* y0;
+  const T *rl, *ru;
+  T* yh;
+  T* yb;
+  T sg, tw, a, b;
+  int write_hat;
+};
+
+template <class T>
+struct EpiDual {
+  DualArgs<T> p;
+  __device__ void operator()(int i, bool has, double s) {
+    if (!has) return;
+    const T ax = static_cast<T>(s);
+    const T yi = p.y[i];
+    const T q = ax - yi / p.sg;
+    const T h = yi - p.sg * ax + p.sg * clampv<T>(q, p.rl[i], p.ru[i]);
+    if (p.write_hat) {
+      p.yh[i] = h;
+      p.yb[i] = 2 * h - yi;
+    }
+    p.y[i] = p.tw * (p.a * h + p.b * yi) + (1 - p.tw) * p.y0[i];
+  }
+};
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_iter_dual(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
+                                                      const int* __restrict__ col, const T* __restrict__ val,
+                                                      const T* __restrict__ xb, DualArgs<T> p) {
+  EpiDual<T> e{p};
+  tiled_rows<T, T>(tiles, ntiles, ptr, col, val, xb, e);
+}
+
+template <class T>
+__global__ void k_restart(T* __restrict__ x, T* __restrict__ x0, const T* __restrict__ xh, long long n) {
+  const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const T v = xh[j];
+  x[j] = v;
+  x0[j] = v;
+}
+
+// ------------------------------------------------------------------ check kernels
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Order-independent maximum of non-negative values (NaN ignored, as std::max(acc, NaN) does).
+// Every lane of the wavefront must reach this call.
+__device__ inline void max_into(unsigned long long* slot, double v) {
+  v = v >= 0.0 ? v : 0.0;
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(slot, static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+__device__ inline double dmaxd(double a, double b) { return a < b ? b : a; }  // std::max
+__device__ inline double dmind(double a, double b) { return b < a ? b : a; }  // std::min
+
+__device__ inline double elem_viol(double v, double lo, double up) {  // cpu_backend.cpp:248-252
+  if (v < lo) return (lo - v) / (1.0 + fabs(lo));
+  if (v > up) return (v - up) / (1.0 + fabs(up));
+  return 0.0;
+}
+
+template <class T>
+struct CheckArgs {
+  unsigned flags;
+  int m, n;
+  // primal side
+  const T *xh, *xb, *x0;
+  T* dx;            // F_RES: Δx = x̄ − x̂ is written here; otherwise the ray direction is read here
+  // dual side
+  const T *yh, *yb, *y0;
+  T* dy;
+  T* ax;            // Ã·Δx (stored like the reference's work vector)
+  // matrices
+  const int *a_ptr, *a_col, *at_ptr, *ato_col;
+  const T* a_val;   // scaled Ã (working precision)
+  const double *a_orig, *ato_val;
+  // scaling and original data
+  const double *rs, *cs, *clo, *cuo, *rlo, *ruo, *cmin;
+  double bs, os;
+  // scratch
+  double *xo, *yo, *rv, *dv, *rv2, *dv2;
+  double* terms;
+  long long tl;
+  unsigned long long* maxes;
+};
+
+template <class T>
+__device__ inline double* term(const CheckArgs<T>& a, int k) {
+  return a.terms + static_cast<size_t>(k) * a.tl;
+}
+
+// Primal ray from a scaled direction (engine.cpp:132-139).
+__device__ inline double primal_ray_entry(double dxs, double cs, double bs, double lo, double up) {
+  double v = cs * dxs / bs;
+  if (isfinite(lo)) v = dmaxd(v, 0.0);
+  if (isfinite(up)) v = dmind(v, 0.0);
+  return v;
+}
+// Dual ray from a scaled direction (engine.cpp:90-97).
+__device__ inline double dual_ray_entry(double dys, double rs, double os, double lo, double up) {
+  double v = rs * dys / os;
+  if (!isfinite(lo)) v = dmind(v, 0.0);
+  if (!isfinite(up)) v = dmaxd(v, 0.0);
+  return v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_check_n(CheckArgs<T> a) {
+  const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool in = j < a.n;
+  double dmax_ray = 0, dmax_ray2 = 0;
+  if (in) {
+    T d = 0;
+    if (a.flags & F_RES) {
+      d = a.xb[j] - a.xh[j];
+      a.dx[j] = d;
+      term(a, T_DX2)[j] = static_cast<double>(d) * static_cast<double>(d);
+    } else if (a.flags & F_RAY) {
+      d = a.dx[j];
+    }
+    if (a.flags & F_KKT)  // unscale_primal (scaling.cpp:144-150)
+      a.xo[j] = dmind(dmaxd(a.cs[j] * static_cast<double>(a.xh[j]) / a.bs, a.clo[j]), a.cuo[j]);
+    if (a.flags & F_RAY) {
+      const double v = primal_ray_entry(static_cast<double>(d), a.cs[j], a.bs, a.clo[j], a.cuo[j]);
+      a.dv[j] = v;
+      dmax_ray = fabs(v);
+      term(a, T_CD)[j] = a.cmin[j] * v;
+      term(a, T_CMAG)[j] = fabs(a.cmin[j] * v);
+    }
+    if (a.flags & F_DRIFT) {
+      const T dd = a.xh[j] - a.x0[j];
+      const double v = primal_ray_entry(static_cast<double>(dd), a.cs[j], a.bs, a.clo[j], a.cuo[j]);
+      a.dv2[j] = v;
+      dmax_ray2 = fabs(v);
+      term(a, T_CD2)[j] = a.cmin[j] * v;
+      term(a, T_CMAG2)[j] = fabs(a.cmin[j] * v);
+    }
+    if (a.flags & F_PID) {
+      const double t = static_cast<double>(a.xh[j]) - static_cast<double>(a.x0[j]);
+      term(a, T_PIDX)[j] = t * t;
+    }
+  }
+  if (a.flags & F_RAY) max_into(a.maxes + M_DMAXRAY, dmax_ray);
+  if (a.flags & F_DRIFT) max_into(a.maxes + M_DMAXRAY2, dmax_ray2);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_check_m(CheckArgs<T> a) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool in = i < a.m;
+  double rmax = 0, rmax2 = 0;
+  if (in) {
+    T d = 0;
+    if (a.flags & F_RES) {
+      d = a.yb[i] - a.yh[i];
+      a.dy[i] = d;
+      term(a, T_DY2)[i] = static_cast<double>(d) * static_cast<double>(d);
+    } else if (a.flags & F_RAY) {
+      d = a.dy[i];
+    }
+    if (a.flags & F_KKT) a.yo[i] = a.rs[i] * static_cast<double>(a.yh[i]) / a.os;  // scaling.cpp:152-155
+    const double lo = a.rlo[i], up = a.ruo[i];
+    if (a.flags & F_RAY) {
+      const double r = dual_ray_entry(static_cast<double>(d), a.rs[i], a.os, lo, up);
+      a.rv[i] = r;
+      rmax = fabs(r);
+      const double t = r > 0 ? lo * r : (r < 0 ? up * r : 0.0);
+      term(a, T_ROBJ_ROW)[i] = t;
+      term(a, T_RMAG_ROW)[i] = fabs(t);
+    }
+    if (a.flags & F_DRIFT) {
+      const T dd = a.yh[i] - a.y0[i];
+      const double r = dual_ray_entry(static_cast<double>(dd), a.rs[i], a.os, lo, up);
+      a.rv2[i] = r;
+      rmax2 = fabs(r);
+      const double t = r > 0 ? lo * r : (r < 0 ? up * r : 0.0);
+      term(a, T_ROBJ_ROW2)[i] = t;
+      term(a, T_RMAG_ROW2)[i] = fabs(t);
+    }
+    if (a.flags & F_PID) {
+      const double t = static_cast<double>(a.yh[i]) - static_cast<double>(a.y0[i]);
+      term(a, T_PIDY)[i] = t * t;
+    }
+  }
+  if (a.flags & F_RAY) max_into(a.maxes + M_RMAX, rmax);
+  if (a.flags & F_DRIFT) max_into(a.maxes + M_RMAX2, rmax2);
+}
+
+// One thread per row of A (Ã and the original A share the structure).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool in = i < a.m;
+  double pmax = 0, dmax = 0, pviol = 0, pviol2 = 0;
+  if (in) {
+    const int b = a.a_ptr[i], e = a.a_ptr[i + 1];
+    if (a.flags & F_RES) {  // Ã·Δx (fp64 accumulate, stored in T), cross term Δyᵀ(ÃΔx)
+      double s = 0.0;
+      for (int p = b; p < e; ++p) s += static_cast<double>(a.a_val[p]) * static_cast<double>(a.dx[a.a_col[p]]);
+      const T ax = static_cast<T>(s);
+      a.ax[i] = ax;
+      term(a, T_CROSS)[i] = static_cast<double>(a.dy[i]) * static_cast<double>(ax);
+    }
+    const double lo = a.rlo[i], up = a.ruo[i];
+    if (a.flags & F_KKT) {  // row part of kkt_general (cpu_backend.cpp:254-270)
+      double s = 0.0;
+      for (int p = b; p < e; ++p) s += a.a_orig[p] * a.xo[a.a_col[p]];
+      const double viol = s < lo ? lo - s : (s > up ? s - up : 0.0);
+      term(a, T_RP2)[i] = viol * viol;
+      pmax = elem_viol(s, lo, up);
+      const double bl = isfinite(lo) ? fabs(lo) : 0.0, bu = isfinite(up) ? fabs(up) : 0.0;
+      term(a, T_B2)[i] = dmaxd(bl, bu) * dmaxd(bl, bu);
+      const double y = a.yo[i];
+      double dt = 0.0, rd = 0.0;
+      if (y > 0) {
+        if (isfinite(lo)) dt = y * lo;
+        else rd = y * y, dmax = y;
+      } else if (y < 0) {
+        if (isfinite(up)) dt = y * up;
+        else rd = y * y, dmax = -y;
+      }
+      term(a, T_DROW)[i] = dt;
+      term(a, T_RD2ROW)[i] = rd;
+    }
+    if (a.flags & F_RAY) {  // A·d against the finite row bounds (engine.cpp:140-145)
+      double s = 0.0;
+      for (int p = b; p < e; ++p) s += a.a_orig[p] * a.dv[a.a_col[p]];
+      if (isfinite(lo)) pviol = dmaxd(pviol, -s);
+      if (isfinite(up)) pviol = dmaxd(pviol, s);
+    }
+    if (a.flags & F_DRIFT) {
+      double s = 0.0;
+      for (int p = b; p < e; ++p) s += a.a_orig[p] * a.dv2[a.a_col[p]];
+      if (isfinite(lo)) pviol2 = dmaxd(pviol2, -s);
+      if (isfinite(up)) pviol2 = dmaxd(pviol2, s);
+    }
+  }
+  if (a.flags & F_KKT) {
+    max_into(a.maxes + M_PMAX, pmax);
+    max_into(a.maxes + M_DMAX, dmax);
+  }
+  if (a.flags & F_RAY) max_into(a.maxes + M_PVIOL, pviol);
+  if (a.flags & F_DRIFT) max_into(a.maxes + M_PVIOL2, pviol2);
+}
+
+// Dual-ray column terms of λ = −Aᵀr (engine.cpp:98-119).
+__device__ inline void ray_col_terms(double g, double gabs, double lo, double up, double& obj, double& mag, double& viol) {
+  const double lam = -g;
+  obj = 0.0;
+  mag = 0.0;
+  if (lam > 0) {
+    if (isfinite(lo)) obj = lo * lam, mag = gabs * fabs(lo);
+    else viol = dmaxd(viol, lam);
+  } else if (lam < 0) {
+    if (isfinite(up)) obj = up * lam, mag = gabs * fabs(up);
+    else viol = dmaxd(viol, -lam);
+  }
+}
+
+// One thread per row of the original Aᵀ (the model's CSC columns).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a) {
+  const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool in = j < a.n;
+  double pmax = 0, dmax = 0, cinf = 0, dviol = 0, dviol2 = 0;
+  if (in) {
+    const int b = a.at_ptr[j], e = a.at_ptr[j + 1];
+    const double lo = a.clo[j], up = a.cuo[j];
+    if (a.flags & F_KKT) {  // column part of kkt_general (cpu_backend.cpp:271-282)
+      double s = 0.0;
+      for (int p = b; p < e; ++p) s += a.ato_val[p] * a.yo[a.ato_col[p]];
+      const double cj = a.cmin[j], xj = a.xo[j];
+      term(a, T_C2)[j] = cj * cj;
+      cinf = fabs(cj);
+      term(a, T_P)[j] = cj * xj;
+      pmax = elem_viol(xj, lo, up);
+      const double lam = cj - s;
+      double z;
+      if (isfinite(lo) && isfinite(up)) z = lam;
+      else if (isfinite(lo)) z = dmaxd(lam, 0.0);
+      else if (isfinite(up)) z = dmind(lam, 0.0);
+      else z = 0.0;
+      term(a, T_RD2COL)[j] = (lam - z) * (lam - z);
+      dmax = fabs(lam - z);
+      term(a, T_DCOL)[j] = z > 0 ? z * lo : (z < 0 ? z * up : 0.0);
+    }
+    if (a.flags & F_RAY) {
+      double g = 0.0, gabs = 0.0;
+      for (int p = b; p < e; ++p) {
+        const double r = a.rv[a.ato_col[p]];
+        g += a.ato_val[p] * r;
+        gabs += fabs(a.ato_val[p] * r);
+      }
+      double obj, mag;
+      ray_col_terms(g, gabs, lo, up, obj, mag, dviol);
+      term(a, T_ROBJ_COL)[j] = obj;
+      term(a, T_RMAG_COL)[j] = mag;
+    }
+    if (a.flags & F_DRIFT) {
+      double g = 0.0, gabs = 0.0;
+      for (int p = b; p < e; ++p) {
+        const double r = a.rv2[a.ato_col[p]];
+        g += a.ato_val[p] * r;
+        gabs += fabs(a.ato_val[p] * r);
+      }
+      double obj, mag;
+      ray_col_terms(g, gabs, lo, up, obj, mag, dviol2);
+      term(a, T_ROBJ_COL2)[j] = obj;
+      term(a, T_RMAG_COL2)[j] = mag;
+    }
+  }
+  if (a.flags & F_KKT) {
+    max_into(a.maxes + M_PMAX, pmax);
+    max_into(a.maxes + M_DMAX, dmax);
+    max_into(a.maxes + M_CINF, cinf);
+  }
+  if (a.flags & F_RAY) max_into(a.maxes + M_DVIOL, dviol);
+  if (a.flags & F_DRIFT) max_into(a.maxes + M_DVIOL2, dviol2);
+}
+
+__global__ void k_zero(double* p, int n) {
+  if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0.0;
+}
+
+template <class T>
+T* sp(psh_ctx* c, int s) {
+  if constexpr (sizeof(T) == 8) return c->vd[s];
+  else return c->vf[s];
+}
+
+template <class T>
+CheckArgs<T> make_args(psh_ctx* c, unsigned flags, int xh, int xb, int x0, int dx, int yh, int yb, int y0, int dy) {
+  CheckArgs<T> a{};
+  a.flags = flags;
+  a.m = c->m;
+  a.n = c->n;
+  a.xh = sp<T>(c, xh);
+  a.xb = sp<T>(c, xb);
+  a.x0 = sp<T>(c, x0);
+  a.dx = sp<T>(c, dx);
+  a.yh = sp<T>(c, yh);
+  a.yb = sp<T>(c, yb);
+  a.y0 = sp<T>(c, y0);
+  a.dy = sp<T>(c, dy);
+  a.ax = sp<T>(c, PSH_AX);
+  a.a_ptr = c->A.ptr;
+  a.a_col = c->A.col;
+  a.at_ptr = c->At.ptr;
+  a.ato_col = c->ato_col;
+  if constexpr (sizeof(T) == 8) a.a_val = c->A.val;
+  else a.a_val = c->A.valf;
+  a.a_orig = c->a_orig;
+  a.ato_val = c->ato_val;
+  a.rs = c->rs;
+  a.cs = c->cs;
+  a.clo = c->clo;
+  a.cuo = c->cuo;
+  a.rlo = c->rlo;
+  a.ruo = c->ruo;
+  a.cmin = c->cmin;
+  a.bs = c->bs;
+  a.os = c->os;
+  a.xo = c->xo;
+  a.yo = c->yo;
+  a.rv = c->rv;
+  a.dv = c->dv;
+  a.rv2 = c->rv2;
+  a.dv2 = c->dv2;
+  a.terms = c->terms;
+  a.tl = c->term_len;
+  a.maxes = reinterpret_cast<unsigned long long*>(c->results + R_MAX0);
+  return a;
+}
+
+// Launches the four check kernels for `flags`; the dual-side pass runs before the row pass
+// because the row pass reads Δy, ŷ (unscaled) and the dual ray.
+template <class T>
+int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool zero_maxes) {
+  if (zero_maxes) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c->stream, c->results + R_MAX0, static_cast<int>(kMaxCount));
+  const unsigned nf = a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT | F_PID);
+  const unsigned mf = nf;
+  if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
+  if (mf && c->m > 0) hipLaunchKernelGGL((k_check_m<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
+  if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->m > 0)
+    hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
+  if ((a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->n > 0)
+    hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
+  PSH_LAUNCHED(c, "check kernels");
+  return PSH_OK;
+}
+
+template <class F>
+int with_t(psh_ctx* c, F&& f) {
+  if (c->fp32) return f(static_cast<float*>(nullptr));
+  return f(static_cast<double*>(nullptr));
+}
+
+double max_of(const double* r, int k) { return r[R_MAX0 + k]; }
+
+void fill_kkt(const double* r, psh_kkt* k) {  // termination.h quantities from the sums
+  k->primal_residual = std::sqrt(r[R_RP2]);
+  k->dual_residual = std::sqrt(r[R_RD2]);
+  k->primal_obj = r[R_P];
+  k->dual_obj = r[R_D];
+  k->b_norm = std::sqrt(r[R_B2]);
+  k->c_norm = std::sqrt(r[R_C2]);
+  k->primal_max_rel = max_of(r, M_PMAX);
+  k->dual_max_rel = max_of(r, M_DMAX) / (1.0 + max_of(r, M_CINF));
+}
+
+void fill_ray(const double* r, int second, psh_ray* o) {
+  o->rmax = max_of(r, second ? M_RMAX2 : M_RMAX);
+  o->dual_obj = r[second ? R_ROBJ2 : R_ROBJ];
+  o->dual_mag = r[second ? R_RMAG2 : R_RMAG];
+  o->dual_viol = max_of(r, second ? M_DVIOL2 : M_DVIOL);
+  o->dmax = max_of(r, second ? M_DMAXRAY2 : M_DMAXRAY);
+  o->primal_cd = r[second ? R_CD2 : R_CD];
+  o->primal_cmag = r[second ? R_CMAG2 : R_CMAG];
+  o->primal_viol = max_of(r, second ? M_PVIOL2 : M_PVIOL);
+}
+
+int check_slots(psh_ctx* c) {
+  if (!c) return PSH_E_ARG;
+  if (!c->loaded) return fail(c, PSH_E_STATE, "no problem loaded");
+  return PSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
+  int rc = check_slots(c);
+  if (rc) return rc;
+  if (!g || g->iterations <= 0) return fail(c, PSH_E_ARG, "chunk needs at least one iteration");
+  return with_t(c, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    const T* at_val;
+    const T* a_val;
+    if constexpr (sizeof(T) == 8) at_val = c->At.val, a_val = c->A.val;
+    else at_val = c->At.valf, a_val = c->A.valf;
+    const T *cc, *ll, *uu, *rl, *ru;
+    if constexpr (sizeof(T) == 8) cc = c->c, ll = c->l, uu = c->u, rl = c->rl, ru = c->ru;
+    else cc = c->cf, ll = c->lf, uu = c->uf, rl = c->rlf, ru = c->ruf;
+    const T a2 = static_cast<T>(2 * g->rho), b2 = static_cast<T>(1 - 2 * g->rho);
+    for (int s = 0; s < g->iterations; ++s) {
+      const long long inner = g->inner0 + s;
+      const double w = static_cast<double>(inner + 1) / static_cast<double>(inner + 2);
+      const bool epoch_start = inner == 0;
+      const int hat = epoch_start || (s == g->iterations - 1 && g->want_hat_last) ? 1 : 0;
+      PrimalArgs<T> pa{sp<T>(c, PSH_X), sp<T>(c, PSH_X0), cc, ll, uu, sp<T>(c, PSH_XH), sp<T>(c, PSH_XB),
+                       static_cast<T>(g->tau), static_cast<T>(w), a2, b2, hat};
+      DualArgs<T> da{sp<T>(c, PSH_Y), sp<T>(c, PSH_Y0), rl, ru, sp<T>(c, PSH_YH), sp<T>(c, PSH_YB),
+                     static_cast<T>(g->sigma), static_cast<T>(w), a2, b2, hat};
+      hipEvent_t ev;
+      if (c->At.ntiles > 0) {
+        time_begin(c, 0, &ev);
+        hipLaunchKernelGGL((k_iter_primal<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
+                           c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, at_val, sp<T>(c, PSH_Y), pa);
+        time_end(c, 0, ev);
+      }
+      if (c->A.ntiles > 0) {
+        time_begin(c, 1, &ev);
+        hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, c->A.tiles,
+                           c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
+        time_end(c, 1, ev);
+      }
+      PSH_LAUNCHED(c, "iteration kernels");
+      if (epoch_start) {  // residual at the epoch's anchor (r2hpdhg.cpp:75), kept on the device
+        const CheckArgs<T> a = make_args<T>(c, F_RES, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
+        int r;
+        if ((r = launch_check<T>(c, a, false))) return r;
+        int err = PSH_OK;
+        const JobTable* t = check_table(c, g->mode, F_RES, &err);
+        if (!t) return err;
+        JobTable r0;
+        r0.dev = t->dev;
+        r0.host.assign(t->host.begin(), t->host.begin() + 3);
+        if ((r = reduce_table(c, r0, R_R0))) return r;
+        c->have_r0 = true;
+      }
+    }
+    return PSH_OK;
+  });
+}
+
+int psh_check(psh_ctx* c, const psh_check_args* g, psh_stats* out) {
+  int rc = check_slots(c);
+  if (rc) return rc;
+  if (!g || !out) return fail(c, PSH_E_ARG, "null check arguments");
+  unsigned flags = F_RES | F_KKT;
+  if (g->want_ray) flags |= F_RAY;
+  if (g->want_drift) flags |= F_DRIFT;
+  if (g->want_pid) flags |= F_PID;
+  rc = with_t(c, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    const CheckArgs<T> a = make_args<T>(c, flags, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
+    hipEvent_t ev;
+    time_begin(c, 2, &ev);
+    int r = launch_check<T>(c, a, true);
+    time_end(c, 2, ev);
+    return r;
+  });
+  if (rc) return rc;
+  int err = PSH_OK;
+  const JobTable* t = check_table(c, g->mode, flags, &err);
+  if (!t) return err;
+  if ((rc = reduce_table(c, *t, 0))) return rc;
+  double r[kResultCount];
+  if ((rc = fetch_results(c, kResultCount, r))) return rc;
+  out->px = r[R_PX];
+  out->py = r[R_PY];
+  out->cross = r[R_CROSS];
+  out->have_r0 = c->have_r0 ? 1 : 0;
+  out->r0_px = r[R_R0];
+  out->r0_py = r[R_R0 + 1];
+  out->r0_cross = r[R_R0 + 2];
+  c->have_r0 = false;
+  fill_kkt(r, &out->kkt);
+  fill_ray(r, 0, &out->ray);
+  fill_ray(r, 1, &out->drift);
+  out->dx_norm = std::sqrt(r[R_PIDX]);
+  out->dy_norm = std::sqrt(r[R_PIDY]);
+  bool fin = true;
+  for (int k = 0; k < kSumCount; ++k) fin = fin && std::isfinite(r[k]);
+  out->finite = fin ? 1 : 0;
+  return PSH_OK;
+}
+
+int psh_kkt_of(psh_ctx* c, int xs, int ys, int mode, psh_kkt* out) {
+  int rc = check_slots(c);
+  if (rc) return rc;
+  if (!out || xs < 0 || xs >= PSH_NSLOT || ys < 0 || ys >= PSH_NSLOT || !is_primal_slot(xs) || is_primal_slot(ys))
+    return fail(c, PSH_E_ARG, "kkt needs a primal and a dual slot");
+  rc = with_t(c, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    const CheckArgs<T> a = make_args<T>(c, F_KKT, xs, PSH_XB, PSH_X0, PSH_DX, ys, PSH_YB, PSH_Y0, PSH_DY);
+    return launch_check<T>(c, a, true);
+  });
+  if (rc) return rc;
+  int err = PSH_OK;
+  const JobTable* t = check_table(c, mode, F_KKT, &err);
+  if (!t) return err;
+  if ((rc = reduce_table(c, *t, 0))) return rc;
+  double r[kResultCount];
+  if ((rc = fetch_results(c, kResultCount, r))) return rc;
+  fill_kkt(r, out);
+  return PSH_OK;
+}
+
+int psh_ray_of(psh_ctx* c, int dxs, int dys, psh_ray* out) {
+  int rc = check_slots(c);
+  if (rc) return rc;
+  if (!out || dxs < 0 || dxs >= PSH_NSLOT || dys < 0 || dys >= PSH_NSLOT || !is_primal_slot(dxs) || is_primal_slot(dys))
+    return fail(c, PSH_E_ARG, "ray test needs a primal and a dual slot");
+  rc = with_t(c, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    const CheckArgs<T> a = make_args<T>(c, F_RAY, PSH_XH, PSH_XB, PSH_X0, dxs, PSH_YH, PSH_YB, PSH_Y0, dys);
+    return launch_check<T>(c, a, true);
+  });
+  if (rc) return rc;
+  int err = PSH_OK;
+  const JobTable* t = check_table(c, PSH_RED_TREE, F_RAY, &err);
+  if (!t) return err;
+  if ((rc = reduce_table(c, *t, 0))) return rc;
+  double r[kResultCount];
+  if ((rc = fetch_results(c, kResultCount, r))) return rc;
+  fill_ray(r, 0, out);
+  return PSH_OK;
+}
+
+int psh_ray_vectors(psh_ctx* c, int drift, double* dual_ray, double* primal_ray) {
+  int rc = check_slots(c);
+  if (rc) return rc;
+  if (dual_ray && c->m > 0)
+    PSH_TRY(c, hipMemcpyAsync(dual_ray, drift ? c->rv2 : c->rv, sizeof(double) * c->m, hipMemcpyDeviceToHost, c->stream));
+  if (primal_ray && c->n > 0)
+    PSH_TRY(c, hipMemcpyAsync(primal_ray, drift ? c->dv2 : c->dv, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  return PSH_OK;
+}
+
+int psh_restart(psh_ctx* c) {
+  int rc = check_slots(c);
+  if (rc) return rc;
+  return with_t(c, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    if (c->n > 0)
+      hipLaunchKernelGGL((k_restart<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, sp<T>(c, PSH_X),
+                         sp<T>(c, PSH_X0), sp<T>(c, PSH_XH), (long long)c->n);
+    if (c->m > 0)
+      hipLaunchKernelGGL((k_restart<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, sp<T>(c, PSH_Y),
+                         sp<T>(c, PSH_Y0), sp<T>(c, PSH_YH), (long long)c->m);
+    PSH_LAUNCHED(c, "restart");
+    return PSH_OK;
+  });
+}
+
+}  // extern "C"

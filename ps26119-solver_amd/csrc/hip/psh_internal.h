@@ -1,0 +1,227 @@
+// psh_internal.h — device-side state and helpers shared by the HIP translation units of
+// libps26119_hip.so. Not part of the public C-ABI (include/ps26119_hip.h).
+//
+// Design (MI355X / gfx950):
+//  * Sparse products are row-tiled: a tile is ≤ 256 consecutive rows holding ≤ kTileNnz
+//    nonzeros (or a single longer row). One 256-thread workgroup per tile stages the tile's
+//    products val·x[col] in LDS with coalesced loads of col/val and parallel gathers, then each
+//    thread sums ITS row sequentially in CSR order — the reference CPU order (csr.h:36-39), so
+//    fp64 results are bit-identical to the CPU backend — and runs the fused epilogue.
+//  * Workgroup → tile mapping is XCD-aware: consecutive tiles go to the same XCD (dispatch is
+//    round-robin over the 8 XCDs), so gathers of neighbouring tiles share that XCD's L2.
+//  * All arithmetic on the iterate follows the reference expressions operation for operation,
+//    compiled with -ffp-contract=off (no FMA contraction), so it matches the CPU bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ps26119_hip.h"
+
+namespace psh {
+
+constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
+constexpr int kTileNnz = 2048;   // products staged in LDS per tile pass (16 KiB of fp64)
+constexpr int kXcd = 8;          // XCDs per MI355X
+constexpr int kTreeBlocks = 512; // fixed first-level block count of tree reductions
+
+struct DevMat {
+  int rows = 0;
+  int* ptr = nullptr;     // rows+1
+  int* col = nullptr;     // nnz
+  double* val = nullptr;  // nnz, fp64
+  float* valf = nullptr;  // nnz, fp32 copy (allocated on first switch to fp32)
+  int ntiles = 0;
+  int* tiles = nullptr;   // ntiles+1 tile start rows
+};
+
+// A reduction job: Σ or max over t[0..na) followed by u[0..nb) (u optional).
+struct RedJob {
+  const double* a;
+  const double* b;
+  long long na, nb;
+  int op;    // 0 sum, 1 max
+  int mode;  // 0 tree, 1 64-chunk CPU order, 2 single sequential loop
+};
+
+struct JobTable {
+  RedJob* dev = nullptr;
+  std::vector<RedJob> host;
+};
+
+// Per-element term arrays of one check pass (each c->term_len doubles in c->terms).
+enum Term {
+  T_DX2, T_DY2, T_CROSS,                               // fixed-point residual
+  T_RP2, T_B2, T_DROW, T_RD2ROW,                       // KKT, row part
+  T_C2, T_P, T_DCOL, T_RD2COL,                         // KKT, column part
+  T_PIDX, T_PIDY,                                      // ‖x̂ − x0‖², ‖ŷ − y0‖²
+  T_ROBJ_ROW, T_RMAG_ROW, T_ROBJ_COL, T_RMAG_COL, T_CD, T_CMAG,        // ray test of T(z) − z
+  T_ROBJ_ROW2, T_RMAG_ROW2, T_ROBJ_COL2, T_RMAG_COL2, T_CD2, T_CMAG2,  // ray test of the drift
+  kTermCount
+};
+// Result slots in c->results.
+enum Result {
+  R_PX, R_PY, R_CROSS, R_RP2, R_B2, R_D, R_RD2, R_C2, R_P, R_PIDX, R_PIDY,
+  R_ROBJ, R_RMAG, R_CD, R_CMAG, R_ROBJ2, R_RMAG2, R_CD2, R_CMAG2, kSumCount,
+  R_MAX0 = 24,  // maxima, stored as fp64 bit patterns by atomicMax
+  R_R0 = 40,    // residual sums of the last epoch start (px, py, cross)
+  kResultCount = 44
+};
+enum Max { M_PMAX, M_DMAX, M_CINF, M_RMAX, M_DVIOL, M_DMAXRAY, M_PVIOL, M_RMAX2, M_DVIOL2, M_DMAXRAY2, M_PVIOL2, kMaxCount };
+enum Flag : unsigned { F_RES = 1, F_KKT = 2, F_RAY = 4, F_DRIFT = 8, F_PID = 16 };
+
+}  // namespace psh
+
+struct psh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int m = 0, n = 0;
+  long long nnz = 0;
+  bool loaded = false, fp32 = false;
+  size_t bytes = 0;
+
+  psh::DevMat A, At;             // scaled Ã (m×n) and Ãᵀ (n×m)
+  double* a_orig = nullptr;      // original A values, Ã's structure
+  int* ato_col = nullptr;        // original Aᵀ (CSC) row indices (aliases At.col when equal)
+  bool ato_alias = false;
+  double* ato_val = nullptr;
+
+  double *c = nullptr, *l = nullptr, *u = nullptr, *rl = nullptr, *ru = nullptr;
+  float *cf = nullptr, *lf = nullptr, *uf = nullptr, *rlf = nullptr, *ruf = nullptr;
+  double *rs = nullptr, *cs = nullptr;
+  double bs = 1, os = 1;
+  double *clo = nullptr, *cuo = nullptr, *rlo = nullptr, *ruo = nullptr, *cmin = nullptr;
+
+  double* vd[PSH_NSLOT] = {};
+  float* vf[PSH_NSLOT] = {};
+
+  // check-pass scratch
+  double *xo = nullptr, *yo = nullptr;          // unscaled iterate
+  double *rv = nullptr, *dv = nullptr;          // dual / primal ray of (Δx, Δy)
+  double *rv2 = nullptr, *dv2 = nullptr;        // rays of the drift
+  double* terms = nullptr;                      // kTerms arrays of max(m, n)
+  long long term_len = 0;
+  double* partials = nullptr;                   // tree first level
+  double* results = nullptr;                    // device results
+  double* host_results = nullptr;               // pinned
+  psh::RedJob* jobs_dev = nullptr;
+  psh::RedJob* jobs_host = nullptr;             // pinned
+
+  // Lanczos
+  double *lv = nullptr, *lvp = nullptr, *lw = nullptr, *lt = nullptr;
+
+  // epoch-start residual sums kept on the device (in `results`) until the next check
+  double* r0_dev = nullptr;
+  bool have_r0 = false;
+  // persistent reduction tables, keyed by (mode, check flags)
+  std::map<int, psh::JobTable> tables;
+
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending { int cls; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  long long t_launch[3] = {0, 0, 0};
+  double t_ms[3] = {0, 0, 0};
+};
+
+namespace psh {
+
+constexpr int kTerms = kTermCount;  // term arrays in the scratch area
+constexpr int kMaxJobs = 64;
+
+int fail(psh_ctx* c, int code, const std::string& what);
+int hip_fail(psh_ctx* c, hipError_t e, const char* where);
+#define PSH_TRY(ctx, call)                                   \
+  do {                                                       \
+    hipError_t e_ = (call);                                  \
+    if (e_ != hipSuccess) return psh::hip_fail(ctx, e_, #call); \
+  } while (0)
+#define PSH_LAUNCHED(ctx, where)                                   \
+  do {                                                             \
+    hipError_t e_ = hipGetLastError();                             \
+    if (e_ != hipSuccess) return psh::hip_fail(ctx, e_, where);    \
+  } while (0)
+
+int dev_alloc(psh_ctx* c, void** p, size_t bytes);
+template <class T>
+int alloc_n(psh_ctx* c, T** p, long long count) {
+  return dev_alloc(c, reinterpret_cast<void**>(p), static_cast<size_t>(count > 0 ? count : 1) * sizeof(T));
+}
+void dev_free(void* p);
+
+std::vector<int> build_tiles(const int* ptr, int rows);
+inline int grid_for_tiles(int ntiles) { return ((ntiles + kXcd - 1) / kXcd) * kXcd; }
+inline int grid_for(long long n) {
+  long long g = (n + kBlock - 1) / kBlock;
+  return static_cast<int>(g < 1 ? 1 : g);
+}
+bool is_primal_slot(int s);
+long long slot_len(const psh_ctx* c, int s);
+
+// Runs `njobs` reductions (sums in the job's mode, maxes as trees); results land in
+// c->results[first .. first+njobs) on the device (not synchronised).
+int reduce(psh_ctx* c, const RedJob* jobs, int njobs, int first);
+// Same, with a table already on the device (no upload, no synchronisation).
+int reduce_table(psh_ctx* c, const JobTable& t, int first);
+// The table for a check pass with `flags` in reduction `mode` (created on first use).
+const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err);
+// Copies `count` device results to the host (synchronises the stream).
+int fetch_results(psh_ctx* c, int count, double* out);
+
+// timing helpers
+void time_begin(psh_ctx* c, int cls, hipEvent_t* a);
+void time_end(psh_ctx* c, int cls, hipEvent_t a);
+
+}  // namespace psh
+
+// ------------------------------------------------------------------ device helpers
+namespace psh {
+
+// Workgroup → tile: the dispatcher deals workgroups round-robin to the 8 XCDs, so give each XCD
+// a contiguous run of tiles.
+__device__ inline int tile_of_block(int bid, int nblocks) {
+  const int per = nblocks / kXcd;
+  return (bid % kXcd) * per + bid / kXcd;
+}
+
+template <class T>
+__device__ inline T clampv(T v, T lo, T hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Row-tiled sparse product with an fp64 row-sequential accumulator (CSR order) and a fused
+// epilogue epi(row, has_row, sum) run by every thread of the tile's workgroup.
+template <class V, class X, class Epi>
+__device__ inline void tiled_rows(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
+                                  const int* __restrict__ col, const V* __restrict__ val, const X* __restrict__ x,
+                                  Epi& epi) {
+  __shared__ double prod[kTileNnz];
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  if (tile >= ntiles) return;
+  const int r0 = tiles[tile], r1 = tiles[tile + 1];
+  const int p0 = ptr[r0], p1 = ptr[r1];
+  const int row = r0 + static_cast<int>(threadIdx.x);
+  const bool has = row < r1;
+  const int rb = has ? ptr[row] : 0, re = has ? ptr[row + 1] : 0;
+  double s = 0.0;
+  for (int cb = p0; cb < p1; cb += kTileNnz) {
+    const int ce = p1 - cb > kTileNnz ? cb + kTileNnz : p1;
+    for (int p = cb + static_cast<int>(threadIdx.x); p < ce; p += kBlock)
+      prod[p - cb] = static_cast<double>(val[p]) * static_cast<double>(x[col[p]]);
+    __syncthreads();
+    if (has) {
+      const int b = rb > cb ? rb : cb, e = re < ce ? re : ce;
+      for (int p = b; p < e; ++p) s += prod[p - cb];
+    }
+    __syncthreads();
+  }
+  epi(row, has, s);
+}
+
+}  // namespace psh

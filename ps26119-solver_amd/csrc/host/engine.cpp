@@ -1,0 +1,630 @@
+// engine.cpp — r²HPDHG on the MI355X. The driver follows the reference loop decision for
+// decision (src/pdhg/r2hpdhg.cpp:11-162, src/pdhg/engine.cpp:56-307); what changes is where the
+// work runs: each block of K iterations is one psh_chunk (2 fused kernels per iteration, no host
+// round trip), and each check is one psh_check (residual, KKT, ray-test scalars and PID
+// distances in one device pass, one ~350-byte copy back). Only decisions run on the host.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <new>
+#include <stdexcept>
+
+#include "la.h"
+
+namespace ps26119::amd {
+
+// ------------------------------------------------------------------ options
+EngineOptions engine_options_from(const Options& o) {
+  EngineOptions e;
+  e.tolerance = o.tolerance;
+  e.time_limit = o.time_limit;
+  e.iteration_limit = o.iteration_limit;
+  e.check_every = o.termination_check_every > 0 ? o.termination_check_every : 64;
+  e.precision = o.precision;
+  e.verbosity = o.verbosity;
+  e.scaling.ruiz_iterations = o.ruiz_iterations;
+  e.scaling.pock_chambolle = o.pock_chambolle;
+  e.warm_x = o.warm_x;
+  e.warm_y = o.warm_y;
+  e.warm_primal_weight = o.warm_primal_weight;
+  e.kkt_b_norm = o.kkt_b_norm;
+  e.kkt_c_norm = o.kkt_c_norm;
+  e.kkt_obj_shift = o.kkt_obj_shift;
+  return e;
+}
+
+std::string apply_engine_params(const Options& o, EngineOptions& e) {  // reference engine.cpp:34-53
+  for (const auto& [name, v] : o.engine_params) {
+    if (name == "reflection") e.reflection = v;
+    else if (name == "restart_sufficient") e.restart_sufficient = v;
+    else if (name == "restart_necessary") e.restart_necessary = v;
+    else if (name == "restart_artificial") e.restart_artificial = v;
+    else if (name == "pid_kp") e.pid_kp = v;
+    else if (name == "pid_ki") e.pid_ki = v;
+    else if (name == "pid_kd") e.pid_kd = v;
+    else if (name == "pid_integral_decay") e.pid_integral_decay = v;
+    else if (name == "pid_max_log_step") e.pid_max_log_step = v;
+    else if (name == "bound_objective_rescaling") e.scaling.bound_objective_rescaling = v != 0;
+    else if (name == "geometric_mean_iterations") e.scaling.geometric_mean_iterations = static_cast<int>(v);
+    else if (name == "geometric_mean_min_log10_range") e.scaling.geometric_mean_min_log10_range = v;
+    else if (name == "ruiz_iterations") e.scaling.ruiz_iterations = static_cast<int>(v);
+    else if (name == "pock_chambolle") e.scaling.pock_chambolle = v != 0;
+    else if (name == "hip_cpu_order") e.cpu_order = v != 0 ? 1 : 0;
+    else if (name == "hip_device") e.device = static_cast<int>(v);
+    else if (name == "hip_timing") e.timing = v != 0 ? 1 : 0;
+    else return name;
+  }
+  return {};
+}
+
+// ------------------------------------------------------------------ KKT
+double KktStats::rel_gap() const {
+  return std::fabs(primal_obj - dual_obj) / (1.0 + std::fabs(primal_obj + obj_shift) + std::fabs(dual_obj + obj_shift));
+}
+double KktStats::rel_kkt() const { return std::max({rel_primal(), rel_dual(), rel_gap()}); }
+bool KktStats::converged(double eps) const {
+  if (rel_kkt() > eps) return false;
+  if (eps >= tol::kVerifyPrimal) return true;
+  return primal_max_rel <= tol::kVerifyPrimal && dual_max_rel <= tol::kVerifyDual && rel_gap() <= tol::kVerifyGap;
+}
+bool KktStats::finite() const {
+  return std::isfinite(primal_residual) && std::isfinite(dual_residual) && std::isfinite(primal_obj) && std::isfinite(dual_obj);
+}
+
+KktStats kkt_on_original(const ScaledProblem& sp, const std::vector<double>& x, const std::vector<double>& y) {
+  const Model& M = *sp.original;
+  const int m = sp.m, n = sp.n;
+  const double sense = M.sense;
+  std::vector<double> ax(m), aty(n);
+  Csr A;  // the original A in Ã's layout
+  A.rows = m;
+  A.cols = n;
+  A.ptr = sp.A.ptr;
+  A.col = sp.A.col;
+  A.val = sp.a_orig;
+  multiply(A, x.data(), ax.data());
+  const Csr At = csc_as_transpose(M);
+  multiply(At, y.data(), aty.data());
+  double rp2 = 0, rd2 = 0, b2 = 0, c2 = 0, p = 0, d = 0, cinf = 0, pmax = 0, dmax = 0;
+  auto elem_viol = [](double v, double lo, double up) {
+    if (v < lo) return (lo - v) / (1.0 + std::fabs(lo));
+    if (v > up) return (v - up) / (1.0 + std::fabs(up));
+    return 0.0;
+  };
+  for (int i = 0; i < m; ++i) {
+    const double lo = M.row_lower[i], up = M.row_upper[i];
+    const double viol = ax[i] < lo ? lo - ax[i] : (ax[i] > up ? ax[i] - up : 0.0);
+    rp2 += viol * viol;
+    pmax = std::max(pmax, elem_viol(ax[i], lo, up));
+    const double bl = std::isfinite(lo) ? std::fabs(lo) : 0.0, bu = std::isfinite(up) ? std::fabs(up) : 0.0;
+    b2 += std::max(bl, bu) * std::max(bl, bu);
+    if (y[i] > 0) {
+      if (std::isfinite(lo)) d += y[i] * lo;
+      else rd2 += y[i] * y[i], dmax = std::max(dmax, y[i]);
+    } else if (y[i] < 0) {
+      if (std::isfinite(up)) d += y[i] * up;
+      else rd2 += y[i] * y[i], dmax = std::max(dmax, -y[i]);
+    }
+  }
+  for (int j = 0; j < n; ++j) {
+    const double cj = sense * M.obj[j];
+    c2 += cj * cj;
+    cinf = std::max(cinf, std::fabs(cj));
+    p += cj * x[j];
+    pmax = std::max(pmax, elem_viol(x[j], M.col_lower[j], M.col_upper[j]));
+    const double lam = cj - aty[j];
+    const double lo = M.col_lower[j], up = M.col_upper[j];
+    double z;
+    if (std::isfinite(lo) && std::isfinite(up)) z = lam;
+    else if (std::isfinite(lo)) z = std::max(lam, 0.0);
+    else if (std::isfinite(up)) z = std::min(lam, 0.0);
+    else z = 0.0;
+    rd2 += (lam - z) * (lam - z);
+    dmax = std::max(dmax, std::fabs(lam - z));
+    if (z > 0) d += z * lo;
+    else if (z < 0) d += z * up;
+  }
+  KktStats k;
+  k.primal_residual = std::sqrt(rp2);
+  k.dual_residual = std::sqrt(rd2);
+  k.primal_obj = p;
+  k.dual_obj = d;
+  k.b_norm = std::sqrt(b2);
+  k.c_norm = std::sqrt(c2);
+  k.primal_max_rel = pmax;
+  k.dual_max_rel = dmax / (1.0 + cinf);
+  return k;
+}
+
+static KktStats from_device(const psh_kkt& d) {
+  KktStats k;
+  k.primal_residual = d.primal_residual;
+  k.dual_residual = d.dual_residual;
+  k.primal_obj = d.primal_obj;
+  k.dual_obj = d.dual_obj;
+  k.b_norm = d.b_norm;
+  k.c_norm = d.c_norm;
+  k.primal_max_rel = d.primal_max_rel;
+  k.dual_max_rel = d.dual_max_rel;
+  return k;
+}
+
+// ------------------------------------------------------------------ device
+std::unique_ptr<Device> Device::open(int device, std::string& error) {
+  psh_ctx* c = nullptr;
+  const int rc = psh_create(device, &c);
+  if (rc != PSH_OK) {
+    error = rc == PSH_E_NO_DEVICE ? "HIP backend unavailable: no usable MI355X/gfx950 device is visible"
+                                  : "HIP backend unavailable: device context creation failed";
+    return nullptr;
+  }
+  return std::unique_ptr<Device>(new Device(c));
+}
+
+Device::~Device() { psh_destroy(ctx_); }
+
+void Device::check(int rc, const char* what) const {
+  if (rc == PSH_OK) return;
+  if (rc == PSH_E_OOM) throw std::bad_alloc();
+  throw std::runtime_error(std::string("HIP backend: ") + what + ": " + psh_error(ctx_));
+}
+
+void Device::load(const ScaledProblem& sp) {
+  const Model& M = *sp.original;
+  std::vector<double> cmin(sp.n);
+  for (int j = 0; j < sp.n; ++j) cmin[j] = M.sense * M.obj[j];
+  psh_problem p{};
+  p.m = sp.m;
+  p.n = sp.n;
+  p.nnz = sp.A.nnz();
+  p.a_ptr = sp.A.ptr.data();
+  p.a_col = sp.A.col.data();
+  p.a_val = sp.A.val.data();
+  p.a_val_orig = sp.a_orig.data();
+  p.at_ptr = sp.At.ptr.data();
+  p.at_col = sp.At.col.data();
+  p.ato_col = M.row_index.data();
+  p.at_val = sp.At.val.data();
+  p.ato_val = M.value.data();
+  p.c = sp.c.data();
+  p.l = sp.col_lower.data();
+  p.u = sp.col_upper.data();
+  p.rl = sp.row_lower.data();
+  p.ru = sp.row_upper.data();
+  p.row_scale = sp.row_scale.data();
+  p.col_scale = sp.col_scale.data();
+  p.bound_scale = sp.bound_scale;
+  p.obj_scale = sp.obj_scale;
+  p.cl_o = M.col_lower.data();
+  p.cu_o = M.col_upper.data();
+  p.rl_o = M.row_lower.data();
+  p.ru_o = M.row_upper.data();
+  p.c_min = cmin.data();
+  check(psh_load(ctx_, &p), "upload");
+}
+
+// ------------------------------------------------------------------ ‖Ã‖₂
+namespace {
+
+// Largest eigenvalue of the symmetric tridiagonal (a, b) by Sturm-count bisection
+// (reference src/la/csr.cpp:44-69).
+double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& b, int k) {
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < k; ++i) {
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  auto count_below = [&](double x) {
+    int c = 0;
+    double d = 1.0;
+    for (int i = 0; i < k; ++i) {
+      const double bb = i > 0 ? b[i - 1] * b[i - 1] : 0.0;
+      d = (a[i] - x) - (i > 0 ? bb / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(hi)); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) == k) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+}  // namespace
+
+double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter, double rel_tol) {
+  psh_ctx* c = dev.ctx();
+  if (n == 0) return 0.0;
+  const int K = std::min(std::max(max_iter, 20), 400);
+  std::vector<double> v(n);
+  std::uint64_t state = 0x9E3779B97F4A7C15ULL;  // fixed xorshift64* start → uniform(−1, 1)
+  double nv = 0;
+  for (int j = 0; j < n; ++j) {
+    state ^= state >> 12, state ^= state << 25, state ^= state >> 27;
+    v[j] = static_cast<double>((state * 0x2545F4914F6CDD1DULL) >> 11) / 9007199254740992.0 * 2.0 - 1.0;
+    nv += v[j] * v[j];
+  }
+  nv = std::sqrt(nv);
+  for (double& e : v) e /= nv;
+  dev.check(psh_lanczos_begin(c, v.data()), "lanczos");
+  std::vector<double> alpha, beta;
+  double theta = 0, theta_old = -1;
+  int stagnant = 0;
+  const int mode = cpu_order ? PSH_RED_CPU : PSH_RED_TREE;
+  for (int k = 0; k < K; ++k) {
+    double al = 0, bsq = 0;
+    dev.check(psh_lanczos_step(c, beta.empty() ? 0.0 : beta.back(), mode, &al, &bsq), "lanczos");
+    const double bn = std::sqrt(bsq);
+    alpha.push_back(al);
+    theta = tridiag_max_eig(alpha, beta, static_cast<int>(alpha.size()));
+    if (bn <= 1e-14 * std::max(1.0, theta)) break;
+    beta.push_back(bn);
+    if (theta - theta_old <= rel_tol * theta) {
+      if (++stagnant >= 5 && k >= 20) break;
+    } else {
+      stagnant = 0;
+    }
+    theta_old = theta;
+    dev.check(psh_lanczos_advance(c, bn), "lanczos");
+  }
+  return std::sqrt(std::max(theta, 0.0)) * (1.0 + 1e-4);
+}
+
+// ------------------------------------------------------------------ engine
+namespace {
+
+class Context {  // reference EngineContext (engine.cpp:195-307)
+ public:
+  Context(const Model& model, const EngineOptions& opt)
+      : model_(model), opt_(opt), t0_(std::chrono::steady_clock::now()) {
+    sp_ = make_scaled_problem(model, opt.scaling);
+    if (sp_.A.nnz() >= (1LL << 31)) {
+      error_ = "matrix has 2^31 or more nonzeros (beyond the 32-bit device index range)";
+      return;
+    }
+    dev_ = Device::open(opt.device, error_);
+    if (!dev_) return;
+    dev_->load(sp_);
+    if (opt.timing) dev_->check(psh_timing(dev_->ctx(), 1), "timing");
+    const bool empty = sp_.n == 0 || sp_.A.nnz() == 0;
+    const double norm = empty ? 0.0 : estimate_norm2(*dev_, sp_.n, opt.cpu_order);
+    eta_ = norm > 0 ? 0.998 / norm : 1.0;
+    dev_->check(psh_sync(dev_->ctx()), "setup");
+    setup_seconds_ = elapsed();
+  }
+  bool ok() const { return dev_ != nullptr; }
+  const std::string& error() const { return error_; }
+  Device& dev() { return *dev_; }
+  psh_ctx* ctx() { return dev_->ctx(); }
+  const ScaledProblem& sp() const { return sp_; }
+  double eta() const { return eta_; }
+  int mode() const { return opt_.cpu_order ? PSH_RED_CPU : PSH_RED_TREE; }
+  double elapsed() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count(); }
+  bool out_of_time() const { return elapsed() > opt_.time_limit; }
+  double initial_primal_weight() const {
+    if (opt_.warm_primal_weight > 0) return opt_.warm_primal_weight;
+    if (opt_.scaling.bound_objective_rescaling) return 1.0;
+    double c2 = 0, b2 = 0;
+    for (double v : sp_.c) c2 += v * v;
+    for (int i = 0; i < sp_.m; ++i) {
+      const double lo = std::isfinite(sp_.row_lower[i]) ? std::fabs(sp_.row_lower[i]) : 0.0;
+      const double up = std::isfinite(sp_.row_upper[i]) ? std::fabs(sp_.row_upper[i]) : 0.0;
+      b2 += std::max(lo, up) * std::max(lo, up);
+    }
+    return (std::sqrt(c2) + 1.0) / (std::sqrt(b2) + 1.0);
+  }
+  KktStats adjust(KktStats k) const {
+    if (opt_.kkt_b_norm >= 0) k.b_norm = opt_.kkt_b_norm;
+    if (opt_.kkt_c_norm >= 0) k.c_norm = opt_.kkt_c_norm;
+    k.obj_shift = opt_.kkt_obj_shift;
+    return k;
+  }
+  KktStats device_kkt(int xs, int ys) {
+    psh_kkt d{};
+    dev_->check(psh_kkt_of(ctx(), xs, ys, mode(), &d), "kkt");
+    return adjust(from_device(d));
+  }
+  bool record(const KktStats& k, std::int64_t it) {
+    const double e = k.rel_kkt();
+    if (fast_iterations_ < 0 && e <= tol::kFirstOrderFast) {
+      fast_iterations_ = it;
+      fast_seconds_ = elapsed();
+    }
+    if (opt_.verbosity >= 2)
+      std::fprintf(stderr, "r2hpdhg-hip it %8lld  t %7.2fs  pobj %+.10e  dobj %+.10e  rp %.2e  rd %.2e  gap %.2e  %s\n",
+                   static_cast<long long>(it), elapsed(), k.primal_obj, k.dual_obj, k.rel_primal(), k.rel_dual(),
+                   k.rel_gap(), psh_is_fp32(ctx()) ? "fp32" : "fp64");
+    return k.converged(opt_.tolerance);
+  }
+  void note_primal_weight(double w) { primal_weight_ = w; }
+
+  // Ray-test verdict from the device scalars (reference engine.cpp:82-184).
+  Status ray_verdict(const psh_ray& t, bool drift, const KktStats& cur, std::string& msg) {
+    char buf[200];
+    if (t.rmax > 0) {
+      const double scale = std::max(t.rmax, t.dual_viol);
+      const double obj = t.dual_obj / scale, viol = t.dual_viol / scale;
+      if (obj > 0 && t.dual_obj > tol::kFirstOrderInfeasible * t.dual_mag && viol <= tol::kFirstOrderInfeasible * obj) {
+        dual_ray_.assign(sp_.m, 0.0);
+        dev_->check(psh_ray_vectors(ctx(), drift ? 1 : 0, dual_ray_.data(), nullptr), "ray vectors");
+        std::snprintf(buf, sizeof buf, "primal infeasible: dual ray certificate (objective %f, violation %f)", obj, viol);
+        msg = buf;
+        return Status::Infeasible;
+      }
+    }
+    if (t.dmax > 0) {
+      const double scale = std::max(t.dmax, t.primal_viol);
+      const double obj = t.primal_cd / scale, viol = t.primal_viol / scale;
+      if (obj < 0 && -t.primal_cd > tol::kFirstOrderInfeasible * t.primal_cmag && viol <= tol::kFirstOrderInfeasible * -obj) {
+        const bool feasible = cur.rel_primal() <= tol::kVerifyPrimal && cur.primal_max_rel <= tol::kVerifyPrimal;
+        primal_ray_.assign(sp_.n, 0.0);
+        dev_->check(psh_ray_vectors(ctx(), drift ? 1 : 0, nullptr, primal_ray_.data()), "ray vectors");
+        std::snprintf(buf, sizeof buf, "unbounded: primal ray certificate (c'd %f, violation %f)%s", obj, viol,
+                      feasible ? " at a primal-feasible iterate" : "; iterate not primal-feasible yet");
+        msg = buf;
+        return Status::Unbounded;
+      }
+    }
+    return Status::NotSolved;
+  }
+
+  // Warm start (reference engine.cpp:237-266); returns true when one was applied.
+  bool apply_warm_start(bool mixed_now, bool& promote) {
+    promote = false;
+    if (opt_.warm_x.empty() && opt_.warm_y.empty()) return false;
+    const Model& M = model_;
+    std::vector<double> xo, yo(sp_.m, 0.0);
+    if (!opt_.warm_x.empty()) {
+      xo.resize(sp_.n);
+      for (int j = 0; j < sp_.n; ++j) xo[j] = std::min(std::max(opt_.warm_x[j], M.col_lower[j]), M.col_upper[j]);
+    } else {
+      std::vector<double> xs(sp_.n);
+      dev_->check(psh_download(ctx(), PSH_X, xs.data()), "download");
+      sp_.unscale_primal(xs, xo);
+    }
+    if (!opt_.warm_y.empty())
+      for (int i = 0; i < sp_.m; ++i) {
+        double v = M.sense * opt_.warm_y[i];
+        if (!std::isfinite(M.row_lower[i])) v = std::min(v, 0.0);
+        if (!std::isfinite(M.row_upper[i])) v = std::max(v, 0.0);
+        yo[i] = v;
+      }
+    // precision decided BEFORE the upload (an fp32 copy would lose the warm point's accuracy)
+    const double rel = adjust(kkt_on_original(sp_, xo, yo)).rel_kkt();
+    if (mixed_now && rel <= tol::kMixedPrecisionSwitch && opt_.tolerance < tol::kMixedPrecisionSwitch) promote = true;
+    if (promote) dev_->check(psh_set_fp32(ctx(), 0), "precision");
+    std::vector<double> xs(sp_.n), ys(sp_.m);
+    for (int j = 0; j < sp_.n; ++j) xs[j] = sp_.bound_scale * xo[j] / sp_.col_scale[j];
+    for (int i = 0; i < sp_.m; ++i) ys[i] = sp_.obj_scale * yo[i] / sp_.row_scale[i];
+    dev_->check(psh_upload(ctx(), PSH_X, xs.data()), "upload");
+    dev_->check(psh_upload(ctx(), PSH_Y, ys.data()), "upload");
+    return true;
+  }
+
+  Solution finish(Status status, int xs_slot, int ys_slot, std::int64_t iterations, const std::string& message) {
+    Solution sol;
+    sol.engine = "r2hpdhg";
+    sol.precision = to_string(opt_.precision);
+    sol.iterations = iterations;
+    sol.message = message;
+    std::vector<double> xs(sp_.n), ys(sp_.m), y_min;
+    dev_->check(psh_download(ctx(), xs_slot, xs.data()), "download");
+    dev_->check(psh_download(ctx(), ys_slot, ys.data()), "download");
+    sp_.unscale_primal(xs, sol.x);
+    sp_.unscale_dual(ys, y_min);
+    const KktStats k = adjust(kkt_on_original(sp_, sol.x, y_min));
+    if (!k.finite()) status = Status::NumericalError;
+    if (status == Status::Optimal && !k.converged(opt_.tolerance)) status = Status::NumericalError;
+    const double sense = model_.sense;
+    sol.y.resize(sp_.m);
+    for (int i = 0; i < sp_.m; ++i) sol.y[i] = sense * y_min[i];
+    sol.z = model_.obj;  // z = c − Aᵀy (original sense)
+    parallel_for(sp_.n, [&](std::int64_t b, std::int64_t e) {
+      for (std::int64_t j = b; j < e; ++j)
+        for (int p = model_.col_start[j]; p < model_.col_start[j + 1]; ++p)
+          sol.z[j] -= model_.value[p] * sol.y[model_.row_index[p]];
+    });
+    {
+      Csr A;
+      A.rows = sp_.m;
+      A.cols = sp_.n;
+      A.ptr = sp_.A.ptr;
+      A.col = sp_.A.col;
+      A.val = sp_.a_orig;
+      sol.row_activity.resize(sp_.m);
+      multiply(A, sol.x.data(), sol.row_activity.data());
+    }
+    sol.objective = model_.objective_value(sol.x);
+    sol.dual_objective = sense * k.dual_obj + model_.obj_offset;
+    sol.primal_residual = k.rel_primal();
+    sol.dual_residual = k.rel_dual();
+    sol.gap = k.rel_gap();
+    sol.status = status;
+    sol.iterations_to_fast = fast_iterations_;
+    sol.seconds_to_fast = fast_seconds_;
+    sol.seconds = elapsed();
+    sol.setup_seconds = setup_seconds_;
+    sol.primal_weight = primal_weight_;
+    if (status == Status::Infeasible) sol.dual_ray = dual_ray_;
+    if (status == Status::Unbounded) sol.primal_ray = primal_ray_;
+    if (status == Status::Infeasible || status == Status::Unbounded) sol.objective = sol.dual_objective = kNaN;
+    if (opt_.timing)
+      for (int k = 0; k < 3; ++k) dev_->check(psh_timing_get(ctx(), k, &sol.kernel_launches[k], &sol.kernel_ms[k]), "timing");
+    sol.device_bytes = static_cast<double>(psh_device_bytes(ctx()));
+    return sol;
+  }
+
+ private:
+  const Model& model_;
+  const EngineOptions& opt_;
+  ScaledProblem sp_;
+  std::unique_ptr<Device> dev_;
+  std::string error_;
+  double eta_ = 1.0, setup_seconds_ = 0, primal_weight_ = kNaN, fast_seconds_ = -1;
+  std::int64_t fast_iterations_ = -1;
+  std::chrono::steady_clock::time_point t0_;
+  std::vector<double> dual_ray_, primal_ray_;
+};
+
+// Mixed precision (reference engine.cpp:56-79): fp32 until fp32 has done what it can.
+class PrecisionPolicy {
+ public:
+  PrecisionPolicy(Context& ctx, const EngineOptions& o)
+      : ctx_(ctx), mixed_(o.precision == Precision::Mixed), target_(o.tolerance) {
+    ctx_.dev().check(psh_set_fp32(ctx_.ctx(), mixed_ ? 1 : 0), "precision");
+  }
+  bool fp32() { return psh_is_fp32(ctx_.ctx()) != 0; }
+  bool on_check(double rel) {
+    if (!mixed_ || !fp32()) return false;
+    if (rel < 0.9 * best_) best_ = rel, stale_ = 0;
+    else ++stale_;
+    const bool reached = rel <= tol::kMixedPrecisionSwitch && target_ < tol::kMixedPrecisionSwitch;
+    if (reached || stale_ >= 10) {
+      ctx_.dev().check(psh_set_fp32(ctx_.ctx(), 0), "precision");
+      return true;
+    }
+    return false;
+  }
+
+ private:
+  Context& ctx_;
+  bool mixed_;
+  double target_, best_ = 1e300;
+  int stale_ = 0;
+};
+
+}  // namespace
+
+Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
+  Context ctx(model, opt);
+  if (!ctx.ok()) {
+    Solution s;
+    s.engine = "r2hpdhg";
+    s.status = Status::NotSolved;
+    s.message = ctx.error();
+    return s;
+  }
+  Device& dev = ctx.dev();
+  psh_ctx* c = ctx.ctx();
+  PrecisionPolicy policy(ctx, opt);
+  const double eta = ctx.eta(), rho = opt.reflection;
+  const int mode = ctx.mode();
+  double omega = ctx.initial_primal_weight();
+
+  // x = proj_X(0), y = 0 (reference r2hpdhg.cpp:31-34)
+  dev.check(psh_fill(c, PSH_DX, 0.0), "fill");
+  dev.check(psh_fill(c, PSH_ATY, 0.0), "fill");
+  dev.check(psh_primal_step(c, PSH_DX, PSH_ATY, 0.0, PSH_X), "init");
+  dev.check(psh_fill(c, PSH_Y, 0.0), "fill");
+  bool promoted_by_warm = false;
+  const bool warm = ctx.apply_warm_start(policy.fp32(), promoted_by_warm);
+  ctx.note_primal_weight(omega);
+  if (warm) {
+    const KktStats kw = ctx.device_kkt(PSH_X, PSH_Y);
+    if (kw.finite() && ctx.record(kw, 0)) return ctx.finish(Status::Optimal, PSH_X, PSH_Y, 0, "warm start already optimal");
+  }
+  dev.check(psh_copy(c, PSH_X0, PSH_X), "copy");
+  dev.check(psh_copy(c, PSH_Y0, PSH_Y), "copy");
+  dev.check(psh_copy(c, PSH_XH, PSH_X), "copy");
+  dev.check(psh_copy(c, PSH_YH, PSH_Y), "copy");
+
+  double pid_integral = 0.0, pid_last = 0.0, best_omega = omega, best_balance = 1e300;
+  auto residual = [&](double px, double py, double cross) {
+    const double r2 = omega * px + py / omega + 2.0 * eta * cross;
+    return std::sqrt(std::max(r2, 0.0));
+  };
+  const int K = opt.check_every;
+  std::int64_t it = 0, inner = 0;
+  double r0 = -1.0, r_last = kInf;
+  std::string msg = warm ? "warm start" : "";
+
+  for (;;) {
+    psh_chunk_args ch{};
+    ch.iterations = K;
+    ch.inner0 = inner;
+    ch.tau = eta / omega;
+    ch.sigma = eta * omega;
+    ch.rho = rho;
+    ch.eta = eta;
+    ch.omega = omega;
+    ch.want_hat_last = 1;
+    ch.mode = mode;
+    dev.check(psh_chunk(c, &ch), "iteration");
+    inner += K;
+    it += K;
+
+    psh_check_args ca{};
+    ca.omega = omega;
+    ca.eta = eta;
+    ca.want_ray = it >= 4 * K ? 1 : 0;
+    ca.want_drift = it >= 4 * K && inner >= K && (it / K) % 4 == 0 ? 1 : 0;
+    ca.want_pid = mode == PSH_RED_TREE ? 1 : 0;
+    ca.mode = mode;
+    psh_stats st{};
+    dev.check(psh_check(c, &ca, &st), "check");
+    if (st.have_r0) r0 = residual(st.r0_px, st.r0_py, st.r0_cross);  // residual at the epoch's anchor
+    const double r = residual(st.px, st.py, st.cross);
+    const KktStats k = ctx.adjust(from_device(st.kkt));
+    if (!k.finite() || !std::isfinite(r)) return ctx.finish(Status::NumericalError, PSH_XH, PSH_YH, it, "non-finite iterate");
+    if (ctx.record(k, it)) return ctx.finish(Status::Optimal, PSH_XH, PSH_YH, it, msg);
+    if (it >= opt.iteration_limit) return ctx.finish(Status::IterationLimit, PSH_XH, PSH_YH, it, msg);
+    if (ctx.out_of_time()) return ctx.finish(Status::TimeLimit, PSH_XH, PSH_YH, it, msg);
+    if (it >= 4 * K) {  // ray tests on T(z) − z, then on the drift since the anchor
+      std::string why;
+      Status verdict = ctx.ray_verdict(st.ray, false, k, why);
+      if (verdict == Status::NotSolved && ca.want_drift) {
+        verdict = ctx.ray_verdict(st.drift, true, k, why);
+        if (verdict != Status::NotSolved) why += " (drift since the restart anchor)";
+      }
+      if (verdict != Status::NotSolved) return ctx.finish(verdict, PSH_XH, PSH_YH, it, why);
+    }
+    const bool promoted = policy.on_check(k.rel_kkt());
+    if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
+    const bool ratios_ok = r0 > 0 && std::isfinite(r0) && r > 0;
+    const bool restart = promoted ||
+                         (ratios_ok && (r <= opt.restart_sufficient * r0 || (r <= opt.restart_necessary * r0 && r > r_last))) ||
+                         static_cast<double>(inner) >= opt.restart_artificial * static_cast<double>(it);
+    r_last = r;
+    if (!restart) continue;
+
+    // PID primal weight (reference r2hpdhg.cpp:127-148)
+    double dxn = st.dx_norm, dyn = st.dy_norm;
+    if (mode == PSH_RED_CPU || promoted) {
+      dev.check(psh_diff_norm2(c, PSH_XH, PSH_X0, mode, &dxn), "pid");
+      dev.check(psh_diff_norm2(c, PSH_YH, PSH_Y0, mode, &dyn), "pid");
+    }
+    const bool both_pos = k.rel_dual() > 0 && k.rel_primal() > 0;
+    const double ratio = both_pos ? k.rel_dual() / k.rel_primal() : 1.0;
+    if (dxn > 1e-16 && dyn > 1e-16 && dxn < 1e12 && dyn < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+      const double e = std::log(dyn) - std::log(dxn) - std::log(omega);
+      pid_integral = opt.pid_integral_decay * pid_integral + e;
+      const double step = opt.pid_kp * e + opt.pid_ki * pid_integral + opt.pid_kd * (e - pid_last);
+      omega *= std::exp(std::clamp(step, -opt.pid_max_log_step, opt.pid_max_log_step));
+      pid_last = e;
+    } else {
+      omega = best_omega;
+      pid_integral = pid_last = 0.0;
+    }
+    const double balance = std::fabs(std::log10(ratio));
+    if (both_pos && balance < best_balance) {
+      best_balance = balance;
+      best_omega = omega;
+    }
+    ctx.note_primal_weight(omega);
+    if (opt.verbosity >= 2)
+      std::fprintf(stderr, "  restart at %lld (inner %lld): r/r0 %.3f  omega %.4e  dx %.3e dy %.3e\n",
+                   static_cast<long long>(it), static_cast<long long>(inner), r / r0, omega, dxn, dyn);
+    dev.check(psh_restart(c), "restart");
+    inner = 0;
+    r_last = kInf;
+  }
+}
+
+}  // namespace ps26119::amd

@@ -1,0 +1,81 @@
+// engine.h — the r²HPDHG engine on the MI355X (reference src/pdhg/r2hpdhg.{h,cpp},
+// engine.{h,cpp}): same restarts, PID primal weight, precision policy, ray tests, warm start
+// and finish, with the iteration and the per-check reductions on the GPU.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "contract.h"
+#include "ps26119_hip.h"
+#include "scaling.h"
+
+namespace ps26119::amd {
+
+struct EngineOptions {  // reference src/pdhg/engine.h:27-48
+  double tolerance = tol::kFirstOrderHigh;
+  std::vector<double> warm_x, warm_y;
+  double warm_primal_weight = 0.0;
+  double kkt_b_norm = -1, kkt_c_norm = -1, kkt_obj_shift = 0;
+  double time_limit = 3600.0;
+  std::int64_t iteration_limit = 100'000'000;
+  int check_every = 64;
+  Precision precision = Precision::Fp64;
+  int verbosity = 0;
+  ScalingOptions scaling;
+  double reflection = 1.0;
+  double restart_sufficient = 0.2, restart_necessary = 0.5, restart_artificial = 0.36;
+  double pid_kp = 0.99, pid_ki = 0.01, pid_kd = 0.0, pid_integral_decay = 0.3;
+  double pid_max_log_step = 2.302585092994046;
+  // MI355X knobs (engine_params names hip_cpu_order, hip_device):
+  //   cpu_order = 1: every reduction in the reference CPU order (bit-identical trajectory to the
+  //   reference CPU backend, slower checks); 0: fixed-shape device trees for the KKT sums and the
+  //   residual / PID norms. The Lanczos dots follow the same switch.
+  int cpu_order = 0;
+  int device = 0;
+  int timing = 0;  // hip_timing: per-kernel HIP event timing into the Solution
+};
+
+EngineOptions engine_options_from(const Options& o);
+// Applies Options::engine_params; returns the first unknown name or "".
+std::string apply_engine_params(const Options& o, EngineOptions& e);
+
+// KKT statistics (reference src/pdhg/termination.h:35-60), fp64, original problem, min form.
+struct KktStats {
+  double primal_residual = 0, dual_residual = 0, primal_obj = 0, dual_obj = 0, b_norm = 0, c_norm = 0;
+  double primal_max_rel = 0, dual_max_rel = 0, obj_shift = 0;
+  double rel_primal() const { return primal_residual / (1.0 + b_norm); }
+  double rel_dual() const { return dual_residual / (1.0 + c_norm); }
+  double rel_gap() const;
+  double rel_kkt() const;
+  bool converged(double eps) const;
+  bool finite() const;
+};
+
+// Host fp64 KKT of an original-space point (x, y_min) — the exact reference computation
+// (reference src/pdhg/cpu_backend.cpp:226-284), used for warm starts and the final report.
+KktStats kkt_on_original(const ScaledProblem& sp, const std::vector<double>& x, const std::vector<double>& y_min);
+
+// RAII device context for one solve. Errors become exceptions: std::bad_alloc for device
+// out-of-memory (solve() reports NotSolved "out of memory"), std::runtime_error otherwise.
+class Device {
+ public:
+  static std::unique_ptr<Device> open(int device, std::string& error);
+  ~Device();
+  psh_ctx* ctx() const { return ctx_; }
+  void check(int rc, const char* what) const;
+  void load(const ScaledProblem& sp);
+
+ private:
+  explicit Device(psh_ctx* c) : ctx_(c) {}
+  psh_ctx* ctx_;
+};
+
+// ‖Ã‖₂ by Lanczos on ÃᵀÃ with the device products (reference src/la/csr.cpp:86-132).
+double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter = 400, double rel_tol = 1e-12);
+
+Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt);
+
+}  // namespace ps26119::amd
