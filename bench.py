@@ -1,0 +1,136 @@
+"""Benchmark of the MI355X r²HPDHG backend on the 1M-row random LP.
+
+Workload (one GPU): the random LP with a planted optimum, m = 1,000,000 rows, n = 2,000,000
+columns, exactly 6 nonzeros per row (6,000,000 nnz), seed 1, fp64, relative KKT tolerance 1e-8,
+solved through the C API (presolve, scaling, ‖Ã‖₂, r²HPDHG on the GPU, postsolve, in-process
+check and certified bound — the whole solve() call, as the reference times it).
+
+A step is one full solve to 1e-8 with the inputs already in host memory; `value` is seconds per
+solve (lower is better). The warmup steps are short iteration-limited solves of the same model.
+Per-kernel durations come from HIP events recorded on the solve's stream around every fused
+iteration kernel; `roofline` reports the fused Ãᵀy + primal-update kernel against the 8 TB/s HBM
+peak. `cpu_baseline` times the reference CPU solver (oracle/_ref, all host cores) on a bounded
+sample of the same model and extrapolates its time to 1e-8 from the GPU run's iteration count
+(the fp64 trajectories are the same).
+
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--m 1000000] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "ps26119-solver_amd", "python"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+METRIC = "time-to-1e-8 KKT (s) + HBM GB/s on 1M-row LP; served solves/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(m, n, nnz):
+    """Algorithmic bytes of one launch of each fused kernel (fp64 values and vectors, int32
+    indices; the gathered vector counted once).
+      primal: Ãᵀ (nnz·(8+4) + (n+1)·4), gather y (8m), read x, x0, c, l, u (40n), write x̄, x (16n)
+      dual:   Ã  (nnz·(8+4) + (m+1)·4), gather x̄ (8n), read y, y0, rl, ru (32m), write y (8m)
+    The x̂ / ŷ, ȳ stores happen only on the launches before a check or at an epoch start and are
+    not counted (under 2% of launches)."""
+    primal = nnz * 12 + (n + 1) * 4 + 8 * m + 56 * n
+    dual = nnz * 12 + (m + 1) * 4 + 8 * n + 40 * m
+    return primal, dual
+
+
+def cpu_baseline(lp, gpu_iterations, gpu_solve):
+    """Reference CPU solver on all host cores, bounded sample: setup plus 256 iterations."""
+    import orc
+    if not orc.available("ref"):
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    ref = orc.load("ref")
+    sample_it = 256
+    t = time.time()
+    r, _ = ref.solve(lp, presolve=True, iteration_limit=sample_it, threads=0)
+    wall = time.time() - t
+    per_it = (r["seconds"] - r["setup_seconds"]) / max(1, r["iterations"])
+    est = r["setup_seconds"] + per_it * gpu_iterations + max(0.0, wall - r["seconds"])
+    return {"value": est, "unit": "s", "cores": cores, "kind": "reference",
+            "sample": f"reference r2HPDHG (oracle/_ref, threads=all) on the same 1M-row LP, {r['iterations']} iterations "
+                      f"in {r['seconds']:.1f} s (setup {r['setup_seconds']:.1f} s, {per_it * 1e3:.2f} ms/iteration), "
+                      f"extrapolated to the GPU run's {gpu_iterations} iterations",
+            "ms_per_iteration": per_it * 1e3, "setup_seconds": r["setup_seconds"],
+            "speedup_vs_cpu": est / gpu_solve if gpu_solve > 0 else None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--time-limit", type=float, default=600.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py measures one GPU: the r2HPDHG path here runs on a single MI355X; --gpus {args.gpus} "
+              "is not supported (multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import ps26119
+    from ps26119_amd import lpgen
+    if ps26119.device_count() <= 0:
+        log("no GPU visible to the HIP backend")
+        sys.exit(1)
+    t = time.time()
+    lp = lpgen.random_lp(args.m, seed=args.seed)
+    log(f"generated m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t:.1f}s")
+    for w in range(args.warmup):
+        r = ps26119.solve_model(lp, iteration_limit=64)
+        log(f"warmup {w}: {r.status_name} {r.seconds:.2f}s")
+    times, res = [], None
+    for s in range(args.steps):
+        t = time.time()
+        res = ps26119.solve_model(lp, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit)
+        times.append(time.time() - t)
+        log(f"step {s}: {res.status_name} it={res.iterations} solve={res.seconds:.2f}s wall={times[-1]:.2f}s "
+            f"obj={res.objective!r} planted={lp.optimum!r} check={res.check} {res.message[:120]}")
+    st = res.stats
+    launches, kms = st["kernel_launches"], st["kernel_ms"]
+    avg_us = [kms[k] / launches[k] * 1e3 if launches[k] else float("nan") for k in range(3)]
+    pb, db = kernel_bytes(lp.m, lp.n - 0, lp.nnz)
+    gbs_p = pb / (avg_us[0] * 1e-6) / 1e9 if launches[0] else float("nan")
+    gbs_d = db / (avg_us[1] * 1e-6) / 1e9 if launches[1] else float("nan")
+    iter_s = (res.seconds - st["setup_seconds"]) / max(1, res.iterations)
+    value = float(np.mean([r for r in times]))
+    line = {
+        "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"random LP with planted optimum, m={lp.m} n={lp.n} nnz={lp.nnz} (6 per row), seed "
+                               f"{args.seed}, fp64, tolerance 1e-8, presolve on", "m": lp.m, "n": lp.n, "nnz": lp.nnz},
+        "roofline": {"bound": "hbm", "achieved": gbs_p, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_p / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_iter_primal (fused Aᵀy + projection/reflection/Halpern)",
+                     "bytes_per_launch": pb, "avg_launch_us": avg_us[0]},
+        "status": res.status_name, "check": res.check, "iterations": res.iterations,
+        "objective_rel_error_vs_planted": abs(res.objective - lp.optimum) / abs(lp.optimum),
+        "solve_seconds": res.seconds, "setup_seconds": st["setup_seconds"], "ms_per_iteration": iter_s * 1e3,
+        "dual_kernel": {"achieved_GBs": gbs_d, "avg_launch_us": avg_us[1], "bytes_per_launch": db},
+        "check_pass": {"launches": launches[2], "avg_us": avg_us[2]},
+        "spmv_phase_GBs": (pb + db) / ((avg_us[0] + avg_us[1]) * 1e-6) / 1e9 if launches[0] and launches[1] else None,
+        "device_bytes": st["device_bytes"],
+    }
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, value)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+"""Kernel-level parity of the HIP path (through the psh_* C-ABI) with the oracle, on the GPU.
+
+fp64 and fp32 (reference "mixed") results of the reference Backend operations must be
+bit-identical to the CPU restatement: row-sequential SpMV in CSR order, the r²HPDHG steps with
+the same expression order, CPU-order reductions and KKT sums. The fused chunk must equal the
+unfused op sequence bit for bit, and the fused check's sums must equal the separate reductions.
+"""
+import numpy as np
+import pytest
+
+from conftest import netlib
+
+pytestmark = pytest.mark.gpu
+
+
+def long_row_model():
+    """Row 0 touches all 3000 columns and column 0 all 1500 rows (both beyond one LDS tile)."""
+    from ps26119_amd import lpgen
+    m, n = 1500, 3000
+    rows, cols, vals = [], [], []
+    for j in range(n):
+        rows.append(0), cols.append(j), vals.append(1.0 + (j % 5) * 0.25)
+        i = 1 + j % (m - 1)
+        rows.append(i), cols.append(j), vals.append(0.5 + (j % 7) * 0.1)
+        if j == 0:
+            for r in range(1, m):
+                if r != i:
+                    rows.append(r), cols.append(0), vals.append(0.3)
+    cs, ri, v = lpgen._csc_from_triplets(m, n, np.array(rows), np.array(cols), np.array(vals))
+    c = -1.0 - (np.arange(n) % 13) * 0.1
+    return lpgen.Lp(m=m, n=n, c=c, col_lower=np.zeros(n), col_upper=np.full(n, 10.0), row_lower=np.full(m, -np.inf),
+                    row_upper=np.full(m, 50.0), col_start=cs, row_index=ri, value=v, name="long-rows")
+
+
+def models():
+    from ps26119_amd import lpgen
+    return {"afiro": netlib("afiro"), "stocfor1": netlib("stocfor1"), "rand300": lpgen.random_lp(300, seed=3),
+            "long_rows": long_row_model(), "rand20k": lpgen.random_lp(20000, seed=5)}
+
+
+MODELS = ["afiro", "stocfor1", "rand300", "long_rows", "rand20k"]
+_cache = {}
+
+
+def model(name):
+    if name not in _cache:
+        _cache[name] = models()[name]
+    return _cache[name]
+
+
+def context(oracle, lp, prec):
+    from ps26119_amd import psh
+    ctx = psh.Context(psh.scaled_problem(oracle.scale(lp), lp))
+    ctx.set_fp32(prec)
+    return ctx
+
+
+def run_ops(ctx, lp, tau=0.3, sigma=0.4, rho=1.0, iters=20):
+    from ps26119_amd import psh
+    ctx.upload(psh.X, np.sin(1.0 + np.arange(lp.n)))
+    ctx.upload(psh.Y, np.cos(2.0 + np.arange(lp.m)))
+    ctx.copy(psh.X0, psh.X)
+    ctx.copy(psh.Y0, psh.Y)
+    for it in range(iters):
+        w = (it + 1.0) / (it + 2.0)
+        ctx.spmv_t(psh.Y, psh.ATY)
+        ctx.r2h_primal(psh.X, psh.X0, psh.ATY, tau, w, rho, psh.XH, psh.XB)
+        ctx.spmv(psh.XB, psh.AX)
+        ctx.r2h_dual(psh.Y, psh.Y0, psh.AX, sigma, w, rho, psh.YH, psh.YB)
+
+
+@pytest.mark.parametrize("name", MODELS)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_backend_ops_bit_identical(gpu_available, oracle, name, prec):
+    from ps26119_amd import psh
+    lp = model(name)
+    ctx = context(oracle, lp, prec)
+    run_ops(ctx, lp)
+    ctx.primal_step(psh.X, psh.ATY, 0.15, psh.XB)
+    ctx.dual_step(psh.Y, psh.AX, 0.5, psh.YB)
+    ctx.axpby(0.3, psh.XH, -1.7, psh.XB)
+    got = {k: ctx.download(s) for k, s in [("x", psh.X), ("xh", psh.XH), ("xb", psh.XB), ("y", psh.Y), ("yh", psh.YH),
+                                            ("yb", psh.YB)]}
+    red = [ctx.dot(psh.X, psh.XH, psh.RED_CPU), np.sqrt(ctx.dot(psh.Y, psh.Y, psh.RED_CPU)),
+           ctx.diff_norm2(psh.X, psh.X0, psh.RED_CPU)] + ctx.kkt(psh.XH, psh.YH, psh.RED_CPU)
+    exp, ered = oracle.backend_ops(lp, precision=prec, iters=20, tau=0.3, sigma=0.4, rho=1.0)
+    for k in got:
+        assert np.array_equal(got[k], exp[k]), (k, float(np.max(np.abs(got[k] - exp[k]))))
+    assert [float(v).hex() for v in red] == [float(v).hex() for v in ered]
+    # tree reductions: deterministic and within rounding of the CPU order
+    tree = [ctx.dot(psh.X, psh.XH), np.sqrt(ctx.dot(psh.Y, psh.Y)), ctx.diff_norm2(psh.X, psh.X0)] + ctx.kkt(psh.XH, psh.YH)
+    again = [ctx.dot(psh.X, psh.XH), np.sqrt(ctx.dot(psh.Y, psh.Y)), ctx.diff_norm2(psh.X, psh.X0)] + ctx.kkt(psh.XH, psh.YH)
+    assert tree == again
+    assert np.allclose(tree, ered, rtol=1e-12, atol=1e-300)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["stocfor1", "rand300", "long_rows", "rand20k"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fused_chunk_equals_unfused_ops(gpu_available, oracle, name, prec):
+    """psh_chunk (2 fused kernels per iteration) against the op-by-op sequence, bit for bit,
+    including the hat vectors written at the epoch start and on the last iteration."""
+    from ps26119_amd import psh
+    lp = model(name)
+    K, tau, sigma, rho = 7, 0.3, 0.4, 1.0
+    a = context(oracle, lp, prec)
+    b = context(oracle, lp, prec)
+    for ctx in (a, b):
+        ctx.upload(psh.X, np.sin(1.0 + np.arange(lp.n)))
+        ctx.upload(psh.Y, np.cos(2.0 + np.arange(lp.m)))
+        ctx.copy(psh.X0, psh.X)
+        ctx.copy(psh.Y0, psh.Y)
+    for inner0 in (0, K):
+        a.chunk(K, inner0, tau, sigma, rho)
+        for s in range(K):
+            w = (inner0 + s + 1.0) / (inner0 + s + 2.0)
+            b.spmv_t(psh.Y, psh.ATY)
+            b.r2h_primal(psh.X, psh.X0, psh.ATY, tau, w, rho, psh.XH, psh.XB)
+            b.spmv(psh.XB, psh.AX)
+            b.r2h_dual(psh.Y, psh.Y0, psh.AX, sigma, w, rho, psh.YH, psh.YB)
+        for slot in (psh.X, psh.XH, psh.XB, psh.Y, psh.YH, psh.YB):
+            assert np.array_equal(a.download(slot), b.download(slot)), (inner0, slot)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("name", ["stocfor1", "rand300", "rand20k"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_check_equals_separate_reductions(gpu_available, oracle, name, mode):
+    from ps26119_amd import psh
+    lp = model(name)
+    ctx = context(oracle, lp, 0)
+    run_ops(ctx, lp, iters=5)
+    st = ctx.check(omega=1.3, eta=0.9, want_ray=1, want_drift=1, want_pid=1, mode=mode)
+    # residual pieces against the reference-interface ops
+    ctx2 = context(oracle, lp, 0)
+    run_ops(ctx2, lp, iters=5)
+    ctx2.copy(psh.DX, psh.XB)
+    ctx2.axpby(-1.0, psh.XH, 1.0, psh.DX)
+    ctx2.copy(psh.DY, psh.YB)
+    ctx2.axpby(-1.0, psh.YH, 1.0, psh.DY)
+    ctx2.spmv(psh.DX, psh.AX)
+    px, py, cr = ctx2.dot(psh.DX, psh.DX, mode), ctx2.dot(psh.DY, psh.DY, mode), ctx2.dot(psh.DY, psh.AX, mode)
+    assert (st.px, st.py, st.cross) == (px, py, cr)
+    assert st.dx_norm == ctx2.diff_norm2(psh.XH, psh.X0, mode)
+    assert st.dy_norm == ctx2.diff_norm2(psh.YH, psh.Y0, mode)
+    kk = ctx2.kkt(psh.XH, psh.YH, mode)
+    assert [st.kkt.primal_residual, st.kkt.dual_residual, st.kkt.primal_obj, st.kkt.dual_obj] == kk[:4]
+    assert st.finite == 1
+    ctx.close()
+    ctx2.close()
+
+
+def test_kkt_cpu_order_matches_oracle_on_original_point(gpu_available, oracle):
+    """KKT of the unscaled iterate, CPU order on the device vs the oracle's kkt_general."""
+    from ps26119_amd import psh
+    lp = model("rand20k")
+    sc = oracle.scale(lp)
+    ctx = context(oracle, lp, 0)
+    run_ops(ctx, lp, iters=10)
+    got = ctx.kkt(psh.XH, psh.YH, psh.RED_CPU)
+    xs, ys = ctx.download(psh.XH), ctx.download(psh.YH)
+    x = np.minimum(np.maximum(sc["col_scale"] * xs / sc["bound_scale"], lp.col_lower), lp.col_upper)
+    y = sc["row_scale"] * ys / sc["obj_scale"]
+    e = oracle.kkt(lp, x, y)
+    exp = [e[k] for k in ("primal_residual", "dual_residual", "primal_obj", "dual_obj", "b_norm", "c_norm",
+                          "primal_max_rel", "dual_max_rel")]
+    assert [float(v).hex() for v in got] == [float(v).hex() for v in exp]
+    ctx.close()

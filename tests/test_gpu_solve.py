@@ -1,0 +1,146 @@
+"""End-to-end parity of the GPU solve (through the C API) with the reference.
+
+With CPU-order reductions (engine knob hip_cpu_order = 1) the whole fp64 / fp32 trajectory is
+the reference CPU backend's: same iteration count, same final primal weight, bit-identical x,
+y, objective and certified bound as the fixtures written from the reference's code
+(tests/golden/reference_runs.json). With the default device-tree reductions the answer must be
+Optimal, pass the in-process check on the original model, and agree with the reference objective
+to 1e-9 relative (fp64); iteration counts are compared too.
+"""
+import numpy as np
+import pytest
+
+from conftest import NETLIB, fnv, hand_models, netlib
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(name):
+    from ps26119_amd import lpgen
+    if name in NETLIB:
+        return netlib(name)
+    if name.startswith("random"):
+        m, s = name[len("random"):].split("_s")
+        return lpgen.random_lp(int(m), seed=int(s))
+    return hand_models()[name]
+
+
+SMALL = NETLIB + ["wyndor", "mixed_rows", "max_ranged", "random300_s3"]
+
+
+@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("presolve", [1, 0])
+def test_cpu_order_trajectory_is_the_reference(gpu_available, golden, name, prec, presolve):
+    import ps26119
+    lp = _model(name)
+    exp = golden["solves"][f"{name}|presolve={presolve}|{prec}"]
+    r = ps26119.solve_model(lp, precision=prec, presolve=bool(presolve), params={"hip_cpu_order": 1}, stats=True,
+                            iteration_limit=3_000_000)
+    assert r.status == exp["status"], r.message
+    assert r.iterations == exp["iterations"]
+    assert float(r.objective).hex() == exp["objective"]
+    assert float(r.certified_bound).hex() == exp["certified_bound"]
+    assert float(r.stats["primal_weight"]).hex() == exp["primal_weight"]
+    assert r.stats["iterations_to_fast"] == exp["iterations_to_fast"]
+    assert {"PASS": 1, "FAIL": 0, "": -1}[r.check] == exp["check"]
+    if exp["x_hash"] is not None:
+        assert fnv(r.x) == exp["x_hash"] and fnv(r.y) == exp["y_hash"]
+
+
+@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_tree_reductions_reach_the_reference_answer(gpu_available, golden, name, prec):
+    import ps26119
+    lp = _model(name)
+    exp = golden["solves"][f"{name}|presolve=1|{prec}"]
+    r = ps26119.solve_model(lp, precision=prec, iteration_limit=3_000_000)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    ref_obj = float.fromhex(exp["objective"])
+    tol = 1e-9 if prec == "fp64" else 1e-6
+    assert abs(r.objective - ref_obj) <= tol * (1 + abs(ref_obj))
+    # the reference's own GPU-vs-CPU criterion (its tests/unit/test_gpu.cpp:120): within 25% + 256
+    assert abs(r.iterations - exp["iterations"]) <= 0.25 * exp["iterations"] + 256
+
+
+@pytest.mark.parametrize("key", ["random2000_s4|presolve=1|fp64", "random2000_s4|presolve=0|mixed",
+                                 "random10000_s1|presolve=1|fp64", "random10000_s1|presolve=0|fp64"])
+def test_long_solves_follow_the_reference(gpu_available, golden, key):
+    """Hundreds of thousands of iterations: the CPU-order run stays on the reference trajectory."""
+    import ps26119
+    name, pre, prec = key.split("|")
+    if key not in golden["solves"]:
+        pytest.skip("fixture not generated")
+    lp = _model(name)
+    exp = golden["solves"][key]
+    r = ps26119.solve_model(lp, precision=prec, presolve=pre.endswith("1"), params={"hip_cpu_order": 1})
+    assert r.status == exp["status"] and r.iterations == exp["iterations"]
+    assert float(r.objective).hex() == exp["objective"]
+    assert fnv(r.x) == exp["x_hash"]
+    assert abs(r.objective - lp.optimum) <= 1e-9 * abs(lp.optimum)
+
+
+def test_default_reductions_on_config0(gpu_available, golden):
+    import ps26119
+    lp = _model("random10000_s1")
+    exp = golden["solves"]["random10000_s1|presolve=1|fp64"]
+    r = ps26119.solve_model(lp)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    assert abs(r.objective - float.fromhex(exp["objective"])) <= 1e-9 * abs(lp.optimum)
+    assert abs(r.objective - lp.optimum) <= 1e-9 * abs(lp.optimum)
+    assert r.certified_bound <= lp.optimum + 1e-9 * abs(lp.optimum)
+
+
+def test_wyndor_duals_and_bound(gpu_available):
+    import ps26119
+    r = ps26119.solve_model(hand_models()["wyndor"])
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS"
+    assert abs(r.objective - 36) < 1e-6
+    assert np.allclose(r.x, [2, 6], atol=1e-5) and np.allclose(r.y, [0, 1.5, 1], atol=1e-5)
+    assert 36 - 1e-9 <= r.certified_bound < 36 + 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_infeasible_and_unbounded_are_certified(gpu_available, prec):
+    import ps26119
+    from ps26119_amd import lpgen
+    inf = np.inf
+    cases = [(hand_models()["infeasible"], ps26119.INFEASIBLE),
+             (lpgen.from_dense([1], [[1]], [-inf], [3], [5], [inf]), ps26119.INFEASIBLE),
+             (lpgen.from_dense([1, 2], [[1, 1], [1, 1]], [1, 2], [1, 2], [-inf, -inf], [inf, inf]), ps26119.INFEASIBLE),
+             (hand_models()["unbounded"], ps26119.UNBOUNDED),
+             (lpgen.from_dense([1, 1], [[1, -1], [-1, 1]], [-inf, -inf], [2, 1], [0, 0], [inf, inf], sense=-1),
+              ps26119.UNBOUNDED)]
+    for lp, status in cases:
+        r = ps26119.solve_model(lp, precision=prec, iteration_limit=200000)
+        assert r.status == status, r.message
+        assert r.check == "PASS", r.message
+        if status == ps26119.INFEASIBLE:
+            assert r.dual_ray is not None
+        else:
+            assert r.primal_ray is not None and r.x is not None
+
+
+def test_warm_start_from_optimum_stops_early(gpu_available):
+    import ps26119
+    for name in ("stocfor1", "share2b", "adlittle"):
+        lp = netlib(name)
+        cold = ps26119.solve_model(lp)
+        warm = ps26119.solve_model(lp, warm_x=cold.x, warm_y=cold.y, precision="mixed")
+        assert warm.status == ps26119.OPTIMAL, warm.message
+        assert warm.iterations < cold.iterations and warm.iterations <= 4 * 64
+        assert "warm start" in warm.message
+        assert abs(warm.objective - cold.objective) <= 1e-6 * (1 + abs(cold.objective))
+
+
+def test_limits_and_determinism(gpu_available):
+    import ps26119
+    lp = netlib("share2b")
+    r = ps26119.solve_model(lp, iteration_limit=128)
+    assert r.status == ps26119.ITERATION_LIMIT and r.x is not None and len(r.x) == lp.n
+    a, b = ps26119.solve_model(netlib("afiro")), ps26119.solve_model(netlib("afiro"))
+    assert a.iterations == b.iterations and np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    r = ps26119.solve_model(lp, warm_x=np.full(lp.n, np.nan))
+    assert r.status == ps26119.NOT_SOLVED and "non-finite" in r.message
+    r = ps26119.solve_model(lp, params={"no_such_knob": 1.0})
+    assert r.status == ps26119.NOT_SOLVED and "no_such_knob" in r.message

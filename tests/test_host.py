@@ -1,0 +1,190 @@
+"""Host side of the MI355X backend, on the CPU: the C-ABI libraries load and export every
+declared entry point, the C API layout matches the reference header, argument validation and
+error statuses follow the reference's C API, and the host stages of the path (scaling,
+presolve, certified bound, in-process check) reproduce the oracle bit for bit."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, hand_models, netlib
+
+HEADERS = [os.path.join(ROOT, "include", "ps26119", "ps26119.h"), os.path.join(ROOT, "include", "ps26119", "ps26119_amd.h"),
+           os.path.join(ROOT, "include", "ps26119_hip.h")]
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:ps26119|psh)_\w+)\s*\(", text)))
+
+
+def test_libraries_export_every_declared_symbol():
+    import ps26119
+    from ps26119_amd import psh
+    main = ctypes.CDLL(ps26119.library_path)
+    hip = ctypes.CDLL(psh.HIP_LIB)
+    for h in HEADERS:
+        lib = hip if h.endswith("ps26119_hip.h") else main
+        names = declared_functions(h)
+        assert names, h
+        missing = [n for n in names if not hasattr(lib, n)]
+        assert not missing, (h, missing)
+
+
+def test_c_header_compiles_as_c_and_matches_reference_layout(tmp_path):
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "abi.c"
+    src.write_text(r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "ps26119/ps26119.h"
+#define F(T, f) printf(#T "." #f " %zu %zu\n", offsetof(T, f), sizeof(((T*)0)->f));
+int main(void) {
+  printf("options %zu result %zu\n", sizeof(ps26119_options), sizeof(ps26119_result));
+  F(ps26119_options, algorithm) F(ps26119_options, precision) F(ps26119_options, use_gpu) F(ps26119_options, tolerance)
+  F(ps26119_options, time_limit) F(ps26119_options, iteration_limit) F(ps26119_options, verbosity)
+  F(ps26119_options, threads) F(ps26119_options, warm_x) F(ps26119_options, warm_y)
+  F(ps26119_result, status) F(ps26119_result, objective) F(ps26119_result, dual_objective)
+  F(ps26119_result, primal_residual) F(ps26119_result, dual_residual) F(ps26119_result, gap)
+  F(ps26119_result, certified_bound) F(ps26119_result, iterations) F(ps26119_result, seconds)
+  F(ps26119_result, engine) F(ps26119_result, message) F(ps26119_result, check)
+  printf("enums %d %d %d %d %d %d %d %d %d %d\n", PS26119_OPTIMAL, PS26119_INFEASIBLE, PS26119_UNBOUNDED,
+         PS26119_ITERATION_LIMIT, PS26119_TIME_LIMIT, PS26119_NUMERICAL_ERROR, PS26119_NOT_SOLVED,
+         PS26119_INVALID_ARGUMENT, PS26119_ALG_R2HPDHG, PS26119_PREC_MIXED);
+  return 0;
+}
+""")
+
+    def layout(include):
+        exe = tmp_path / ("abi_" + str(abs(hash(include))))
+        subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", f"-I{include}", str(src), "-o", str(exe)], check=True)
+        return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+
+    ours = layout(os.path.join(ROOT, "include"))
+    assert "options 64 result 256" in ours
+    if os.path.isdir("/root/reference/include"):
+        assert ours == layout("/root/reference/include")
+
+
+def _arrays(lp):
+    return dict(c=np.ascontiguousarray(lp.c, float), cl=np.ascontiguousarray(lp.col_lower, float),
+                cu=np.ascontiguousarray(lp.col_upper, float), rl=np.ascontiguousarray(lp.row_lower, float),
+                ru=np.ascontiguousarray(lp.row_upper, float), cs=np.ascontiguousarray(lp.col_start, np.int32),
+                ri=np.ascontiguousarray(lp.row_index, np.int32), v=np.ascontiguousarray(lp.value, float))
+
+
+def _p(a, t=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _model_args(lp, a, offset=True):
+    head = [lp.m, lp.n, lp.sense] + ([lp.offset] if offset else [])
+    return head + [_p(a["c"]), _p(a["cl"]), _p(a["cu"]), _p(a["rl"]), _p(a["ru"]), _p(a["cs"], ctypes.c_int),
+                   _p(a["ri"], ctypes.c_int), _p(a["v"])]
+
+
+@pytest.mark.parametrize("name", ["afiro", "adlittle", "random"])
+def test_host_scaling_is_bit_identical_to_oracle(oracle, name):
+    import ps26119
+    from ps26119_amd import lpgen
+    lp = lpgen.random_lp(3000, seed=8) if name == "random" else netlib(name)
+    a = _arrays(lp)
+    rs, cs, sc = np.zeros(lp.m), np.zeros(lp.n), np.zeros(4)
+    av, atv = np.zeros(lp.nnz), np.zeros(lp.nnz)
+    assert ps26119.lib().ps26119_amd_scale(*_model_args(lp, a, offset=False), _p(rs), _p(cs), _p(sc), _p(av), _p(atv)) == 0
+    e = oracle.scale(lp)
+    assert np.array_equal(rs, e["row_scale"]) and np.array_equal(cs, e["col_scale"])
+    assert np.array_equal(av, e["a_vals"]) and np.array_equal(atv, e["at_vals"])
+    assert sc[0] == e["bound_scale"] and sc[1] == e["obj_scale"] and bool(sc[3]) == e["geometric_mean_applied"]
+
+
+@pytest.mark.parametrize("name", ["stocfor1", "wyndor", "random"])
+def test_host_verification_is_bit_identical_to_oracle(oracle, name):
+    import ps26119
+    from ps26119_amd import lpgen
+    lp = lpgen.random_lp(2000, seed=3) if name == "random" else (hand_models()[name] if name == "wyndor" else netlib(name))
+    r, v = oracle.solve(lp, presolve=False)
+    a = _arrays(lp)
+    rng = np.random.default_rng(1)
+    for y in (v["y"], v["y"] + 1e-3 * rng.standard_normal(lp.m)):
+        y = np.ascontiguousarray(y)
+        got = ps26119.lib().ps26119_amd_certified_bound(*_model_args(lp, a), _p(y), 1)
+        assert got == oracle.certified_bound(lp, y)
+    out = np.zeros(6)
+    assert ps26119.lib().ps26119_amd_check_solution(*_model_args(lp, a), _p(v["x"]), _p(np.ascontiguousarray(v["y"])), _p(out)) == 0
+    assert list(out) == list(oracle.check_solution(lp, v["x"], v["y"]).values())
+
+
+def test_host_presolve_counts(oracle):
+    import ps26119
+    from ps26119_amd import lpgen
+    lp = lpgen.random_lp(2000, seed=4)
+    a = _arrays(lp)
+    out = np.zeros(7)
+    assert ps26119.lib().ps26119_amd_presolve_info(*_model_args(lp, a), _p(out)) == 0
+    empty_cols = int(np.sum(np.diff(lp.col_start) == 0))
+    assert out[0] == 0 and out[1] == 0 and out[2] == empty_cols  # random LP: presolve drops the empty columns
+    assert out[4] == lp.n - empty_cols and out[5] == lp.nnz
+
+
+def _solve_raw(lp, opt=None, result=True, **over):
+    import ps26119
+    L = ps26119.lib()
+    a = _arrays(lp)
+    for k, v in over.items():
+        a[k] = v
+    res = ps26119.CResult() if result else None
+    st = L.ps26119_solve_lp(*_model_args(lp, a), ctypes.byref(opt) if opt is not None else None,
+                            ctypes.byref(res) if res is not None else None, None, None, None)
+    return st, res
+
+
+def test_invalid_arguments_never_crash():
+    import ps26119
+    lp = hand_models()["wyndor"]
+    st, _ = _solve_raw(lp, result=False)
+    assert st == ps26119.INVALID_ARGUMENT
+    st, res = _solve_raw(lp, ri=np.array([0, 7, 1, 2], np.int32))
+    assert st == ps26119.INVALID_ARGUMENT and b"invalid model" in res.message
+    for field, value in [("time_limit", -1.0), ("time_limit", float("nan")), ("tolerance", -1e-8),
+                         ("tolerance", float("nan")), ("iteration_limit", -5), ("threads", -3), ("precision", 7),
+                         ("algorithm", 99)]:
+        opt = ps26119.Options()
+        ps26119.lib().ps26119_default_options(ctypes.byref(opt))
+        setattr(opt, field, value)
+        st, res = _solve_raw(lp, opt)
+        assert st == ps26119.INVALID_ARGUMENT, field
+        assert len(res.message) > 0
+
+
+def test_cpu_requests_and_other_engines_are_not_solved():
+    import ps26119
+    lp = hand_models()["wyndor"]
+    r = ps26119.solve_model(lp, gpu=False)
+    assert r.status == ps26119.NOT_SOLVED and "use_gpu" in r.message
+    r = ps26119.solve_model(lp, algorithm="simplex")
+    assert r.status == ps26119.NOT_SOLVED and "r2hpdhg" in r.message
+    assert ps26119.lib().ps26119_solve_mps(b"x.mps", None, ctypes.byref(ps26119.CResult()), None) == ps26119.INVALID_ARGUMENT
+
+
+def test_no_device_is_not_solved_with_reason():
+    import ps26119
+    if ps26119.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    r = ps26119.solve_model(hand_models()["wyndor"])
+    assert r.status == ps26119.NOT_SOLVED and "HIP backend unavailable" in r.message
+
+
+def test_version_and_defaults():
+    import ps26119
+    assert ps26119.version()
+    opt = ps26119.Options()
+    ps26119.lib().ps26119_default_options(ctypes.byref(opt))
+    assert (opt.algorithm, opt.precision, opt.use_gpu, opt.tolerance, opt.threads) == (0, 0, 0, 1e-8, 1)
