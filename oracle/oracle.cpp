@@ -1803,6 +1803,292 @@ void gate(const Lp& l, double tol, Sol& s) {  // solve.cpp:119-185 (LP, first-or
   if (tol < 1e-6) s.status = NumericalError;
 }
 
+// ------------------------------------------------------------------ batched scenarios
+// Lockstep r²HPDHG over K scenarios sharing A (reference src/pdhg/batch.cpp:129-441): shared
+// scaling without the bound/objective scalars, per-scenario bs/os, interleaved n×W layout,
+// SpMM, per-scenario restarts / PID / termination, compaction of finished scenarios.
+struct Scen {
+  std::vector<double> c, cl, cu, rl, ru;
+};
+
+static void spmm(const Csr<double>& M, const std::vector<double>& X, std::vector<double>& Y, int W) {  // batch.cpp:40-91
+  std::vector<double> acc(W);
+  for (int i = 0; i < M.rows; ++i) {
+    std::fill(acc.begin(), acc.end(), 0.0);
+    for (std::int64_t p = M.ptr[i]; p < M.ptr[i + 1]; ++p) {
+      const double a = M.val[p];
+      const double* x = X.data() + static_cast<std::size_t>(M.idx[p]) * W;
+      for (int k = 0; k < W; ++k) acc[k] += a * x[k];
+    }
+    std::copy(acc.begin(), acc.end(), Y.begin() + static_cast<std::size_t>(i) * W);
+  }
+}
+
+static void compact(std::vector<double>& v, std::size_t rows, int W, const std::vector<int>& keep) {
+  const int W2 = static_cast<int>(keep.size());
+  std::vector<double> nv(rows * W2);
+  for (std::size_t r = 0; r < rows; ++r)
+    for (int t = 0; t < W2; ++t) nv[r * W2 + t] = v[r * W + keep[t]];
+  v.swap(nv);
+}
+
+std::vector<Sol> solve_batch(const Lp& base, const std::vector<Scen>& S, const Opt& eo) {
+  const auto t0 = std::chrono::steady_clock::now();
+  auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  const int K = static_cast<int>(S.size());
+  std::vector<Sol> out(K);
+  if (K == 0) return out;
+  const int m = base.m, n = base.n;
+  std::vector<Scen> D(K);
+  for (int k = 0; k < K; ++k) {
+    auto pick = [](const std::vector<double>& v, const std::vector<double>& d, std::size_t len, bool& ok) {
+      if (v.empty()) return d;
+      ok = ok && v.size() == len;
+      return v;
+    };
+    bool ok = true;
+    D[k].c = pick(S[k].c, base.c, n, ok);
+    D[k].cl = pick(S[k].cl, base.cl, n, ok);
+    D[k].cu = pick(S[k].cu, base.cu, n, ok);
+    D[k].rl = pick(S[k].rl, base.rl, m, ok);
+    D[k].ru = pick(S[k].ru, base.ru, m, ok);
+    for (int j = 0; ok && j < n; ++j) ok = std::isfinite(D[k].c[j]) && !std::isnan(D[k].cl[j]) && !std::isnan(D[k].cu[j]);
+    for (int i = 0; ok && i < m; ++i) ok = !std::isnan(D[k].rl[i]) && !std::isnan(D[k].ru[i]);
+    bool crossed = false;
+    for (int j = 0; ok && !crossed && j < n; ++j) crossed = D[k].cl[j] > D[k].cu[j];
+    for (int i = 0; ok && !crossed && i < m; ++i) crossed = D[k].rl[i] > D[k].ru[i];
+    if (!ok) {
+      out[k].status = NotSolved;
+      out[k].message = "scenario has wrong vector sizes or non-finite data";
+    } else if (crossed) {
+      out[k].status = Infeasible;
+      out[k].message = "bounds cross: scenario has a lower bound above its upper bound";
+    }
+  }
+  ScaleOpt so = eo.sc;
+  so.bor = false;
+  const Scaled sp = scale(base, so);
+  const double nrm = norm2(sp.A, sp.At);
+  const double eta = nrm > 0 ? 0.998 / nrm : 1.0;
+  const double setup = elapsed();
+  const std::size_t NK = static_cast<std::size_t>(n) * K, MK = static_cast<std::size_t>(m) * K;
+  std::vector<double> C(NK), L(NK), U(NK), RL(MK), RU(MK), bs(K, 1.0), os(K, 1.0);
+  for (int k = 0; k < K; ++k) {
+    if (!out[k].message.empty()) continue;
+    if (eo.sc.bor) {
+      double b2 = 0, c2 = 0;
+      for (int i = 0; i < m; ++i) {
+        const double lo = std::isfinite(D[k].rl[i]) ? std::fabs(D[k].rl[i]) : 0.0;
+        const double up = std::isfinite(D[k].ru[i]) ? std::fabs(D[k].ru[i]) : 0.0;
+        const double b = sp.rs[i] * std::max(lo, up);
+        b2 += b * b;
+      }
+      for (int j = 0; j < n; ++j) {
+        const double c = sp.cs[j] * D[k].c[j];
+        c2 += c * c;
+      }
+      bs[k] = 1.0 / (std::sqrt(b2) + 1.0);
+      os[k] = 1.0 / (std::sqrt(c2) + 1.0);
+    }
+    for (int j = 0; j < n; ++j) {
+      const std::size_t q = static_cast<std::size_t>(j) * K + k;
+      C[q] = os[k] * sp.cs[j] * base.sense * D[k].c[j];
+      L[q] = bs[k] * D[k].cl[j] / sp.cs[j];
+      U[q] = bs[k] * D[k].cu[j] / sp.cs[j];
+    }
+    for (int i = 0; i < m; ++i) {
+      const std::size_t q = static_cast<std::size_t>(i) * K + k;
+      RL[q] = bs[k] * sp.rs[i] * D[k].rl[i];
+      RU[q] = bs[k] * sp.rs[i] * D[k].ru[i];
+    }
+  }
+  std::vector<int> sid;
+  for (int k = 0; k < K; ++k)
+    if (out[k].message.empty()) sid.push_back(k);
+  {
+    std::vector<int> keep(sid.begin(), sid.end());
+    for (auto* v : {&C, &L, &U}) compact(*v, n, K, keep);
+    for (auto* v : {&RL, &RU}) compact(*v, m, K, keep);
+  }
+  int W = static_cast<int>(sid.size());
+  std::vector<double> X(static_cast<std::size_t>(n) * W), X0, XH, XB(X.size()), ATY(X.size()), DX(X.size());
+  std::vector<double> Y(static_cast<std::size_t>(m) * W, 0.0), Y0, YH, YB(Y.size()), AX(Y.size()), DY(Y.size()), ADX(Y.size());
+  for (std::size_t q = 0; q < X.size(); ++q) X[q] = clampv<double>(0.0, L[q], U[q]);
+  X0 = X;
+  XH = X;
+  Y0 = Y;
+  YH = Y;
+  std::vector<double> omega(K, 1.0), best_w(K, 1.0), best_bal(K, 1e300), integ(K, 0.0), last_e(K, 0.0);
+  std::vector<double> r0(K, -1.0), r_last(K, kInf);
+  std::vector<std::int64_t> inner(K, 0), fast(K, -1);
+  const double rho = eo.rho, a2 = 2 * rho, b2c = 1 - 2 * rho;
+  const int Kc = eo.K;
+  std::int64_t it = 0;
+  std::vector<double> ct, cg, cw, resid, xk, yk;
+  auto unscale = [&](int t, std::vector<double>& x, std::vector<double>& y) {
+    const int k = sid[t];
+    x.resize(n);
+    y.resize(m);
+    for (int j = 0; j < n; ++j)
+      x[j] = clampv<double>(sp.cs[j] * XH[static_cast<std::size_t>(j) * W + t] / bs[k], D[k].cl[j], D[k].cu[j]);
+    for (int i = 0; i < m; ++i) y[i] = sp.rs[i] * YH[static_cast<std::size_t>(i) * W + t] / os[k];
+  };
+  auto finish = [&](int k, Status st, const Kkt& kk, const std::vector<double>& x, const std::vector<double>& ym) {
+    Sol& s = out[k];
+    s.status = st;
+    s.x = x;
+    s.y.resize(m);
+    for (int i = 0; i < m; ++i) s.y[i] = base.sense * ym[i];
+    s.z = D[k].c;
+    for (int j = 0; j < n; ++j)
+      for (int p = base.cs[j]; p < base.cs[j + 1]; ++p) s.z[j] -= base.v[p] * s.y[base.ri[p]];
+    double obj = base.offset;
+    for (int j = 0; j < n; ++j) obj += D[k].c[j] * x[j];
+    s.objective = obj;
+    s.dual_objective = base.sense * kk.dobj + base.offset;
+    s.rp = kk.rel_p();
+    s.rd = kk.rel_d();
+    s.gap = kk.rel_gap();
+    s.iterations = it;
+    s.seconds = elapsed();
+    s.setup_seconds = setup;
+    s.it_fast = fast[k];
+    s.primal_weight = omega[k];
+    s.engine = "r2hpdhg-batch";
+  };
+  auto residuals = [&]() {
+    for (std::size_t q = 0; q < X.size(); ++q) DX[q] = XB[q] - XH[q];
+    for (std::size_t q = 0; q < Y.size(); ++q) DY[q] = YB[q] - YH[q];
+    spmm(sp.A, DX, ADX, W);
+    std::vector<double> px(W, 0.0), py(W, 0.0), cr(W, 0.0);
+    for (std::size_t q = 0; q < X.size(); ++q) px[q % W] += DX[q] * DX[q];
+    for (std::size_t q = 0; q < Y.size(); ++q) py[q % W] += DY[q] * DY[q], cr[q % W] += DY[q] * ADX[q];
+    resid.assign(W, 0.0);
+    for (int t = 0; t < W; ++t) {
+      const double om = omega[sid[t]];
+      resid[t] = std::sqrt(std::max(om * px[t] + py[t] / om + 2.0 * eta * cr[t], 0.0));
+    }
+  };
+  while (W > 0) {
+    for (int step = 0; step < Kc; ++step) {
+      ct.resize(W);
+      cg.resize(W);
+      cw.resize(W);
+      for (int t = 0; t < W; ++t) {
+        const int k = sid[t];
+        ct[t] = eta / omega[k];
+        cg[t] = eta * omega[k];
+        cw[t] = static_cast<double>(inner[k] + 1) / static_cast<double>(inner[k] + 2);
+      }
+      spmm(sp.At, Y, ATY, W);
+      for (int j = 0; j < n; ++j)
+        for (int t = 0; t < W; ++t) {
+          const std::size_t q = static_cast<std::size_t>(j) * W + t;
+          const double h = clampv<double>(X[q] - ct[t] * (C[q] - ATY[q]), L[q], U[q]);
+          XH[q] = h;
+          XB[q] = 2 * h - X[q];
+          X[q] = cw[t] * (a2 * h + b2c * X[q]) + (1 - cw[t]) * X0[q];
+        }
+      spmm(sp.A, XB, AX, W);
+      for (int i = 0; i < m; ++i)
+        for (int t = 0; t < W; ++t) {
+          const std::size_t q = static_cast<std::size_t>(i) * W + t;
+          const double sg = cg[t];
+          const double v = AX[q] - Y[q] / sg;
+          const double h = Y[q] - sg * AX[q] + sg * clampv<double>(v, RL[q], RU[q]);
+          YH[q] = h;
+          YB[q] = 2 * h - Y[q];
+          Y[q] = cw[t] * (a2 * h + b2c * Y[q]) + (1 - cw[t]) * Y0[q];
+        }
+      bool any_new = false;
+      for (int t = 0; t < W; ++t) any_new = any_new || inner[sid[t]] == 0;
+      if (any_new) {
+        residuals();
+        for (int t = 0; t < W; ++t)
+          if (inner[sid[t]] == 0) r0[sid[t]] = resid[t];
+      }
+      for (int t = 0; t < W; ++t) ++inner[sid[t]];
+    }
+    it += Kc;
+    residuals();
+    const bool over_iter = it >= eo.iter_limit, over_time = elapsed() > eo.time_limit;
+    std::vector<int> keep;
+    for (int t = 0; t < W; ++t) {
+      const int k = sid[t];
+      unscale(t, xk, yk);
+      const View lv{m, n, base.sense, D[k].c.data(), D[k].cl.data(), D[k].cu.data(), D[k].rl.data(), D[k].ru.data()};
+      const Kkt kk = kkt_of(sp.A0, sp.At0, lv, xk, yk);
+      if (!kk.finite()) {
+        finish(k, NumericalError, kk, xk, yk);
+        out[k].message = "non-finite iterate";
+        continue;
+      }
+      if (fast[k] < 0 && kk.rel() <= 1e-4) fast[k] = it;
+      if (kk.converged(eo.tol)) {
+        finish(k, Optimal, kk, xk, yk);
+        continue;
+      }
+      if (over_iter || over_time) {
+        finish(k, over_iter ? IterationLimit : TimeLimit, kk, xk, yk);
+        continue;
+      }
+      keep.push_back(t);
+      const double r = resid[t];
+      const bool ok = r0[k] > 0 && std::isfinite(r0[k]) && r > 0;
+      const bool restart = (ok && (r <= eo.r_suff * r0[k] || (r <= eo.r_nec * r0[k] && r > r_last[k]))) ||
+                           static_cast<double>(inner[k]) >= eo.r_art * static_cast<double>(it);
+      r_last[k] = r;
+      if (!restart) continue;
+      double dxn = 0, dyn = 0;
+      for (int j = 0; j < n; ++j) {
+        const std::size_t q = static_cast<std::size_t>(j) * W + t;
+        dxn += (XH[q] - X0[q]) * (XH[q] - X0[q]);
+      }
+      for (int i = 0; i < m; ++i) {
+        const std::size_t q = static_cast<std::size_t>(i) * W + t;
+        dyn += (YH[q] - Y0[q]) * (YH[q] - Y0[q]);
+      }
+      dxn = std::sqrt(dxn);
+      dyn = std::sqrt(dyn);
+      const bool both = kk.rel_d() > 0 && kk.rel_p() > 0;
+      const double ratio = both ? kk.rel_d() / kk.rel_p() : 1.0;
+      if (dxn > 1e-16 && dyn > 1e-16 && dxn < 1e12 && dyn < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double e = std::log(dyn) - std::log(dxn) - std::log(omega[k]);
+        integ[k] = eo.idecay * integ[k] + e;
+        const double stp = eo.kp * e + eo.ki * integ[k] + eo.kd * (e - last_e[k]);
+        omega[k] *= std::exp(std::clamp(stp, -eo.max_log_step, eo.max_log_step));
+        last_e[k] = e;
+      } else {
+        omega[k] = best_w[k];
+        integ[k] = last_e[k] = 0.0;
+      }
+      if (both && std::fabs(std::log10(ratio)) < best_bal[k]) {
+        best_bal[k] = std::fabs(std::log10(ratio));
+        best_w[k] = omega[k];
+      }
+      for (int j = 0; j < n; ++j) {
+        const std::size_t q = static_cast<std::size_t>(j) * W + t;
+        X[q] = X0[q] = XH[q];
+      }
+      for (int i = 0; i < m; ++i) {
+        const std::size_t q = static_cast<std::size_t>(i) * W + t;
+        Y[q] = Y0[q] = YH[q];
+      }
+      inner[k] = 0;
+      r_last[k] = kInf;
+    }
+    if (static_cast<int>(keep.size()) != W) {
+      for (auto* v : {&C, &L, &U, &X, &X0, &XH, &XB, &ATY, &DX}) compact(*v, n, W, keep);
+      for (auto* v : {&RL, &RU, &Y, &Y0, &YH, &YB, &AX, &DY, &ADX}) compact(*v, m, W, keep);
+      std::vector<int> ns;
+      for (int t : keep) ns.push_back(sid[t]);
+      sid.swap(ns);
+      W = static_cast<int>(sid.size());
+    }
+  }
+  return out;
+}
+
 }  // namespace orc
 
 // ================================================================== extern "C" surface
@@ -1940,6 +2226,50 @@ int orc_backend_ops(const orc_lp* lp, int precision, int iters, double tau, doub
   const Kkt k = b.kkt(xh, yh);
   const double kv[8] = {k.rp, k.rd, k.pobj, k.dobj, k.bn, k.cn, k.pmax, k.dmax};
   std::memcpy(red + 3, kv, sizeof kv);
+  return 0;
+}
+
+int orc_solve_batch(const orc_lp* lp, int K, const orc_scenario* sc, const orc_opts* o, orc_result* results, double* xs,
+                    double* ys) {
+  const Lp base = from_abi(lp);
+  std::vector<Scen> S(K);
+  auto take = [](const double* p, int len) { return p ? std::vector<double>(p, p + len) : std::vector<double>{}; };
+  for (int k = 0; k < K; ++k) {
+    S[k].c = take(sc[k].obj, lp->n);
+    S[k].cl = take(sc[k].col_lower, lp->n);
+    S[k].cu = take(sc[k].col_upper, lp->n);
+    S[k].rl = take(sc[k].row_lower, lp->m);
+    S[k].ru = take(sc[k].row_upper, lp->m);
+  }
+  Opt e;
+  if (o->check_every > 0) e.K = o->check_every;
+  if (o->tolerance > 0) e.tol = o->tolerance;
+  if (o->time_limit > 0) e.time_limit = o->time_limit;
+  if (o->iteration_limit > 0) e.iter_limit = o->iteration_limit;
+  std::vector<std::pair<std::string, double>> ps;
+  for (int k = 0; k < o->n_params; ++k) ps.emplace_back(o->param_names[k], o->param_values[k]);
+  apply_params(e, ps);
+  const std::vector<Sol> out = solve_batch(base, S, e);
+  for (int k = 0; k < K; ++k) {
+    const Sol& s = out[k];
+    orc_result* r = results + k;
+    std::memset(r, 0, sizeof *r);
+    r->status = s.status;
+    r->iterations = s.iterations;
+    r->iterations_to_fast = s.it_fast;
+    r->objective = s.objective;
+    r->dual_objective = s.dual_objective;
+    r->primal_residual = s.rp;
+    r->dual_residual = s.rd;
+    r->gap = s.gap;
+    r->certified_bound = s.certified;
+    r->primal_weight = s.primal_weight;
+    r->seconds = s.seconds;
+    r->check = -1;
+    std::snprintf(r->message, sizeof r->message, "%s", s.message.c_str());
+    if (xs && static_cast<int>(s.x.size()) == lp->n) std::memcpy(xs + static_cast<size_t>(k) * lp->n, s.x.data(), sizeof(double) * lp->n);
+    if (ys && static_cast<int>(s.y.size()) == lp->m) std::memcpy(ys + static_cast<size_t>(k) * lp->m, s.y.data(), sizeof(double) * lp->m);
+  }
   return 0;
 }
 

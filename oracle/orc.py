@@ -68,6 +68,10 @@ class LpHandle:
                        _p(cs, ctypes.c_int), _p(ri, ctypes.c_int), _p(v))
 
 
+class OrcScenario(ctypes.Structure):
+    _fields_ = [("obj", D), ("col_lower", D), ("col_upper", D), ("row_lower", D), ("row_upper", D)]
+
+
 class Checker:
     def __init__(self, kind: str):
         path = PATHS[kind]
@@ -92,6 +96,8 @@ class Checker:
         self._ops = fn("backend_ops", [L, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, D, D, D, D, D, D, D])
         self._bound = fn("certified_bound", [L, D, ctypes.c_int], ctypes.c_double)
+        self._batch = fn("solve_batch", [L, ctypes.c_int, ctypes.POINTER(OrcScenario), ctypes.POINTER(OrcOpts),
+                                         ctypes.POINTER(OrcResult), D, D])
         self._check = fn("check_solution", [L, D, D, D])
 
     def solve(self, lp, precision=0, presolve=True, tolerance=1e-8, iteration_limit=0, time_limit=0.0,
@@ -109,6 +115,30 @@ class Checker:
         dr, pr = np.full(lp.m, np.nan), np.full(lp.n, np.nan)
         self._solve(ctypes.byref(h.s), ctypes.byref(o), ctypes.byref(r), _p(x), _p(y), _p(z), _p(dr), _p(pr))
         return r.as_dict(), {"x": x, "y": y, "z": z, "dual_ray": dr, "primal_ray": pr}
+
+    def solve_batch(self, lp, scenarios, tolerance=1e-8, iteration_limit=0, check_every=0, params=None):
+        """scenarios: list of dicts with optional keys obj, col_lower, col_upper, row_lower, row_upper."""
+        h = LpHandle(lp)
+        K = len(scenarios)
+        keep = []
+        arr = (OrcScenario * max(1, K))()
+        for k, sc in enumerate(scenarios):
+            vals = []
+            for key in ("obj", "col_lower", "col_upper", "row_lower", "row_upper"):
+                a = sc.get(key)
+                a = None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                keep.append(a)
+                vals.append(_p(a))
+            arr[k] = OrcScenario(*vals)
+        names = [k.encode() for k in (params or {})]
+        pv = np.array(list((params or {}).values()), dtype=np.float64)
+        name_arr = (ctypes.c_char_p * max(1, len(names)))(*names)
+        o = OrcOpts(3, 0, 0, 1, check_every, tolerance, 0.0, iteration_limit, None, None, 0.0, len(names), name_arr,
+                    _p(pv) if len(names) else None)
+        res = (OrcResult * max(1, K))()
+        xs, ys = np.full(K * lp.n, np.nan), np.full(K * lp.m, np.nan)
+        self._batch(ctypes.byref(h.s), K, arr, ctypes.byref(o), res, _p(xs), _p(ys))
+        return [res[k].as_dict() for k in range(K)], xs.reshape(K, lp.n), ys.reshape(K, lp.m)
 
     def scale(self, lp, geo_iters=12, geo_min_range=4.5, ruiz=10, pock_chambolle=True, bound_objective_rescaling=True):
         h = LpHandle(lp)

@@ -48,6 +48,11 @@ typedef struct {
   char message[512];
 } orc_result;
 
+/* One scenario of a batch sharing the base matrix; NULL vectors mean "same as the base". */
+typedef struct {
+  const double *obj, *col_lower, *col_upper, *row_lower, *row_upper;
+} orc_scenario;
+
 #ifdef __cplusplus
 }
 #endif

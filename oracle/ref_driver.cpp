@@ -13,6 +13,7 @@
 #include "la/csr.h"
 #include "pdhg/backend.h"
 #include "pdhg/scaling.h"
+#include "ps26119/batch.h"
 #include "ps26119/solve.h"
 #include "../oracle/orc_abi.h"
 
@@ -189,6 +190,51 @@ int ref_backend_ops(const orc_lp* lp, int precision, int iters, double tau, doub
   const double kv[8] = {k.primal_residual, k.dual_residual, k.primal_obj, k.dual_obj,
                         k.b_norm,          k.c_norm,        k.primal_max_rel, k.dual_max_rel};
   std::memcpy(red + 3, kv, sizeof kv);
+  return 0;
+}
+
+// Batched scenarios through the reference's solve_batch (src/pdhg/batch.cpp:129-441).
+// results[k], xs[k·n ..], ys[k·m ..] per scenario.
+int ref_solve_batch(const orc_lp* lp, int K, const orc_scenario* sc, const orc_opts* o, orc_result* results, double* xs,
+                    double* ys) {
+  const Model base = to_model(lp);
+  std::vector<Scenario> scen(K);
+  auto take = [](const double* p, int len) { return p ? std::vector<double>(p, p + len) : std::vector<double>{}; };
+  for (int k = 0; k < K; ++k) {
+    scen[k].obj = take(sc[k].obj, lp->n);
+    scen[k].col_lower = take(sc[k].col_lower, lp->n);
+    scen[k].col_upper = take(sc[k].col_upper, lp->n);
+    scen[k].row_lower = take(sc[k].row_lower, lp->m);
+    scen[k].row_upper = take(sc[k].row_upper, lp->m);
+  }
+  Options opt;
+  opt.threads = o->threads;
+  if (o->check_every > 0) opt.termination_check_every = o->check_every;
+  if (o->tolerance > 0) opt.tolerance = o->tolerance;
+  if (o->time_limit > 0) opt.time_limit = o->time_limit;
+  if (o->iteration_limit > 0) opt.iteration_limit = o->iteration_limit;
+  for (int k = 0; k < o->n_params; ++k) opt.engine_params.emplace_back(o->param_names[k], o->param_values[k]);
+  const std::vector<Solution> out = solve_batch(base, scen, opt);
+  for (int k = 0; k < K; ++k) {
+    const Solution& s = out[k];
+    orc_result* r = results + k;
+    std::memset(r, 0, sizeof *r);
+    r->status = static_cast<int>(s.status);
+    r->iterations = s.iterations;
+    r->iterations_to_fast = s.iterations_to_fast;
+    r->objective = s.objective;
+    r->dual_objective = s.dual_objective;
+    r->primal_residual = s.primal_residual;
+    r->dual_residual = s.dual_residual;
+    r->gap = s.gap;
+    r->certified_bound = s.certified_bound;
+    r->primal_weight = s.primal_weight;
+    r->seconds = s.seconds;
+    r->check = -1;
+    std::snprintf(r->message, sizeof r->message, "%s", s.message.c_str());
+    if (xs && static_cast<int>(s.x.size()) == lp->n) std::memcpy(xs + static_cast<size_t>(k) * lp->n, s.x.data(), sizeof(double) * lp->n);
+    if (ys && static_cast<int>(s.y.size()) == lp->m) std::memcpy(ys + static_cast<size_t>(k) * lp->m, s.y.data(), sizeof(double) * lp->m);
+  }
   return 0;
 }
 

@@ -186,3 +186,71 @@ def from_dense(c, A, rl, ru, cl, cu, sense: int = 1, name: str = "") -> Lp:
     return Lp(m=m, n=n, c=np.asarray(c, float), col_lower=np.asarray(cl, float), col_upper=np.asarray(cu, float),
               row_lower=np.asarray(rl, float), row_upper=np.asarray(ru, float), col_start=col_start,
               row_index=cols.astype(np.int32), value=A.T[rows, cols].astype(np.float64), sense=sense, name=name)
+
+
+def refinery_lp(T: int = 8760, seed: int = 3) -> Lp:
+    """Multi-period planning LP in block-staircase form, feasible by construction.
+
+    Each period t has 60 columns and 49 rows:
+      columns  0..10  inventory of product q at the end of period t (bounds [0, U(10, 1000)])
+               11..21 purchase of product q, 22..32 sale of product q
+               33..59 process activities
+      rows     0..10  balance of product q (equality):
+                      yields of the activities + inv[t-1, q] − inv[t, q] + purchase − sale = demand
+               11..48 capacity rows (<=) over 2–4 activities
+    Yield / capacity coefficients are log-uniform in [1e-3, 1e3] with random signs on the yields;
+    the inventory columns link periods t and t+1 with +1 / −1. Demands are U(1, 100) times a daily
+    sinusoid (period 24); column upper bounds U(10, 1000); costs U(−5, 20). A point x0 inside the
+    bounds is drawn first and every capacity row gets rhs = (A x0)_i + slack, purchases and sales
+    absorb the balance residual (their bounds widened to contain it), so x0 is feasible; all
+    columns are bounded, so the LP has an optimum (not known in closed form)."""
+    rng = np.random.default_rng(seed)
+    P, R = 60, 49
+    n, m = T * P, T * R
+    rows, cols, vals = [], [], []
+
+    def logu(k):
+        return np.exp(rng.uniform(np.log(1e-3), np.log(1e3), k))
+
+    cap = rng.uniform(10.0, 1000.0, n)
+    x0 = rng.uniform(0.0, 1.0, n) * cap * 0.5
+    t_idx = np.arange(T)
+    phase = rng.uniform(0, 2 * np.pi, 11)
+    demand = rng.uniform(1.0, 100.0, (T, 11)) * (1.0 + 0.5 * np.sin(2 * np.pi * t_idx[:, None] / 24.0 + phase[None, :]))
+    # structure of one period (same pattern every period, fresh coefficients)
+    bal_acts = [rng.choice(np.arange(33, 60), size=rng.integers(2, 5), replace=False) for _ in range(11)]
+    cap_acts = [rng.choice(np.arange(33, 60), size=rng.integers(2, 5), replace=False) for _ in range(38)]
+    rl = np.empty(m)
+    ru = np.empty(m)
+    for t in range(T):
+        c0, r0 = t * P, t * R
+        for q in range(11):
+            r = r0 + q
+            acts = bal_acts[q]
+            coef = logu(len(acts)) * rng.choice([-1.0, 1.0], len(acts))
+            s = float(np.dot(coef, x0[c0 + acts]))
+            rows.extend([r] * len(acts)), cols.extend((c0 + acts).tolist()), vals.extend(coef.tolist())
+            rows.append(r), cols.append(c0 + q), vals.append(-1.0)          # inventory out of period t
+            s -= x0[c0 + q]
+            if t > 0:
+                rows.append(r), cols.append(c0 - P + q), vals.append(1.0)   # inventory from period t-1
+                s += x0[c0 - P + q]
+            rows.append(r), cols.append(c0 + 11 + q), vals.append(1.0)      # purchase
+            rows.append(r), cols.append(c0 + 22 + q), vals.append(-1.0)     # sale
+            gap = demand[t, q] - s
+            buy, sell = (gap, 0.0) if gap >= 0 else (0.0, -gap)
+            x0[c0 + 11 + q], x0[c0 + 22 + q] = buy, sell
+            cap[c0 + 11 + q] = max(cap[c0 + 11 + q], 2.0 * buy)
+            cap[c0 + 22 + q] = max(cap[c0 + 22 + q], 2.0 * sell)
+            rl[r] = ru[r] = demand[t, q]
+        for k in range(38):
+            r = r0 + 11 + k
+            acts = cap_acts[k]
+            coef = logu(len(acts))
+            rows.extend([r] * len(acts)), cols.extend((c0 + acts).tolist()), vals.extend(coef.tolist())
+            rl[r] = -INF
+            ru[r] = float(np.dot(coef, x0[c0 + acts])) * rng.uniform(1.05, 2.0)
+    cs, ri, v = _csc_from_triplets(m, n, np.asarray(rows), np.asarray(cols), np.asarray(vals))
+    c = rng.uniform(-5.0, 20.0, n)
+    return Lp(m=m, n=n, c=c, col_lower=np.zeros(n), col_upper=cap, row_lower=rl, row_upper=ru, col_start=cs,
+              row_index=ri, value=v, name=f"refinery-T{T}-s{seed}", x_star=x0)
