@@ -43,6 +43,27 @@ int ps26119_amd_solve_lp(int num_rows, int num_cols, int sense, double obj_offse
                          ps26119_result* result, ps26119_amd_stats* stats, double* x, double* y, double* z,
                          double* dual_ray, double* primal_ray);
 
+/* One scenario of a batch: any array left NULL keeps the base model's. */
+typedef struct {
+  const double* obj;        /* n */
+  const double* col_lower;  /* n */
+  const double* col_upper;  /* n */
+  const double* row_lower;  /* m */
+  const double* row_upper;  /* m */
+} ps26119_amd_scenario;
+
+/* Scenarios sharing the base model's constraint matrix, solved in lockstep on the GPU (replaces
+ * the reference's C++-only ps26119::solve_batch, include/ps26119/batch.h:32-33 there, which runs
+ * on the CPU). opt->use_gpu must be 1; opt->warm_x / warm_y, when set, start every scenario from
+ * that point. results[k] (required), xs[k·n + j] and ys[k·m + i] (may be NULL) receive scenario k.
+ * Returns PS26119_OPTIMAL when every scenario is optimal, else the first other status. */
+int ps26119_amd_solve_batch(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                            const double* col_lower, const double* col_upper, const double* row_lower,
+                            const double* row_upper, const int* col_start, const int* row_index,
+                            const double* value, int num_scenarios, const ps26119_amd_scenario* scenarios,
+                            const ps26119_options* opt, const ps26119_amd_options* ext, ps26119_result* results,
+                            double* xs, double* ys);
+
 /* Number of visible GPUs the backend can use (0 when none). */
 int ps26119_amd_device_count(void);
 

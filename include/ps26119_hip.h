@@ -152,6 +152,40 @@ int psh_lanczos_begin(psh_ctx* ctx, const double* v0);
 int psh_lanczos_step(psh_ctx* ctx, double beta_prev, int mode, double* alpha, double* beta_sq);
 int psh_lanczos_advance(psh_ctx* ctx, double beta);
 
+/* ---- batched scenarios sharing Ã (reference src/pdhg/batch.cpp:129-441), on a context that has
+ * the shared scaled problem loaded (psh_load). Block vectors are interleaved: entry (row r,
+ * slot t) at r·W + t, W ≤ PSH_BATCH_MAX_W slots. Scaled scenario data: c, l, u (n·W), rl, ru
+ * (m·W); original-space data for the KKT: c_min = sense·c, cl, cu (n·W), rl, ru (m·W); per-slot
+ * bound / objective scalars bs, os (W). */
+enum { PSH_BATCH_MAX_W = 256 };
+/* Per-slot statistics written by psh_batch_check, PSH_BSTAT_COUNT doubles per slot:
+ * residual sums, KKT sums and maxima (termination.h), PID sums, residual sums of the last
+ * epoch start. */
+enum {
+  PSH_BSTAT_PX, PSH_BSTAT_PY, PSH_BSTAT_CROSS, PSH_BSTAT_RP2, PSH_BSTAT_B2, PSH_BSTAT_D, PSH_BSTAT_RD2,
+  PSH_BSTAT_C2, PSH_BSTAT_P, PSH_BSTAT_PIDX, PSH_BSTAT_PIDY, PSH_BSTAT_PMAX, PSH_BSTAT_DMAX, PSH_BSTAT_CINF,
+  PSH_BSTAT_R0_PX, PSH_BSTAT_R0_PY, PSH_BSTAT_R0_CROSS, PSH_BSTAT_COUNT
+};
+int psh_batch_begin(psh_ctx* ctx, int W, const double* c, const double* l, const double* u, const double* rl,
+                    const double* ru, const double* c_min, const double* cl_o, const double* cu_o,
+                    const double* rl_o, const double* ru_o, const double* bs, const double* os);
+/* x (n·W) and y (m·W), scaled: x = x0 = x̂, y = y0 = ŷ. */
+int psh_batch_set_iterate(psh_ctx* ctx, const double* x, const double* y);
+/* `iterations` lockstep r²HPDHG steps; per-slot tau, sigma (W) and Halpern index of the first step
+ * inner0 (W). When some slot starts an epoch (inner0 = 0) the residual sums of every slot after
+ * the first step are kept for the next psh_batch_check (reference batch.cpp:347-354). */
+int psh_batch_chunk(psh_ctx* ctx, int iterations, const double* tau, const double* sigma, const long long* inner0,
+                    double rho, int mode);
+/* Per-slot statistics into out (W · PSH_BSTAT_COUNT); mode PSH_RED_* (CPU order = the reference's
+ * sequential per-slot loops). */
+int psh_batch_check(psh_ctx* ctx, int mode, double* out);
+/* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ for the slots with mask[t] != 0. */
+int psh_batch_restart(psh_ctx* ctx, const int* mask);
+/* Keeps the slots keep[0..W2) in that order (finished scenarios leave the batch). */
+int psh_batch_compact(psh_ctx* ctx, const int* keep, int W2);
+/* Unscaled point of slot t: x = clip(C x̂ / bs, l, u) (n), y_min = R ŷ / os (m). */
+int psh_batch_point(psh_ctx* ctx, int t, double* x, double* y_min);
+
 /* ---- per-kernel timing (HIP events on the context's stream), for the benchmark. */
 int psh_timing(psh_ctx* ctx, int on);
 /* kernel classes: 0 fused primal (Ãᵀy + projection/reflection/Halpern), 1 fused dual (Ãx̄ + ...),

@@ -124,9 +124,12 @@ class Checker:
         arr = (OrcScenario * max(1, K))()
         for k, sc in enumerate(scenarios):
             vals = []
-            for key in ("obj", "col_lower", "col_upper", "row_lower", "row_upper"):
+            for key, length in (("obj", lp.n), ("col_lower", lp.n), ("col_upper", lp.n), ("row_lower", lp.m),
+                                ("row_upper", lp.m)):
                 a = sc.get(key)
-                a = None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                a = None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+                if a is not None and a.size != length:  # same verdict as a wrong size: "non-finite data"
+                    a = np.full(length, np.nan)
                 keep.append(a)
                 vals.append(_p(a))
             arr[k] = OrcScenario(*vals)

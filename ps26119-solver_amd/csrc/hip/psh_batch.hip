@@ -1,0 +1,567 @@
+// psh_batch.hip — lockstep r²HPDHG over W scenarios sharing Ã (reference src/pdhg/batch.cpp).
+//
+// Block vectors are interleaved, entry (row r, slot t) at r·W + t, so one thread per (row, slot)
+// with consecutive threads on consecutive slots turns every gather of the SpMM into a contiguous
+// W-element read, and each nonzero of Ã / Ãᵀ is fetched once per iteration for all scenarios.
+// Each thread sums its (row, slot) product sequentially in CSR order and runs the fused update —
+// the reference's per-scenario expressions (batch.cpp:319-346), so the bits are the reference's.
+// Per-slot sums (residual, KKT, PID) go through a segmented reducer: fixed-shape trees, or one
+// sequential loop per slot in the reference order (batch.cpp:293-305, 390-400; kkt_general).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "psh_internal.h"
+
+using namespace psh;
+
+namespace {
+
+// Batch term arrays.
+enum BTerm {
+  B_DX2, B_DY2, B_CROSS, B_RP2, B_B2, B_DROW, B_RD2ROW, B_C2, B_P, B_DCOL, B_RD2COL, B_PIDX, B_PIDY,
+  B_PMAXR, B_PMAXC, B_DMAXR, B_DMAXC, B_CINF, kBTerms
+};
+constexpr int kBTreeBlocks = 64;
+
+struct BArgs {
+  int m, n, W, it, hat;
+  double rho2, rho1;  // 2ρ, 1 − 2ρ
+  const int *a_ptr, *a_col, *at_ptr, *at_col, *ato_col;
+  const double *a_val, *at_val, *a_orig, *ato_val;
+  const double *rs, *cs;
+  BatchState b;
+};
+
+__device__ inline double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ inline double dmaxd(double a, double b) { return a < b ? b : a; }
+__device__ inline double dmind(double a, double b) { return b < a ? b : a; }
+
+__device__ inline double* bterm(const BArgs& a, int k) {
+  return a.b.terms + static_cast<size_t>(k) * static_cast<size_t>(a.b.rows_max) * a.b.cap;
+}
+
+__device__ inline double halpern_w(long long inner0, int it) {
+  const long long k = inner0 + it;
+  return static_cast<double>(k + 1) / static_cast<double>(k + 2);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bprimal(BArgs a) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long j = g / a.W;
+  if (j >= a.n) return;
+  const int t = static_cast<int>(g - j * a.W);
+  double s = 0.0;
+  for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) s += a.at_val[p] * a.b.y[static_cast<size_t>(a.at_col[p]) * a.W + t];
+  const size_t q = static_cast<size_t>(g);
+  const double w = halpern_w(a.b.inner0[t], a.it);
+  const double xq = a.b.x[q];
+  const double h = clampd(xq - a.b.tau[t] * (a.b.c[q] - s), a.b.l[q], a.b.u[q]);
+  if (a.hat) a.b.xh[q] = h;
+  a.b.xb[q] = 2 * h - xq;
+  a.b.x[q] = w * (a.rho2 * h + a.rho1 * xq) + (1 - w) * a.b.x0[q];
+}
+
+__global__ void __launch_bounds__(kBlock) k_bdual(BArgs a) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long i = g / a.W;
+  if (i >= a.m) return;
+  const int t = static_cast<int>(g - i * a.W);
+  double s = 0.0;
+  for (int p = a.a_ptr[i]; p < a.a_ptr[i + 1]; ++p) s += a.a_val[p] * a.b.xb[static_cast<size_t>(a.a_col[p]) * a.W + t];
+  const size_t q = static_cast<size_t>(g);
+  const double w = halpern_w(a.b.inner0[t], a.it);
+  const double sg = a.b.sigma[t];
+  const double yq = a.b.y[q];
+  const double v = s - yq / sg;
+  const double h = yq - sg * s + sg * clampd(v, a.b.rl[q], a.b.ru[q]);
+  if (a.hat) {
+    a.b.yh[q] = h;
+    a.b.yb[q] = 2 * h - yq;
+  }
+  a.b.y[q] = w * (a.rho2 * h + a.rho1 * yq) + (1 - w) * a.b.y0[q];
+}
+
+// ---- residual and check passes (one thread per (row, slot))
+__global__ void __launch_bounds__(kBlock) k_bcheck_n(BArgs a, int kkt) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long j = g / a.W;
+  if (j >= a.n) return;
+  const int t = static_cast<int>(g - j * a.W);
+  const size_t q = static_cast<size_t>(g);
+  const double d = a.b.xb[q] - a.b.xh[q];
+  a.b.dx[q] = d;
+  bterm(a, B_DX2)[q] = d * d;
+  if (kkt) {
+    a.b.xo[q] = clampd(a.cs[j] * a.b.xh[q] / a.b.bs[t], a.b.clo[q], a.b.cuo[q]);
+    const double e = a.b.xh[q] - a.b.x0[q];
+    bterm(a, B_PIDX)[q] = e * e;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bcheck_m(BArgs a, int kkt) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long i = g / a.W;
+  if (i >= a.m) return;
+  const int t = static_cast<int>(g - i * a.W);
+  const size_t q = static_cast<size_t>(g);
+  const double d = a.b.yb[q] - a.b.yh[q];
+  a.b.dy[q] = d;
+  bterm(a, B_DY2)[q] = d * d;
+  if (kkt) {
+    a.b.yo[q] = a.rs[i] * a.b.yh[q] / a.b.os[t];
+    const double e = a.b.yh[q] - a.b.y0[q];
+    bterm(a, B_PIDY)[q] = e * e;
+  }
+}
+
+__device__ inline double elem_viol(double v, double lo, double up) {
+  if (v < lo) return (lo - v) / (1.0 + fabs(lo));
+  if (v > up) return (v - up) / (1.0 + fabs(up));
+  return 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bcheck_rows(BArgs a, int kkt) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long i = g / a.W;
+  if (i >= a.m) return;
+  const int t = static_cast<int>(g - i * a.W);
+  const size_t q = static_cast<size_t>(g);
+  const int b = a.a_ptr[i], e = a.a_ptr[i + 1];
+  double s = 0.0;
+  for (int p = b; p < e; ++p) s += a.a_val[p] * a.b.dx[static_cast<size_t>(a.a_col[p]) * a.W + t];
+  a.b.adx[q] = s;
+  bterm(a, B_CROSS)[q] = a.b.dy[q] * s;
+  if (!kkt) return;
+  double ax = 0.0;
+  for (int p = b; p < e; ++p) ax += a.a_orig[p] * a.b.xo[static_cast<size_t>(a.a_col[p]) * a.W + t];
+  const double lo = a.b.rlo[q], up = a.b.ruo[q];
+  const double viol = ax < lo ? lo - ax : (ax > up ? ax - up : 0.0);
+  bterm(a, B_RP2)[q] = viol * viol;
+  bterm(a, B_PMAXR)[q] = elem_viol(ax, lo, up);
+  const double bl = isfinite(lo) ? fabs(lo) : 0.0, bu = isfinite(up) ? fabs(up) : 0.0;
+  bterm(a, B_B2)[q] = dmaxd(bl, bu) * dmaxd(bl, bu);
+  const double y = a.b.yo[q];
+  double dt = 0.0, rd = 0.0, dm = 0.0;
+  if (y > 0) {
+    if (isfinite(lo)) dt = y * lo;
+    else rd = y * y, dm = y;
+  } else if (y < 0) {
+    if (isfinite(up)) dt = y * up;
+    else rd = y * y, dm = -y;
+  }
+  bterm(a, B_DROW)[q] = dt;
+  bterm(a, B_RD2ROW)[q] = rd;
+  bterm(a, B_DMAXR)[q] = dm;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bcheck_cols(BArgs a) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long j = g / a.W;
+  if (j >= a.n) return;
+  const int t = static_cast<int>(g - j * a.W);
+  const size_t q = static_cast<size_t>(g);
+  double s = 0.0;
+  for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) s += a.ato_val[p] * a.b.yo[static_cast<size_t>(a.ato_col[p]) * a.W + t];
+  const double cj = a.b.cmin[q], xj = a.b.xo[q], lo = a.b.clo[q], up = a.b.cuo[q];
+  bterm(a, B_C2)[q] = cj * cj;
+  bterm(a, B_CINF)[q] = fabs(cj);
+  bterm(a, B_P)[q] = cj * xj;
+  bterm(a, B_PMAXC)[q] = elem_viol(xj, lo, up);
+  const double lam = cj - s;
+  double z;
+  if (isfinite(lo) && isfinite(up)) z = lam;
+  else if (isfinite(lo)) z = dmaxd(lam, 0.0);
+  else if (isfinite(up)) z = dmind(lam, 0.0);
+  else z = 0.0;
+  bterm(a, B_RD2COL)[q] = (lam - z) * (lam - z);
+  bterm(a, B_DMAXC)[q] = fabs(lam - z);
+  bterm(a, B_DCOL)[q] = z > 0 ? z * lo : (z < 0 ? z * up : 0.0);
+}
+
+// ---- segmented reducer over interleaved arrays: job → per-slot value at out[t·stride + slot]
+struct BJob {
+  int ta, tb;               // term indices (tb < 0: none)
+  long long ra, rb;         // rows of each
+  int op;                   // 0 sum, 1 max
+  int stat;                 // PSH_BSTAT_* index
+};
+
+struct BRed {
+  BJob jobs[20];
+  int njobs, W, mode;
+  const double* terms;
+  long long stride;  // rows_max · cap
+  double* partials;
+  double* stats;
+};
+
+__device__ inline double bval(const BRed& r, const BJob& J, long long row, int t) {
+  const int k = row < J.ra ? J.ta : J.tb;
+  const long long rr = row < J.ra ? row : row - J.ra;
+  return r.terms[static_cast<size_t>(k) * r.stride + static_cast<size_t>(rr) * r.W + t];
+}
+
+// Tree, level 1: block b of job j covers rows [R·b/B, R·(b+1)/B); threads are (slot, group).
+__global__ void __launch_bounds__(kBlock) k_bred1(BRed r) {
+  const BJob J = r.jobs[blockIdx.y];
+  if (J.op == 0 && r.mode != 0) return;
+  const int W = r.W, G = kBlock / W;
+  const int t = threadIdx.x % W, grp = threadIdx.x / W;
+  const long long R = J.ra + J.rb;
+  const long long lo = R * blockIdx.x / kBTreeBlocks, hi = R * (blockIdx.x + 1) / kBTreeBlocks;
+  double acc = 0.0;
+  if (grp < G)
+    for (long long row = lo + grp; row < hi; row += G) {
+      const double v = bval(r, J, row, t);
+      acc = J.op == 0 ? acc + v : fmax(acc, v);
+    }
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (grp == 0) {
+    for (int k = 1; k < G; ++k) acc = J.op == 0 ? acc + red[k * W + t] : fmax(acc, red[k * W + t]);
+    r.partials[(static_cast<size_t>(blockIdx.y) * kBTreeBlocks + blockIdx.x) * W + t] = acc;
+  }
+}
+
+__global__ void k_bred2(BRed r) {
+  const int j = blockIdx.x, t = threadIdx.x;
+  const BJob J = r.jobs[j];
+  if (t >= r.W || (J.op == 0 && r.mode != 0)) return;
+  double acc = 0.0;
+  for (int b = 0; b < kBTreeBlocks; ++b) {
+    const double v = r.partials[(static_cast<size_t>(j) * kBTreeBlocks + b) * r.W + t];
+    acc = J.op == 0 ? acc + v : fmax(acc, v);
+  }
+  r.stats[static_cast<size_t>(t) * PSH_BSTAT_COUNT + J.stat] = acc;
+}
+
+// CPU order: one sequential loop per slot (lanes of a wave read consecutive addresses).
+__global__ void k_bred_seq(BRed r) {
+  const int j = blockIdx.x, t = threadIdx.x;
+  const BJob J = r.jobs[j];
+  if (t >= r.W || J.op != 0 || r.mode == 0) return;
+  double s = 0.0;
+  const long long R = J.ra + J.rb;
+  for (long long row = 0; row < R; ++row) s += bval(r, J, row, t);
+  r.stats[static_cast<size_t>(t) * PSH_BSTAT_COUNT + J.stat] = s;
+}
+
+__global__ void k_bgather(const double* __restrict__ src, double* __restrict__ dst, long long rows, int W, int W2,
+                          const int* __restrict__ keep) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= rows * W2) return;
+  const long long r = g / W2;
+  const int t2 = static_cast<int>(g - r * W2);
+  dst[g] = src[r * W + keep[t2]];
+}
+
+__global__ void k_brestart(double* __restrict__ x, double* __restrict__ x0, const double* __restrict__ xh,
+                           const int* __restrict__ mask, long long rows, int W) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= rows * W) return;
+  const int t = static_cast<int>(g % W);
+  if (!mask[t]) return;
+  x[g] = xh[g];
+  x0[g] = xh[g];
+}
+
+__global__ void k_bpoint(const double* __restrict__ xh, const double* __restrict__ yh, const double* __restrict__ cs,
+                         const double* __restrict__ rs, const double* __restrict__ clo, const double* __restrict__ cuo,
+                         double bs, double os, int t, int W, int m, int n, double* __restrict__ x, double* __restrict__ y) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g < n) {
+    const size_t q = static_cast<size_t>(g) * W + t;
+    x[g] = clampd(cs[g] * xh[q] / bs, clo[q], cuo[q]);
+  }
+  if (g < m) y[g] = rs[g] * yh[static_cast<size_t>(g) * W + t] / os;
+}
+
+BArgs make_bargs(psh_ctx* c, int it, int hat, double rho) {
+  BArgs a{};
+  a.m = c->m;
+  a.n = c->n;
+  a.W = c->batch->W;
+  a.it = it;
+  a.hat = hat;
+  a.rho2 = 2 * rho;
+  a.rho1 = 1 - 2 * rho;
+  a.a_ptr = c->A.ptr;
+  a.a_col = c->A.col;
+  a.at_ptr = c->At.ptr;
+  a.at_col = c->At.col;
+  a.ato_col = c->ato_col;
+  a.a_val = c->A.val;
+  a.at_val = c->At.val;
+  a.a_orig = c->a_orig;
+  a.ato_val = c->ato_val;
+  a.rs = c->rs;
+  a.cs = c->cs;
+  a.b = *c->batch;
+  return a;
+}
+
+int grid_rows(long long rows, int W) { return grid_for(rows * W); }
+
+void free_batch(psh_ctx* c) {
+  BatchState* b = c->batch;
+  if (!b) return;
+  void* ptrs[] = {b->c, b->l, b->u, b->rl, b->ru, b->cmin, b->clo, b->cuo, b->rlo, b->ruo, b->bs, b->os, b->x, b->x0,
+                  b->xh, b->xb, b->y, b->y0, b->yh, b->yb, b->dx, b->dy, b->adx, b->xo, b->yo, b->tau, b->sigma,
+                  b->inner0, b->mask, b->keep, b->terms, b->partials, b->stats, b->tmp};
+  for (void* p : ptrs) dev_free(p);
+  if (b->pinned) (void)hipHostFree(b->pinned);
+  delete b;
+  c->batch = nullptr;
+}
+
+// Pinned staging layout (bytes): params (tau, sigma, inner0) | mask | keep | stats.
+constexpr size_t kPinParams = 0;
+constexpr size_t kPinMask = 3 * 8 * PSH_BATCH_MAX_W;
+constexpr size_t kPinKeep = kPinMask + 4 * PSH_BATCH_MAX_W;
+constexpr size_t kPinStats = kPinKeep + 4 * PSH_BATCH_MAX_W;
+constexpr size_t kPinBytes = kPinStats + 8 * PSH_BATCH_MAX_W * PSH_BSTAT_COUNT;
+
+int batch_reduce(psh_ctx* c, int mode, bool kkt, bool only_residual, int r0) {
+  BatchState& b = *c->batch;
+  BRed r{};
+  r.W = b.W;
+  r.mode = mode;
+  r.terms = b.terms;
+  r.stride = b.rows_max * b.cap;
+  r.partials = b.partials;
+  r.stats = b.stats;
+  const long long m = c->m, n = c->n;
+  int k = 0;
+  auto add = [&](int ta, long long ra, int op, int stat, int tb = -1, long long rb = 0) {
+    r.jobs[k++] = BJob{ta, tb < 0 ? ta : tb, ra, rb, op, stat};
+  };
+  add(B_DX2, n, 0, r0 ? PSH_BSTAT_R0_PX : PSH_BSTAT_PX);
+  add(B_DY2, m, 0, r0 ? PSH_BSTAT_R0_PY : PSH_BSTAT_PY);
+  add(B_CROSS, m, 0, r0 ? PSH_BSTAT_R0_CROSS : PSH_BSTAT_CROSS);
+  if (kkt && !only_residual) {
+    add(B_RP2, m, 0, PSH_BSTAT_RP2);
+    add(B_B2, m, 0, PSH_BSTAT_B2);
+    add(B_DROW, m, 0, PSH_BSTAT_D, B_DCOL, n);
+    add(B_RD2ROW, m, 0, PSH_BSTAT_RD2, B_RD2COL, n);
+    add(B_C2, n, 0, PSH_BSTAT_C2);
+    add(B_P, n, 0, PSH_BSTAT_P);
+    add(B_PIDX, n, 0, PSH_BSTAT_PIDX);
+    add(B_PIDY, m, 0, PSH_BSTAT_PIDY);
+    add(B_PMAXR, m, 1, PSH_BSTAT_PMAX, B_PMAXC, n);
+    add(B_DMAXR, m, 1, PSH_BSTAT_DMAX, B_DMAXC, n);
+    add(B_CINF, n, 1, PSH_BSTAT_CINF);
+  }
+  r.njobs = k;
+  hipLaunchKernelGGL(k_bred1, dim3(kBTreeBlocks, k), dim3(kBlock), 0, c->stream, r);
+  hipLaunchKernelGGL(k_bred2, dim3(k), dim3(PSH_BATCH_MAX_W), 0, c->stream, r);
+  if (mode != 0) hipLaunchKernelGGL(k_bred_seq, dim3(k), dim3(PSH_BATCH_MAX_W), 0, c->stream, r);
+  PSH_LAUNCHED(c, "batch reductions");
+  return PSH_OK;
+}
+
+int batch_residual_pass(psh_ctx* c, bool kkt, int mode, bool r0) {
+  BatchState& b = *c->batch;
+  const BArgs a = make_bargs(c, 0, 0, 1.0);
+  if (c->n > 0) hipLaunchKernelGGL(k_bcheck_n, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a, kkt ? 1 : 0);
+  if (c->m > 0) {
+    hipLaunchKernelGGL(k_bcheck_m, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, a, kkt ? 1 : 0);
+    hipLaunchKernelGGL(k_bcheck_rows, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, a, kkt ? 1 : 0);
+  }
+  if (kkt && c->n > 0) hipLaunchKernelGGL(k_bcheck_cols, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a);
+  PSH_LAUNCHED(c, "batch check kernels");
+  return batch_reduce(c, mode, kkt, !kkt, r0 ? 1 : 0);
+}
+
+int need_batch(psh_ctx* c) {
+  if (!c) return PSH_E_ARG;
+  if (!c->batch) return fail(c, PSH_E_STATE, "psh_batch_begin not called");
+  return PSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psh_batch_begin(psh_ctx* c, int W, const double* cc, const double* l, const double* u, const double* rl,
+                    const double* ru, const double* c_min, const double* cl_o, const double* cu_o, const double* rl_o,
+                    const double* ru_o, const double* bs, const double* os) {
+  if (!c) return PSH_E_ARG;
+  if (!c->loaded) return fail(c, PSH_E_STATE, "no problem loaded");
+  if (W < 1 || W > PSH_BATCH_MAX_W) return fail(c, PSH_E_ARG, "batch width out of range");
+  free_batch(c);
+  c->batch = new BatchState;
+  BatchState& b = *c->batch;
+  b.W = b.cap = W;
+  const long long m = c->m, n = c->n;
+  b.rows_max = std::max<long long>(std::max(m, n), 1);
+  const long long NW = n * W, MW = m * W, RW = b.rows_max * W;
+  int rc;
+  auto up = [&](double** dst, const double* src, long long count) -> int {
+    int r = alloc_n(c, dst, count);
+    if (r) return r;
+    if (count > 0 && src) PSH_TRY(c, hipMemcpyAsync(*dst, src, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+    return PSH_OK;
+  };
+  if ((rc = up(&b.c, cc, NW)) || (rc = up(&b.l, l, NW)) || (rc = up(&b.u, u, NW)) || (rc = up(&b.rl, rl, MW)) ||
+      (rc = up(&b.ru, ru, MW)) || (rc = up(&b.cmin, c_min, NW)) || (rc = up(&b.clo, cl_o, NW)) ||
+      (rc = up(&b.cuo, cu_o, NW)) || (rc = up(&b.rlo, rl_o, MW)) || (rc = up(&b.ruo, ru_o, MW)) ||
+      (rc = up(&b.bs, bs, W)) || (rc = up(&b.os, os, W)))
+    return rc;
+  for (double** v : {&b.x, &b.x0, &b.xh, &b.xb, &b.dx, &b.xo})
+    if ((rc = alloc_n(c, v, NW))) return rc;
+  for (double** v : {&b.y, &b.y0, &b.yh, &b.yb, &b.dy, &b.adx, &b.yo})
+    if ((rc = alloc_n(c, v, MW))) return rc;
+  if ((rc = alloc_n(c, &b.tau, W)) || (rc = alloc_n(c, &b.sigma, W)) || (rc = alloc_n(c, &b.inner0, W)) ||
+      (rc = alloc_n(c, &b.mask, W)) || (rc = alloc_n(c, &b.keep, W)) || (rc = alloc_n(c, &b.terms, RW * kBTerms)) ||
+      (rc = alloc_n(c, &b.partials, static_cast<long long>(kBTreeBlocks) * 20 * W)) ||
+      (rc = alloc_n(c, &b.stats, static_cast<long long>(W) * PSH_BSTAT_COUNT)) ||
+      (rc = alloc_n(c, &b.tmp, std::max(RW, m + n))))  // gather staging, and one slot's x | y_min
+    return rc;
+  if (hipHostMalloc(reinterpret_cast<void**>(&b.pinned), kPinBytes) != hipSuccess) return fail(c, PSH_E_OOM, "pinned staging");
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  return PSH_OK;
+}
+
+int psh_batch_set_iterate(psh_ctx* c, const double* x, const double* y) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  const size_t NW = static_cast<size_t>(c->n) * b.W, MW = static_cast<size_t>(c->m) * b.W;
+  for (double* d : {b.x, b.x0, b.xh})
+    if (NW) PSH_TRY(c, hipMemcpyAsync(d, x, sizeof(double) * NW, hipMemcpyHostToDevice, c->stream));
+  for (double* d : {b.y, b.y0, b.yh})
+    if (MW) PSH_TRY(c, hipMemcpyAsync(d, y, sizeof(double) * MW, hipMemcpyHostToDevice, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  return PSH_OK;
+}
+
+int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double* sigma, const long long* inner0, double rho,
+                    int mode) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  const int W = b.W;
+  if (iterations < 1 || W < 1) return fail(c, PSH_E_ARG, "empty batch chunk");
+  char* pin = b.pinned + kPinParams;
+  std::memcpy(pin, tau, 8 * W);
+  std::memcpy(pin + 8 * W, sigma, 8 * W);
+  std::memcpy(pin + 16 * W, inner0, 8 * W);
+  PSH_TRY(c, hipMemcpyAsync(b.tau, pin, 8 * W, hipMemcpyHostToDevice, c->stream));
+  PSH_TRY(c, hipMemcpyAsync(b.sigma, pin + 8 * W, 8 * W, hipMemcpyHostToDevice, c->stream));
+  PSH_TRY(c, hipMemcpyAsync(b.inner0, pin + 16 * W, 8 * W, hipMemcpyHostToDevice, c->stream));
+  bool any_new = false;
+  for (int t = 0; t < W; ++t) any_new = any_new || inner0[t] == 0;
+  for (int s = 0; s < iterations; ++s) {
+    const int hat = (s == 0 && any_new) || s == iterations - 1 ? 1 : 0;
+    const BArgs a = make_bargs(c, s, hat, rho);
+    hipEvent_t ev;
+    if (c->n > 0) {
+      time_begin(c, 0, &ev);
+      hipLaunchKernelGGL(k_bprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
+      time_end(c, 0, ev);
+    }
+    if (c->m > 0) {
+      time_begin(c, 1, &ev);
+      hipLaunchKernelGGL(k_bdual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
+      time_end(c, 1, ev);
+    }
+    PSH_LAUNCHED(c, "batch iteration");
+    if (s == 0 && any_new) {  // residuals of every slot after the first step (reference batch.cpp:347-354)
+      if ((rc = batch_residual_pass(c, false, mode, true))) return rc;
+      b.have_r0 = true;
+    }
+  }
+  return PSH_OK;
+}
+
+int psh_batch_check(psh_ctx* c, int mode, double* out) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  hipEvent_t ev;
+  time_begin(c, 2, &ev);
+  rc = batch_residual_pass(c, true, mode, false);
+  time_end(c, 2, ev);
+  if (rc) return rc;
+  double* pin = reinterpret_cast<double*>(b.pinned + kPinStats);
+  const size_t bytes = sizeof(double) * b.W * PSH_BSTAT_COUNT;
+  PSH_TRY(c, hipMemcpyAsync(pin, b.stats, bytes, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  std::memcpy(out, pin, bytes);
+  b.have_r0 = false;
+  return PSH_OK;
+}
+
+int psh_batch_restart(psh_ctx* c, const int* mask) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  int* pin = reinterpret_cast<int*>(b.pinned + kPinMask);
+  std::memcpy(pin, mask, sizeof(int) * b.W);
+  PSH_TRY(c, hipMemcpyAsync(b.mask, pin, sizeof(int) * b.W, hipMemcpyHostToDevice, c->stream));
+  if (c->n > 0)
+    hipLaunchKernelGGL(k_brestart, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, b.x, b.x0, b.xh, b.mask,
+                       (long long)c->n, b.W);
+  if (c->m > 0)
+    hipLaunchKernelGGL(k_brestart, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, b.y, b.y0, b.yh, b.mask,
+                       (long long)c->m, b.W);
+  PSH_LAUNCHED(c, "batch restart");
+  return PSH_OK;
+}
+
+int psh_batch_compact(psh_ctx* c, const int* keep, int W2) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  if (W2 < 0 || W2 > b.W) return fail(c, PSH_E_ARG, "bad compaction");
+  for (int k = 0; k < W2; ++k)
+    if (keep[k] < 0 || keep[k] >= b.W) return fail(c, PSH_E_ARG, "bad compaction index");
+  if (W2 == b.W) return PSH_OK;
+  int* pin = reinterpret_cast<int*>(b.pinned + kPinKeep);
+  std::memcpy(pin, keep, sizeof(int) * W2);
+  PSH_TRY(c, hipMemcpyAsync(b.keep, pin, sizeof(int) * std::max(W2, 1), hipMemcpyHostToDevice, c->stream));
+  const long long m = c->m, n = c->n;
+  auto squeeze = [&](double* v, long long rows) -> int {
+    if (rows * W2 == 0) return PSH_OK;
+    hipLaunchKernelGGL(k_bgather, dim3(grid_for(rows * W2)), dim3(kBlock), 0, c->stream, v, b.tmp, rows, b.W, W2, b.keep);
+    PSH_TRY(c, hipMemcpyAsync(v, b.tmp, sizeof(double) * rows * W2, hipMemcpyDeviceToDevice, c->stream));
+    return PSH_OK;
+  };
+  for (double* v : {b.c, b.l, b.u, b.cmin, b.clo, b.cuo, b.x, b.x0, b.xh, b.xb})
+    if ((rc = squeeze(v, n))) return rc;
+  for (double* v : {b.rl, b.ru, b.rlo, b.ruo, b.y, b.y0, b.yh, b.yb})
+    if ((rc = squeeze(v, m))) return rc;
+  if ((rc = squeeze(b.bs, 1)) || (rc = squeeze(b.os, 1))) return rc;
+  PSH_LAUNCHED(c, "batch compaction");
+  b.W = W2;
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  return PSH_OK;
+}
+
+int psh_batch_point(psh_ctx* c, int t, double* x, double* y_min) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  if (t < 0 || t >= b.W) return fail(c, PSH_E_ARG, "bad slot");
+  double bs = 1, os = 1;
+  PSH_TRY(c, hipMemcpyAsync(&bs, b.bs + t, 8, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipMemcpyAsync(&os, b.os + t, 8, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  const long long rows = std::max(c->m, c->n);
+  if (rows > 0)
+    hipLaunchKernelGGL(k_bpoint, dim3(grid_for(rows)), dim3(kBlock), 0, c->stream, b.xh, b.yh, c->cs, c->rs, b.clo, b.cuo,
+                       bs, os, t, b.W, c->m, c->n, b.tmp, b.tmp + c->n);
+  PSH_LAUNCHED(c, "batch point");
+  if (c->n > 0) PSH_TRY(c, hipMemcpyAsync(x, b.tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+  if (c->m > 0) PSH_TRY(c, hipMemcpyAsync(y_min, b.tmp + c->n, sizeof(double) * c->m, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  return PSH_OK;
+}
+
+}  // extern "C"
+
+namespace psh {
+void destroy_batch(psh_ctx* c) { free_batch(c); }
+}  // namespace psh

@@ -524,6 +524,7 @@ void psh_destroy(psh_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  psh::destroy_batch(c);
   void* ptrs[] = {c->A.ptr, c->A.col, c->A.val, c->A.valf, c->A.tiles, c->At.ptr, c->At.col, c->At.val, c->At.valf,
                   c->At.tiles, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,

@@ -74,6 +74,26 @@ enum Result {
 enum Max { M_PMAX, M_DMAX, M_CINF, M_RMAX, M_DVIOL, M_DMAXRAY, M_PVIOL, M_RMAX2, M_DVIOL2, M_DMAXRAY2, M_PVIOL2, kMaxCount };
 enum Flag : unsigned { F_RES = 1, F_KKT = 2, F_RAY = 4, F_DRIFT = 8, F_PID = 16 };
 
+// Device state of a batch of scenarios (interleaved block vectors, W slots of `cap`).
+struct BatchState {
+  int W = 0, cap = 0;
+  long long rows_max = 0;
+  double *c = nullptr, *l = nullptr, *u = nullptr, *rl = nullptr, *ru = nullptr;
+  double *cmin = nullptr, *clo = nullptr, *cuo = nullptr, *rlo = nullptr, *ruo = nullptr;
+  double *bs = nullptr, *os = nullptr;
+  double *x = nullptr, *x0 = nullptr, *xh = nullptr, *xb = nullptr, *y = nullptr, *y0 = nullptr, *yh = nullptr,
+         *yb = nullptr, *dx = nullptr, *dy = nullptr, *adx = nullptr, *xo = nullptr, *yo = nullptr;
+  double *tau = nullptr, *sigma = nullptr;
+  long long* inner0 = nullptr;
+  int *mask = nullptr, *keep = nullptr;
+  double* terms = nullptr;     // per-element terms, kBTerms arrays of rows_max·cap
+  double* partials = nullptr;  // tree partials
+  double* stats = nullptr;     // cap · PSH_BSTAT_COUNT
+  double* tmp = nullptr;       // compaction buffer, rows_max·cap
+  char* pinned = nullptr;      // host staging: params | mask | keep | stats
+  bool have_r0 = false;
+};
+
 }  // namespace psh
 
 struct psh_ctx {
@@ -120,6 +140,8 @@ struct psh_ctx {
   bool have_r0 = false;
   // persistent reduction tables, keyed by (mode, check flags)
   std::map<int, psh::JobTable> tables;
+
+  psh::BatchState* batch = nullptr;
 
   // timing
   bool timing = false;
@@ -173,6 +195,9 @@ int reduce_table(psh_ctx* c, const JobTable& t, int first);
 const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err);
 // Copies `count` device results to the host (synchronises the stream).
 int fetch_results(psh_ctx* c, int count, double* out);
+
+// frees the batch state of a context (psh_batch.hip)
+void destroy_batch(psh_ctx* c);
 
 // timing helpers
 void time_begin(psh_ctx* c, int cls, hipEvent_t* a);

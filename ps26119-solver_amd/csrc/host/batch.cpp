@@ -1,0 +1,316 @@
+// batch.cpp — scenarios sharing one matrix, solved in lockstep on the GPU (reference
+// src/pdhg/batch.cpp:129-441; include/ps26119/batch.h:10-33). Decisions per scenario on the host
+// (restarts, PID primal weight, termination, compaction) exactly as the reference; the
+// iterations, residuals and KKT sums of all live scenarios on the device (psh_batch_*).
+// Additions the reference does not have: the GPU, and an optional warm start of every scenario
+// from Options::warm_x / warm_y / warm_primal_weight (e.g. the base model's solution).
+#include "batch.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <new>
+#include <stdexcept>
+
+#include "engine.h"
+#include "la.h"
+#include "verify.h"
+
+namespace ps26119::amd {
+
+namespace {
+
+struct ScenarioData {  // original space, original sense
+  std::vector<double> c, cl, cu, rl, ru;
+};
+
+}  // namespace
+
+std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>& scenarios, const Options& options) {
+  const auto t0 = std::chrono::steady_clock::now();
+  auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  const int K = static_cast<int>(scenarios.size());
+  std::vector<Solution> out(K);
+  const int m = base.num_rows, n = base.num_cols;
+  {
+    Model w = base;  // fingerprint of the model each scenario solves
+    for (int k = 0; k < K; ++k) {
+      const Scenario& s = scenarios[k];
+      w.obj = s.obj.empty() ? base.obj : s.obj;
+      w.col_lower = s.col_lower.empty() ? base.col_lower : s.col_lower;
+      w.col_upper = s.col_upper.empty() ? base.col_upper : s.col_upper;
+      w.row_lower = s.row_lower.empty() ? base.row_lower : s.row_lower;
+      w.row_upper = s.row_upper.empty() ? base.row_upper : s.row_upper;
+      out[k].engine = "r2hpdhg-batch";
+      out[k].precision = "fp64";
+      out[k].model_fingerprint = w.fingerprint_hex();
+    }
+  }
+  auto fail_all = [&](const std::string& why) {
+    for (auto& s : out) s.status = Status::NotSolved, s.message = why;
+    return out;
+  };
+  if (K == 0) return out;
+  if (auto err = base.validate(); !err.empty()) return fail_all("invalid base model: " + err);
+  EngineOptions eo = engine_options_from(options);
+  if (auto bad = apply_engine_params(options, eo); !bad.empty()) return fail_all("unknown engine parameter '" + bad + "'");
+  if (!options.use_gpu) return fail_all("the MI355X backend serves GPU solves: set use_gpu = 1");
+  const std::vector<double>& warm_x = options.warm_x;
+  const std::vector<double>& warm_y = options.warm_y;
+  if ((!warm_x.empty() && static_cast<int>(warm_x.size()) != n) || (!warm_y.empty() && static_cast<int>(warm_y.size()) != m))
+    return fail_all("warm start vectors have the wrong size");
+  set_host_threads(options.threads);
+
+  std::vector<ScenarioData> D(K);
+  for (int k = 0; k < K; ++k) {  // reference batch.cpp:152-179
+    const Scenario& s = scenarios[k];
+    auto pick = [](const std::vector<double>& v, const std::vector<double>& d, std::size_t len, bool& ok) {
+      if (v.empty()) return d;
+      ok = ok && v.size() == len;
+      return v;
+    };
+    bool ok = true;
+    D[k].c = pick(s.obj, base.obj, n, ok);
+    D[k].cl = pick(s.col_lower, base.col_lower, n, ok);
+    D[k].cu = pick(s.col_upper, base.col_upper, n, ok);
+    D[k].rl = pick(s.row_lower, base.row_lower, m, ok);
+    D[k].ru = pick(s.row_upper, base.row_upper, m, ok);
+    for (int j = 0; ok && j < n; ++j) ok = std::isfinite(D[k].c[j]) && !std::isnan(D[k].cl[j]) && !std::isnan(D[k].cu[j]);
+    for (int i = 0; ok && i < m; ++i) ok = !std::isnan(D[k].rl[i]) && !std::isnan(D[k].ru[i]);
+    bool crossed = false;
+    for (int j = 0; ok && !crossed && j < n; ++j) crossed = D[k].cl[j] > D[k].cu[j];
+    for (int i = 0; ok && !crossed && i < m; ++i) crossed = D[k].rl[i] > D[k].ru[i];
+    if (!ok) {
+      out[k].status = Status::NotSolved;
+      out[k].message = "scenario has wrong vector sizes or non-finite data";
+    } else if (crossed) {
+      out[k].status = Status::Infeasible;
+      out[k].message = "bounds cross: scenario has a lower bound above its upper bound";
+    }
+  }
+
+  try {
+    ScalingOptions so = eo.scaling;
+    so.bound_objective_rescaling = false;  // per scenario below (reference batch.cpp:182-221)
+    const ScaledProblem sp = make_scaled_problem(base, so);
+    std::string err;
+    std::unique_ptr<Device> dev = Device::open(eo.device, err);
+    if (!dev) return fail_all(err);
+    dev->load(sp);
+    psh_ctx* c = dev->ctx();
+    if (eo.timing) dev->check(psh_timing(c, 1), "timing");
+    const bool empty = n == 0 || sp.A.nnz() == 0;
+    const double norm = empty ? 0.0 : estimate_norm2(*dev, n, eo.cpu_order);
+    const double eta = norm > 0 ? 0.998 / norm : 1.0;
+    const double setup_shared = elapsed();
+    const int mode = eo.cpu_order ? PSH_RED_CPU : PSH_RED_TREE;
+    const double rho = eo.reflection;
+    const int Kc = eo.check_every;
+
+    std::vector<int> valid;
+    for (int k = 0; k < K; ++k)
+      if (out[k].message.empty() && out[k].status == Status::NotSolved) valid.push_back(k);
+    // groups of at most PSH_BATCH_MAX_W scenarios share one device batch
+    for (std::size_t g0 = 0; g0 < valid.size(); g0 += PSH_BATCH_MAX_W) {
+      std::vector<int> sid(valid.begin() + g0, valid.begin() + std::min(valid.size(), g0 + PSH_BATCH_MAX_W));
+      int W = static_cast<int>(sid.size());
+      std::vector<double> bs(W, 1.0), os(W, 1.0);
+      const std::size_t NW = static_cast<std::size_t>(n) * W, MW = static_cast<std::size_t>(m) * W;
+      std::vector<double> C(NW), L(NW), U(NW), RL(MW), RU(MW), CM(NW), CLO(NW), CUO(NW), RLO(MW), RUO(MW);
+      for (int t = 0; t < W; ++t) {
+        const ScenarioData& d = D[sid[t]];
+        if (eo.scaling.bound_objective_rescaling) {
+          double b2 = 0, c2 = 0;
+          for (int i = 0; i < m; ++i) {
+            const double lo = std::isfinite(d.rl[i]) ? std::fabs(d.rl[i]) : 0.0;
+            const double up = std::isfinite(d.ru[i]) ? std::fabs(d.ru[i]) : 0.0;
+            const double b = sp.row_scale[i] * std::max(lo, up);
+            b2 += b * b;
+          }
+          for (int j = 0; j < n; ++j) {
+            const double cc = sp.col_scale[j] * d.c[j];
+            c2 += cc * cc;
+          }
+          bs[t] = 1.0 / (std::sqrt(b2) + 1.0);
+          os[t] = 1.0 / (std::sqrt(c2) + 1.0);
+        }
+        for (int j = 0; j < n; ++j) {
+          const std::size_t q = static_cast<std::size_t>(j) * W + t;
+          C[q] = os[t] * sp.col_scale[j] * base.sense * d.c[j];
+          L[q] = bs[t] * d.cl[j] / sp.col_scale[j];
+          U[q] = bs[t] * d.cu[j] / sp.col_scale[j];
+          CM[q] = base.sense * d.c[j];
+          CLO[q] = d.cl[j];
+          CUO[q] = d.cu[j];
+        }
+        for (int i = 0; i < m; ++i) {
+          const std::size_t q = static_cast<std::size_t>(i) * W + t;
+          RL[q] = bs[t] * sp.row_scale[i] * d.rl[i];
+          RU[q] = bs[t] * sp.row_scale[i] * d.ru[i];
+          RLO[q] = d.rl[i];
+          RUO[q] = d.ru[i];
+        }
+      }
+      dev->check(psh_batch_begin(c, W, C.data(), L.data(), U.data(), RL.data(), RU.data(), CM.data(), CLO.data(),
+                                 CUO.data(), RLO.data(), RUO.data(), bs.data(), os.data()),
+                 "batch");
+      // initial iterate: x = proj(0), y = 0 (reference batch.cpp:238-242), or the warm point
+      std::vector<double> X(NW), Y(MW, 0.0);
+      for (int t = 0; t < W; ++t) {
+        const ScenarioData& d = D[sid[t]];
+        for (int j = 0; j < n; ++j) {
+          const std::size_t q = static_cast<std::size_t>(j) * W + t;
+          if (!warm_x.empty()) {
+            const double xo = std::min(std::max(warm_x[j], d.cl[j]), d.cu[j]);
+            X[q] = bs[t] * xo / sp.col_scale[j];
+          } else {
+            X[q] = 0.0 < L[q] ? L[q] : (0.0 > U[q] ? U[q] : 0.0);
+          }
+        }
+        if (!warm_y.empty())
+          for (int i = 0; i < m; ++i) {
+            double v = base.sense * warm_y[i];
+            if (!std::isfinite(d.rl[i])) v = std::min(v, 0.0);
+            if (!std::isfinite(d.ru[i])) v = std::max(v, 0.0);
+            Y[static_cast<std::size_t>(i) * W + t] = os[t] * v / sp.row_scale[i];
+          }
+      }
+      dev->check(psh_batch_set_iterate(c, X.data(), Y.data()), "batch");
+
+      const double w0 = options.warm_primal_weight > 0 ? options.warm_primal_weight : 1.0;
+      std::vector<double> omega(K, w0), best_omega(K, w0), best_balance(K, 1e300), pid_int(K, 0.0), pid_last(K, 0.0);
+      std::vector<double> r0(K, -1.0), r_last(K, kInf), fast_s(K, -1.0);
+      std::vector<std::int64_t> inner(K, 0), fast_it(K, -1);
+      std::int64_t it = 0;
+      std::vector<double> tau, sigma, stats, x, ym;
+      std::vector<long long> in0;
+      std::vector<int> mask;
+      const int SC = PSH_BSTAT_COUNT;
+      while (W > 0) {
+        tau.resize(W);
+        sigma.resize(W);
+        in0.resize(W);
+        for (int t = 0; t < W; ++t) {
+          tau[t] = eta / omega[sid[t]];
+          sigma[t] = eta * omega[sid[t]];
+          in0[t] = inner[sid[t]];
+        }
+        dev->check(psh_batch_chunk(c, Kc, tau.data(), sigma.data(), in0.data(), rho, mode), "batch iteration");
+        for (int t = 0; t < W; ++t) inner[sid[t]] += Kc;
+        it += Kc;
+        stats.resize(static_cast<std::size_t>(W) * SC);
+        dev->check(psh_batch_check(c, mode, stats.data()), "batch check");
+        const bool over_iter = it >= eo.iteration_limit, over_time = elapsed() > eo.time_limit;
+        std::vector<int> keep;
+        mask.assign(W, 0);
+        for (int t = 0; t < W; ++t) {
+          const int k = sid[t];
+          const double* st = stats.data() + static_cast<std::size_t>(t) * SC;
+          const double om = omega[k];
+          auto resid = [&](double px, double py, double cr) { return std::sqrt(std::max(om * px + py / om + 2.0 * eta * cr, 0.0)); };
+          if (in0[t] == 0) r0[k] = resid(st[PSH_BSTAT_R0_PX], st[PSH_BSTAT_R0_PY], st[PSH_BSTAT_R0_CROSS]);
+          KktStats kk;
+          kk.primal_residual = std::sqrt(st[PSH_BSTAT_RP2]);
+          kk.dual_residual = std::sqrt(st[PSH_BSTAT_RD2]);
+          kk.primal_obj = st[PSH_BSTAT_P];
+          kk.dual_obj = st[PSH_BSTAT_D];
+          kk.b_norm = std::sqrt(st[PSH_BSTAT_B2]);
+          kk.c_norm = std::sqrt(st[PSH_BSTAT_C2]);
+          kk.primal_max_rel = st[PSH_BSTAT_PMAX];
+          kk.dual_max_rel = st[PSH_BSTAT_DMAX] / (1.0 + st[PSH_BSTAT_CINF]);
+          auto finish = [&](Status status, const std::string& msg) {
+            Solution& s = out[k];
+            x.assign(n, 0.0);
+            ym.assign(m, 0.0);
+            dev->check(psh_batch_point(c, t, x.data(), ym.data()), "batch point");
+            s.status = status;
+            s.message = msg;
+            s.x = x;
+            s.y.resize(m);
+            for (int i = 0; i < m; ++i) s.y[i] = base.sense * ym[i];
+            s.z = D[k].c;
+            parallel_for(n, [&](std::int64_t b, std::int64_t e) {
+              for (std::int64_t j = b; j < e; ++j)
+                for (int p = base.col_start[j]; p < base.col_start[j + 1]; ++p) s.z[j] -= base.value[p] * s.y[base.row_index[p]];
+            });
+            s.row_activity = base.row_activity(x);
+            double obj = base.obj_offset;
+            for (int j = 0; j < n; ++j) obj += D[k].c[j] * x[j];
+            s.objective = obj;
+            s.dual_objective = base.sense * kk.dual_obj + base.obj_offset;
+            s.primal_residual = kk.rel_primal();
+            s.dual_residual = kk.rel_dual();
+            s.gap = kk.rel_gap();
+            s.iterations = it;
+            s.seconds = elapsed();
+            s.setup_seconds = setup_shared;
+            s.iterations_to_fast = fast_it[k];
+            s.seconds_to_fast = fast_s[k];
+            s.primal_weight = omega[k];
+          };
+          if (!kk.finite()) {
+            finish(Status::NumericalError, "non-finite iterate");
+            continue;
+          }
+          if (fast_it[k] < 0 && kk.rel_kkt() <= tol::kFirstOrderFast) fast_it[k] = it, fast_s[k] = elapsed();
+          if (kk.converged(eo.tolerance)) {
+            finish(Status::Optimal, "");
+            continue;
+          }
+          if (over_iter || over_time) {
+            finish(over_iter ? Status::IterationLimit : Status::TimeLimit, "");
+            continue;
+          }
+          keep.push_back(t);
+          const double r = resid(st[PSH_BSTAT_PX], st[PSH_BSTAT_PY], st[PSH_BSTAT_CROSS]);
+          const bool ratios_ok = r0[k] > 0 && std::isfinite(r0[k]) && r > 0;
+          const bool restart = (ratios_ok && (r <= eo.restart_sufficient * r0[k] ||
+                                              (r <= eo.restart_necessary * r0[k] && r > r_last[k]))) ||
+                               static_cast<double>(inner[k]) >= eo.restart_artificial * static_cast<double>(it);
+          r_last[k] = r;
+          if (!restart) continue;
+          const double dxn = std::sqrt(st[PSH_BSTAT_PIDX]), dyn = std::sqrt(st[PSH_BSTAT_PIDY]);
+          const bool both_pos = kk.rel_dual() > 0 && kk.rel_primal() > 0;
+          const double ratio = both_pos ? kk.rel_dual() / kk.rel_primal() : 1.0;
+          if (dxn > 1e-16 && dyn > 1e-16 && dxn < 1e12 && dyn < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+            const double e = std::log(dyn) - std::log(dxn) - std::log(omega[k]);
+            pid_int[k] = eo.pid_integral_decay * pid_int[k] + e;
+            const double step = eo.pid_kp * e + eo.pid_ki * pid_int[k] + eo.pid_kd * (e - pid_last[k]);
+            omega[k] *= std::exp(std::clamp(step, -eo.pid_max_log_step, eo.pid_max_log_step));
+            pid_last[k] = e;
+          } else {
+            omega[k] = best_omega[k];
+            pid_int[k] = pid_last[k] = 0.0;
+          }
+          if (both_pos && std::fabs(std::log10(ratio)) < best_balance[k]) {
+            best_balance[k] = std::fabs(std::log10(ratio));
+            best_omega[k] = omega[k];
+          }
+          mask[t] = 1;
+          inner[k] = 0;
+          r_last[k] = kInf;
+        }
+        dev->check(psh_batch_restart(c, mask.data()), "batch restart");
+        if (static_cast<int>(keep.size()) != W) {
+          dev->check(psh_batch_compact(c, keep.data(), static_cast<int>(keep.size())), "batch compaction");
+          std::vector<int> ns;
+          for (int t : keep) ns.push_back(sid[t]);
+          sid.swap(ns);
+          W = static_cast<int>(sid.size());
+        }
+        if (eo.verbosity >= 2)
+          std::fprintf(stderr, "batch it %lld  t %.2fs  running %d/%d\n", static_cast<long long>(it), elapsed(), W, K);
+      }
+    }
+  } catch (const std::bad_alloc&) {
+    for (auto& s : out)
+      if (s.status == Status::NotSolved && s.message.empty()) s.message = "out of memory";
+  } catch (const std::exception& e) {
+    for (auto& s : out)
+      if (s.status == Status::NotSolved && s.message.empty()) s.status = Status::NumericalError, s.message = e.what();
+  }
+  return out;
+}
+
+}  // namespace ps26119::amd

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <exception>
 
+#include "batch.h"
 #include "engine.h"
 #include "presolve.h"
 #include "ps26119/ps26119.h"
@@ -234,6 +235,66 @@ int ps26119_amd_solve_lp(int num_rows, int num_cols, int sense, double obj_offse
                          ps26119_amd_stats* stats, double* x, double* y, double* z, double* dual_ray, double* primal_ray) {
   return solve_common(num_rows, num_cols, sense, obj_offset, c, col_lower, col_upper, row_lower, row_upper, col_start,
                       row_index, value, opt, ext, result, stats, x, y, z, dual_ray, primal_ray);
+}
+
+int ps26119_amd_solve_batch(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                            const double* col_lower, const double* col_upper, const double* row_lower,
+                            const double* row_upper, const int* col_start, const int* row_index, const double* value,
+                            int num_scenarios, const ps26119_amd_scenario* scenarios, const ps26119_options* opt,
+                            const ps26119_amd_options* ext, ps26119_result* results, double* xs, double* ys) {
+  if (!results || num_scenarios < 0 || (num_scenarios > 0 && !scenarios)) return PS26119_INVALID_ARGUMENT;
+  for (int k = 0; k < num_scenarios; ++k) clear(&results[k]);
+  auto invalid_all = [&](const std::string& why) {
+    for (int k = 0; k < num_scenarios; ++k) {
+      results[k].status = PS26119_INVALID_ARGUMENT;
+      set_text(results[k].message, sizeof results[k].message, why);
+    }
+    return static_cast<int>(PS26119_INVALID_ARGUMENT);
+  };
+  ps26119_result scratch;
+  clear(&scratch);
+  const int rc = guarded(&scratch, [&]() -> int {
+    Model m;
+    std::string err;
+    if (const char* bad = make_model(num_rows, num_cols, sense, obj_offset, c, col_lower, col_upper, row_lower, row_upper,
+                                     col_start, row_index, value, m, err))
+      return invalid_all(bad);
+    Options o;
+    if (const char* bad = to_options(opt, num_rows, num_cols, o)) return invalid_all(bad);
+    if (ext) {
+      if (ext->check_every > 0) o.termination_check_every = ext->check_every;
+      for (int k = 0; k < ext->n_params; ++k) {
+        if (!ext->param_names || !ext->param_values || !ext->param_names[k]) return invalid_all("engine parameter list is NULL");
+        o.engine_params.emplace_back(ext->param_names[k], ext->param_values[k]);
+      }
+    }
+    std::vector<Scenario> S(num_scenarios);
+    for (int k = 0; k < num_scenarios; ++k) {
+      const ps26119_amd_scenario& q = scenarios[k];
+      if (q.obj) S[k].obj.assign(q.obj, q.obj + num_cols);
+      if (q.col_lower) S[k].col_lower.assign(q.col_lower, q.col_lower + num_cols);
+      if (q.col_upper) S[k].col_upper.assign(q.col_upper, q.col_upper + num_cols);
+      if (q.row_lower) S[k].row_lower.assign(q.row_lower, q.row_lower + num_rows);
+      if (q.row_upper) S[k].row_upper.assign(q.row_upper, q.row_upper + num_rows);
+    }
+    const std::vector<Solution> sol = amd::solve_batch(m, S, o);
+    int first = PS26119_OPTIMAL;
+    for (int k = 0; k < num_scenarios; ++k) {
+      fill(sol[k], &results[k]);
+      if (first == PS26119_OPTIMAL && results[k].status != PS26119_OPTIMAL) first = results[k].status;
+      const bool have = sol[k].x.size() == static_cast<std::size_t>(num_cols) && sol[k].y.size() == static_cast<std::size_t>(num_rows);
+      if (xs && have && num_cols > 0)
+        std::memcpy(xs + static_cast<std::size_t>(k) * num_cols, sol[k].x.data(), sizeof(double) * num_cols);
+      if (ys && have && num_rows > 0)
+        std::memcpy(ys + static_cast<std::size_t>(k) * num_rows, sol[k].y.data(), sizeof(double) * num_rows);
+    }
+    return first;
+  });
+  if (scratch.status == PS26119_NUMERICAL_ERROR && scratch.message[0]) {  // an exception escaped the solver
+    for (int k = 0; k < num_scenarios; ++k) results[k] = scratch;
+    return PS26119_NUMERICAL_ERROR;
+  }
+  return rc;
 }
 
 int ps26119_amd_device_count(void) { return psh_device_count(); }

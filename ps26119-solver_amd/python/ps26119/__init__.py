@@ -11,7 +11,8 @@ the GPU path, ``gpu`` defaults to True here.
 
 Extensions: ``presolve``, ``check_every``, ``params`` (named engine knobs, e.g.
 ``{"hip_cpu_order": 1}``) and ``stats=True`` (setup time, kernel timing) go through
-``ps26119_amd_solve_lp``.
+``ps26119_amd_solve_lp``; ``solve_batch`` (scenarios sharing one matrix) goes through
+``ps26119_amd_solve_batch``.
 """
 from __future__ import annotations
 
@@ -21,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["solve_lp", "Result", "version", "device_count", "library_path", "OPTIMAL", "INFEASIBLE", "UNBOUNDED",
+__all__ = ["solve_lp", "solve_batch", "Result", "version", "device_count", "library_path", "OPTIMAL", "INFEASIBLE", "UNBOUNDED",
            "ITERATION_LIMIT", "TIME_LIMIT", "NUMERICAL_ERROR", "NOT_SOLVED", "INVALID_ARGUMENT"]
 
 OPTIMAL, INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR, NOT_SOLVED, INVALID_ARGUMENT = range(8)
@@ -75,6 +76,10 @@ class AmdStats(ctypes.Structure):  # ps26119_amd_stats
                 "device_bytes": self.device_bytes}
 
 
+class Scenario(ctypes.Structure):  # ps26119_amd_scenario
+    _fields_ = [("obj", D), ("col_lower", D), ("col_upper", D), ("row_lower", D), ("row_upper", D)]
+
+
 def _load():
     path = os.environ.get("PS26119_LIB", library_path)
     if not os.path.exists(path):
@@ -88,6 +93,9 @@ def _load():
     lib.ps26119_amd_solve_lp.argtypes = model + [ctypes.POINTER(Options), ctypes.POINTER(AmdOptions),
                                                  ctypes.POINTER(CResult), ctypes.POINTER(AmdStats), D, D, D, D, D]
     lib.ps26119_amd_solve_lp.restype = ctypes.c_int
+    lib.ps26119_amd_solve_batch.argtypes = model + [ctypes.c_int, ctypes.POINTER(Scenario), ctypes.POINTER(Options),
+                                                    ctypes.POINTER(AmdOptions), ctypes.POINTER(CResult), D, D]
+    lib.ps26119_amd_solve_batch.restype = ctypes.c_int
     lib.ps26119_default_options.argtypes = [ctypes.POINTER(Options)]
     lib.ps26119_amd_default_options.argtypes = [ctypes.POINTER(AmdOptions)]
     lib.ps26119_solve_mps.argtypes = [ctypes.c_char_p, ctypes.POINTER(Options), ctypes.POINTER(CResult),
@@ -232,7 +240,117 @@ def solve_lp(c, A, row_lower, row_upper, col_lower=None, col_upper=None, sense=1
                   st_out.as_dict() if stats else None)
 
 
+def _options(L, algorithm, precision, gpu, tolerance, time_limit, iteration_limit, verbosity, threads, wx, wy):
+    opt = Options()
+    L.ps26119_default_options(ctypes.byref(opt))
+    opt.algorithm = _lookup(_ALG, algorithm, "algorithm")
+    opt.precision = _lookup(_PREC, precision, "precision")
+    opt.use_gpu = int(bool(gpu))
+    opt.tolerance = tolerance
+    opt.time_limit = time_limit
+    opt.iteration_limit = iteration_limit
+    opt.verbosity = verbosity
+    opt.threads = threads
+    opt.warm_x = _ptr(wx)
+    opt.warm_y = _ptr(wy)
+    return opt
+
+
+def _ext(L, presolve, check_every, params, keep):
+    ext = AmdOptions()
+    L.ps26119_amd_default_options(ctypes.byref(ext))
+    ext.presolve = int(bool(presolve))
+    ext.check_every = int(check_every)
+    names = [k.encode() for k in (params or {})]
+    vals = np.array([float(x) for x in (params or {}).values()], dtype=np.float64)
+    name_arr = (ctypes.c_char_p * max(1, len(names)))(*names)
+    keep += [names, vals, name_arr]
+    ext.n_params = len(names)
+    ext.param_names = name_arr
+    ext.param_values = _ptr(vals) if len(names) else None
+    return ext
+
+
+def solve_batch(c, A, row_lower, row_upper, col_lower=None, col_upper=None, scenarios=(), sense=1, offset=0.0,
+                gpu=True, tolerance=1e-8, time_limit=3600.0, iteration_limit=0, warm_x=None, warm_y=None, verbosity=0,
+                threads=1, check_every=0, params=None) -> list:
+    """Scenarios of one LP sharing A (the reference's ps26119::solve_batch, include/ps26119/batch.h).
+
+    ``scenarios`` is a list of dicts with any of ``obj``, ``col_lower``, ``col_upper``,
+    ``row_lower``, ``row_upper`` (missing = the base's). Returns one Result per scenario. A vector
+    of the wrong length makes only that scenario NotSolved, as in the reference."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    n = c.size
+    rl = np.ascontiguousarray(row_lower, dtype=np.float64)
+    ru = np.ascontiguousarray(row_upper, dtype=np.float64)
+    m = rl.size
+    cl = np.zeros(n) if col_lower is None else np.ascontiguousarray(col_lower, dtype=np.float64)
+    cu = np.full(n, np.inf) if col_upper is None else np.ascontiguousarray(col_upper, dtype=np.float64)
+    cs, ri, v = csc(A, m, n)
+    L = lib()
+    keep = []
+    wx = None if warm_x is None else np.ascontiguousarray(warm_x, dtype=np.float64)
+    wy = None if warm_y is None else np.ascontiguousarray(warm_y, dtype=np.float64)
+    opt = _options(L, "r2hpdhg", "fp64", gpu, tolerance, time_limit, iteration_limit, verbosity, threads, wx, wy)
+    ext = _ext(L, False, check_every, params, keep)
+    K = len(scenarios)
+    arr = (Scenario * max(1, K))()
+    for k, sc in enumerate(scenarios):
+        ptrs = []
+        for key, length in (("obj", n), ("col_lower", n), ("col_upper", n), ("row_lower", m), ("row_upper", m)):
+            a = sc.get(key)
+            if a is not None:
+                a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+                if a.size != length:  # the library's "wrong sizes or non-finite data" verdict
+                    a = np.full(length, np.nan)
+                keep.append(a)
+            ptrs.append(_ptr(a))
+        arr[k] = Scenario(*ptrs)
+    res = (CResult * max(1, K))()
+    xs, ys = np.full(K * n, np.nan), np.full(K * m, np.nan)
+    L.ps26119_amd_solve_batch(m, n, int(sense), float(offset), _ptr(c), _ptr(cl), _ptr(cu), _ptr(rl), _ptr(ru),
+                              _ptr(cs, ctypes.c_int), _ptr(ri, ctypes.c_int), _ptr(v), K, arr, ctypes.byref(opt),
+                              ctypes.byref(ext), res, _ptr(xs), _ptr(ys))
+    out = []
+    for k in range(K):
+        r = res[k]
+        st = r.status
+        have = st in (OPTIMAL, ITERATION_LIMIT, TIME_LIMIT)
+        out.append(Result(st, _STATUS[st] if 0 <= st < len(_STATUS) else str(st), r.objective, r.dual_objective,
+                          r.primal_residual, r.dual_residual, r.gap, r.certified_bound, r.iterations, r.seconds,
+                          r.engine.decode(), r.message.decode(), _CHECK.get(r.check, ""),
+                          xs[k * n:(k + 1) * n].copy() if have else None,
+                          ys[k * m:(k + 1) * m].copy() if have else None))
+    return out
+
+
+def solve_model_batch(lp, scenarios, **kw) -> list:
+    """solve_batch for an object with the model fields (see solve_model)."""
+    return solve_batch(lp.c, (lp.col_start, lp.row_index, lp.value), lp.row_lower, lp.row_upper, lp.col_lower,
+                       lp.col_upper, scenarios=scenarios, sense=lp.sense, offset=lp.offset, **kw)
+
+
 def solve_model(lp, **kw) -> Result:
     """solve_lp for an object with the model fields (m, n, c, col_lower, ..., value, sense, offset)."""
     return solve_lp(lp.c, (lp.col_start, lp.row_index, lp.value), lp.row_lower, lp.row_upper, lp.col_lower,
                     lp.col_upper, sense=lp.sense, offset=lp.offset, **kw)
+
+
+def check_solution(lp, x, y) -> dict:
+    """In-process verifier-grade check of (x, y) on an lp-like object (ps26119_amd_check_solution)."""
+    L = lib()
+    c, cl, cu, rl, ru, v = (np.ascontiguousarray(a, dtype=np.float64) for a in
+                            (lp.c, lp.col_lower, lp.col_upper, lp.row_lower, lp.row_upper, lp.value))
+    cs, ri = (np.ascontiguousarray(a, dtype=np.int32) for a in (lp.col_start, lp.row_index))
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(6)
+    rc = L.ps26119_amd_check_solution(lp.m, lp.n, int(lp.sense), float(lp.offset), _ptr(c), _ptr(cl), _ptr(cu),
+                                      _ptr(rl), _ptr(ru), _ptr(cs, ctypes.c_int), _ptr(ri, ctypes.c_int), _ptr(v),
+                                      _ptr(x), _ptr(y), _ptr(out))
+    if rc != OPTIMAL:
+        raise ValueError(f"check_solution: invalid model or vectors (status {rc})")
+    keys = ["max_primal_violation", "max_dual_violation", "relative_gap", "primal_objective", "dual_objective"]
+    d = dict(zip(keys, out[:5].tolist()))
+    d["passed"] = bool(out[5])
+    return d

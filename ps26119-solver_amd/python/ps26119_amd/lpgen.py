@@ -126,6 +126,19 @@ def scenario_set(base: Lp, k: int, sigma: float = 0.05, seed: int = 7):
     return out
 
 
+def perturbation_scenarios(base: Lp):
+    """The four scenarios of the reference's batch test (tests/unit/test_batch.cpp:28-42 there):
+    unchanged; objective scaled by 1 + 0.05·((j mod 3) − 1); finite, non-equality row upper
+    bounds loosened by 0.02·(1 + |u|); both changes."""
+    j = np.arange(base.n)
+    obj = base.c * (1.0 + 0.05 * ((j % 3) - 1.0))
+    ru = base.row_upper.copy()
+    loosen = np.isfinite(ru) & (base.row_lower != ru)
+    ru[loosen] += 0.02 * (1 + np.abs(ru[loosen]))
+    rows = {"row_lower": base.row_lower.copy(), "row_upper": ru}
+    return [{}, {"obj": obj}, dict(rows), dict(rows, obj=obj)]
+
+
 def read_lpm(path: str) -> Lp:
     """Reader for the reference's `.lpm` text format, version 1 (sections as in its
     tools/lpm.py specification: header keys, then OBJ, COL_LOWER, COL_UPPER, ROW_LOWER,

@@ -61,6 +61,21 @@ def solve_case(ref, lp, **kw):
     return out
 
 
+def batch_case(ref, lp, scenarios):
+    res, xs, ys = ref.solve_batch(lp, scenarios, iteration_limit=3_000_000)
+    out = []
+    for k, r in enumerate(res):
+        d = {key: r[key] for key in ("status", "iterations", "iterations_to_fast")}
+        for key in ("objective", "primal_weight", "dual_objective"):
+            d[key] = hexf(r[key])
+        have = r["status"] in (0, 3, 4)
+        d["x_hash"] = fnv(xs[k]) if have else None
+        d["y_hash"] = fnv(ys[k]) if have else None
+        d["message"] = r["message"][:120]
+        out.append(d)
+    return out
+
+
 def main():
     big = "--big" in sys.argv
     ref = orc.load("ref")
@@ -81,6 +96,17 @@ def main():
             key = f"random10000_s1|presolve={int(pre)}|fp64"
             solves[key] = solve_case(ref, lp, presolve=pre, precision=0)
             print(key, solves[key]["status"], solves[key]["iterations"], flush=True)
+    batches = {}
+    for name in ("afiro", "sc50a", "stocfor1", "share2b", "random300_s3"):
+        lp = models[name]
+        batches[name] = batch_case(ref, lp, lpgen.perturbation_scenarios(lp))
+        print("batch", name, [b["iterations"] for b in batches[name]], flush=True)
+    mr = models["max_ranged"]
+    cu = mr.col_upper.copy()
+    cu[0] = 2.0
+    batches["max_ranged"] = batch_case(ref, mr, [{}, {"obj": [1.0]}, {"col_upper": cu}])
+    batches["random300_s3|scenario_set"] = batch_case(ref, models["random300_s3"],
+                                                     lpgen.scenario_set(models["random300_s3"], 6))
     kernels = {}
     for name in ("afiro", "stocfor1", "random300_s3"):
         lp = models[name]
@@ -102,7 +128,8 @@ def main():
             if k.startswith("random10000"):
                 solves[k] = v
     with open(out_path, "w") as f:
-        json.dump({"generator": "tests/golden/make_golden.py", "solves": solves, "kernels": kernels}, f, indent=1,
+        json.dump({"generator": "tests/golden/make_golden.py", "solves": solves, "kernels": kernels,
+                   "batches": batches}, f, indent=1,
                   sort_keys=True)
     print("wrote", out_path)
 
