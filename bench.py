@@ -14,6 +14,7 @@ sample of the same model and extrapolates its time to 1e-8 from the GPU run's it
 (the fp64 trajectories are the same).
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--m 1000000] [--no-cpu-baseline]
+                    [--iteration-limit N]   (profiling: same per-launch work, N iterations)
 """
 import argparse
 import json
@@ -78,6 +79,8 @@ def main():
     ap.add_argument("--m", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--time-limit", type=float, default=600.0)
+    ap.add_argument("--iteration-limit", type=int, default=0,
+                    help="stop each timed solve after this many iterations (profiling runs; 0 = to 1e-8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -99,7 +102,8 @@ def main():
     times, res = [], None
     for s in range(args.steps):
         t = time.time()
-        res = ps26119.solve_model(lp, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit)
+        res = ps26119.solve_model(lp, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit,
+                                  iteration_limit=args.iteration_limit, verbosity=1)
         times.append(time.time() - t)
         log(f"step {s}: {res.status_name} it={res.iterations} solve={res.seconds:.2f}s wall={times[-1]:.2f}s "
             f"obj={res.objective!r} planted={lp.optimum!r} check={res.check} {res.message[:120]}")

@@ -13,7 +13,9 @@ extern "C" {
 /* Named engine knobs (the reference's Options::engine_params, options.h:70-82 there), e.g.
  * "restart_sufficient", "pid_kp", ..., plus the MI355X ones: "hip_cpu_order" (1 = every
  * reduction in the reference CPU order, bit-identical trajectory), "hip_timing" (1 = per-kernel
- * HIP event timing into ps26119_amd_stats), "hip_device". presolve: 1 (default) / 0. */
+ * HIP event timing into ps26119_amd_stats), "hip_graphs" (1 = default: replay captured HIP
+ * graphs of the iteration chunks; timing runs launch eagerly), "hip_device".
+ * presolve: 1 (default) / 0. */
 typedef struct {
   int presolve;
   int check_every;            /* iterations between checks (0 = 64) */

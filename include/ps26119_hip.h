@@ -139,7 +139,13 @@ int psh_ray_of(psh_ctx* ctx, int dx_slot, int dy_slot, psh_ray* out);
 int psh_ray_vectors(psh_ctx* ctx, int drift, double* dual_ray, double* primal_ray);
 
 /* ---- fused fast path */
+/* `iterations` r²HPDHG steps, two fused launches each (reference src/pdhg/r2hpdhg.cpp:66-85).
+ * τ, σ and the epoch counter are written to device memory first, so every launch of a chunk
+ * has fixed arguments: unless per-kernel timing is on or psh_graphs(ctx, 0) was called, the
+ * launches are captured once per (precision, length, epoch start, last-step hats, mode) into a
+ * HIP graph and replayed. */
 int psh_chunk(psh_ctx* ctx, const psh_chunk_args* a);
+int psh_graphs(psh_ctx* ctx, int on); /* 1 (default): replay captured chunk graphs */
 int psh_check(psh_ctx* ctx, const psh_check_args* a, psh_stats* out);
 int psh_restart(psh_ctx* ctx); /* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ */
 

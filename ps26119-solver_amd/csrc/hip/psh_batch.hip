@@ -311,6 +311,7 @@ void free_batch(psh_ctx* c) {
                   b->xh, b->xb, b->y, b->y0, b->yh, b->yb, b->dx, b->dy, b->adx, b->xo, b->yo, b->tau, b->sigma,
                   b->inner0, b->mask, b->keep, b->terms, b->partials, b->stats, b->tmp};
   for (void* p : ptrs) dev_free(p);
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   if (b->pinned) (void)hipHostFree(b->pinned);
   delete b;
   c->batch = nullptr;
@@ -372,6 +373,16 @@ int batch_residual_pass(psh_ctx* c, bool kkt, int mode, bool r0) {
   if (kkt && c->n > 0) hipLaunchKernelGGL(k_bcheck_cols, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a);
   PSH_LAUNCHED(c, "batch check kernels");
   return batch_reduce(c, mode, kkt, !kkt, r0 ? 1 : 0);
+}
+
+// Each pinned staging region (1 params, 2 mask, 4 keep) is reused by the next call of the same
+// kind: wait for a copy that may still read it.
+int pinned_free(psh_ctx* c, unsigned region) {
+  BatchState& b = *c->batch;
+  if (!(b.pin_pending & region)) return PSH_OK;
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  b.pin_pending = 0;
+  return PSH_OK;
 }
 
 int need_batch(psh_ctx* c) {
@@ -444,6 +455,7 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
   BatchState& b = *c->batch;
   const int W = b.W;
   if (iterations < 1 || W < 1) return fail(c, PSH_E_ARG, "empty batch chunk");
+  if ((rc = pinned_free(c, 1))) return rc;
   char* pin = b.pinned + kPinParams;
   std::memcpy(pin, tau, 8 * W);
   std::memcpy(pin + 8 * W, sigma, 8 * W);
@@ -451,28 +463,59 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
   PSH_TRY(c, hipMemcpyAsync(b.tau, pin, 8 * W, hipMemcpyHostToDevice, c->stream));
   PSH_TRY(c, hipMemcpyAsync(b.sigma, pin + 8 * W, 8 * W, hipMemcpyHostToDevice, c->stream));
   PSH_TRY(c, hipMemcpyAsync(b.inner0, pin + 16 * W, 8 * W, hipMemcpyHostToDevice, c->stream));
+  b.pin_pending |= 1;
   bool any_new = false;
   for (int t = 0; t < W; ++t) any_new = any_new || inner0[t] == 0;
-  for (int s = 0; s < iterations; ++s) {
-    const int hat = (s == 0 && any_new) || s == iterations - 1 ? 1 : 0;
-    const BArgs a = make_bargs(c, s, hat, rho);
-    hipEvent_t ev;
-    if (c->n > 0) {
-      time_begin(c, 0, &ev);
-      hipLaunchKernelGGL(k_bprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
-      time_end(c, 0, ev);
+  // Launch arguments depend only on (length, epoch start in any slot, width, mode): replay a
+  // captured graph unless per-kernel timing is on.
+  auto enqueue = [&]() -> int {
+    for (int s = 0; s < iterations; ++s) {
+      const int hat = (s == 0 && any_new) || s == iterations - 1 ? 1 : 0;
+      const BArgs a = make_bargs(c, s, hat, rho);
+      hipEvent_t ev;
+      if (c->n > 0) {
+        time_begin(c, 0, &ev);
+        hipLaunchKernelGGL(k_bprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
+        time_end(c, 0, ev);
+      }
+      if (c->m > 0) {
+        time_begin(c, 1, &ev);
+        hipLaunchKernelGGL(k_bdual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
+        time_end(c, 1, ev);
+      }
+      PSH_LAUNCHED(c, "batch iteration");
+      if (s == 0 && any_new) {  // residuals of every slot after the first step (reference batch.cpp:347-354)
+        const int r = batch_residual_pass(c, false, mode, true);
+        if (r) return r;
+      }
     }
-    if (c->m > 0) {
-      time_begin(c, 1, &ev);
-      hipLaunchKernelGGL(k_bdual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
-      time_end(c, 1, ev);
+    return PSH_OK;
+  };
+  if (c->timing || !c->graphs) {
+    if ((rc = enqueue())) return rc;
+  } else {
+    const long long key = (static_cast<long long>(iterations) << 16) | (static_cast<long long>(W) << 4) |
+                          (any_new ? 1 : 0) | (static_cast<long long>(mode & 7) << 1);
+    auto it = b.graphs.find(key);
+    if (it == b.graphs.end()) {
+      PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int erc = enqueue();
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (erc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return erc;
+      }
+      if (ec != hipSuccess) return hip_fail(c, ec, "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return hip_fail(c, ei, "hipGraphInstantiate");
+      it = b.graphs.emplace(key, exec).first;
     }
-    PSH_LAUNCHED(c, "batch iteration");
-    if (s == 0 && any_new) {  // residuals of every slot after the first step (reference batch.cpp:347-354)
-      if ((rc = batch_residual_pass(c, false, mode, true))) return rc;
-      b.have_r0 = true;
-    }
+    PSH_TRY(c, hipGraphLaunch(it->second, c->stream));
   }
+  if (any_new) b.have_r0 = true;
   return PSH_OK;
 }
 
@@ -490,6 +533,7 @@ int psh_batch_check(psh_ctx* c, int mode, double* out) {
   PSH_TRY(c, hipMemcpyAsync(pin, b.stats, bytes, hipMemcpyDeviceToHost, c->stream));
   PSH_TRY(c, hipStreamSynchronize(c->stream));
   std::memcpy(out, pin, bytes);
+  b.pin_pending = 0;
   b.have_r0 = false;
   return PSH_OK;
 }
@@ -498,9 +542,11 @@ int psh_batch_restart(psh_ctx* c, const int* mask) {
   int rc = need_batch(c);
   if (rc) return rc;
   BatchState& b = *c->batch;
+  if ((rc = pinned_free(c, 2))) return rc;
   int* pin = reinterpret_cast<int*>(b.pinned + kPinMask);
   std::memcpy(pin, mask, sizeof(int) * b.W);
   PSH_TRY(c, hipMemcpyAsync(b.mask, pin, sizeof(int) * b.W, hipMemcpyHostToDevice, c->stream));
+  b.pin_pending |= 2;
   if (c->n > 0)
     hipLaunchKernelGGL(k_brestart, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, b.x, b.x0, b.xh, b.mask,
                        (long long)c->n, b.W);
@@ -519,6 +565,7 @@ int psh_batch_compact(psh_ctx* c, const int* keep, int W2) {
   for (int k = 0; k < W2; ++k)
     if (keep[k] < 0 || keep[k] >= b.W) return fail(c, PSH_E_ARG, "bad compaction index");
   if (W2 == b.W) return PSH_OK;
+  if ((rc = pinned_free(c, 4))) return rc;
   int* pin = reinterpret_cast<int*>(b.pinned + kPinKeep);
   std::memcpy(pin, keep, sizeof(int) * W2);
   PSH_TRY(c, hipMemcpyAsync(b.keep, pin, sizeof(int) * std::max(W2, 1), hipMemcpyHostToDevice, c->stream));
@@ -537,6 +584,7 @@ int psh_batch_compact(psh_ctx* c, const int* keep, int W2) {
   PSH_LAUNCHED(c, "batch compaction");
   b.W = W2;
   PSH_TRY(c, hipStreamSynchronize(c->stream));
+  b.pin_pending = 0;
   return PSH_OK;
 }
 

@@ -512,7 +512,8 @@ int psh_create(int device, psh_ctx** out) {
   }
   int rc;
   if ((rc = alloc_n(c, &c->results, 128)) || (rc = alloc_n(c, &c->jobs_dev, kMaxJobs)) ||
-      (rc = alloc_n(c, &c->partials, static_cast<long long>(kTreeBlocks) * kMaxJobs)) || (rc = alloc_n(c, &c->r0_dev, 4))) {
+      (rc = alloc_n(c, &c->partials, static_cast<long long>(kTreeBlocks) * kMaxJobs)) || (rc = alloc_n(c, &c->r0_dev, 4)) ||
+      (rc = alloc_n(c, &c->prm, 4))) {
     psh_destroy(c);
     return rc;
   }
@@ -525,11 +526,12 @@ void psh_destroy(psh_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   psh::destroy_batch(c);
+  for (auto& kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
   void* ptrs[] = {c->A.ptr, c->A.col, c->A.val, c->A.valf, c->A.tiles, c->At.ptr, c->At.col, c->At.val, c->At.valf,
                   c->At.tiles, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
-                  c->lvp, c->lw, c->lt, c->r0_dev};
+                  c->lvp, c->lw, c->lt, c->r0_dev, c->prm};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);

@@ -24,6 +24,23 @@ using namespace psh;
 namespace {
 
 // ------------------------------------------------------------------ iteration kernels
+// Step sizes and the epoch counter live in device memory (`prm` = {τ, σ, inner₀}, written by
+// k_set_params before each chunk), so a chunk's launches have fixed arguments and can be
+// replayed from a captured graph. The Halpern weight of step s is (k+1)/(k+2), k = inner₀ + s,
+// computed in fp64 (IEEE division, the host's bits) and then rounded to T like the host value.
+__device__ inline double halpern_weight(const double* prm, int step) {
+  const double k = prm[2] + static_cast<double>(step);
+  return (k + 1.0) / (k + 2.0);
+}
+
+__global__ void k_set_params(double* prm, double tau, double sigma, double inner0) {
+  if (threadIdx.x == 0) {
+    prm[0] = tau;
+    prm[1] = sigma;
+    prm[2] = inner0;
+  }
+}
+
 template <class T>
 struct PrimalArgs {
   T* x;
@@ -31,21 +48,26 @@ struct PrimalArgs {
   const T *c, *l, *u;
   T* xh;
   T* xb;
-  T tau, tw, a, b;
+  const double* prm;
+  int step;
+  T a, b;
   int write_hat;
 };
 
 template <class T>
 struct EpiPrimal {
   PrimalArgs<T> p;
+  T tau, tw;
+  __device__ explicit EpiPrimal(const PrimalArgs<T>& a)
+      : p(a), tau(static_cast<T>(a.prm[0])), tw(static_cast<T>(halpern_weight(a.prm, a.step))) {}
   __device__ void operator()(int j, bool has, double s) {
     if (!has) return;
     const T g = static_cast<T>(s);
     const T xj = p.x[j];
-    const T h = clampv<T>(xj - p.tau * (p.c[j] - g), p.l[j], p.u[j]);
+    const T h = clampv<T>(xj - tau * (p.c[j] - g), p.l[j], p.u[j]);
     if (p.write_hat) p.xh[j] = h;
     p.xb[j] = 2 * h - xj;
-    p.x[j] = p.tw * (p.a * h + p.b * xj) + (1 - p.tw) * p.x0[j];
+    p.x[j] = tw * (p.a * h + p.b * xj) + (1 - tw) * p.x0[j];
   }
 };
 
@@ -53,7 +75,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_iter_primal(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
                                                         const int* __restrict__ col, const T* __restrict__ val,
                                                         const T* __restrict__ y, PrimalArgs<T> p) {
-  EpiPrimal<T> e{p};
+  EpiPrimal<T> e(p);
   tiled_rows<T, T>(tiles, ntiles, ptr, col, val, y, e);
 }
 
@@ -64,24 +86,29 @@ struct DualArgs {
   const T *rl, *ru;
   T* yh;
   T* yb;
-  T sg, tw, a, b;
+  const double* prm;
+  int step;
+  T a, b;
   int write_hat;
 };
 
 template <class T>
 struct EpiDual {
   DualArgs<T> p;
+  T sg, tw;
+  __device__ explicit EpiDual(const DualArgs<T>& a)
+      : p(a), sg(static_cast<T>(a.prm[1])), tw(static_cast<T>(halpern_weight(a.prm, a.step))) {}
   __device__ void operator()(int i, bool has, double s) {
     if (!has) return;
     const T ax = static_cast<T>(s);
     const T yi = p.y[i];
-    const T q = ax - yi / p.sg;
-    const T h = yi - p.sg * ax + p.sg * clampv<T>(q, p.rl[i], p.ru[i]);
+    const T q = ax - yi / sg;
+    const T h = yi - sg * ax + sg * clampv<T>(q, p.rl[i], p.ru[i]);
     if (p.write_hat) {
       p.yh[i] = h;
       p.yb[i] = 2 * h - yi;
     }
-    p.y[i] = p.tw * (p.a * h + p.b * yi) + (1 - p.tw) * p.y0[i];
+    p.y[i] = tw * (p.a * h + p.b * yi) + (1 - tw) * p.y0[i];
   }
 };
 
@@ -89,7 +116,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_iter_dual(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
                                                       const int* __restrict__ col, const T* __restrict__ val,
                                                       const T* __restrict__ xb, DualArgs<T> p) {
-  EpiDual<T> e{p};
+  EpiDual<T> e(p);
   tiled_rows<T, T>(tiles, ntiles, ptr, col, val, xb, e);
 }
 
@@ -499,55 +526,98 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
   int rc = check_slots(c);
   if (rc) return rc;
   if (!g || g->iterations <= 0) return fail(c, PSH_E_ARG, "chunk needs at least one iteration");
-  return with_t(c, [&](auto* tag) -> int {
-    using T = std::remove_pointer_t<decltype(tag)>;
-    const T* at_val;
-    const T* a_val;
-    if constexpr (sizeof(T) == 8) at_val = c->At.val, a_val = c->A.val;
-    else at_val = c->At.valf, a_val = c->A.valf;
-    const T *cc, *ll, *uu, *rl, *ru;
-    if constexpr (sizeof(T) == 8) cc = c->c, ll = c->l, uu = c->u, rl = c->rl, ru = c->ru;
-    else cc = c->cf, ll = c->lf, uu = c->uf, rl = c->rlf, ru = c->ruf;
-    const T a2 = static_cast<T>(2 * g->rho), b2 = static_cast<T>(1 - 2 * g->rho);
-    for (int s = 0; s < g->iterations; ++s) {
-      const long long inner = g->inner0 + s;
-      const double w = static_cast<double>(inner + 1) / static_cast<double>(inner + 2);
-      const bool epoch_start = inner == 0;
-      const int hat = epoch_start || (s == g->iterations - 1 && g->want_hat_last) ? 1 : 0;
-      PrimalArgs<T> pa{sp<T>(c, PSH_X), sp<T>(c, PSH_X0), cc, ll, uu, sp<T>(c, PSH_XH), sp<T>(c, PSH_XB),
-                       static_cast<T>(g->tau), static_cast<T>(w), a2, b2, hat};
-      DualArgs<T> da{sp<T>(c, PSH_Y), sp<T>(c, PSH_Y0), rl, ru, sp<T>(c, PSH_YH), sp<T>(c, PSH_YB),
-                     static_cast<T>(g->sigma), static_cast<T>(w), a2, b2, hat};
-      hipEvent_t ev;
-      if (c->At.ntiles > 0) {
-        time_begin(c, 0, &ev);
-        hipLaunchKernelGGL((k_iter_primal<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
-                           c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, at_val, sp<T>(c, PSH_Y), pa);
-        time_end(c, 0, ev);
+  if (g->inner0 < 0 || g->inner0 > (1LL << 52)) return fail(c, PSH_E_ARG, "epoch counter out of range");
+  const bool epoch_start = g->inner0 == 0;
+  const bool hat_last = g->want_hat_last != 0;
+  if (epoch_start) {  // the residual's reduction table must exist before any capture
+    int err = PSH_OK;
+    if (!check_table(c, g->mode, F_RES, &err)) return err;
+  }
+  hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, c->stream, c->prm, g->tau, g->sigma,
+                     static_cast<double>(g->inner0));
+  PSH_LAUNCHED(c, "chunk parameters");
+  // The launches of one chunk depend only on (precision, length, epoch start, last-step hats,
+  // reduction mode): replay a captured graph unless per-kernel timing is on.
+  auto enqueue = [&]() -> int {
+    return with_t(c, [&](auto* tag) -> int {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      const T* at_val;
+      const T* a_val;
+      if constexpr (sizeof(T) == 8) at_val = c->At.val, a_val = c->A.val;
+      else at_val = c->At.valf, a_val = c->A.valf;
+      const T *cc, *ll, *uu, *rl, *ru;
+      if constexpr (sizeof(T) == 8) cc = c->c, ll = c->l, uu = c->u, rl = c->rl, ru = c->ru;
+      else cc = c->cf, ll = c->lf, uu = c->uf, rl = c->rlf, ru = c->ruf;
+      const T a2 = static_cast<T>(2 * g->rho), b2 = static_cast<T>(1 - 2 * g->rho);
+      for (int s = 0; s < g->iterations; ++s) {
+        const bool first = s == 0 && epoch_start;
+        const int hat = first || (s == g->iterations - 1 && hat_last) ? 1 : 0;
+        PrimalArgs<T> pa{sp<T>(c, PSH_X), sp<T>(c, PSH_X0), cc, ll, uu, sp<T>(c, PSH_XH), sp<T>(c, PSH_XB), c->prm, s,
+                         a2, b2, hat};
+        DualArgs<T> da{sp<T>(c, PSH_Y), sp<T>(c, PSH_Y0), rl, ru, sp<T>(c, PSH_YH), sp<T>(c, PSH_YB), c->prm, s,
+                       a2, b2, hat};
+        hipEvent_t ev;
+        if (c->At.ntiles > 0) {
+          time_begin(c, 0, &ev);
+          hipLaunchKernelGGL((k_iter_primal<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
+                             c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, at_val, sp<T>(c, PSH_Y), pa);
+          time_end(c, 0, ev);
+        }
+        if (c->A.ntiles > 0) {
+          time_begin(c, 1, &ev);
+          hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
+                             c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
+          time_end(c, 1, ev);
+        }
+        PSH_LAUNCHED(c, "iteration kernels");
+        if (first) {  // residual at the epoch's anchor (r2hpdhg.cpp:75), kept on the device
+          const CheckArgs<T> a = make_args<T>(c, F_RES, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
+          int r;
+          if ((r = launch_check<T>(c, a, false))) return r;
+          int err = PSH_OK;
+          const JobTable* t = check_table(c, g->mode, F_RES, &err);
+          if (!t) return err;
+          JobTable r0;
+          r0.dev = t->dev;
+          r0.host.assign(t->host.begin(), t->host.begin() + 3);
+          if ((r = reduce_table(c, r0, R_R0))) return r;
+        }
       }
-      if (c->A.ntiles > 0) {
-        time_begin(c, 1, &ev);
-        hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, c->A.tiles,
-                           c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
-        time_end(c, 1, ev);
+      return PSH_OK;
+    });
+  };
+  if (c->timing || !c->graphs) {
+    if ((rc = enqueue())) return rc;
+  } else {
+    const long long key = (static_cast<long long>(g->iterations) << 8) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
+                          (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3);
+    auto it = c->graph_cache.find(key);
+    if (it == c->graph_cache.end()) {
+      PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int erc = enqueue();
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (erc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return erc;
       }
-      PSH_LAUNCHED(c, "iteration kernels");
-      if (epoch_start) {  // residual at the epoch's anchor (r2hpdhg.cpp:75), kept on the device
-        const CheckArgs<T> a = make_args<T>(c, F_RES, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
-        int r;
-        if ((r = launch_check<T>(c, a, false))) return r;
-        int err = PSH_OK;
-        const JobTable* t = check_table(c, g->mode, F_RES, &err);
-        if (!t) return err;
-        JobTable r0;
-        r0.dev = t->dev;
-        r0.host.assign(t->host.begin(), t->host.begin() + 3);
-        if ((r = reduce_table(c, r0, R_R0))) return r;
-        c->have_r0 = true;
-      }
+      if (ec != hipSuccess) return hip_fail(c, ec, "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return hip_fail(c, ei, "hipGraphInstantiate");
+      it = c->graph_cache.emplace(key, exec).first;
     }
-    return PSH_OK;
-  });
+    PSH_TRY(c, hipGraphLaunch(it->second, c->stream));
+  }
+  if (epoch_start) c->have_r0 = true;
+  return PSH_OK;
+}
+
+int psh_graphs(psh_ctx* c, int on) {
+  if (!c) return PSH_E_ARG;
+  c->graphs = on != 0;
+  return PSH_OK;
 }
 
 int psh_check(psh_ctx* c, const psh_check_args* g, psh_stats* out) {

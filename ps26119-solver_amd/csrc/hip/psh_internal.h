@@ -92,6 +92,8 @@ struct BatchState {
   double* tmp = nullptr;       // compaction buffer, rows_max·cap
   char* pinned = nullptr;      // host staging: params | mask | keep | stats
   bool have_r0 = false;
+  unsigned pin_pending = 0;                    // regions of `pinned` a queued copy may still read
+  std::map<long long, hipGraphExec_t> graphs;  // captured chunks
 };
 
 }  // namespace psh
@@ -142,6 +144,11 @@ struct psh_ctx {
   std::map<int, psh::JobTable> tables;
 
   psh::BatchState* batch = nullptr;
+
+  // chunk parameters {τ, σ, inner₀} on the device, and captured chunk graphs
+  double* prm = nullptr;
+  bool graphs = true;
+  std::map<long long, hipGraphExec_t> graph_cache;
 
   // timing
   bool timing = false;

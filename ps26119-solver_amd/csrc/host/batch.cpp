@@ -100,6 +100,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
     dev->load(sp);
     psh_ctx* c = dev->ctx();
     if (eo.timing) dev->check(psh_timing(c, 1), "timing");
+    dev->check(psh_graphs(c, eo.graphs), "graphs");
     const bool empty = n == 0 || sp.A.nnz() == 0;
     const double norm = empty ? 0.0 : estimate_norm2(*dev, n, eo.cpu_order);
     const double eta = norm > 0 ? 0.998 / norm : 1.0;

@@ -55,6 +55,7 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_cpu_order") e.cpu_order = v != 0 ? 1 : 0;
     else if (name == "hip_device") e.device = static_cast<int>(v);
     else if (name == "hip_timing") e.timing = v != 0 ? 1 : 0;
+    else if (name == "hip_graphs") e.graphs = v != 0 ? 1 : 0;
     else return name;
   }
   return {};
@@ -293,6 +294,7 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
     if (!dev_) return;
     dev_->load(sp_);
     if (opt.timing) dev_->check(psh_timing(dev_->ctx(), 1), "timing");
+    dev_->check(psh_graphs(dev_->ctx(), opt.graphs), "graphs");
     const bool empty = sp_.n == 0 || sp_.A.nnz() == 0;
     const double norm = empty ? 0.0 : estimate_norm2(*dev_, sp_.n, opt.cpu_order);
     eta_ = norm > 0 ? 0.998 / norm : 1.0;
@@ -337,10 +339,13 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
       fast_iterations_ = it;
       fast_seconds_ = elapsed();
     }
-    if (opt_.verbosity >= 2)
+    // verbosity 1: a progress line at most once a second; 2: every check
+    if (opt_.verbosity >= 2 || (opt_.verbosity == 1 && elapsed() - last_log_ >= 1.0)) {
+      last_log_ = elapsed();
       std::fprintf(stderr, "r2hpdhg-hip it %8lld  t %7.2fs  pobj %+.10e  dobj %+.10e  rp %.2e  rd %.2e  gap %.2e  %s\n",
-                   static_cast<long long>(it), elapsed(), k.primal_obj, k.dual_obj, k.rel_primal(), k.rel_dual(),
+                   static_cast<long long>(it), last_log_, k.primal_obj, k.dual_obj, k.rel_primal(), k.rel_dual(),
                    k.rel_gap(), psh_is_fp32(ctx()) ? "fp32" : "fp64");
+    }
     return k.converged(opt_.tolerance);
   }
   void note_primal_weight(double w) { primal_weight_ = w; }
@@ -467,7 +472,7 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   ScaledProblem sp_;
   std::unique_ptr<Device> dev_;
   std::string error_;
-  double eta_ = 1.0, setup_seconds_ = 0, primal_weight_ = kNaN, fast_seconds_ = -1;
+  double eta_ = 1.0, setup_seconds_ = 0, primal_weight_ = kNaN, fast_seconds_ = -1, last_log_ = 0;
   std::int64_t fast_iterations_ = -1;
   std::chrono::steady_clock::time_point t0_;
   std::vector<double> dual_ray_, primal_ray_;

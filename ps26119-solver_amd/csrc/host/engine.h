@@ -36,6 +36,7 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   int cpu_order = 0;
   int device = 0;
   int timing = 0;  // hip_timing: per-kernel HIP event timing into the Solution
+  int graphs = 1;  // hip_graphs: replay captured chunk graphs (off while timing)
 };
 
 EngineOptions engine_options_from(const Options& o);

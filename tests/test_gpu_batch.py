@@ -130,3 +130,14 @@ def test_warm_start_from_base_solution(gpu_available):
         assert c.status == w.status == ps26119.OPTIMAL
         assert abs(c.objective - w.objective) <= 1e-6 * (1 + abs(c.objective))
     assert sum(w.iterations for w in warm) <= sum(c.iterations for c in cold)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_batch_graph_replay_equals_eager_launches(gpu_available, order):
+    import ps26119
+    base, scen = _scenarios("stocfor1")
+    eager = ps26119.solve_model_batch(base, scen, params={"hip_graphs": 0, "hip_cpu_order": order})
+    graph = ps26119.solve_model_batch(base, scen, params={"hip_cpu_order": order})
+    for e, g in zip(eager, graph):
+        assert e.status == g.status == ps26119.OPTIMAL
+        assert e.iterations == g.iterations and fnv(e.x) == fnv(g.x) and fnv(e.y) == fnv(g.y)

@@ -77,7 +77,9 @@ def test_long_solves_follow_the_reference(gpu_available, golden, key):
     assert r.status == exp["status"] and r.iterations == exp["iterations"]
     assert float(r.objective).hex() == exp["objective"]
     assert fnv(r.x) == exp["x_hash"]
-    assert abs(r.objective - lp.optimum) <= 1e-9 * abs(lp.optimum)
+    # sanity against the planted optimum: a relative KKT error of 1e-8 bounds the objective
+    # error only to that order (the reference's own answer is 3.3e-9 off on random2000_s4)
+    assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
 
 
 def test_default_reductions_on_config0(gpu_available, golden):
@@ -144,3 +146,19 @@ def test_limits_and_determinism(gpu_available):
     assert r.status == ps26119.NOT_SOLVED and "non-finite" in r.message
     r = ps26119.solve_model(lp, params={"no_such_knob": 1.0})
     assert r.status == ps26119.NOT_SOLVED and "no_such_knob" in r.message
+
+
+@pytest.mark.parametrize("name", ["afiro", "stocfor1", "random300_s3"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("order", [0, 1])
+def test_graph_replay_equals_eager_launches(gpu_available, name, prec, order):
+    """Chunks replayed from captured HIP graphs (the default) give the bits of eager launches:
+    every replay runs between eager checks and restarts, so a stale or skipped node would show
+    up as a different trajectory."""
+    import ps26119
+    lp = _model(name)
+    eager = ps26119.solve_model(lp, precision=prec, params={"hip_graphs": 0, "hip_cpu_order": order})
+    graph = ps26119.solve_model(lp, precision=prec, params={"hip_cpu_order": order})
+    assert eager.status == graph.status == ps26119.OPTIMAL
+    assert eager.iterations == graph.iterations
+    assert fnv(eager.x) == fnv(graph.x) and fnv(eager.y) == fnv(graph.y)
