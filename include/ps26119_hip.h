@@ -146,6 +146,13 @@ int psh_ray_vectors(psh_ctx* ctx, int drift, double* dual_ray, double* primal_ra
  * HIP graph and replayed. */
 int psh_chunk(psh_ctx* ctx, const psh_chunk_args* a);
 int psh_graphs(psh_ctx* ctx, int on); /* 1 (default): replay captured chunk graphs */
+/* Mixed precision in increment form (no counterpart in the reference; its mixed mode is
+ * psh_set_fp32): the iterate is the fp64 anchor in X0 / Y0 plus an fp32 increment, the matrix
+ * is read in fp32, and the anchor constants are rebuilt in fp64 at every restart, so the fp32
+ * rounding is relative to the increment and each epoch refines the last. psh_chunk writes
+ * x̂, x̄, ŷ, ȳ in fp64 on its hat steps, so psh_check / psh_restart work unchanged. Off
+ * writes the current point into X / Y. Exclusive with psh_set_fp32(ctx, 1). */
+int psh_delta(psh_ctx* ctx, int on);
 int psh_check(psh_ctx* ctx, const psh_check_args* a, psh_stats* out);
 int psh_restart(psh_ctx* ctx); /* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ */
 

@@ -422,20 +422,37 @@ int check_slot(psh_ctx* c, int s) {
   return PSH_OK;
 }
 
+}  // namespace
+
+// fp32 copies of Ã's and Ãᵀ's values (shared by fp32 mode and the increment form).
+int psh::ensure_fp32_matrix(psh_ctx* c) {
+  if (c->A.valf) return PSH_OK;
+  int rc;
+  if ((rc = alloc_n(c, &c->A.valf, c->nnz)) || (rc = alloc_n(c, &c->At.valf, c->nnz))) return rc;
+  if (c->nnz > 0) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->A.val,
+                       c->A.valf, c->nnz);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->At.val,
+                       c->At.valf, c->nnz);
+  }
+  PSH_LAUNCHED(c, "fp32 matrix");
+  return PSH_OK;
+}
+
+namespace {
+
 int ensure_fp32(psh_ctx* c) {
   if (c->vf[0]) return PSH_OK;
   int rc;
   for (int s = 0; s < PSH_NSLOT; ++s)
     if ((rc = alloc_n(c, &c->vf[s], slot_len(c, s)))) return rc;
-  if ((rc = alloc_n(c, &c->A.valf, c->nnz)) || (rc = alloc_n(c, &c->At.valf, c->nnz)) || (rc = alloc_n(c, &c->cf, c->n)) ||
+  if ((rc = ensure_fp32_matrix(c)) || (rc = alloc_n(c, &c->cf, c->n)) ||
       (rc = alloc_n(c, &c->lf, c->n)) || (rc = alloc_n(c, &c->uf, c->n)) || (rc = alloc_n(c, &c->rlf, c->m)) ||
       (rc = alloc_n(c, &c->ruf, c->m)))
     return rc;
   auto conv = [&](const double* a, float* b, long long n) {
     if (n > 0) hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, b, n);
   };
-  conv(c->A.val, c->A.valf, c->nnz);
-  conv(c->At.val, c->At.valf, c->nnz);
   conv(c->c, c->cf, c->n);
   conv(c->l, c->lf, c->n);
   conv(c->u, c->uf, c->n);
@@ -527,6 +544,11 @@ void psh_destroy(psh_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   psh::destroy_batch(c);
   for (auto& kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
+  {
+    DeltaState& d = c->delta;
+    void* dp[] = {d.dx, d.dxh, d.dxb, d.G, d.LX, d.UX, d.dy, d.dyh, d.dyb, d.BL, d.BU, d.KY};
+    for (void* p : dp) dev_free(p);
+  }
   void* ptrs[] = {c->A.ptr, c->A.col, c->A.val, c->A.valf, c->A.tiles, c->At.ptr, c->At.col, c->At.val, c->At.valf,
                   c->At.tiles, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
@@ -615,6 +637,7 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
 int psh_set_fp32(psh_ctx* c, int on) {
   if (!c || !c->loaded) return c ? fail(c, PSH_E_STATE, "no problem loaded") : PSH_E_ARG;
   const bool want = on != 0;
+  if (want && c->delta.on) return fail(c, PSH_E_STATE, "fp32 storage and the increment form are exclusive");
   if (want == c->fp32) return PSH_OK;
   int rc;
   if (want && (rc = ensure_fp32(c))) return rc;

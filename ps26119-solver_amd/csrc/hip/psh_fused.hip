@@ -120,6 +120,122 @@ __global__ void __launch_bounds__(kBlock) k_iter_dual(const int* __restrict__ ti
   tiled_rows<T, T>(tiles, ntiles, ptr, col, val, xb, e);
 }
 
+// ------------------------------------------------------------------ increment form (mixed precision)
+// The iterate is held as an fp64 anchor plus an fp32 increment, z = z₀ + δ, where z₀ is the
+// epoch's restart point — which is also the Halpern anchor, so the anchor term of the averaging
+// vanishes in δ. Per epoch the anchor terms are folded into fp32 constants computed in fp64:
+//   G = c̃ − Ãᵀȳ₀, LX = l̃ − x₀, UX = ũ − x₀;   BL = Ãx₀ − r̃l, BU = Ãx₀ − r̃u, KY = ȳ₀/σ.
+// Primal:  δx̂ = clamp(δx − τ(G − Ãᵀδy), LX, UX),  δx̄ = 2δx̂ − δx,  δx ← w(2ρδx̂ + (1−2ρ)δx)
+// Dual:    a = Ãδx̄,  t = clamp(KY + δy/σ, BU + a, BL + a)  (= Ãx̄ − proj(Ãx̄ − y/σ)),
+//          δŷ = δy − σt,  δȳ = 2δŷ − δy,  δy ← w(2ρδŷ + (1−2ρ)δy)
+// which is the r²HPDHG step translated by z₀ (reference cpu_backend.cpp:140-178). Rounding is
+// relative to |δ|, and every restart re-anchors in fp64, so the fp32 error does not
+// accumulate: each epoch refines the last. Ã's values are read in fp32 (half the bytes).
+struct DeltaP {
+  float *dx, *dxh, *dxb;
+  const float *G, *LX, *UX;
+  const double* prm;
+  int step;
+  float a, b;
+  int write_hat;
+};
+
+struct EpiDeltaP {
+  DeltaP p;
+  float tau, tw;
+  __device__ explicit EpiDeltaP(const DeltaP& a)
+      : p(a), tau(static_cast<float>(a.prm[0])), tw(static_cast<float>(halpern_weight(a.prm, a.step))) {}
+  __device__ void operator()(int j, bool has, double s) {
+    if (!has) return;
+    const float g = static_cast<float>(s);
+    const float xj = p.dx[j];
+    const float h = clampv<float>(xj - tau * (p.G[j] - g), p.LX[j], p.UX[j]);
+    if (p.write_hat) p.dxh[j] = h;
+    p.dxb[j] = 2 * h - xj;
+    p.dx[j] = tw * (p.a * h + p.b * xj);
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_delta_primal(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
+                                                         const int* __restrict__ col, const float* __restrict__ val,
+                                                         const float* __restrict__ dy, DeltaP p) {
+  EpiDeltaP e(p);
+  tiled_rows<float, float>(tiles, ntiles, ptr, col, val, dy, e);
+}
+
+struct DeltaD {
+  float *dy, *dyh, *dyb;
+  const float *BL, *BU, *KY;
+  const double* prm;
+  int step;
+  float a, b;
+  int write_hat;
+};
+
+struct EpiDeltaD {
+  DeltaD p;
+  float sg, tw;
+  __device__ explicit EpiDeltaD(const DeltaD& a)
+      : p(a), sg(static_cast<float>(a.prm[1])), tw(static_cast<float>(halpern_weight(a.prm, a.step))) {}
+  __device__ void operator()(int i, bool has, double s) {
+    if (!has) return;
+    const float ax = static_cast<float>(s);
+    const float yi = p.dy[i];
+    const float t = clampv<float>(p.KY[i] + yi / sg, p.BU[i] + ax, p.BL[i] + ax);
+    const float h = yi - sg * t;
+    if (p.write_hat) {
+      p.dyh[i] = h;
+      p.dyb[i] = 2 * h - yi;
+    }
+    p.dy[i] = tw * (p.a * h + p.b * yi);
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_delta_dual(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
+                                                       const int* __restrict__ col, const float* __restrict__ val,
+                                                       const float* __restrict__ dxb, DeltaD p) {
+  EpiDeltaD e(p);
+  tiled_rows<float, float>(tiles, ntiles, ptr, col, val, dxb, e);
+}
+
+// Anchor constants of an epoch (fp64 arithmetic, one rounding to fp32); resets δ when asked.
+__global__ void k_delta_anchor_n(const double* __restrict__ c, const double* __restrict__ l, const double* __restrict__ u,
+                                 const double* __restrict__ x0, const double* __restrict__ aty, float* __restrict__ G,
+                                 float* __restrict__ LX, float* __restrict__ UX, float* __restrict__ dx,
+                                 float* __restrict__ dxh, float* __restrict__ dxb, long long n, int reset) {
+  const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double x = x0[j];
+  G[j] = static_cast<float>(c[j] - aty[j]);
+  LX[j] = static_cast<float>(l[j] - x);
+  UX[j] = static_cast<float>(u[j] - x);
+  if (reset) dx[j] = dxh[j] = dxb[j] = 0.0f;
+}
+
+__global__ void k_delta_anchor_m(const double* __restrict__ rl, const double* __restrict__ ru, const double* __restrict__ ax,
+                                 const double* __restrict__ y0, double sigma, float* __restrict__ BL,
+                                 float* __restrict__ BU, float* __restrict__ KY, float* __restrict__ dy,
+                                 float* __restrict__ dyh, float* __restrict__ dyb, long long m, int reset) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  BL[i] = static_cast<float>(ax[i] - rl[i]);
+  BU[i] = static_cast<float>(ax[i] - ru[i]);
+  KY[i] = static_cast<float>(y0[i] / sigma);
+  if (reset) dy[i] = dyh[i] = dyb[i] = 0.0f;
+}
+
+// fp64 points from anchor + increment: ẑ, z̄ (and z when `cur` is set) into the fp64 slots.
+__global__ void k_delta_point(const double* __restrict__ z0, const float* __restrict__ dh, const float* __restrict__ db,
+                              const float* __restrict__ dc, double* __restrict__ zh, double* __restrict__ zb,
+                              double* __restrict__ zc, long long n) {
+  const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double a = z0[j];
+  zh[j] = a + static_cast<double>(dh[j]);
+  zb[j] = a + static_cast<double>(db[j]);
+  if (zc) zc[j] = a + static_cast<double>(dc[j]);
+}
+
 template <class T>
 __global__ void k_restart(T* __restrict__ x, T* __restrict__ x0, const T* __restrict__ xh, long long n) {
   const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -512,6 +628,54 @@ void fill_ray(const double* r, int second, psh_ray* o) {
   o->primal_viol = max_of(r, second ? M_PVIOL2 : M_PVIOL);
 }
 
+// Anchor constants for the increment form: fp64 products of the anchors in X0 / Y0 (into the
+// AX / ATY slots), then the fp32 constants; δ restarts from 0 when the anchor moved.
+int delta_prepare(psh_ctx* c, double sigma) {
+  DeltaState& d = c->delta;
+  if (!d.dirty && sigma == d.sigma) return PSH_OK;
+  int rc;
+  if ((rc = psh_spmv_t(c, PSH_Y0, PSH_ATY)) || (rc = psh_spmv(c, PSH_X0, PSH_AX))) return rc;
+  const int reset = d.dirty ? 1 : 0;
+  if (c->n > 0)
+    hipLaunchKernelGGL(k_delta_anchor_n, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->c, c->l, c->u,
+                       c->vd[PSH_X0], c->vd[PSH_ATY], d.G, d.LX, d.UX, d.dx, d.dxh, d.dxb, (long long)c->n, reset);
+  if (c->m > 0)
+    hipLaunchKernelGGL(k_delta_anchor_m, dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, c->rl, c->ru,
+                       c->vd[PSH_AX], c->vd[PSH_Y0], sigma, d.BL, d.BU, d.KY, d.dy, d.dyh, d.dyb, (long long)c->m, reset);
+  PSH_LAUNCHED(c, "increment-form anchor");
+  d.dirty = false;
+  d.sigma = sigma;
+  return PSH_OK;
+}
+
+// x̂, x̄, ŷ, ȳ (and x, y when `cur`) in fp64 from the anchors and the increments.
+int delta_points(psh_ctx* c, bool cur) {
+  DeltaState& d = c->delta;
+  if (c->n > 0)
+    hipLaunchKernelGGL(k_delta_point, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->vd[PSH_X0], d.dxh, d.dxb,
+                       d.dx, c->vd[PSH_XH], c->vd[PSH_XB], cur ? c->vd[PSH_X] : nullptr, (long long)c->n);
+  if (c->m > 0)
+    hipLaunchKernelGGL(k_delta_point, dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, c->vd[PSH_Y0], d.dyh, d.dyb,
+                       d.dy, c->vd[PSH_YH], c->vd[PSH_YB], cur ? c->vd[PSH_Y] : nullptr, (long long)c->m);
+  PSH_LAUNCHED(c, "increment-form points");
+  return PSH_OK;
+}
+
+// Residual sums at the epoch's anchor (after its first step) into R_R0, on the device.
+template <class T>
+int epoch_residual(psh_ctx* c, int mode) {
+  const CheckArgs<T> a = make_args<T>(c, F_RES, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
+  int r;
+  if ((r = launch_check<T>(c, a, false))) return r;
+  int err = PSH_OK;
+  const JobTable* t = check_table(c, mode, F_RES, &err);
+  if (!t) return err;
+  JobTable r0;
+  r0.dev = t->dev;
+  r0.host.assign(t->host.begin(), t->host.begin() + 3);
+  return reduce_table(c, r0, R_R0);
+}
+
 int check_slots(psh_ctx* c) {
   if (!c) return PSH_E_ARG;
   if (!c->loaded) return fail(c, PSH_E_STATE, "no problem loaded");
@@ -533,9 +697,40 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
     int err = PSH_OK;
     if (!check_table(c, g->mode, F_RES, &err)) return err;
   }
+  const bool delta = c->delta.on;
+  if (delta && (rc = delta_prepare(c, g->sigma))) return rc;
   hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, c->stream, c->prm, g->tau, g->sigma,
                      static_cast<double>(g->inner0));
   PSH_LAUNCHED(c, "chunk parameters");
+  auto enqueue_delta = [&]() -> int {
+    DeltaState& d = c->delta;
+    const float a2 = static_cast<float>(2 * g->rho), b2 = static_cast<float>(1 - 2 * g->rho);
+    for (int s = 0; s < g->iterations; ++s) {
+      const bool first = s == 0 && epoch_start;
+      const bool last_hat = s == g->iterations - 1 && hat_last;
+      const int hat = first || last_hat ? 1 : 0;
+      const DeltaP pa{d.dx, d.dxh, d.dxb, d.G, d.LX, d.UX, c->prm, s, a2, b2, hat};
+      const DeltaD da{d.dy, d.dyh, d.dyb, d.BL, d.BU, d.KY, c->prm, s, a2, b2, hat};
+      hipEvent_t ev;
+      if (c->At.ntiles > 0) {
+        time_begin(c, 0, &ev);
+        hipLaunchKernelGGL(k_delta_primal, dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream, c->At.tiles,
+                           c->At.ntiles, c->At.ptr, c->At.col, c->At.valf, d.dy, pa);
+        time_end(c, 0, ev);
+      }
+      if (c->A.ntiles > 0) {
+        time_begin(c, 1, &ev);
+        hipLaunchKernelGGL(k_delta_dual, dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, c->A.tiles,
+                           c->A.ntiles, c->A.ptr, c->A.col, c->A.valf, d.dxb, da);
+        time_end(c, 1, ev);
+      }
+      PSH_LAUNCHED(c, "increment-form iteration kernels");
+      int r;
+      if (hat && (r = delta_points(c, false))) return r;
+      if (first && (r = epoch_residual<double>(c, g->mode))) return r;
+    }
+    return PSH_OK;
+  };
   // The launches of one chunk depend only on (precision, length, epoch start, last-step hats,
   // reduction mode): replay a captured graph unless per-kernel timing is on.
   auto enqueue = [&]() -> int {
@@ -571,30 +766,22 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
         }
         PSH_LAUNCHED(c, "iteration kernels");
         if (first) {  // residual at the epoch's anchor (r2hpdhg.cpp:75), kept on the device
-          const CheckArgs<T> a = make_args<T>(c, F_RES, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
-          int r;
-          if ((r = launch_check<T>(c, a, false))) return r;
-          int err = PSH_OK;
-          const JobTable* t = check_table(c, g->mode, F_RES, &err);
-          if (!t) return err;
-          JobTable r0;
-          r0.dev = t->dev;
-          r0.host.assign(t->host.begin(), t->host.begin() + 3);
-          if ((r = reduce_table(c, r0, R_R0))) return r;
+          const int r = epoch_residual<T>(c, g->mode);
+          if (r) return r;
         }
       }
       return PSH_OK;
     });
   };
   if (c->timing || !c->graphs) {
-    if ((rc = enqueue())) return rc;
+    if ((rc = delta ? enqueue_delta() : enqueue())) return rc;
   } else {
     const long long key = (static_cast<long long>(g->iterations) << 8) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
-                          (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3);
+                          (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3) | (delta ? 1LL << 6 : 0);
     auto it = c->graph_cache.find(key);
     if (it == c->graph_cache.end()) {
       PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const int erc = enqueue();
+      const int erc = delta ? enqueue_delta() : enqueue();
       hipGraph_t graph = nullptr;
       const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
       if (erc) {
@@ -617,6 +804,31 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
 int psh_graphs(psh_ctx* c, int on) {
   if (!c) return PSH_E_ARG;
   c->graphs = on != 0;
+  return PSH_OK;
+}
+
+int psh_delta(psh_ctx* c, int on) {
+  int rc = check_slots(c);
+  if (rc) return rc;
+  DeltaState& d = c->delta;
+  const bool want = on != 0;
+  if (want == d.on) return PSH_OK;
+  if (want) {
+    if (c->fp32) return fail(c, PSH_E_STATE, "fp32 storage and the increment form are exclusive");
+    if ((rc = ensure_fp32_matrix(c))) return rc;
+    if (!d.dx) {
+      for (float** v : {&d.dx, &d.dxh, &d.dxb, &d.G, &d.LX, &d.UX})
+        if ((rc = alloc_n(c, v, c->n))) return rc;
+      for (float** v : {&d.dy, &d.dyh, &d.dyb, &d.BL, &d.BU, &d.KY})
+        if ((rc = alloc_n(c, v, c->m))) return rc;
+    }
+    d.on = true;
+    d.dirty = true;  // anchors are read from X0 / Y0 at the next chunk
+    return PSH_OK;
+  }
+  // leaving: the current point, x̂, x̄, ŷ, ȳ back into the fp64 slots (only valid after a chunk)
+  if (!d.dirty && (rc = delta_points(c, true))) return rc;
+  d.on = false;
   return PSH_OK;
 }
 
@@ -719,6 +931,7 @@ int psh_ray_vectors(psh_ctx* c, int drift, double* dual_ray, double* primal_ray)
 int psh_restart(psh_ctx* c) {
   int rc = check_slots(c);
   if (rc) return rc;
+  if (c->delta.on) c->delta.dirty = true;  // the fp64 slots hold the new anchor
   return with_t(c, [&](auto* tag) -> int {
     using T = std::remove_pointer_t<decltype(tag)>;
     if (c->n > 0)

@@ -75,6 +75,14 @@ enum Max { M_PMAX, M_DMAX, M_CINF, M_RMAX, M_DVIOL, M_DMAXRAY, M_PVIOL, M_RMAX2,
 enum Flag : unsigned { F_RES = 1, F_KKT = 2, F_RAY = 4, F_DRIFT = 8, F_PID = 16 };
 
 // Device state of a batch of scenarios (interleaved block vectors, W slots of `cap`).
+// Mixed precision in increment form: fp32 δ around the fp64 anchors in slots X0 / Y0.
+struct DeltaState {
+  bool on = false, dirty = true;
+  double sigma = 0.0;  // σ the dual constants were built with
+  float *dx = nullptr, *dxh = nullptr, *dxb = nullptr, *G = nullptr, *LX = nullptr, *UX = nullptr;
+  float *dy = nullptr, *dyh = nullptr, *dyb = nullptr, *BL = nullptr, *BU = nullptr, *KY = nullptr;
+};
+
 struct BatchState {
   int W = 0, cap = 0;
   long long rows_max = 0;
@@ -145,6 +153,8 @@ struct psh_ctx {
 
   psh::BatchState* batch = nullptr;
 
+  psh::DeltaState delta;
+
   // chunk parameters {τ, σ, inner₀} on the device, and captured chunk graphs
   double* prm = nullptr;
   bool graphs = true;
@@ -202,6 +212,9 @@ int reduce_table(psh_ctx* c, const JobTable& t, int first);
 const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err);
 // Copies `count` device results to the host (synchronises the stream).
 int fetch_results(psh_ctx* c, int count, double* out);
+
+// fp32 copies of the matrix values (psh_core.hip)
+int ensure_fp32_matrix(psh_ctx* c);
 
 // frees the batch state of a context (psh_batch.hip)
 void destroy_batch(psh_ctx* c);

@@ -56,8 +56,11 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_device") e.device = static_cast<int>(v);
     else if (name == "hip_timing") e.timing = v != 0 ? 1 : 0;
     else if (name == "hip_graphs") e.graphs = v != 0 ? 1 : 0;
+    else if (name == "hip_refine") e.refine = v != 0 ? 1 : 0;
+    else if (name == "hip_refine_patience") e.refine_patience = std::max(1, static_cast<int>(v));
     else return name;
   }
+  if (e.refine < 0) e.refine = e.cpu_order ? 0 : 1;
   return {};
 }
 
@@ -478,15 +481,33 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   std::vector<double> dual_ray_, primal_ray_;
 };
 
-// Mixed precision (reference engine.cpp:56-79): fp32 until fp32 has done what it can.
+// Mixed precision. With refine = 0 the reference's policy (engine.cpp:56-79): fp32 storage until
+// fp32 has done what it can, then fp64. With refine = 1 (the default outside CPU-order mode)
+// the increment form: fp32 matrix and increments around fp64 anchors that every restart
+// rebuilds, to the full tolerance; fp64 takes over only if progress stops for `refine_patience`
+// checks once the relative KKT error is below 1e-5.
 class PrecisionPolicy {
  public:
   PrecisionPolicy(Context& ctx, const EngineOptions& o)
-      : ctx_(ctx), mixed_(o.precision == Precision::Mixed), target_(o.tolerance) {
-    ctx_.dev().check(psh_set_fp32(ctx_.ctx(), mixed_ ? 1 : 0), "precision");
+      : ctx_(ctx), mixed_(o.precision == Precision::Mixed), refine_(mixed_ && o.refine != 0), target_(o.tolerance),
+        patience_(o.refine_patience) {
+    if (refine_) ctx_.dev().check(psh_delta(ctx_.ctx(), 1), "precision");
+    else ctx_.dev().check(psh_set_fp32(ctx_.ctx(), mixed_ ? 1 : 0), "precision");
   }
   bool fp32() { return psh_is_fp32(ctx_.ctx()) != 0; }
+  bool refining() const { return refine_ && !left_; }
   bool on_check(double rel) {
+    if (refine_) {
+      if (left_) return false;
+      if (rel < 0.9 * best_) best_ = rel, stale_ = 0;
+      else ++stale_;
+      if (rel <= 1e-5 && stale_ >= patience_) {
+        ctx_.dev().check(psh_delta(ctx_.ctx(), 0), "precision");
+        left_ = true;
+        return true;
+      }
+      return false;
+    }
     if (!mixed_ || !fp32()) return false;
     if (rel < 0.9 * best_) best_ = rel, stale_ = 0;
     else ++stale_;
@@ -500,9 +521,9 @@ class PrecisionPolicy {
 
  private:
   Context& ctx_;
-  bool mixed_;
+  bool mixed_, refine_, left_ = false;
   double target_, best_ = 1e300;
-  int stale_ = 0;
+  int stale_ = 0, patience_ = 40;
 };
 
 }  // namespace
@@ -549,6 +570,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
   std::int64_t it = 0, inner = 0;
   double r0 = -1.0, r_last = kInf;
   std::string msg = warm ? "warm start" : "";
+  if (policy.refining()) msg += std::string(msg.empty() ? "" : "; ") + "fp32 increments on fp64 anchors";
 
   for (;;) {
     psh_chunk_args ch{};

@@ -37,6 +37,10 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   int device = 0;
   int timing = 0;  // hip_timing: per-kernel HIP event timing into the Solution
   int graphs = 1;  // hip_graphs: replay captured chunk graphs (off while timing)
+  // hip_refine: mixed precision as fp32 increments on fp64 anchors (default) instead of the
+  // reference's fp32-then-fp64 switch; CPU-order mode (hip_cpu_order = 1) defaults it to 0.
+  int refine = -1;
+  int refine_patience = 40;  // hip_refine_patience: stalled checks before fp64 takes over
 };
 
 EngineOptions engine_options_from(const Options& o);

@@ -162,3 +162,33 @@ def test_graph_replay_equals_eager_launches(gpu_available, name, prec, order):
     assert eager.status == graph.status == ps26119.OPTIMAL
     assert eager.iterations == graph.iterations
     assert fnv(eager.x) == fnv(graph.x) and fnv(eager.y) == fnv(graph.y)
+
+
+@pytest.mark.parametrize("name", SMALL + ["random2000_s4"])
+def test_refined_mixed_precision_reaches_the_fp64_answer(gpu_available, golden, name):
+    """Mixed precision in increment form (fp32 matrix and increments, fp64 anchors rebuilt at
+    every restart; the default for precision="mixed" outside CPU-order mode) reaches the full
+    1e-8 tolerance: Optimal, the in-process check passes, and the objective is within 1e-7
+    relative of the reference's fp64 answer (both answers carry a 1e-8 KKT error; the
+    difference is printed per model)."""
+    import ps26119
+    lp = _model(name)
+    exp = golden["solves"][f"{name}|presolve=1|fp64"]
+    r = ps26119.solve_model(lp, precision="mixed", iteration_limit=3_000_000)
+    ref_obj = float.fromhex(exp["objective"])
+    err = abs(r.objective - ref_obj) / (1 + abs(ref_obj))
+    print(f"{name}: increment-form mixed it={r.iterations} (fp64 reference {exp['iterations']}), "
+          f"objective error vs fp64 {err:.2e}; {r.message}", flush=True)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    assert "fp32 increments" in r.message
+    assert err <= 1e-7
+
+
+def test_refined_mixed_on_config0(gpu_available):
+    import ps26119
+    lp = _model("random10000_s1")
+    r = ps26119.solve_model(lp, precision="mixed")
+    print(f"C0 increment-form mixed: it={r.iterations} t={r.seconds:.2f}s obj={r.objective!r} "
+          f"planted={lp.optimum!r}; {r.message}", flush=True)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS"
+    assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
