@@ -1,25 +1,36 @@
-"""Benchmark of the MI355X r²HPDHG backend on the 1M-row random LP.
+"""Benchmark of the MI355X r²HPDHG backend.
 
-Workload (one GPU): the random LP with a planted optimum, m = 1,000,000 rows, n = 2,000,000
-columns, exactly 6 nonzeros per row (6,000,000 nnz), seed 1, fp64, relative KKT tolerance 1e-8,
-solved through the C API (presolve, scaling, ‖Ã‖₂, r²HPDHG on the GPU, postsolve, in-process
-check and certified bound — the whole solve() call, as the reference times it).
+Default workload (`--workload c1`, one GPU): the random LP with a planted optimum, m = 1,000,000
+rows, n = 2,000,000 columns, exactly 6 nonzeros per row (6,000,000 nnz), seed 1, fp64, relative
+KKT tolerance 1e-8, solved through the C API (presolve, scaling, ‖Ã‖₂, r²HPDHG on the GPU,
+postsolve, in-process check and certified bound — the whole solve() call, as the reference times
+it). A step is one full solve with the inputs already in host memory; `value` is seconds per
+solve (lower is better). Per-kernel durations come from HIP events recorded on the solve's stream
+around every fused iteration kernel; `roofline` reports the fused Ãᵀy + primal-update kernel
+against the 8 TB/s HBM peak. `cpu_baseline` times the reference CPU solver (oracle/_ref, all host
+cores) on a bounded sample of the same model and extrapolates its time from the GPU run's
+iteration count (the fp64 trajectories of the two are the same algorithm).
 
-A step is one full solve to 1e-8 with the inputs already in host memory; `value` is seconds per
-solve (lower is better). The warmup steps are short iteration-limited solves of the same model.
-Per-kernel durations come from HIP events recorded on the solve's stream around every fused
-iteration kernel; `roofline` reports the fused Ãᵀy + primal-update kernel against the 8 TB/s HBM
-peak. `cpu_baseline` times the reference CPU solver (oracle/_ref, all host cores) on a bounded
-sample of the same model and extrapolates its time to 1e-8 from the GPU run's iteration count
-(the fp64 trajectories are the same).
+Other workloads (each prints one JSON line of the same shape):
+  c2  refinery planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
+      versus plain fp64, both to 1e-8; value = mixed seconds, with the fp64 seconds, the speedup
+      and the objective difference between the two.
+  c3  64 scenarios sharing one 100,000 × 200,000 random matrix (b, c, u scaled by lognormal(0,
+      0.05) factors), warm-started from the base solution and solved as one batch (SpMM);
+      value = scenarios per second at the tolerance, against sequential single solves.
+  c4  serving: Poisson arrivals of random LPs with m log-uniform in [m_lo, m_hi], half at 1e-6
+      and half at 1e-8, admitted under an HBM byte budget; value = solves per second, with p50 /
+      p99 latency, against sequential plain fp64 solves of the same requests.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--m 1000000] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload c1|c2|c3|c4] [--m M]
+                    [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
                     [--iteration-limit N]   (profiling: same per-launch work, N iterations)
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -36,8 +47,8 @@ def log(*a):
 
 
 def kernel_bytes(m, n, nnz):
-    """Algorithmic bytes of one launch of each fused kernel (fp64 values and vectors, int32
-    indices; the gathered vector counted once).
+    """Algorithmic bytes of one launch of each fused fp64 kernel (int32 indices; the gathered
+    vector counted once).
       primal: Ãᵀ (nnz·(8+4) + (n+1)·4), gather y (8m), read x, x0, c, l, u (40n), write x̄, x (16n)
       dual:   Ã  (nnz·(8+4) + (m+1)·4), gather x̄ (8n), read y, y0, rl, ru (32m), write y (8m)
     The x̂ / ŷ, ȳ stores happen only on the launches before a check or at an epoch start and are
@@ -47,15 +58,27 @@ def kernel_bytes(m, n, nnz):
     return primal, dual
 
 
-def cpu_baseline(lp, gpu_iterations, gpu_solve):
+def delta_kernel_bytes(m, n, nnz):
+    """The same for the increment-form (mixed precision) kernels: fp32 values and increments.
+      primal: Ãᵀ (nnz·(4+4) + (n+1)·4), gather δy (4m), read δx, G, LX, UX (16n), write δx̄, δx (8n)
+      dual:   Ã  (nnz·(4+4) + (m+1)·4), gather δx̄ (4n), read δy, BL, BU, KY (16m), write δy (4m)"""
+    primal = nnz * 8 + (n + 1) * 4 + 4 * m + 24 * n
+    dual = nnz * 8 + (m + 1) * 4 + 4 * n + 20 * m
+    return primal, dual
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(lp, gpu_iterations, gpu_solve, what):
     """Reference CPU solver on all host cores, bounded sample: setup plus 256 iterations."""
     import orc
     if not orc.available("ref"):
         return None
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
     ref = orc.load("ref")
     sample_it = 256
     t = time.time()
@@ -63,12 +86,199 @@ def cpu_baseline(lp, gpu_iterations, gpu_solve):
     wall = time.time() - t
     per_it = (r["seconds"] - r["setup_seconds"]) / max(1, r["iterations"])
     est = r["setup_seconds"] + per_it * gpu_iterations + max(0.0, wall - r["seconds"])
-    return {"value": est, "unit": "s", "cores": cores, "kind": "reference",
-            "sample": f"reference r2HPDHG (oracle/_ref, threads=all) on the same 1M-row LP, {r['iterations']} iterations "
+    return {"value": est, "unit": "s", "cores": host_cores(), "kind": "reference",
+            "sample": f"reference r2HPDHG (oracle/_ref, threads=all) on the same {what}, {r['iterations']} iterations "
                       f"in {r['seconds']:.1f} s (setup {r['setup_seconds']:.1f} s, {per_it * 1e3:.2f} ms/iteration), "
                       f"extrapolated to the GPU run's {gpu_iterations} iterations",
             "ms_per_iteration": per_it * 1e3, "setup_seconds": r["setup_seconds"],
             "speedup_vs_cpu": est / gpu_solve if gpu_solve > 0 else None}
+
+
+def base_line(args, value, unit, higher, dtype, config):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": value * 1e3 if unit == "s" else None, "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def roofline_of(st, pb, name):
+    launches, kms = st["kernel_launches"], st["kernel_ms"]
+    avg_us = [kms[k] / launches[k] * 1e3 if launches[k] else float("nan") for k in range(3)]
+    gbs = pb / (avg_us[0] * 1e-6) / 1e9 if launches[0] else float("nan")
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None, "kernel": name, "bytes_per_launch": pb, "avg_launch_us": avg_us[0]}, avg_us
+
+
+# ------------------------------------------------------------------ c1
+def bench_c1(args, ps26119, lpgen):
+    t = time.time()
+    lp = lpgen.random_lp(args.m, seed=args.seed)
+    log(f"generated m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t:.1f}s")
+    for w in range(args.warmup):
+        r = ps26119.solve_model(lp, iteration_limit=64, precision=args.precision)
+        log(f"warmup {w}: {r.status_name} {r.seconds:.2f}s")
+    times, res = [], None
+    for s in range(args.steps):
+        t = time.time()
+        res = ps26119.solve_model(lp, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit,
+                                  iteration_limit=args.iteration_limit, verbosity=1, precision=args.precision)
+        times.append(time.time() - t)
+        log(f"step {s}: {res.status_name} it={res.iterations} solve={res.seconds:.2f}s wall={times[-1]:.2f}s "
+            f"obj={res.objective!r} planted={lp.optimum!r} check={res.check} {res.message[:160]}")
+    st = res.stats
+    mixed = args.precision == "mixed"
+    pb, db = (delta_kernel_bytes if mixed else kernel_bytes)(lp.m, lp.n, lp.nnz)
+    roof, avg_us = roofline_of(st, pb, "k_delta_primal (fp32 increments: Ãᵀδy + projection/reflection/Halpern)" if mixed
+                               else "k_iter_primal (fused Ãᵀy + projection/reflection/Halpern)")
+    value = float(np.mean(times))
+    line = base_line(args, value, "s", False, "f32+f64" if mixed else "f64",
+                     {"workload": f"random LP with planted optimum, m={lp.m} n={lp.n} nnz={lp.nnz} (6 per row), seed "
+                                  f"{args.seed}, {args.precision}, tolerance 1e-8, presolve on",
+                      "m": lp.m, "n": lp.n, "nnz": lp.nnz})
+    iter_s = (res.seconds - st["setup_seconds"]) / max(1, res.iterations)
+    line.update({
+        "roofline": roof, "status": res.status_name, "check": res.check, "iterations": res.iterations,
+        "objective_rel_error_vs_planted": abs(res.objective - lp.optimum) / abs(lp.optimum),
+        "solve_seconds": res.seconds, "setup_seconds": st["setup_seconds"], "ms_per_iteration": iter_s * 1e3,
+        "dual_kernel": {"achieved_GBs": db / (avg_us[1] * 1e-6) / 1e9, "avg_launch_us": avg_us[1], "bytes_per_launch": db},
+        "check_pass": {"launches": st["kernel_launches"][2], "avg_us": avg_us[2]},
+        "spmv_phase_GBs": (pb + db) / ((avg_us[0] + avg_us[1]) * 1e-6) / 1e9,
+        "device_bytes": st["device_bytes"], "message": res.message[:200],
+    })
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, value, "1M-row LP")
+    return line
+
+
+# ------------------------------------------------------------------ c2
+def bench_c2(args, ps26119, lpgen):
+    t = time.time()
+    lp = lpgen.refinery_lp(T=args.T, seed=3)
+    log(f"refinery T={args.T}: m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t:.1f}s")
+    out = {}
+    for prec in ("mixed", "fp64"):
+        r = ps26119.solve_model(lp, precision=prec, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit,
+                                iteration_limit=args.iteration_limit, verbosity=1)
+        log(f"{prec}: {r.status_name} it={r.iterations} t={r.seconds:.2f}s obj={r.objective!r} check={r.check} "
+            f"{r.message[:160]}")
+        out[prec] = r
+    mx, fp = out["mixed"], out["fp64"]
+    pb, db = delta_kernel_bytes(lp.m, lp.n, lp.nnz)
+    roof, avg_us = roofline_of(mx.stats, pb, "k_delta_primal (fp32 increments)")
+    line = base_line(args, mx.seconds, "s", False, "f32+f64",
+                     {"workload": f"refinery planning LP T={args.T} (49 rows x 60 columns per period), mixed precision "
+                                  f"in increment form vs fp64, tolerance 1e-8", "m": lp.m, "n": lp.n, "nnz": lp.nnz})
+    line.update({
+        "roofline": roof, "status": mx.status_name, "check": mx.check, "iterations": mx.iterations,
+        "fp64": {"seconds": fp.seconds, "iterations": fp.iterations, "status": fp.status_name, "check": fp.check,
+                 "ms_per_iteration": (fp.seconds - fp.stats["setup_seconds"]) / max(1, fp.iterations) * 1e3},
+        "mixed_ms_per_iteration": (mx.seconds - mx.stats["setup_seconds"]) / max(1, mx.iterations) * 1e3,
+        "speedup_mixed_vs_fp64": fp.seconds / mx.seconds if mx.seconds > 0 else None,
+        "objective_rel_diff_mixed_vs_fp64": abs(mx.objective - fp.objective) / (1 + abs(fp.objective)),
+        "message": mx.message[:200],
+    })
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(lp, fp.iterations, fp.seconds,
+                                                                           f"refinery T={args.T}")
+    return line
+
+
+# ------------------------------------------------------------------ c3
+def bench_c3(args, ps26119, lpgen):
+    m = args.m if args.m != 1_000_000 else 100_000
+    base = lpgen.random_lp(m, seed=args.seed)
+    scen = lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)
+    log(f"c3 base m={base.m} n={base.n} nnz={base.nnz}, {len(scen)} scenarios")
+    t = time.time()
+    b = ps26119.solve_model(base, time_limit=args.time_limit)
+    base_s = time.time() - t
+    log(f"base solve: {b.status_name} it={b.iterations} {base_s:.2f}s")
+    results = {}
+    for tol in (1e-6, 1e-8):
+        t = time.time()
+        res = ps26119.solve_model_batch(base, scen, tolerance=tol, warm_x=b.x, warm_y=b.y, time_limit=args.time_limit)
+        wall = time.time() - t
+        ok = sum(r.status == ps26119.OPTIMAL for r in res)
+        its = [r.iterations for r in res]
+        log(f"batch tol {tol:g}: {ok}/{len(res)} optimal, iterations max {max(its)} mean {np.mean(its):.0f}, {wall:.2f}s")
+        results[tol] = (wall, ok, its, res)
+    # sequential single solves of the first few scenarios (same warm start), extrapolated
+    k_seq = min(args.seq_sample, len(scen))
+    seq = {}
+    for tol in (1e-6, 1e-8):
+        t = time.time()
+        for sc in scen[:k_seq]:
+            mk = lpgen.with_scenario(base, sc)
+            ps26119.solve_model(mk, tolerance=tol, warm_x=b.x, warm_y=b.y, presolve=False, time_limit=args.time_limit)
+        seq[tol] = (time.time() - t) / k_seq * len(scen)
+        log(f"sequential tol {tol:g}: {k_seq} solves, estimated {seq[tol]:.2f}s for {len(scen)}")
+    wall8, ok8, its8, res8 = results[1e-8]
+    value = len(scen) / wall8
+    line = base_line(args, value, "scenarios/s", True, "f64",
+                     {"workload": f"{len(scen)} scenarios sharing a random {base.m}x{base.n} matrix (6 per row), b/c/u "
+                                  f"scaled by lognormal(0, 0.05), warm start from the base solution, fp64, tolerance "
+                                  f"1e-8, one batched solve (SpMM)", "m": base.m, "n": base.n, "nnz": base.nnz,
+                      "scenarios": len(scen)})
+    line.update({
+        "roofline": None, "optimal": ok8, "iterations_max": max(its8), "iterations_mean": float(np.mean(its8)),
+        "batch_seconds": {"1e-6": results[1e-6][0], "1e-8": wall8},
+        "sequential_seconds_estimate": {"1e-6": seq[1e-6], "1e-8": seq[1e-8], "sample": k_seq},
+        "speedup_vs_sequential": {"1e-6": seq[1e-6] / results[1e-6][0], "1e-8": seq[1e-8] / wall8},
+        "base_solve_seconds": base_s,
+    })
+    line["cpu_baseline"] = None
+    return line
+
+
+# ------------------------------------------------------------------ c4
+def bench_c4(args, ps26119, lpgen):
+    rng = np.random.default_rng(args.seed)
+    N = args.requests
+    ms = np.exp(rng.uniform(np.log(args.m_lo), np.log(args.m_hi), N)).astype(int)
+    tols = np.where(rng.random(N) < 0.5, 1e-6, 1e-8)
+    gaps = rng.exponential(1.0 / args.rate, N)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    lps = [lpgen.random_lp(int(m), seed=1000 + k) for k, m in enumerate(ms)]
+    log(f"c4: {N} requests, m in [{ms.min()}, {ms.max()}], rate {args.rate}/s, budget {args.budget_gb} GB")
+    srv = ps26119.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_instances=256, setup_threads=8)
+    ids = [None] * N
+
+    def feed():
+        t0 = time.time()
+        for k in range(N):
+            d = arrivals[k] - (time.time() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=args.time_limit)
+
+    t = time.time()
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    res = [srv.wait(i) for i in ids]
+    served = time.time() - t
+    srv.close()
+    lat = np.array([r.stats["latency"] for r in res])
+    ok = sum(r.status == ps26119.OPTIMAL for r in res)
+    log(f"served: {ok}/{N} optimal in {served:.2f}s, p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+    # sequential plain fp64 solves of the same requests, back to back
+    t = time.time()
+    seq_lat = []
+    for lp, tol in zip(lps, tols):
+        r = ps26119.solve_model(lp, tolerance=float(tol), time_limit=args.time_limit)
+        seq_lat.append(r.seconds)
+    sequential = time.time() - t
+    log(f"sequential: {sequential:.2f}s")
+    value = N / served
+    line = base_line(args, value, "solves/s", True, "f64",
+                     {"workload": f"serving: {N} Poisson arrivals at {args.rate}/s of random LPs (6 per row, n=2m), m "
+                                  f"log-uniform in [{args.m_lo}, {args.m_hi}], half at tolerance 1e-6 and half at 1e-8, "
+                                  f"HBM budget {args.budget_gb} GB, continuous batching", "requests": N})
+    line.update({
+        "roofline": None, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)), "served_seconds": served,
+        "sequential": {"seconds": sequential, "solves_per_s": N / sequential},
+        "speedup_vs_sequential": sequential / served,
+    })
+    line["cpu_baseline"] = None
+    return line
 
 
 def main():
@@ -76,12 +286,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--m", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--time-limit", type=float, default=600.0)
     ap.add_argument("--iteration-limit", type=int, default=0,
                     help="stop each timed solve after this many iterations (profiling runs; 0 = to 1e-8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="c1: mixed = fp32 increments on fp64 anchors (the line's dtype says which)")
+    ap.add_argument("--T", type=int, default=8760, help="c2: planning periods")
+    ap.add_argument("--scenarios", type=int, default=64, help="c3")
+    ap.add_argument("--seq-sample", type=int, default=4, help="c3: sequential solves timed for the comparison")
+    ap.add_argument("--requests", type=int, default=48, help="c4")
+    ap.add_argument("--rate", type=float, default=8.0, help="c4: arrivals per second")
+    ap.add_argument("--m-lo", type=int, default=1000, help="c4")
+    ap.add_argument("--m-hi", type=int, default=200000, help="c4")
+    ap.add_argument("--budget-gb", type=float, default=16.0, help="c4: HBM byte budget of the server")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU: the r2HPDHG path here runs on a single MI355X; --gpus {args.gpus} "
@@ -93,47 +314,8 @@ def main():
     if ps26119.device_count() <= 0:
         log("no GPU visible to the HIP backend")
         sys.exit(1)
-    t = time.time()
-    lp = lpgen.random_lp(args.m, seed=args.seed)
-    log(f"generated m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t:.1f}s")
-    for w in range(args.warmup):
-        r = ps26119.solve_model(lp, iteration_limit=64)
-        log(f"warmup {w}: {r.status_name} {r.seconds:.2f}s")
-    times, res = [], None
-    for s in range(args.steps):
-        t = time.time()
-        res = ps26119.solve_model(lp, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit,
-                                  iteration_limit=args.iteration_limit, verbosity=1)
-        times.append(time.time() - t)
-        log(f"step {s}: {res.status_name} it={res.iterations} solve={res.seconds:.2f}s wall={times[-1]:.2f}s "
-            f"obj={res.objective!r} planted={lp.optimum!r} check={res.check} {res.message[:120]}")
-    st = res.stats
-    launches, kms = st["kernel_launches"], st["kernel_ms"]
-    avg_us = [kms[k] / launches[k] * 1e3 if launches[k] else float("nan") for k in range(3)]
-    pb, db = kernel_bytes(lp.m, lp.n - 0, lp.nnz)
-    gbs_p = pb / (avg_us[0] * 1e-6) / 1e9 if launches[0] else float("nan")
-    gbs_d = db / (avg_us[1] * 1e-6) / 1e9 if launches[1] else float("nan")
-    iter_s = (res.seconds - st["setup_seconds"]) / max(1, res.iterations)
-    value = float(np.mean([r for r in times]))
-    line = {
-        "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"random LP with planted optimum, m={lp.m} n={lp.n} nnz={lp.nnz} (6 per row), seed "
-                               f"{args.seed}, fp64, tolerance 1e-8, presolve on", "m": lp.m, "n": lp.n, "nnz": lp.nnz},
-        "roofline": {"bound": "hbm", "achieved": gbs_p, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_p / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_iter_primal (fused Aᵀy + projection/reflection/Halpern)",
-                     "bytes_per_launch": pb, "avg_launch_us": avg_us[0]},
-        "status": res.status_name, "check": res.check, "iterations": res.iterations,
-        "objective_rel_error_vs_planted": abs(res.objective - lp.optimum) / abs(lp.optimum),
-        "solve_seconds": res.seconds, "setup_seconds": st["setup_seconds"], "ms_per_iteration": iter_s * 1e3,
-        "dual_kernel": {"achieved_GBs": gbs_d, "avg_launch_us": avg_us[1], "bytes_per_launch": db},
-        "check_pass": {"launches": launches[2], "avg_us": avg_us[2]},
-        "spmv_phase_GBs": (pb + db) / ((avg_us[0] + avg_us[1]) * 1e-6) / 1e9 if launches[0] and launches[1] else None,
-        "device_bytes": st["device_bytes"],
-    }
-    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, value)
-    print(json.dumps(line), flush=True)
+    fn = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload]
+    print(json.dumps(fn(args, ps26119, lpgen)), flush=True)
 
 
 if __name__ == "__main__":

@@ -66,6 +66,39 @@ int ps26119_amd_solve_batch(int num_rows, int num_cols, int sense, double obj_of
                             const ps26119_options* opt, const ps26119_amd_options* ext, ps26119_result* results,
                             double* xs, double* ys);
 
+/* ---- serving: continuous batching of independent LP solves on one GPU (no counterpart in the
+ * reference). Requests are prepared on host threads, admitted while the HBM byte budget has room,
+ * and advanced together as one packed block-diagonal system; each finishes on its own schedule.
+ * Per request: fp64 r²HPDHG without presolve, the reference's restart / PID rules, KKT on the
+ * original model, certified bound and in-process check; the result of a request does not depend
+ * on the other requests in the mix. */
+typedef struct ps26119_amd_server ps26119_amd_server;
+typedef struct {
+  int device;
+  double budget_bytes;   /* HBM arena (0 = 8 GiB) */
+  int max_instances;     /* live at once (0 = 256) */
+  int check_every;       /* 0 = 64 */
+  int setup_threads;     /* host workers for preparation and finish (0 = 4) */
+} ps26119_amd_server_options;
+
+/* NULL on failure, with the reason in error (may be NULL). */
+ps26119_amd_server* ps26119_amd_server_create(const ps26119_amd_server_options* o, char* error, int error_len);
+/* Abandons requests still in flight. */
+void ps26119_amd_server_destroy(ps26119_amd_server* s);
+/* Queues a solve (tolerance, iteration_limit, time_limit and verbosity of opt are honoured;
+ * ext may carry engine parameters). Returns the request id (>= 0) or -1 for invalid arguments. */
+long long ps26119_amd_server_submit(ps26119_amd_server* s, int num_rows, int num_cols, int sense, double obj_offset,
+                                    const double* c, const double* col_lower, const double* col_upper,
+                                    const double* row_lower, const double* row_upper, const int* col_start,
+                                    const int* row_index, const double* value, const ps26119_options* opt,
+                                    const ps26119_amd_options* ext);
+/* Waits for request `id` (timeout_s < 0: forever). Returns 0 when done (result, x, y, z filled as
+ * in ps26119_solve_lp; times = {submitted, admitted, done, latency} in seconds since the server
+ * started, may be NULL), 1 on timeout, -1 for an unknown id. */
+int ps26119_amd_server_wait(ps26119_amd_server* s, long long id, double timeout_s, ps26119_result* result, double* x,
+                            double* y, double* z, double* times);
+int ps26119_amd_server_in_flight(ps26119_amd_server* s);
+
 /* Number of visible GPUs the backend can use (0 when none). */
 int ps26119_amd_device_count(void);
 

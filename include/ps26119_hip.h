@@ -205,6 +205,32 @@ int psh_timing(psh_ctx* ctx, int on);
  * 2 check pass; returns launches and total milliseconds. */
 int psh_timing_get(psh_ctx* ctx, int kernel_class, long long* launches, double* total_ms);
 
+/* ---- serving: independent LPs of different sizes as one packed block-diagonal system
+ * (no counterpart in the reference). An arena of `budget_bytes` of device memory holds every
+ * live instance (problem, original data, iterates); one launch of each fused half step advances
+ * all instances that a chunk lists; per-instance sums are reduced inside each tile and then over
+ * the instance's own tiles in a fixed order, so an instance's trajectory does not depend on the
+ * others. Instances are added / removed between chunks; chunks replay a captured graph while the
+ * instance set is unchanged. Host order per round: chunk → check → restart → remove / add. */
+typedef struct psh_server psh_server;
+typedef struct {
+  int id;
+  double tau, sigma;
+  long long inner0;  /* Halpern index of the chunk's first iteration; 0 = the chunk begins an epoch */
+} psh_serve_param;
+int psh_server_create(int device, size_t budget_bytes, int max_instances, psh_server** out);
+void psh_server_destroy(psh_server* s);
+const char* psh_server_error(const psh_server* s);
+size_t psh_server_bytes(int m, int n, long long nnz);  /* arena bytes of one instance */
+size_t psh_server_free_bytes(const psh_server* s);     /* largest free block of the arena */
+int psh_server_add(psh_server* s, const psh_problem* p, int* id); /* PSH_E_OOM when it does not fit */
+int psh_server_remove(psh_server* s, int id);
+int psh_server_set_iterate(psh_server* s, int id, const double* x, const double* y);
+int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count, double rho);
+int psh_server_check(psh_server* s, const int* ids, int count, psh_stats* out);
+int psh_server_restart(psh_server* s, const int* ids, int count);
+int psh_server_download(psh_server* s, int id, double* xh, double* yh);
+
 #ifdef __cplusplus
 }
 #endif

@@ -242,14 +242,11 @@ __device__ inline T clampv(T v, T lo, T hi) {
 
 // Row-tiled sparse product with an fp64 row-sequential accumulator (CSR order) and a fused
 // epilogue epi(row, has_row, sum) run by every thread of the tile's workgroup.
+// The rows [r0, r1) of one tile (the whole workgroup calls it; r1 − r0 ≤ blockDim.x).
 template <class V, class X, class Epi>
-__device__ inline void tiled_rows(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
-                                  const int* __restrict__ col, const V* __restrict__ val, const X* __restrict__ x,
-                                  Epi& epi) {
+__device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
+                                 const V* __restrict__ val, const X* __restrict__ x, Epi& epi) {
   __shared__ double prod[kTileNnz];
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
-  if (tile >= ntiles) return;
-  const int r0 = tiles[tile], r1 = tiles[tile + 1];
   const int p0 = ptr[r0], p1 = ptr[r1];
   const int row = r0 + static_cast<int>(threadIdx.x);
   const bool has = row < r1;
@@ -267,6 +264,15 @@ __device__ inline void tiled_rows(const int* __restrict__ tiles, int ntiles, con
     __syncthreads();
   }
   epi(row, has, s);
+}
+
+template <class V, class X, class Epi>
+__device__ inline void tiled_rows(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
+                                  const int* __restrict__ col, const V* __restrict__ val, const X* __restrict__ x,
+                                  Epi& epi) {
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  if (tile >= ntiles) return;
+  tile_rows<V, X, Epi>(tiles[tile], tiles[tile + 1], ptr, col, val, x, epi);
 }
 
 }  // namespace psh

@@ -180,7 +180,8 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
       dev->check(psh_batch_set_iterate(c, X.data(), Y.data()), "batch");
 
       const double w0 = options.warm_primal_weight > 0 ? options.warm_primal_weight : 1.0;
-      std::vector<double> omega(K, w0), best_omega(K, w0), best_balance(K, 1e300), pid_int(K, 0.0), pid_last(K, 0.0);
+      std::vector<PidState> pid(K);
+      for (auto& q : pid) q.omega = q.best_omega = w0;
       std::vector<double> r0(K, -1.0), r_last(K, kInf), fast_s(K, -1.0);
       std::vector<std::int64_t> inner(K, 0), fast_it(K, -1);
       std::int64_t it = 0;
@@ -193,8 +194,8 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
         sigma.resize(W);
         in0.resize(W);
         for (int t = 0; t < W; ++t) {
-          tau[t] = eta / omega[sid[t]];
-          sigma[t] = eta * omega[sid[t]];
+          tau[t] = eta / pid[sid[t]].omega;
+          sigma[t] = eta * pid[sid[t]].omega;
           in0[t] = inner[sid[t]];
         }
         dev->check(psh_batch_chunk(c, Kc, tau.data(), sigma.data(), in0.data(), rho, mode), "batch iteration");
@@ -208,7 +209,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
         for (int t = 0; t < W; ++t) {
           const int k = sid[t];
           const double* st = stats.data() + static_cast<std::size_t>(t) * SC;
-          const double om = omega[k];
+          const double om = pid[k].omega;
           auto resid = [&](double px, double py, double cr) { return std::sqrt(std::max(om * px + py / om + 2.0 * eta * cr, 0.0)); };
           if (in0[t] == 0) r0[k] = resid(st[PSH_BSTAT_R0_PX], st[PSH_BSTAT_R0_PY], st[PSH_BSTAT_R0_CROSS]);
           KktStats kk;
@@ -248,7 +249,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
             s.setup_seconds = setup_shared;
             s.iterations_to_fast = fast_it[k];
             s.seconds_to_fast = fast_s[k];
-            s.primal_weight = omega[k];
+            s.primal_weight = pid[k].omega;
           };
           if (!kk.finite()) {
             finish(Status::NumericalError, "non-finite iterate");
@@ -265,29 +266,11 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
           }
           keep.push_back(t);
           const double r = resid(st[PSH_BSTAT_PX], st[PSH_BSTAT_PY], st[PSH_BSTAT_CROSS]);
-          const bool ratios_ok = r0[k] > 0 && std::isfinite(r0[k]) && r > 0;
-          const bool restart = (ratios_ok && (r <= eo.restart_sufficient * r0[k] ||
-                                              (r <= eo.restart_necessary * r0[k] && r > r_last[k]))) ||
-                               static_cast<double>(inner[k]) >= eo.restart_artificial * static_cast<double>(it);
+          const bool restart = restart_due(eo, r, r0[k], r_last[k], inner[k], it);
           r_last[k] = r;
           if (!restart) continue;
-          const double dxn = std::sqrt(st[PSH_BSTAT_PIDX]), dyn = std::sqrt(st[PSH_BSTAT_PIDY]);
-          const bool both_pos = kk.rel_dual() > 0 && kk.rel_primal() > 0;
-          const double ratio = both_pos ? kk.rel_dual() / kk.rel_primal() : 1.0;
-          if (dxn > 1e-16 && dyn > 1e-16 && dxn < 1e12 && dyn < 1e12 && ratio > 1e-8 && ratio < 1e8) {
-            const double e = std::log(dyn) - std::log(dxn) - std::log(omega[k]);
-            pid_int[k] = eo.pid_integral_decay * pid_int[k] + e;
-            const double step = eo.pid_kp * e + eo.pid_ki * pid_int[k] + eo.pid_kd * (e - pid_last[k]);
-            omega[k] *= std::exp(std::clamp(step, -eo.pid_max_log_step, eo.pid_max_log_step));
-            pid_last[k] = e;
-          } else {
-            omega[k] = best_omega[k];
-            pid_int[k] = pid_last[k] = 0.0;
-          }
-          if (both_pos && std::fabs(std::log10(ratio)) < best_balance[k]) {
-            best_balance[k] = std::fabs(std::log10(ratio));
-            best_omega[k] = omega[k];
-          }
+          pid_update(pid[k], eo, std::sqrt(st[PSH_BSTAT_PIDX]), std::sqrt(st[PSH_BSTAT_PIDY]), kk.rel_primal(),
+                     kk.rel_dual());
           mask[t] = 1;
           inner[k] = 0;
           r_last[k] = kInf;

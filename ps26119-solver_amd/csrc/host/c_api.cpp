@@ -4,6 +4,9 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
 
 #include "batch.h"
 #include "engine.h"
@@ -12,6 +15,7 @@
 #include "ps26119/ps26119_amd.h"
 #include "ps26119_hip.h"
 #include "scaling.h"
+#include "serve.h"
 #include "solve.h"
 #include "verify.h"
 
@@ -296,6 +300,108 @@ int ps26119_amd_solve_batch(int num_rows, int num_cols, int sense, double obj_of
   }
   return rc;
 }
+
+struct ps26119_amd_server {
+  std::unique_ptr<amd::Server> srv;
+  std::mutex mu;
+  std::map<long long, std::pair<int, int>> dims;  // id → (m, n)
+};
+
+ps26119_amd_server* ps26119_amd_server_create(const ps26119_amd_server_options* o, char* error, int error_len) {
+  try {
+    amd::ServerOptions so;
+    if (o) {
+      so.device = o->device;
+      if (o->budget_bytes > 0) so.budget_bytes = static_cast<std::size_t>(o->budget_bytes);
+      if (o->max_instances > 0) so.max_instances = o->max_instances;
+      if (o->check_every > 0) so.check_every = o->check_every;
+      if (o->setup_threads > 0) so.setup_threads = o->setup_threads;
+    }
+    auto s = std::make_unique<ps26119_amd_server>();
+    s->srv = std::make_unique<amd::Server>(so);
+    if (!s->srv->ok()) {
+      if (error && error_len > 0) set_text(error, error_len, s->srv->error());
+      return nullptr;
+    }
+    return s.release();
+  } catch (const std::exception& e) {
+    if (error && error_len > 0) set_text(error, error_len, e.what());
+    return nullptr;
+  }
+}
+
+void ps26119_amd_server_destroy(ps26119_amd_server* s) { delete s; }
+
+long long ps26119_amd_server_submit(ps26119_amd_server* s, int num_rows, int num_cols, int sense, double obj_offset,
+                                    const double* c, const double* col_lower, const double* col_upper,
+                                    const double* row_lower, const double* row_upper, const int* col_start,
+                                    const int* row_index, const double* value, const ps26119_options* opt,
+                                    const ps26119_amd_options* ext) {
+  if (!s) return -1;
+  try {
+    Model m;
+    std::string err;
+    if (num_rows < 0 || num_cols < 0 || !col_start) return -1;
+    // an invalid model is still accepted: it finishes at once with its status and message
+    (void)make_model(num_rows, num_cols, sense, obj_offset, c, col_lower, col_upper, row_lower, row_upper, col_start,
+                     row_index, value, m, err);
+    if (m.num_rows != num_rows || m.num_cols != num_cols) return -1;
+    Options o;
+    if (to_options(opt, num_rows, num_cols, o)) return -1;
+    o.use_gpu = true;
+    if (ext) {
+      if (ext->check_every > 0) o.termination_check_every = ext->check_every;
+      for (int k = 0; k < ext->n_params; ++k) {
+        if (!ext->param_names || !ext->param_values || !ext->param_names[k]) return -1;
+        o.engine_params.emplace_back(ext->param_names[k], ext->param_values[k]);
+      }
+    }
+    const long long id = s->srv->submit(std::move(m), o);
+    std::lock_guard<std::mutex> g(s->mu);
+    s->dims[id] = {num_rows, num_cols};
+    return id;
+  } catch (...) {
+    return -1;
+  }
+}
+
+int ps26119_amd_server_wait(ps26119_amd_server* s, long long id, double timeout_s, ps26119_result* result, double* x,
+                            double* y, double* z, double* times) {
+  if (!s) return -1;
+  int m = 0, n = 0;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->dims.find(id);
+    if (it == s->dims.end()) return -1;
+    m = it->second.first;
+    n = it->second.second;
+  }
+  amd::ServedResult r;
+  if (!s->srv->wait(id, r, timeout_s)) return 1;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->dims.erase(id);
+  }
+  if (result) {
+    clear(result);
+    fill(r.solution, result);
+  }
+  auto put = [](double* dst, const std::vector<double>& v, int len) {
+    if (dst && static_cast<int>(v.size()) == len && len > 0) std::memcpy(dst, v.data(), sizeof(double) * len);
+  };
+  put(x, r.solution.x, n);
+  put(y, r.solution.y, m);
+  put(z, r.solution.z, n);
+  if (times) {
+    times[0] = r.submit_time;
+    times[1] = r.admit_time;
+    times[2] = r.done_time;
+    times[3] = r.latency();
+  }
+  return 0;
+}
+
+int ps26119_amd_server_in_flight(ps26119_amd_server* s) { return s ? s->srv->in_flight() : 0; }
 
 int ps26119_amd_device_count(void) { return psh_device_count(); }
 

@@ -177,8 +177,14 @@ void Device::check(int rc, const char* what) const {
 }
 
 void Device::load(const ScaledProblem& sp) {
+  std::vector<double> cmin;
+  const psh_problem p = problem_of(sp, cmin);
+  check(psh_load(ctx_, &p), "upload");
+}
+
+psh_problem problem_of(const ScaledProblem& sp, std::vector<double>& cmin) {
   const Model& M = *sp.original;
-  std::vector<double> cmin(sp.n);
+  cmin.resize(sp.n);
   for (int j = 0; j < sp.n; ++j) cmin[j] = M.sense * M.obj[j];
   psh_problem p{};
   p.m = sp.m;
@@ -207,7 +213,33 @@ void Device::load(const ScaledProblem& sp) {
   p.rl_o = M.row_lower.data();
   p.ru_o = M.row_upper.data();
   p.c_min = cmin.data();
-  check(psh_load(ctx_, &p), "upload");
+  return p;
+}
+
+bool restart_due(const EngineOptions& o, double r, double r0, double r_last, std::int64_t inner, std::int64_t it) {
+  const bool ratios_ok = r0 > 0 && std::isfinite(r0) && r > 0;
+  return (ratios_ok && (r <= o.restart_sufficient * r0 || (r <= o.restart_necessary * r0 && r > r_last))) ||
+         static_cast<double>(inner) >= o.restart_artificial * static_cast<double>(it);
+}
+
+void pid_update(PidState& p, const EngineOptions& o, double dxn, double dyn, double rel_primal, double rel_dual) {
+  const bool both_pos = rel_dual > 0 && rel_primal > 0;
+  const double ratio = both_pos ? rel_dual / rel_primal : 1.0;
+  if (dxn > 1e-16 && dyn > 1e-16 && dxn < 1e12 && dyn < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+    const double e = std::log(dyn) - std::log(dxn) - std::log(p.omega);
+    p.integral = o.pid_integral_decay * p.integral + e;
+    const double step = o.pid_kp * e + o.pid_ki * p.integral + o.pid_kd * (e - p.last);
+    p.omega *= std::exp(std::clamp(step, -o.pid_max_log_step, o.pid_max_log_step));
+    p.last = e;
+  } else {
+    p.omega = p.best_omega;
+    p.integral = p.last = 0.0;
+  }
+  const double balance = std::fabs(std::log10(ratio));
+  if (both_pos && balance < p.best_balance) {
+    p.best_balance = balance;
+    p.best_omega = p.omega;
+  }
 }
 
 // ------------------------------------------------------------------ ‖Ã‖₂
@@ -243,8 +275,13 @@ double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& 
 
 }  // namespace
 
-double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter, double rel_tol) {
-  psh_ctx* c = dev.ctx();
+namespace {
+
+// Lanczos driver over abstract products (reference src/la/csr.cpp:86-132): begin(v0) loads the
+// unit start vector, step(β_prev, α, β²) does w = ÃᵀÃv, α = wᵀv, w ← w − (αv + β_prev v_prev),
+// β² = Σw²; advance(β) sets v_prev ← v, v ← w/β.
+template <class Begin, class Step, class Advance>
+double lanczos_norm(int n, int max_iter, double rel_tol, Begin&& begin, Step&& step, Advance&& advance) {
   if (n == 0) return 0.0;
   const int K = std::min(std::max(max_iter, 20), 400);
   std::vector<double> v(n);
@@ -257,14 +294,13 @@ double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter, double re
   }
   nv = std::sqrt(nv);
   for (double& e : v) e /= nv;
-  dev.check(psh_lanczos_begin(c, v.data()), "lanczos");
+  begin(v);
   std::vector<double> alpha, beta;
   double theta = 0, theta_old = -1;
   int stagnant = 0;
-  const int mode = cpu_order ? PSH_RED_CPU : PSH_RED_TREE;
   for (int k = 0; k < K; ++k) {
     double al = 0, bsq = 0;
-    dev.check(psh_lanczos_step(c, beta.empty() ? 0.0 : beta.back(), mode, &al, &bsq), "lanczos");
+    step(beta.empty() ? 0.0 : beta.back(), al, bsq);
     const double bn = std::sqrt(bsq);
     alpha.push_back(al);
     theta = tridiag_max_eig(alpha, beta, static_cast<int>(alpha.size()));
@@ -276,9 +312,44 @@ double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter, double re
       stagnant = 0;
     }
     theta_old = theta;
-    dev.check(psh_lanczos_advance(c, bn), "lanczos");
+    advance(bn);
   }
   return std::sqrt(std::max(theta, 0.0)) * (1.0 + 1e-4);
+}
+
+}  // namespace
+
+double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter, double rel_tol) {
+  psh_ctx* c = dev.ctx();
+  const int mode = cpu_order ? PSH_RED_CPU : PSH_RED_TREE;
+  return lanczos_norm(
+      n, max_iter, rel_tol, [&](const std::vector<double>& v) { dev.check(psh_lanczos_begin(c, v.data()), "lanczos"); },
+      [&](double bp, double& al, double& bsq) { dev.check(psh_lanczos_step(c, bp, mode, &al, &bsq), "lanczos"); },
+      [&](double bn) { dev.check(psh_lanczos_advance(c, bn), "lanczos"); });
+}
+
+double estimate_norm2_host(const ScaledProblem& sp, int max_iter, double rel_tol) {
+  const int n = sp.n;
+  std::vector<double> v, vp(n, 0.0), w(n), t(sp.m);
+  return lanczos_norm(
+      n, max_iter, rel_tol, [&](const std::vector<double>& v0) { v = v0; },
+      [&](double bp, double& al, double& bsq) {
+        multiply(sp.A, v.data(), t.data());
+        multiply(sp.At, t.data(), w.data());
+        al = 0.0;
+        for (int j = 0; j < n; ++j) al += w[j] * v[j];
+        bsq = 0.0;
+        for (int j = 0; j < n; ++j) {
+          w[j] -= al * v[j] + bp * vp[j];
+          bsq += w[j] * w[j];
+        }
+      },
+      [&](double bn) {
+        for (int j = 0; j < n; ++j) {
+          vp[j] = v[j];
+          v[j] = w[j] / bn;
+        }
+      });
 }
 
 // ------------------------------------------------------------------ engine
@@ -542,7 +613,8 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
   PrecisionPolicy policy(ctx, opt);
   const double eta = ctx.eta(), rho = opt.reflection;
   const int mode = ctx.mode();
-  double omega = ctx.initial_primal_weight();
+  PidState pid;
+  pid.omega = pid.best_omega = ctx.initial_primal_weight();
 
   // x = proj_X(0), y = 0 (reference r2hpdhg.cpp:31-34)
   dev.check(psh_fill(c, PSH_DX, 0.0), "fill");
@@ -551,7 +623,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
   dev.check(psh_fill(c, PSH_Y, 0.0), "fill");
   bool promoted_by_warm = false;
   const bool warm = ctx.apply_warm_start(policy.fp32(), promoted_by_warm);
-  ctx.note_primal_weight(omega);
+  ctx.note_primal_weight(pid.omega);
   if (warm) {
     const KktStats kw = ctx.device_kkt(PSH_X, PSH_Y);
     if (kw.finite() && ctx.record(kw, 0)) return ctx.finish(Status::Optimal, PSH_X, PSH_Y, 0, "warm start already optimal");
@@ -561,9 +633,8 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
   dev.check(psh_copy(c, PSH_XH, PSH_X), "copy");
   dev.check(psh_copy(c, PSH_YH, PSH_Y), "copy");
 
-  double pid_integral = 0.0, pid_last = 0.0, best_omega = omega, best_balance = 1e300;
   auto residual = [&](double px, double py, double cross) {
-    const double r2 = omega * px + py / omega + 2.0 * eta * cross;
+    const double r2 = pid.omega * px + py / pid.omega + 2.0 * eta * cross;
     return std::sqrt(std::max(r2, 0.0));
   };
   const int K = opt.check_every;
@@ -576,11 +647,11 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
     psh_chunk_args ch{};
     ch.iterations = K;
     ch.inner0 = inner;
-    ch.tau = eta / omega;
-    ch.sigma = eta * omega;
+    ch.tau = eta / pid.omega;
+    ch.sigma = eta * pid.omega;
     ch.rho = rho;
     ch.eta = eta;
-    ch.omega = omega;
+    ch.omega = pid.omega;
     ch.want_hat_last = 1;
     ch.mode = mode;
     dev.check(psh_chunk(c, &ch), "iteration");
@@ -588,7 +659,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
     it += K;
 
     psh_check_args ca{};
-    ca.omega = omega;
+    ca.omega = pid.omega;
     ca.eta = eta;
     ca.want_ray = it >= 4 * K ? 1 : 0;
     ca.want_drift = it >= 4 * K && inner >= K && (it / K) % 4 == 0 ? 1 : 0;
@@ -614,10 +685,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
     }
     const bool promoted = policy.on_check(k.rel_kkt());
     if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
-    const bool ratios_ok = r0 > 0 && std::isfinite(r0) && r > 0;
-    const bool restart = promoted ||
-                         (ratios_ok && (r <= opt.restart_sufficient * r0 || (r <= opt.restart_necessary * r0 && r > r_last))) ||
-                         static_cast<double>(inner) >= opt.restart_artificial * static_cast<double>(it);
+    const bool restart = promoted || restart_due(opt, r, r0, r_last, inner, it);
     r_last = r;
     if (!restart) continue;
 
@@ -627,27 +695,11 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
       dev.check(psh_diff_norm2(c, PSH_XH, PSH_X0, mode, &dxn), "pid");
       dev.check(psh_diff_norm2(c, PSH_YH, PSH_Y0, mode, &dyn), "pid");
     }
-    const bool both_pos = k.rel_dual() > 0 && k.rel_primal() > 0;
-    const double ratio = both_pos ? k.rel_dual() / k.rel_primal() : 1.0;
-    if (dxn > 1e-16 && dyn > 1e-16 && dxn < 1e12 && dyn < 1e12 && ratio > 1e-8 && ratio < 1e8) {
-      const double e = std::log(dyn) - std::log(dxn) - std::log(omega);
-      pid_integral = opt.pid_integral_decay * pid_integral + e;
-      const double step = opt.pid_kp * e + opt.pid_ki * pid_integral + opt.pid_kd * (e - pid_last);
-      omega *= std::exp(std::clamp(step, -opt.pid_max_log_step, opt.pid_max_log_step));
-      pid_last = e;
-    } else {
-      omega = best_omega;
-      pid_integral = pid_last = 0.0;
-    }
-    const double balance = std::fabs(std::log10(ratio));
-    if (both_pos && balance < best_balance) {
-      best_balance = balance;
-      best_omega = omega;
-    }
-    ctx.note_primal_weight(omega);
+    pid_update(pid, opt, dxn, dyn, k.rel_primal(), k.rel_dual());
+    ctx.note_primal_weight(pid.omega);
     if (opt.verbosity >= 2)
       std::fprintf(stderr, "  restart at %lld (inner %lld): r/r0 %.3f  omega %.4e  dx %.3e dy %.3e\n",
-                   static_cast<long long>(it), static_cast<long long>(inner), r / r0, omega, dxn, dyn);
+                   static_cast<long long>(it), static_cast<long long>(inner), r / r0, pid.omega, dxn, dyn);
     dev.check(psh_restart(c), "restart");
     inner = 0;
     r_last = kInf;

@@ -80,6 +80,22 @@ class Device {
 
 // ‖Ã‖₂ by Lanczos on ÃᵀÃ with the device products (reference src/la/csr.cpp:86-132).
 double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter = 400, double rel_tol = 1e-12);
+// The same iteration with host products and sequential dots (serving setup, on host threads).
+double estimate_norm2_host(const ScaledProblem& sp, int max_iter = 400, double rel_tol = 1e-12);
+
+// The psh_problem view of a scaled problem; `cmin` receives sense·c and must outlive the view.
+psh_problem problem_of(const ScaledProblem& sp, std::vector<double>& cmin);
+
+// Restart test of r²HPDHG (reference src/pdhg/r2hpdhg.cpp:118-122): sufficient, necessary or
+// artificial restart. r0 < 0 means no epoch-start residual yet.
+bool restart_due(const EngineOptions& o, double r, double r0, double r_last, std::int64_t inner, std::int64_t it);
+
+// PID primal weight with damping and the best-balance fallback (reference r2hpdhg.cpp:127-148).
+struct PidState {
+  double omega = 1.0, best_omega = 1.0, best_balance = 1e300, integral = 0.0, last = 0.0;
+};
+void pid_update(PidState& p, const EngineOptions& o, double dx_norm, double dy_norm, double rel_primal,
+                double rel_dual);
 
 Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt);
 

@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["solve_lp", "solve_batch", "Result", "version", "device_count", "library_path", "OPTIMAL", "INFEASIBLE", "UNBOUNDED",
+__all__ = ["solve_lp", "solve_batch", "Server", "Result", "version", "device_count", "library_path", "OPTIMAL", "INFEASIBLE", "UNBOUNDED",
            "ITERATION_LIMIT", "TIME_LIMIT", "NUMERICAL_ERROR", "NOT_SOLVED", "INVALID_ARGUMENT"]
 
 OPTIMAL, INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR, NOT_SOLVED, INVALID_ARGUMENT = range(8)
@@ -80,6 +80,11 @@ class Scenario(ctypes.Structure):  # ps26119_amd_scenario
     _fields_ = [("obj", D), ("col_lower", D), ("col_upper", D), ("row_lower", D), ("row_upper", D)]
 
 
+class ServerOptions(ctypes.Structure):  # ps26119_amd_server_options
+    _fields_ = [("device", ctypes.c_int), ("budget_bytes", ctypes.c_double), ("max_instances", ctypes.c_int),
+                ("check_every", ctypes.c_int), ("setup_threads", ctypes.c_int)]
+
+
 def _load():
     path = os.environ.get("PS26119_LIB", library_path)
     if not os.path.exists(path):
@@ -96,6 +101,17 @@ def _load():
     lib.ps26119_amd_solve_batch.argtypes = model + [ctypes.c_int, ctypes.POINTER(Scenario), ctypes.POINTER(Options),
                                                     ctypes.POINTER(AmdOptions), ctypes.POINTER(CResult), D, D]
     lib.ps26119_amd_solve_batch.restype = ctypes.c_int
+    lib.ps26119_amd_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.c_char_p, ctypes.c_int]
+    lib.ps26119_amd_server_create.restype = ctypes.c_void_p
+    lib.ps26119_amd_server_destroy.argtypes = [ctypes.c_void_p]
+    lib.ps26119_amd_server_submit.argtypes = [ctypes.c_void_p] + model + [ctypes.POINTER(Options),
+                                                                          ctypes.POINTER(AmdOptions)]
+    lib.ps26119_amd_server_submit.restype = ctypes.c_longlong
+    lib.ps26119_amd_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_double,
+                                            ctypes.POINTER(CResult), D, D, D, D]
+    lib.ps26119_amd_server_wait.restype = ctypes.c_int
+    lib.ps26119_amd_server_in_flight.argtypes = [ctypes.c_void_p]
+    lib.ps26119_amd_server_in_flight.restype = ctypes.c_int
     lib.ps26119_default_options.argtypes = [ctypes.POINTER(Options)]
     lib.ps26119_amd_default_options.argtypes = [ctypes.POINTER(AmdOptions)]
     lib.ps26119_solve_mps.argtypes = [ctypes.c_char_p, ctypes.POINTER(Options), ctypes.POINTER(CResult),
@@ -354,3 +370,84 @@ def check_solution(lp, x, y) -> dict:
     d = dict(zip(keys, out[:5].tolist()))
     d["passed"] = bool(out[5])
     return d
+
+
+class Server:
+    """Continuous batching of LP solves on one GPU (ps26119_amd_server_*).
+
+        with ps26119.Server(budget_bytes=8 << 30) as srv:
+            ids = [srv.submit(lp, tolerance=1e-6) for lp in models]     # returns at once
+            results = [srv.wait(i) for i in ids]                          # Result with .latency
+
+    Requests are prepared on host threads, admitted while the HBM budget has room, advanced
+    together as one packed block-diagonal system and finished on their own schedule."""
+
+    def __init__(self, budget_bytes=8 << 30, max_instances=256, check_every=0, setup_threads=4, device=0):
+        L = lib()
+        o = ServerOptions(device, float(budget_bytes), max_instances, check_every, setup_threads)
+        err = ctypes.create_string_buffer(512)
+        self._h = L.ps26119_amd_server_create(ctypes.byref(o), err, 512)
+        if not self._h:
+            raise RuntimeError(err.value.decode() or "server could not be created")
+        self._dims = {}
+        self._keep = {}
+
+    def submit(self, lp, tolerance=1e-8, iteration_limit=0, time_limit=3600.0, verbosity=0, params=None) -> int:
+        L = lib()
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (lp.c, lp.col_lower, lp.col_upper, lp.row_lower, lp.row_upper, lp.value)]
+        ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (lp.col_start, lp.row_index)]
+        c, cl, cu, rl, ru, v = arrs
+        cs, ri = ints
+        keep = arrs + ints
+        opt = _options(L, "r2hpdhg", "fp64", True, tolerance, time_limit, iteration_limit, verbosity, 1, None, None)
+        ext = _ext(L, False, 0, params, keep)
+        rid = L.ps26119_amd_server_submit(self._h, lp.m, lp.n, int(lp.sense), float(lp.offset), _ptr(c), _ptr(cl),
+                                          _ptr(cu), _ptr(rl), _ptr(ru), _ptr(cs, ctypes.c_int),
+                                          _ptr(ri, ctypes.c_int), _ptr(v), ctypes.byref(opt), ctypes.byref(ext))
+        if rid < 0:
+            raise ValueError("server rejected the request arguments")
+        self._dims[rid] = (lp.m, lp.n)
+        return rid
+
+    def wait(self, rid, timeout=-1.0):
+        """The request's Result (with .stats = {submitted, admitted, done, latency}); None on timeout."""
+        L = lib()
+        m, n = self._dims[rid]
+        res = CResult()
+        x, y, z = np.full(n, np.nan), np.full(m, np.nan), np.full(n, np.nan)
+        times = np.zeros(4)
+        rc = L.ps26119_amd_server_wait(self._h, rid, float(timeout), ctypes.byref(res), _ptr(x), _ptr(y), _ptr(z),
+                                       _ptr(times))
+        if rc == 1:
+            return None
+        if rc != 0:
+            raise KeyError(rid)
+        del self._dims[rid]
+        st = res.status
+        have = st in (OPTIMAL, ITERATION_LIMIT, TIME_LIMIT)
+        return Result(st, _STATUS[st] if 0 <= st < len(_STATUS) else str(st), res.objective, res.dual_objective,
+                      res.primal_residual, res.dual_residual, res.gap, res.certified_bound, res.iterations,
+                      res.seconds, res.engine.decode(), res.message.decode(), _CHECK.get(res.check, ""),
+                      x if have else None, y if have else None, z if have else None, None, None,
+                      {"submitted": times[0], "admitted": times[1], "done": times[2], "latency": times[3]})
+
+    def in_flight(self) -> int:
+        return lib().ps26119_amd_server_in_flight(self._h)
+
+    def close(self):
+        if self._h:
+            lib().ps26119_amd_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

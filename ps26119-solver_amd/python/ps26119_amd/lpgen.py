@@ -126,6 +126,18 @@ def scenario_set(base: Lp, k: int, sigma: float = 0.05, seed: int = 7):
     return out
 
 
+def with_scenario(base: Lp, sc: dict) -> Lp:
+    """The model a scenario describes: `base` with any of obj, col_lower, col_upper, row_lower,
+    row_upper replaced."""
+    import copy
+    m = copy.copy(base)
+    for key, attr in (("obj", "c"), ("col_lower", "col_lower"), ("col_upper", "col_upper"),
+                      ("row_lower", "row_lower"), ("row_upper", "row_upper")):
+        if key in sc:
+            setattr(m, attr, np.asarray(sc[key], dtype=np.float64))
+    return m
+
+
 def perturbation_scenarios(base: Lp):
     """The four scenarios of the reference's batch test (tests/unit/test_batch.cpp:28-42 there):
     unchanged; objective scaled by 1 + 0.05·((j mod 3) − 1); finite, non-equality row upper
