@@ -101,14 +101,15 @@ class Checker:
         self._check = fn("check_solution", [L, D, D, D])
 
     def solve(self, lp, precision=0, presolve=True, tolerance=1e-8, iteration_limit=0, time_limit=0.0,
-              threads=1, check_every=0, warm_x=None, warm_y=None, warm_primal_weight=0.0, params=None):
+              threads=1, check_every=0, warm_x=None, warm_y=None, warm_primal_weight=0.0, params=None, algorithm=3):
+        """algorithm: 3 = r²HPDHG, 2 = PDLP (the reference build only)."""
         h = LpHandle(lp)
         names = [k.encode() for k in (params or {})]
         vals = np.array(list((params or {}).values()), dtype=np.float64)
         name_arr = (ctypes.c_char_p * max(1, len(names)))(*names)
         wx = None if warm_x is None else np.ascontiguousarray(warm_x, dtype=np.float64)
         wy = None if warm_y is None else np.ascontiguousarray(warm_y, dtype=np.float64)
-        o = OrcOpts(3, precision, int(presolve), threads, check_every, tolerance, time_limit, iteration_limit,
+        o = OrcOpts(algorithm, precision, int(presolve), threads, check_every, tolerance, time_limit, iteration_limit,
                     _p(wx), _p(wy), warm_primal_weight, len(names), name_arr, _p(vals) if len(names) else None)
         r = OrcResult()
         x, y, z = np.full(lp.n, np.nan), np.full(lp.m, np.nan), np.full(lp.n, np.nan)

@@ -47,6 +47,8 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "pid_kd") e.pid_kd = v;
     else if (name == "pid_integral_decay") e.pid_integral_decay = v;
     else if (name == "pid_max_log_step") e.pid_max_log_step = v;
+    else if (name == "pdlp_restart_necessary") e.pdlp_restart_necessary = v;
+    else if (name == "pdlp_primal_weight_smoothing") e.pdlp_primal_weight_smoothing = v;
     else if (name == "bound_objective_rescaling") e.scaling.bound_objective_rescaling = v != 0;
     else if (name == "geometric_mean_iterations") e.scaling.geometric_mean_iterations = static_cast<int>(v);
     else if (name == "geometric_mean_min_log10_range") e.scaling.geometric_mean_min_log10_range = v;
@@ -487,9 +489,10 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
     return true;
   }
 
+  void set_engine_name(const char* name) { engine_name_ = name; }
   Solution finish(Status status, int xs_slot, int ys_slot, std::int64_t iterations, const std::string& message) {
     Solution sol;
-    sol.engine = "r2hpdhg";
+    sol.engine = engine_name_;
     sol.precision = to_string(opt_.precision);
     sol.iterations = iterations;
     sol.message = message;
@@ -550,6 +553,7 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   std::int64_t fast_iterations_ = -1;
   std::chrono::steady_clock::time_point t0_;
   std::vector<double> dual_ray_, primal_ray_;
+  const char* engine_name_ = "r2hpdhg";
 };
 
 // Mixed precision. With refine = 0 the reference's policy (engine.cpp:56-79): fp32 storage until
@@ -703,6 +707,126 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
     dev.check(psh_restart(c), "restart");
     inner = 0;
     r_last = kInf;
+  }
+}
+
+Solution solve_pdlp(const Model& model, const EngineOptions& opt_in) {
+  EngineOptions opt = opt_in;
+  opt.refine = 0;  // the increment form lives in the fused chunk; PDLP keeps the reference's policy
+  Context ctx(model, opt);
+  ctx.set_engine_name("pdlp");
+  if (!ctx.ok()) {
+    Solution s;
+    s.engine = "pdlp";
+    s.status = Status::NotSolved;
+    s.message = ctx.error();
+    return s;
+  }
+  Device& dev = ctx.dev();
+  psh_ctx* c = ctx.ctx();
+  PrecisionPolicy policy(ctx, opt);
+  const double eta = ctx.eta();
+  const int mode = ctx.mode();
+  double omega = ctx.initial_primal_weight();
+  // iterate, next iterate, averages, restart anchors, work vectors (reference pdlp.cpp:24-28)
+  int x = PSH_X, xn = PSH_XH, xa = PSH_XB, x0 = PSH_X0, xbar = PSH_DX, aty = PSH_ATY;
+  int y = PSH_Y, yn = PSH_YH, ya = PSH_YB, y0 = PSH_Y0, ax = PSH_AX;
+  auto ok = [&](int rc, const char* what) { dev.check(rc, what); };
+  ok(psh_fill(c, xbar, 0.0), "fill");
+  ok(psh_fill(c, aty, 0.0), "fill");
+  ok(psh_primal_step(c, xbar, aty, 0.0, x), "init");
+  ok(psh_fill(c, y, 0.0), "fill");
+  bool promoted_by_warm = false;
+  const bool warm = ctx.apply_warm_start(policy.fp32(), promoted_by_warm);
+  ctx.note_primal_weight(omega);
+  if (warm) {
+    const KktStats kw = ctx.device_kkt(x, y);
+    if (kw.finite() && ctx.record(kw, 0)) return ctx.finish(Status::Optimal, x, y, 0, "warm start already optimal");
+  }
+  ok(psh_copy(c, x0, x), "copy");
+  ok(psh_copy(c, y0, y), "copy");
+  ok(psh_copy(c, xa, x), "copy");
+  ok(psh_copy(c, ya, y), "copy");
+  const KktStats k0 = ctx.device_kkt(x, y);
+  ctx.record(k0, 0);
+  double restart_kkt = k0.rel_kkt(), last_candidate = 1e300;
+  std::int64_t it = 0, inner = 0;
+  const int K = opt.check_every;
+  std::string msg = warm ? "warm start" : "";
+  for (;;) {
+    const double tau = eta / omega, sigma = eta * omega;
+    for (int s = 0; s < K; ++s) {
+      ok(psh_spmv_t(c, y, aty), "iteration");
+      ok(psh_primal_step(c, x, aty, tau, xn), "iteration");
+      ok(psh_copy(c, xbar, xn), "iteration");
+      ok(psh_axpby(c, -1.0, x, 2.0, xbar), "iteration");  // x̄ = 2x⁺ − x
+      ok(psh_spmv(c, xbar, ax), "iteration");
+      ok(psh_dual_step(c, y, ax, sigma, yn), "iteration");
+      std::swap(x, xn);
+      std::swap(y, yn);
+      ++inner;
+      const double wa = 1.0 / static_cast<double>(inner);
+      ok(psh_axpby(c, wa, x, 1.0 - wa, xa), "iteration");
+      ok(psh_axpby(c, wa, y, 1.0 - wa, ya), "iteration");
+      ++it;
+      if (s + 1 < K && ctx.out_of_time()) {
+        const KktStats kt = ctx.device_kkt(x, y);
+        if (kt.finite() && ctx.record(kt, it)) return ctx.finish(Status::Optimal, x, y, it, msg);
+        return ctx.finish(Status::TimeLimit, x, y, it, msg);
+      }
+    }
+    const KktStats kc = ctx.device_kkt(x, y), ka = ctx.device_kkt(xa, ya);
+    const bool use_avg = ka.finite() && ka.rel_kkt() < kc.rel_kkt();
+    const KktStats& kk = use_avg ? ka : kc;
+    const int cx = use_avg ? xa : x, cy = use_avg ? ya : y;
+    if (!kk.finite()) return ctx.finish(Status::NumericalError, x, y, it, "non-finite KKT statistics");
+    if (ctx.record(kk, it)) return ctx.finish(Status::Optimal, cx, cy, it, msg);
+    if (it >= opt.iteration_limit) return ctx.finish(Status::IterationLimit, cx, cy, it, msg);
+    if (ctx.out_of_time()) return ctx.finish(Status::TimeLimit, cx, cy, it, msg);
+    if (it >= 4 * K) {  // Δ = z_k − z_{k−1}; xn / yn hold z_{k−1} after the swap
+      ok(psh_axpby(c, 1.0, x, -1.0, xn), "ray");
+      ok(psh_axpby(c, 1.0, y, -1.0, yn), "ray");
+      std::string why;
+      psh_ray ray{};
+      ok(psh_ray_of(c, xn, yn, &ray), "ray");
+      Status st = ctx.ray_verdict(ray, false, kc, why);
+      if (st == Status::NotSolved && inner >= K && (it / K) % 4 == 0) {  // the drift since the anchor
+        ok(psh_copy(c, xn, x), "ray");
+        ok(psh_axpby(c, -1.0, x0, 1.0, xn), "ray");
+        ok(psh_copy(c, yn, y), "ray");
+        ok(psh_axpby(c, -1.0, y0, 1.0, yn), "ray");
+        ok(psh_ray_of(c, xn, yn, &ray), "ray");
+        st = ctx.ray_verdict(ray, false, kc, why);
+        if (st != Status::NotSolved) why += " (drift since the restart anchor)";
+      }
+      if (st != Status::NotSolved) return ctx.finish(st, x, y, it, why);
+    }
+    const double e = kk.rel_kkt();
+    const bool promoted = policy.on_check(e);
+    if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
+    const bool restart = promoted || e <= 0.2 * restart_kkt ||
+                         (e <= opt.pdlp_restart_necessary * restart_kkt && e > last_candidate) ||
+                         static_cast<double>(inner) >= 0.36 * static_cast<double>(it);
+    last_candidate = e;
+    if (restart) {
+      double dx = 0, dy = 0;
+      ok(psh_diff_norm2(c, cx, x0, mode, &dx), "pid");
+      ok(psh_diff_norm2(c, cy, y0, mode, &dy), "pid");
+      if (dx > 1e-10 && dy > 1e-10) {
+        const double th = opt.pdlp_primal_weight_smoothing;
+        omega = std::exp(th * std::log(dy / dx) + (1.0 - th) * std::log(omega));
+        ctx.note_primal_weight(omega);
+      }
+      if (cx != x) ok(psh_copy(c, x, cx), "restart");
+      if (cy != y) ok(psh_copy(c, y, cy), "restart");
+      ok(psh_copy(c, x0, x), "restart");
+      ok(psh_copy(c, y0, y), "restart");
+      ok(psh_copy(c, xa, x), "restart");
+      ok(psh_copy(c, ya, y), "restart");
+      inner = 0;
+      restart_kkt = e;
+      last_candidate = 1e300;
+    }
   }
 }
 

@@ -29,6 +29,8 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   double restart_sufficient = 0.2, restart_necessary = 0.5, restart_artificial = 0.36;
   double pid_kp = 0.99, pid_ki = 0.01, pid_kd = 0.0, pid_integral_decay = 0.3;
   double pid_max_log_step = 2.302585092994046;
+  double pdlp_restart_necessary = 0.8;        // PDLP engine (reference engine.h:46-47)
+  double pdlp_primal_weight_smoothing = 0.5;  // θ
   // MI355X knobs (engine_params names hip_cpu_order, hip_device):
   //   cpu_order = 1: every reduction in the reference CPU order (bit-identical trajectory to the
   //   reference CPU backend, slower checks); 0: fixed-shape device trees for the KKT sums and the
@@ -98,5 +100,8 @@ void pid_update(PidState& p, const EngineOptions& o, double dx_norm, double dy_n
                 double rel_dual);
 
 Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt);
+// PDLP-style restarted PDHG with iterate averaging on the same device operations (reference
+// src/pdhg/pdlp.cpp:9-126): unfused Backend ops, KKT of the current and the averaged point.
+Solution solve_pdlp(const Model& model, const EngineOptions& opt);
 
 }  // namespace ps26119::amd

@@ -84,7 +84,7 @@ Solution solve_direct(const Model& model, const Options& options, bool certify) 
       sol.message = "unknown engine parameter '" + bad + "'";
       return sol;
     }
-    sol = solve_r2hpdhg(model, eo);
+    sol = options.algorithm == Algorithm::Pdlp ? solve_pdlp(model, eo) : solve_r2hpdhg(model, eo);
     if (sol.status == Status::Unbounded && !check_unboundedness_certificate(model, sol.x, sol.primal_ray).passed) {
       // a valid ray from a point that is not primal-feasible: find a feasible point with a
       // zero-objective solve within the remaining budget (reference solve.cpp:552-582)
@@ -313,11 +313,12 @@ void gate(const Model& model, const Options& options, Solution& s) {  // referen
 Solution solve(const Model& model, const Options& options) {
   try {
     Solution s;
-    if (options.algorithm != Algorithm::Auto && options.algorithm != Algorithm::R2hpdhg) {
+    if (options.algorithm != Algorithm::Auto && options.algorithm != Algorithm::R2hpdhg &&
+        options.algorithm != Algorithm::Pdlp) {
       s.status = Status::NotSolved;
       s.engine = to_string(options.algorithm);
-      s.message = std::string("the MI355X backend runs r2hpdhg only; algorithm '") + to_string(options.algorithm) +
-                  "' is served by the reference CPU library";
+      s.message = std::string("the MI355X backend runs the first-order engines (r2hpdhg, pdlp); algorithm '") +
+                  to_string(options.algorithm) + "' is served by the reference CPU library";
       s.model_fingerprint = model.fingerprint_hex();
       return s;
     }
@@ -338,7 +339,7 @@ Solution solve(const Model& model, const Options& options) {
     }
     Options resolved = options;
     const bool was_auto = options.algorithm == Algorithm::Auto;
-    resolved.algorithm = Algorithm::R2hpdhg;  // a GPU request is always served by r2hpdhg
+    if (was_auto) resolved.algorithm = Algorithm::R2hpdhg;  // a GPU request defaults to r2hpdhg
     set_host_threads(options.threads);
     s = solve_impl(model, resolved);
     gate(model, resolved, s);
@@ -348,7 +349,7 @@ Solution solve(const Model& model, const Options& options) {
       std::snprintf(buf, sizeof buf, "auto: r2hpdhg (rows*nnz = %.2g)", work);
       s.message = s.message.empty() ? buf : std::string(buf) + "; " + s.message;
     }
-    if (s.engine.empty()) s.engine = "r2hpdhg";
+    if (s.engine.empty()) s.engine = resolved.algorithm == Algorithm::Pdlp ? "pdlp" : "r2hpdhg";
     return s;
   } catch (const std::bad_alloc&) {
     Solution s;

@@ -96,6 +96,14 @@ def main():
             key = f"random10000_s1|presolve={int(pre)}|fp64"
             solves[key] = solve_case(ref, lp, presolve=pre, precision=0)
             print(key, solves[key]["status"], solves[key]["iterations"], flush=True)
+    pdlp = {}
+    for name in ("afiro", "sc50a", "sc105", "stocfor1", "wyndor", "random300_s3"):
+        for pre in (True, False):
+            for prec in (0, 1):
+                key = f"{name}|presolve={int(pre)}|{'mixed' if prec else 'fp64'}"
+                pdlp[key] = solve_case(ref, models[name], presolve=pre, precision=prec, iteration_limit=3_000_000,
+                                       algorithm=2)
+                print("pdlp", key, pdlp[key]["status"], pdlp[key]["iterations"], flush=True)
     batches = {}
     for name in ("afiro", "sc50a", "stocfor1", "share2b", "random300_s3"):
         lp = models[name]
@@ -129,7 +137,7 @@ def main():
                 solves[k] = v
     with open(out_path, "w") as f:
         json.dump({"generator": "tests/golden/make_golden.py", "solves": solves, "kernels": kernels,
-                   "batches": batches}, f, indent=1,
+                   "batches": batches, "pdlp": pdlp}, f, indent=1,
                   sort_keys=True)
     print("wrote", out_path)
 

@@ -192,3 +192,37 @@ def test_refined_mixed_on_config0(gpu_available):
           f"planted={lp.optimum!r}; {r.message}", flush=True)
     assert r.status == ps26119.OPTIMAL and r.check == "PASS"
     assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
+
+
+PDLP_CASES = ["afiro", "sc50a", "sc105", "stocfor1", "wyndor", "random300_s3"]
+
+
+@pytest.mark.parametrize("name", PDLP_CASES)
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("presolve", [1, 0])
+def test_pdlp_cpu_order_trajectory_is_the_reference(gpu_available, golden, name, prec, presolve):
+    """The PDLP engine (reference src/pdhg/pdlp.cpp) on the same device operations: with CPU-order
+    reductions its trajectory is the reference's (fixtures from the reference build)."""
+    import ps26119
+    lp = _model(name)
+    exp = golden["pdlp"][f"{name}|presolve={presolve}|{prec}"]
+    r = ps26119.solve_model(lp, algorithm="pdlp", precision=prec, presolve=bool(presolve),
+                            params={"hip_cpu_order": 1}, stats=True, iteration_limit=3_000_000)
+    assert r.status == exp["status"], r.message
+    assert r.iterations == exp["iterations"]
+    assert float(r.objective).hex() == exp["objective"]
+    assert float(r.stats["primal_weight"]).hex() == exp["primal_weight"]
+    if exp["x_hash"] is not None:
+        assert fnv(r.x) == exp["x_hash"] and fnv(r.y) == exp["y_hash"]
+    assert r.engine == "pdlp"
+
+
+@pytest.mark.parametrize("name", PDLP_CASES)
+def test_pdlp_tree_reductions_reach_the_answer(gpu_available, golden, name):
+    import ps26119
+    lp = _model(name)
+    exp = golden["pdlp"][f"{name}|presolve=1|fp64"]
+    r = ps26119.solve_model(lp, algorithm="pdlp", iteration_limit=3_000_000)
+    ref_obj = float.fromhex(exp["objective"])
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    assert abs(r.objective - ref_obj) <= 1e-7 * (1 + abs(ref_obj))

@@ -250,3 +250,14 @@ def test_verdicts_certified(oracle, name, status):
     r, v = oracle.solve(lp)
     assert r["status"] == status
     assert r["check"] == 1
+
+
+def test_pdlp_fixtures_match_live_reference(ref, golden):
+    """The PDLP fixtures are the reference build's own outputs (reproducible here)."""
+    import itertools
+    for name, pre in itertools.product(["afiro", "stocfor1"], (1, 0)):
+        exp = golden["pdlp"][f"{name}|presolve={pre}|fp64"]
+        r, v = ref.solve(_model(name), presolve=bool(pre), algorithm=2, iteration_limit=3_000_000)
+        assert r["iterations"] == exp["iterations"]
+        assert float(r["objective"]).hex() == exp["objective"]
+        assert fnv(v["x"]) == exp["x_hash"]
