@@ -33,6 +33,7 @@ typedef struct {
   long long kernel_launches[3]; /* hip_timing: fused primal, fused dual, check pass */
   double kernel_ms[3];
   double device_bytes;          /* device memory of the solve */
+  double iterate_seconds;       /* wall time of the iteration loop (setup and finish excluded) */
 } ps26119_amd_stats;
 
 void ps26119_amd_default_options(ps26119_amd_options* o);

@@ -553,7 +553,7 @@ void psh_destroy(psh_ctx* c) {
                   c->At.tiles, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
-                  c->lvp, c->lw, c->lt, c->r0_dev, c->prm};
+                  c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
@@ -622,8 +622,14 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
   c->term_len = mx;
   if ((rc = alloc_n(c, &c->xo, n)) || (rc = alloc_n(c, &c->yo, m)) || (rc = alloc_n(c, &c->rv, m)) ||
       (rc = alloc_n(c, &c->dv, n)) || (rc = alloc_n(c, &c->rv2, m)) || (rc = alloc_n(c, &c->dv2, n)) ||
-      (rc = alloc_n(c, &c->terms, mx * kTerms)))
+      (rc = alloc_n(c, &c->terms, mx * kTerms)) ||
+      (rc = alloc_n(c, &c->mpart_m, static_cast<long long>(grid_for(m)) * kMaxCount)) ||
+      (rc = alloc_n(c, &c->mpart_n, static_cast<long long>(grid_for(n)) * kMaxCount)))
     return rc;
+  hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(static_cast<long long>(grid_for(m)) * kMaxCount)), dim3(kBlock), 0,
+                     c->stream, c->mpart_m, 0.0, static_cast<long long>(grid_for(m)) * kMaxCount);
+  hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(static_cast<long long>(grid_for(n)) * kMaxCount)), dim3(kBlock), 0,
+                     c->stream, c->mpart_n, 0.0, static_cast<long long>(grid_for(n)) * kMaxCount);
   PSH_TRY(c, hipStreamSynchronize(c->stream));  // host arrays may be freed after return
   for (int s = 0; s < PSH_NSLOT; ++s)
     if (slot_len(c, s) > 0)

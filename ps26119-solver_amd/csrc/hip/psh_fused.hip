@@ -251,12 +251,21 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// Order-independent maximum of non-negative values (NaN ignored, as std::max(acc, NaN) does).
-// Every lane of the wavefront must reach this call.
-__device__ inline void max_into(unsigned long long* slot, double v) {
+// Maximum of non-negative values over the workgroup (NaN ignored, as std::max(acc, NaN) does),
+// stored as this workgroup's partial for quantity q; k_max_final combines the partials. Every
+// thread of the workgroup must reach this call (no atomics: one store per workgroup).
+__device__ inline void block_max_into(double* part, int q, double v) {
+  __shared__ double wm[kBlock / 64];
   v = v >= 0.0 ? v : 0.0;
   v = wave_max(v);
-  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(slot, static_cast<unsigned long long>(__double_as_longlong(v)));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mx = wm[0];
+    for (int k = 1; k < kBlock / 64; ++k) mx = fmax(mx, wm[k]);
+    part[static_cast<size_t>(blockIdx.x) * kMaxCount + q] = mx;
+  }
+  __syncthreads();
 }
 
 __device__ inline double dmaxd(double a, double b) { return a < b ? b : a; }  // std::max
@@ -290,7 +299,7 @@ struct CheckArgs {
   double *xo, *yo, *rv, *dv, *rv2, *dv2;
   double* terms;
   long long tl;
-  unsigned long long* maxes;
+  double *mpart_m, *mpart_n;  // per-workgroup maxima of the row-side (m) and column-side (n) passes
 };
 
 template <class T>
@@ -349,8 +358,8 @@ __global__ void __launch_bounds__(kBlock) k_check_n(CheckArgs<T> a) {
       term(a, T_PIDX)[j] = t * t;
     }
   }
-  if (a.flags & F_RAY) max_into(a.maxes + M_DMAXRAY, dmax_ray);
-  if (a.flags & F_DRIFT) max_into(a.maxes + M_DMAXRAY2, dmax_ray2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_n, M_DMAXRAY, dmax_ray);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_n, M_DMAXRAY2, dmax_ray2);
 }
 
 template <class T>
@@ -391,8 +400,8 @@ __global__ void __launch_bounds__(kBlock) k_check_m(CheckArgs<T> a) {
       term(a, T_PIDY)[i] = t * t;
     }
   }
-  if (a.flags & F_RAY) max_into(a.maxes + M_RMAX, rmax);
-  if (a.flags & F_DRIFT) max_into(a.maxes + M_RMAX2, rmax2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_m, M_RMAX, rmax);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_RMAX2, rmax2);
 }
 
 // One thread per row of A (Ã and the original A share the structure).
@@ -445,11 +454,11 @@ __global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a) {
     }
   }
   if (a.flags & F_KKT) {
-    max_into(a.maxes + M_PMAX, pmax);
-    max_into(a.maxes + M_DMAX, dmax);
+    block_max_into(a.mpart_m, M_PMAX, pmax);
+    block_max_into(a.mpart_m, M_DMAX, dmax);
   }
-  if (a.flags & F_RAY) max_into(a.maxes + M_PVIOL, pviol);
-  if (a.flags & F_DRIFT) max_into(a.maxes + M_PVIOL2, pviol2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_m, M_PVIOL, pviol);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_PVIOL2, pviol2);
 }
 
 // Dual-ray column terms of λ = −Aᵀr (engine.cpp:98-119).
@@ -519,12 +528,29 @@ __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a) {
     }
   }
   if (a.flags & F_KKT) {
-    max_into(a.maxes + M_PMAX, pmax);
-    max_into(a.maxes + M_DMAX, dmax);
-    max_into(a.maxes + M_CINF, cinf);
+    block_max_into(a.mpart_n, M_PMAX, pmax);
+    block_max_into(a.mpart_n, M_DMAX, dmax);
+    block_max_into(a.mpart_n, M_CINF, cinf);
   }
-  if (a.flags & F_RAY) max_into(a.maxes + M_DVIOL, dviol);
-  if (a.flags & F_DRIFT) max_into(a.maxes + M_DVIOL2, dviol2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_n, M_DVIOL, dviol);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_n, M_DVIOL2, dviol2);
+}
+
+// Maxima of the per-workgroup partials of both sides into out[0 .. kMaxCount).
+__global__ void __launch_bounds__(kBlock) k_max_final(const double* __restrict__ pm, int nbm, const double* __restrict__ pn,
+                                                      int nbn, double* __restrict__ out) {
+  __shared__ double red[kBlock];
+  const int q = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nbm; b += kBlock) v = fmax(v, pm[static_cast<size_t>(b) * kMaxCount + q]);
+  for (int b = threadIdx.x; b < nbn; b += kBlock) v = fmax(v, pn[static_cast<size_t>(b) * kMaxCount + q]);
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = red[0];
 }
 
 __global__ void k_zero(double* p, int n) {
@@ -577,15 +603,15 @@ CheckArgs<T> make_args(psh_ctx* c, unsigned flags, int xh, int xb, int x0, int d
   a.dv2 = c->dv2;
   a.terms = c->terms;
   a.tl = c->term_len;
-  a.maxes = reinterpret_cast<unsigned long long*>(c->results + R_MAX0);
+  a.mpart_m = c->mpart_m;
+  a.mpart_n = c->mpart_n;
   return a;
 }
 
 // Launches the four check kernels for `flags`; the dual-side pass runs before the row pass
 // because the row pass reads Δy, ŷ (unscaled) and the dual ray.
 template <class T>
-int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool zero_maxes) {
-  if (zero_maxes) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c->stream, c->results + R_MAX0, static_cast<int>(kMaxCount));
+int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool maxes) {
   const unsigned nf = a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT | F_PID);
   const unsigned mf = nf;
   if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
@@ -594,6 +620,9 @@ int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool zero_maxes) {
     hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
   if ((a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->n > 0)
     hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
+  if (maxes)
+    hipLaunchKernelGGL(k_max_final, dim3(kMaxCount), dim3(kBlock), 0, c->stream, c->mpart_m,
+                       c->m > 0 ? grid_for(c->m) : 0, c->mpart_n, c->n > 0 ? grid_for(c->n) : 0, c->results + R_MAX0);
   PSH_LAUNCHED(c, "check kernels");
   return PSH_OK;
 }

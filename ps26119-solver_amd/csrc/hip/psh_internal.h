@@ -135,6 +135,7 @@ struct psh_ctx {
   double *rv = nullptr, *dv = nullptr;          // dual / primal ray of (Δx, Δy)
   double *rv2 = nullptr, *dv2 = nullptr;        // rays of the drift
   double* terms = nullptr;                      // kTerms arrays of max(m, n)
+  double *mpart_m = nullptr, *mpart_n = nullptr; // per-workgroup maxima of the check passes
   long long term_len = 0;
   double* partials = nullptr;                   // tree first level
   double* results = nullptr;                    // device results

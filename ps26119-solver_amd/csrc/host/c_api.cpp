@@ -170,6 +170,7 @@ int solve_common(int num_rows, int num_cols, int sense, double obj_offset, const
       stats->check_gap = s.check_gap;
       for (int k = 0; k < 3; ++k) stats->kernel_launches[k] = s.kernel_launches[k], stats->kernel_ms[k] = s.kernel_ms[k];
       stats->device_bytes = s.device_bytes;
+      stats->iterate_seconds = s.iterate_seconds;
     }
     return result->status;
   });

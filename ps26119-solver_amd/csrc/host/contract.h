@@ -99,6 +99,7 @@ struct Solution {
   long long kernel_launches[3] = {0, 0, 0};
   double kernel_ms[3] = {0, 0, 0};
   double device_bytes = 0;
+  double iterate_seconds = 0;  // wall time of the iteration loop (chunks, checks, restarts)
 };
 
 // Shared-matrix scenario (reference include/ps26119/batch.h:23-29); empty = same as the base.

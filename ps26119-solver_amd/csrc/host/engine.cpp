@@ -490,8 +490,11 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   }
 
   void set_engine_name(const char* name) { engine_name_ = name; }
+  void loop_started() { loop_t0_ = elapsed(); }
   Solution finish(Status status, int xs_slot, int ys_slot, std::int64_t iterations, const std::string& message) {
+    const double loop_end = elapsed();
     Solution sol;
+    sol.iterate_seconds = loop_t0_ >= 0 ? loop_end - loop_t0_ : 0.0;
     sol.engine = engine_name_;
     sol.precision = to_string(opt_.precision);
     sol.iterations = iterations;
@@ -554,6 +557,7 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   std::chrono::steady_clock::time_point t0_;
   std::vector<double> dual_ray_, primal_ray_;
   const char* engine_name_ = "r2hpdhg";
+  double loop_t0_ = -1;
 };
 
 // Mixed precision. With refine = 0 the reference's policy (engine.cpp:56-79): fp32 storage until
@@ -646,6 +650,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
   double r0 = -1.0, r_last = kInf;
   std::string msg = warm ? "warm start" : "";
   if (policy.refining()) msg += std::string(msg.empty() ? "" : "; ") + "fp32 increments on fp64 anchors";
+  ctx.loop_started();
 
   for (;;) {
     psh_chunk_args ch{};
@@ -753,6 +758,7 @@ Solution solve_pdlp(const Model& model, const EngineOptions& opt_in) {
   std::int64_t it = 0, inner = 0;
   const int K = opt.check_every;
   std::string msg = warm ? "warm start" : "";
+  ctx.loop_started();
   for (;;) {
     const double tau = eta / omega, sigma = eta * omega;
     for (int s = 0; s < K; ++s) {

@@ -66,14 +66,14 @@ class AmdStats(ctypes.Structure):  # ps26119_amd_stats
                 ("seconds_to_fast", ctypes.c_double), ("primal_weight", ctypes.c_double),
                 ("check_primal", ctypes.c_double), ("check_dual", ctypes.c_double), ("check_gap", ctypes.c_double),
                 ("kernel_launches", ctypes.c_longlong * 3), ("kernel_ms", ctypes.c_double * 3),
-                ("device_bytes", ctypes.c_double)]
+                ("device_bytes", ctypes.c_double), ("iterate_seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {"setup_seconds": self.setup_seconds, "iterations_to_fast": self.iterations_to_fast,
                 "seconds_to_fast": self.seconds_to_fast, "primal_weight": self.primal_weight,
                 "check_primal": self.check_primal, "check_dual": self.check_dual, "check_gap": self.check_gap,
                 "kernel_launches": list(self.kernel_launches), "kernel_ms": list(self.kernel_ms),
-                "device_bytes": self.device_bytes}
+                "device_bytes": self.device_bytes, "iterate_seconds": self.iterate_seconds}
 
 
 class Scenario(ctypes.Structure):  # ps26119_amd_scenario
