@@ -622,14 +622,19 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
   c->term_len = mx;
   if ((rc = alloc_n(c, &c->xo, n)) || (rc = alloc_n(c, &c->yo, m)) || (rc = alloc_n(c, &c->rv, m)) ||
       (rc = alloc_n(c, &c->dv, n)) || (rc = alloc_n(c, &c->rv2, m)) || (rc = alloc_n(c, &c->dv2, n)) ||
-      (rc = alloc_n(c, &c->terms, mx * kTerms)) ||
-      (rc = alloc_n(c, &c->mpart_m, static_cast<long long>(grid_for(m)) * kMaxCount)) ||
-      (rc = alloc_n(c, &c->mpart_n, static_cast<long long>(grid_for(n)) * kMaxCount)))
+      (rc = alloc_n(c, &c->terms, mx * kTerms)))
     return rc;
-  hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(static_cast<long long>(grid_for(m)) * kMaxCount)), dim3(kBlock), 0,
-                     c->stream, c->mpart_m, 0.0, static_cast<long long>(grid_for(m)) * kMaxCount);
-  hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(static_cast<long long>(grid_for(n)) * kMaxCount)), dim3(kBlock), 0,
-                     c->stream, c->mpart_n, 0.0, static_cast<long long>(grid_for(n)) * kMaxCount);
+  // per-workgroup maxima: the elementwise passes run grid_for(rows) workgroups, the row / column
+  // passes one per tile (padded to the XCD count)
+  c->mpart_rows_m = std::max(m > 0 ? grid_for(m) : 0, c->A.ntiles > 0 ? grid_for_tiles(c->A.ntiles) : 0);
+  c->mpart_rows_n = std::max(n > 0 ? grid_for(n) : 0, c->At.ntiles > 0 ? grid_for_tiles(c->At.ntiles) : 0);
+  {
+    const long long pm = static_cast<long long>(std::max(c->mpart_rows_m, 1)) * kMaxCount;
+    const long long pn = static_cast<long long>(std::max(c->mpart_rows_n, 1)) * kMaxCount;
+    if ((rc = alloc_n(c, &c->mpart_m, pm)) || (rc = alloc_n(c, &c->mpart_n, pn))) return rc;
+    hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(pm)), dim3(kBlock), 0, c->stream, c->mpart_m, 0.0, pm);
+    hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(pn)), dim3(kBlock), 0, c->stream, c->mpart_n, 0.0, pn);
+  }
   PSH_TRY(c, hipStreamSynchronize(c->stream));  // host arrays may be freed after return
   for (int s = 0; s < PSH_NSLOT; ++s)
     if (slot_len(c, s) > 0)

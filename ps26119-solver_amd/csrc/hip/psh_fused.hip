@@ -404,25 +404,55 @@ __global__ void __launch_bounds__(kBlock) k_check_m(CheckArgs<T> a) {
   if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_RMAX2, rmax2);
 }
 
-// One thread per row of A (Ã and the original A share the structure).
+// Row sums of one tile with the products staged in LDS (CSR order, as tile_rows): s = Σ a·x and,
+// with `abs_too`, sa = Σ |a·x|. The whole workgroup calls it.
+template <class V, class X>
+__device__ inline void tile_sums(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
+                                 const V* __restrict__ val, const X* __restrict__ x, bool abs_too, double& s,
+                                 double& sa) {
+  __shared__ double prod[kTileNnz];
+  const int p0 = ptr[r0], p1 = ptr[r1];
+  const int row = r0 + static_cast<int>(threadIdx.x);
+  const bool has = row < r1;
+  const int rb = has ? ptr[row] : 0, re = has ? ptr[row + 1] : 0;
+  s = 0.0;
+  sa = 0.0;
+  for (int cb = p0; cb < p1; cb += kTileNnz) {
+    const int ce = p1 - cb > kTileNnz ? cb + kTileNnz : p1;
+    for (int p = cb + static_cast<int>(threadIdx.x); p < ce; p += kBlock)
+      prod[p - cb] = static_cast<double>(val[p]) * static_cast<double>(x[col[p]]);
+    __syncthreads();
+    if (has) {
+      const int b = rb > cb ? rb : cb, e = re < ce ? re : ce;
+      for (int p = b; p < e; ++p) s += prod[p - cb];
+      if (abs_too)
+        for (int p = b; p < e; ++p) sa += fabs(prod[p - cb]);
+    }
+    __syncthreads();
+  }
+}
+
+// Tiles of A (Ã and the original A share the structure): residual, KKT row part and ray tests.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a) {
-  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool in = i < a.m;
-  double pmax = 0, dmax = 0, pviol = 0, pviol2 = 0;
-  if (in) {
-    const int b = a.a_ptr[i], e = a.a_ptr[i + 1];
-    if (a.flags & F_RES) {  // Ã·Δx (fp64 accumulate, stored in T), cross term Δyᵀ(ÃΔx)
-      double s = 0.0;
-      for (int p = b; p < e; ++p) s += static_cast<double>(a.a_val[p]) * static_cast<double>(a.dx[a.a_col[p]]);
+__global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles) {
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const bool valid = tile < ntiles;  // padding workgroups still write their (zero) maxima
+  const int r0 = valid ? tiles[tile] : 0, r1 = valid ? tiles[tile + 1] : 0;
+  const int i = r0 + static_cast<int>(threadIdx.x);
+  const bool in = i < r1;
+  double pmax = 0, dmax = 0, pviol = 0, pviol2 = 0, s, sa;
+  if (a.flags & F_RES) {  // Ã·Δx (fp64 accumulate, stored in T), cross term Δyᵀ(ÃΔx)
+    tile_sums<T, T>(r0, r1, a.a_ptr, a.a_col, a.a_val, a.dx, false, s, sa);
+    if (in) {
       const T ax = static_cast<T>(s);
       a.ax[i] = ax;
       term(a, T_CROSS)[i] = static_cast<double>(a.dy[i]) * static_cast<double>(ax);
     }
-    const double lo = a.rlo[i], up = a.ruo[i];
-    if (a.flags & F_KKT) {  // row part of kkt_general (cpu_backend.cpp:254-270)
-      double s = 0.0;
-      for (int p = b; p < e; ++p) s += a.a_orig[p] * a.xo[a.a_col[p]];
+  }
+  const double lo = in ? a.rlo[i] : 0.0, up = in ? a.ruo[i] : 0.0;
+  if (a.flags & F_KKT) {  // row part of kkt_general (cpu_backend.cpp:254-270)
+    tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.xo, false, s, sa);
+    if (in) {
       const double viol = s < lo ? lo - s : (s > up ? s - up : 0.0);
       term(a, T_RP2)[i] = viol * viol;
       pmax = elem_viol(s, lo, up);
@@ -440,15 +470,17 @@ __global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a) {
       term(a, T_DROW)[i] = dt;
       term(a, T_RD2ROW)[i] = rd;
     }
-    if (a.flags & F_RAY) {  // A·d against the finite row bounds (engine.cpp:140-145)
-      double s = 0.0;
-      for (int p = b; p < e; ++p) s += a.a_orig[p] * a.dv[a.a_col[p]];
+  }
+  if (a.flags & F_RAY) {  // A·d against the finite row bounds (engine.cpp:140-145)
+    tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv, false, s, sa);
+    if (in) {
       if (isfinite(lo)) pviol = dmaxd(pviol, -s);
       if (isfinite(up)) pviol = dmaxd(pviol, s);
     }
-    if (a.flags & F_DRIFT) {
-      double s = 0.0;
-      for (int p = b; p < e; ++p) s += a.a_orig[p] * a.dv2[a.a_col[p]];
+  }
+  if (a.flags & F_DRIFT) {
+    tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv2, false, s, sa);
+    if (in) {
       if (isfinite(lo)) pviol2 = dmaxd(pviol2, -s);
       if (isfinite(up)) pviol2 = dmaxd(pviol2, s);
     }
@@ -475,18 +507,19 @@ __device__ inline void ray_col_terms(double g, double gabs, double lo, double up
   }
 }
 
-// One thread per row of the original Aᵀ (the model's CSC columns).
+// Tiles of the original Aᵀ (the model's CSC columns; Ãᵀ's tiles).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a) {
-  const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool in = j < a.n;
-  double pmax = 0, dmax = 0, cinf = 0, dviol = 0, dviol2 = 0;
-  if (in) {
-    const int b = a.at_ptr[j], e = a.at_ptr[j + 1];
-    const double lo = a.clo[j], up = a.cuo[j];
-    if (a.flags & F_KKT) {  // column part of kkt_general (cpu_backend.cpp:271-282)
-      double s = 0.0;
-      for (int p = b; p < e; ++p) s += a.ato_val[p] * a.yo[a.ato_col[p]];
+__global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles) {
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const bool valid = tile < ntiles;
+  const int r0 = valid ? tiles[tile] : 0, r1 = valid ? tiles[tile + 1] : 0;
+  const int j = r0 + static_cast<int>(threadIdx.x);
+  const bool in = j < r1;
+  double pmax = 0, dmax = 0, cinf = 0, dviol = 0, dviol2 = 0, s, sa;
+  const double lo = in ? a.clo[j] : 0.0, up = in ? a.cuo[j] : 0.0;
+  if (a.flags & F_KKT) {  // column part of kkt_general (cpu_backend.cpp:271-282)
+    tile_sums<double, double>(r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.yo, false, s, sa);
+    if (in) {
       const double cj = a.cmin[j], xj = a.xo[j];
       term(a, T_C2)[j] = cj * cj;
       cinf = fabs(cj);
@@ -502,27 +535,21 @@ __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a) {
       dmax = fabs(lam - z);
       term(a, T_DCOL)[j] = z > 0 ? z * lo : (z < 0 ? z * up : 0.0);
     }
-    if (a.flags & F_RAY) {
-      double g = 0.0, gabs = 0.0;
-      for (int p = b; p < e; ++p) {
-        const double r = a.rv[a.ato_col[p]];
-        g += a.ato_val[p] * r;
-        gabs += fabs(a.ato_val[p] * r);
-      }
+  }
+  if (a.flags & F_RAY) {
+    tile_sums<double, double>(r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv, true, s, sa);
+    if (in) {
       double obj, mag;
-      ray_col_terms(g, gabs, lo, up, obj, mag, dviol);
+      ray_col_terms(s, sa, lo, up, obj, mag, dviol);
       term(a, T_ROBJ_COL)[j] = obj;
       term(a, T_RMAG_COL)[j] = mag;
     }
-    if (a.flags & F_DRIFT) {
-      double g = 0.0, gabs = 0.0;
-      for (int p = b; p < e; ++p) {
-        const double r = a.rv2[a.ato_col[p]];
-        g += a.ato_val[p] * r;
-        gabs += fabs(a.ato_val[p] * r);
-      }
+  }
+  if (a.flags & F_DRIFT) {
+    tile_sums<double, double>(r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv2, true, s, sa);
+    if (in) {
       double obj, mag;
-      ray_col_terms(g, gabs, lo, up, obj, mag, dviol2);
+      ray_col_terms(s, sa, lo, up, obj, mag, dviol2);
       term(a, T_ROBJ_COL2)[j] = obj;
       term(a, T_RMAG_COL2)[j] = mag;
     }
@@ -616,13 +643,15 @@ int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool maxes) {
   const unsigned mf = nf;
   if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
   if (mf && c->m > 0) hipLaunchKernelGGL((k_check_m<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
-  if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->m > 0)
-    hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
-  if ((a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->n > 0)
-    hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
+  if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->A.ntiles > 0)
+    hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, a, c->A.tiles,
+                       c->A.ntiles);
+  if ((a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->At.ntiles > 0)
+    hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream, a,
+                       c->At.tiles, c->At.ntiles);
   if (maxes)
-    hipLaunchKernelGGL(k_max_final, dim3(kMaxCount), dim3(kBlock), 0, c->stream, c->mpart_m,
-                       c->m > 0 ? grid_for(c->m) : 0, c->mpart_n, c->n > 0 ? grid_for(c->n) : 0, c->results + R_MAX0);
+    hipLaunchKernelGGL(k_max_final, dim3(kMaxCount), dim3(kBlock), 0, c->stream, c->mpart_m, c->mpart_rows_m,
+                       c->mpart_n, c->mpart_rows_n, c->results + R_MAX0);
   PSH_LAUNCHED(c, "check kernels");
   return PSH_OK;
 }

@@ -136,6 +136,7 @@ struct psh_ctx {
   double *rv2 = nullptr, *dv2 = nullptr;        // rays of the drift
   double* terms = nullptr;                      // kTerms arrays of max(m, n)
   double *mpart_m = nullptr, *mpart_n = nullptr; // per-workgroup maxima of the check passes
+  int mpart_rows_m = 0, mpart_rows_n = 0;        // workgroup slots in each
   long long term_len = 0;
   double* partials = nullptr;                   // tree first level
   double* results = nullptr;                    // device results
