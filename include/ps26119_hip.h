@@ -146,6 +146,10 @@ int psh_ray_vectors(psh_ctx* ctx, int drift, double* dual_ray, double* primal_ra
  * HIP graph and replayed. */
 int psh_chunk(psh_ctx* ctx, const psh_chunk_args* a);
 int psh_graphs(psh_ctx* ctx, int on); /* 1 (default): replay captured chunk graphs */
+/* 1 (default): for m >= 65536, fp64 chunks with tree reductions run the dual half step over
+ * column slices of Ã (each XCD gathers from one L2-sized slice of x̄; a row's slice sums are added
+ * in slice order, so its sum is rounded differently from the CSR-order sum of CPU-order mode). */
+int psh_sliced(psh_ctx* ctx, int on);
 /* Mixed precision in increment form (no counterpart in the reference; its mixed mode is
  * psh_set_fp32): the iterate is the fp64 anchor in X0 / Y0 plus an fp32 increment, the matrix
  * is read in fp32, and the anchor constants are rebuilt in fp64 at every restart, so the fp32

@@ -553,7 +553,8 @@ void psh_destroy(psh_ctx* c) {
                   c->At.tiles, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
-                  c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n};
+                  c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->As.seg, c->As.col,
+                  c->As.val, c->As.row, c->As.part};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
@@ -616,6 +617,41 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
   if ((rc = up(&c->A.tiles, ta.data(), static_cast<long long>(ta.size()))) ||
       (rc = up(&c->At.tiles, tt.data(), static_cast<long long>(tt.size()))))
     return rc;
+  bool short_tiles = true;  // a slice segment must fit the LDS product buffer
+  for (int t = 0; t < c->A.ntiles && short_tiles; ++t) short_tiles = p->a_ptr[ta[t + 1]] - p->a_ptr[ta[t]] <= kTileNnz;
+  if (m >= kSliceMinRows && nz > 0 && c->A.ntiles > 0 && short_tiles) {  // Ã by (tile, column slice)
+    Sliced& S = c->As;
+    const int per = kXcd / kSlices;
+    S.slices = kSlices;
+    S.ntp = (c->A.ntiles + per - 1) / per * per;
+    const long long width = (static_cast<long long>(n) + kSlices - 1) / kSlices;
+    std::vector<int> seg(static_cast<size_t>(S.ntp) * kSlices + 1, 0), scol(nz);
+    std::vector<double> sval(nz);
+    std::vector<unsigned char> srow(nz);
+    size_t q = 0;
+    std::vector<int> cut(kSlices + 1);
+    for (int t = 0; t < S.ntp; ++t) {
+      const int r0 = t < c->A.ntiles ? ta[t] : m, r1 = t < c->A.ntiles ? ta[t + 1] : m;
+      for (int k = 0; k < kSlices; ++k) {
+        for (int r = r0; r < r1; ++r) {
+          int b = p->a_ptr[r];
+          const int e = p->a_ptr[r + 1];
+          while (b < e && p->a_col[b] / width < k) ++b;
+          for (; b < e && p->a_col[b] / width == k; ++b, ++q) {
+            scol[q] = p->a_col[b];
+            sval[q] = p->a_val[b];
+            srow[q] = static_cast<unsigned char>(r - r0);
+          }
+        }
+        seg[static_cast<size_t>(t) * kSlices + k + 1] = static_cast<int>(q);
+      }
+    }
+    if (static_cast<long long>(q) != nz || (rc = up(&S.seg, seg.data(), static_cast<long long>(seg.size()))) ||
+        (rc = up(&S.col, scol.data(), nz)) || (rc = up(&S.val, sval.data(), nz)) || (rc = up(&S.row, srow.data(), nz)) ||
+        (rc = alloc_n(c, &S.part, static_cast<long long>(kSlices) * m)))
+      return rc ? rc : fail(c, PSH_E_ARG, "column slices do not cover the matrix");
+    PSH_TRY(c, hipStreamSynchronize(c->stream));  // the staging vectors go out of scope
+  }
   for (int s = 0; s < PSH_NSLOT; ++s)
     if ((rc = alloc_n(c, &c->vd[s], slot_len(c, s)))) return rc;
   const long long mx = std::max<long long>(std::max(m, n), 1);

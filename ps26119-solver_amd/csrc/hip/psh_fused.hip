@@ -236,6 +236,50 @@ __global__ void k_delta_point(const double* __restrict__ z0, const float* __rest
   if (zc) zc[j] = a + static_cast<double>(dc[j]);
 }
 
+// Dual half step over column slices (fp64, tree-reduction mode): workgroup b handles slice
+// k = (b mod 8)·S/8 of tile (b / 8)·(8/S) + (b mod 8) − k·8/S, so the XCD that runs it only gathers
+// from x̄'s k-th slice; products of the (tile, slice) segment go through LDS and each thread writes
+// its row's slice sum (CSR order within the slice).
+__global__ void __launch_bounds__(kBlock) k_slice_partial(Sliced S, const int* __restrict__ tiles, int ntiles, int rows,
+                                                          const double* __restrict__ x) {
+  __shared__ double prod[kTileNnz];
+  __shared__ int rs[kBlock], re[kBlock];
+  const int xcd = static_cast<int>(blockIdx.x) % kXcd;
+  const int per = kXcd / S.slices;
+  const int k = xcd / per;
+  const int tile = static_cast<int>(blockIdx.x) / kXcd * per + (xcd - k * per);
+  rs[threadIdx.x] = 0;
+  re[threadIdx.x] = 0;
+  __syncthreads();
+  const int seg = tile * S.slices + k;
+  const int b0 = S.seg[seg], b1 = S.seg[seg + 1];
+  for (int p = b0 + static_cast<int>(threadIdx.x); p < b1; p += kBlock) {
+    prod[p - b0] = S.val[p] * x[S.col[p]];
+    const int r = S.row[p];
+    if (p == b0 || S.row[p - 1] != r) rs[r] = p - b0;
+    if (p == b1 - 1 || S.row[p + 1] != r) re[r] = p - b0 + 1;
+  }
+  __syncthreads();
+  if (tile >= ntiles) return;
+  const int r0 = tiles[tile], r1 = tiles[tile + 1];
+  const int row = r0 + static_cast<int>(threadIdx.x);
+  if (row < r1) {
+    double v = 0.0;
+    for (int q = rs[threadIdx.x]; q < re[threadIdx.x]; ++q) v += prod[q];
+    S.part[static_cast<size_t>(k) * rows + row] = v;
+  }
+}
+
+// Slice sums added in slice order, then the fused dual update (k_iter_dual's epilogue).
+__global__ void __launch_bounds__(kBlock) k_slice_dual(Sliced S, int rows, DualArgs<double> p) {
+  const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+  EpiDual<double> e(p);
+  if (i >= rows) return;
+  double s = 0.0;
+  for (int k = 0; k < S.slices; ++k) s += S.part[static_cast<size_t>(k) * rows + i];
+  e(i, true, s);
+}
+
 template <class T>
 __global__ void k_restart(T* __restrict__ x, T* __restrict__ x0, const T* __restrict__ xh, long long n) {
   const long long j = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -756,6 +800,8 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
     if (!check_table(c, g->mode, F_RES, &err)) return err;
   }
   const bool delta = c->delta.on;
+  // column-sliced dual half step: fp64, tree reductions (its row sums are not in CSR order)
+  const bool use_slices = c->sliced && c->As.slices > 0 && !c->fp32 && !delta && g->mode == PSH_RED_TREE;
   if (delta && (rc = delta_prepare(c, g->sigma))) return rc;
   hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, c->stream, c->prm, g->tau, g->sigma,
                      static_cast<double>(g->inner0));
@@ -818,8 +864,19 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
         }
         if (c->A.ntiles > 0) {
           time_begin(c, 1, &ev);
-          hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
-                             c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
+          if constexpr (sizeof(T) == 8) {
+            if (use_slices) {
+              hipLaunchKernelGGL(k_slice_partial, dim3(c->As.ntp / (kXcd / c->As.slices) * kXcd), dim3(kBlock), 0,
+                                 c->stream, c->As, c->A.tiles, c->A.ntiles, c->m, sp<T>(c, PSH_XB));
+              hipLaunchKernelGGL(k_slice_dual, dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, c->As, c->m, da);
+            } else {
+              hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
+                                 c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
+            }
+          } else {
+            hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
+                               c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
+          }
           time_end(c, 1, ev);
         }
         PSH_LAUNCHED(c, "iteration kernels");
@@ -835,7 +892,8 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
     if ((rc = delta ? enqueue_delta() : enqueue())) return rc;
   } else {
     const long long key = (static_cast<long long>(g->iterations) << 8) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
-                          (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3) | (delta ? 1LL << 6 : 0);
+                          (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3) | (delta ? 1LL << 6 : 0) |
+                          (use_slices ? 1LL << 7 : 0);
     auto it = c->graph_cache.find(key);
     if (it == c->graph_cache.end()) {
       PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -862,6 +920,12 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
 int psh_graphs(psh_ctx* c, int on) {
   if (!c) return PSH_E_ARG;
   c->graphs = on != 0;
+  return PSH_OK;
+}
+
+int psh_sliced(psh_ctx* c, int on) {
+  if (!c) return PSH_E_ARG;
+  c->sliced = on != 0;
   return PSH_OK;
 }
 

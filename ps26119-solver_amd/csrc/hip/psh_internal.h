@@ -39,6 +39,21 @@ struct DevMat {
   int* tiles = nullptr;   // ntiles+1 tile start rows
 };
 
+// Ã's entries regrouped by (row tile, column slice) for the dual half step with fp64 tree
+// reductions: workgroup b gathers only from column slice (b mod 8)·S/8 of x̄, so the slice stays in
+// its XCD's L2; each row's slice sums are combined in slice order by the fused epilogue kernel.
+struct Sliced {
+  int slices = 0;                 // S (0: not built)
+  int ntp = 0;                    // tiles padded to a multiple of 8/S
+  int* seg = nullptr;             // ntp·S + 1 segment starts
+  int* col = nullptr;             // nnz
+  double* val = nullptr;          // nnz
+  unsigned char* row = nullptr;   // nnz: row within the tile
+  double* part = nullptr;         // S·rows slice sums
+};
+constexpr int kSlices = 4;
+constexpr int kSliceMinRows = 1 << 16;  // below this the extra launch costs more than it saves
+
 // A reduction job: Σ or max over t[0..na) followed by u[0..nb) (u optional).
 struct RedJob {
   const double* a;
@@ -116,6 +131,8 @@ struct psh_ctx {
   size_t bytes = 0;
 
   psh::DevMat A, At;             // scaled Ã (m×n) and Ãᵀ (n×m)
+  psh::Sliced As;                // Ã by (tile, column slice), built for large m
+  bool sliced = true;            // use it for fp64 tree-mode chunks
   double* a_orig = nullptr;      // original A values, Ã's structure
   int* ato_col = nullptr;        // original Aᵀ (CSC) row indices (aliases At.col when equal)
   bool ato_alias = false;

@@ -58,6 +58,7 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_device") e.device = static_cast<int>(v);
     else if (name == "hip_timing") e.timing = v != 0 ? 1 : 0;
     else if (name == "hip_graphs") e.graphs = v != 0 ? 1 : 0;
+    else if (name == "hip_sliced") e.sliced = v != 0 ? 1 : 0;
     else if (name == "hip_refine") e.refine = v != 0 ? 1 : 0;
     else if (name == "hip_refine_patience") e.refine_patience = std::max(1, static_cast<int>(v));
     else return name;
@@ -371,6 +372,7 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
     dev_->load(sp_);
     if (opt.timing) dev_->check(psh_timing(dev_->ctx(), 1), "timing");
     dev_->check(psh_graphs(dev_->ctx(), opt.graphs), "graphs");
+    dev_->check(psh_sliced(dev_->ctx(), opt.sliced), "sliced");
     const bool empty = sp_.n == 0 || sp_.A.nnz() == 0;
     const double norm = empty ? 0.0 : estimate_norm2(*dev_, sp_.n, opt.cpu_order);
     eta_ = norm > 0 ? 0.998 / norm : 1.0;

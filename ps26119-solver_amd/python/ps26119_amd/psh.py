@@ -100,6 +100,9 @@ def lib():
             "psh_lanczos_begin": ([P, D], ctypes.c_int),
             "psh_lanczos_step": ([P, ctypes.c_double, ctypes.c_int, D, D], ctypes.c_int),
             "psh_lanczos_advance": ([P, ctypes.c_double], ctypes.c_int),
+            "psh_graphs": ([P, ctypes.c_int], ctypes.c_int),
+            "psh_sliced": ([P, ctypes.c_int], ctypes.c_int),
+            "psh_delta": ([P, ctypes.c_int], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -216,6 +219,15 @@ class Context:
 
     def restart(self):
         self._ok(lib().psh_restart(self._h))
+
+    def graphs(self, on):
+        self._ok(lib().psh_graphs(self._h, int(on)))
+
+    def sliced(self, on):
+        self._ok(lib().psh_sliced(self._h, int(on)))
+
+    def delta(self, on):
+        self._ok(lib().psh_delta(self._h, int(on)))
 
 
 def scaled_problem(sc: dict, lp) -> dict:

@@ -124,6 +124,36 @@ def test_fused_chunk_equals_unfused_ops(gpu_available, oracle, name, prec):
     b.close()
 
 
+def test_sliced_dual_half_step(gpu_available, oracle):
+    """Above 65,536 rows the fp64 tree-mode chunk runs the dual half step over column slices of Ã
+    (x̄'s slice stays in an XCD's L2). Its row sums add the slice sums in slice order, so they may
+    differ from the CSR-order sums in the last bits only; CPU-order chunks never use slices and stay
+    bit-identical."""
+    from ps26119_amd import lpgen, psh
+    lp = lpgen.random_lp(70000, seed=9)
+    K, tau, sigma, rho = 7, 0.3, 0.4, 1.0
+    a = context(oracle, lp, 0)
+    b = context(oracle, lp, 0)
+    b.sliced(0)
+    for mode in (psh.RED_TREE, psh.RED_CPU):
+        for ctx in (a, b):
+            ctx.upload(psh.X, np.sin(1.0 + np.arange(lp.n)))
+            ctx.upload(psh.Y, np.cos(2.0 + np.arange(lp.m)))
+            ctx.copy(psh.X0, psh.X)
+            ctx.copy(psh.Y0, psh.Y)
+        for inner0 in (0, K):
+            a.chunk(K, inner0, tau, sigma, rho, mode=mode)
+            b.chunk(K, inner0, tau, sigma, rho, mode=mode)
+        for slot in (psh.X, psh.XH, psh.XB, psh.Y, psh.YH, psh.YB):
+            va, vb = a.download(slot), b.download(slot)
+            if mode == psh.RED_CPU:
+                assert np.array_equal(va, vb), slot
+            else:
+                assert np.allclose(va, vb, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(vb).max())), slot
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("name", ["stocfor1", "rand300", "rand20k"])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_fused_check_equals_separate_reductions(gpu_available, oracle, name, mode):

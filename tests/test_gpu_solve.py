@@ -54,7 +54,8 @@ def test_tree_reductions_reach_the_reference_answer(gpu_available, golden, name,
     import ps26119
     lp = _model(name)
     exp = golden["solves"][f"{name}|presolve=1|{prec}"]
-    r = ps26119.solve_model(lp, precision=prec, iteration_limit=3_000_000)
+    # mixed: the reference's precision policy (the increment form has tests of its own)
+    r = ps26119.solve_model(lp, precision=prec, iteration_limit=3_000_000, params={"hip_refine": 0})
     assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
     ref_obj = float.fromhex(exp["objective"])
     tol = 1e-9 if prec == "fp64" else 1e-6
