@@ -201,6 +201,22 @@ def read_lpm(path: str) -> Lp:
               offset=float(hdr.get("OFFSET", "0") or 0), name=hdr.get("NAME", ""))
 
 
+def write_lpm(lp: Lp, path: str) -> None:
+    """Writes `lp` in the `.lpm` text format, version 1 (doubles as repr, so they round-trip)."""
+    def nums(v):
+        return " ".join(repr(float(x)) if np.isfinite(x) else ("inf" if x > 0 else "-inf") for x in v)
+
+    with open(path, "w") as f:
+        f.write(f"LPM 1\nNAME {lp.name or 'model'}\nSENSE {'MAX' if lp.sense < 0 else 'MIN'}\n")
+        f.write(f"ROWS {lp.m}\nCOLS {lp.n}\nNNZ {lp.nnz}\nOFFSET {float(lp.offset)!r}\n")
+        for key, v in (("OBJ", lp.c), ("COL_LOWER", lp.col_lower), ("COL_UPPER", lp.col_upper),
+                       ("ROW_LOWER", lp.row_lower), ("ROW_UPPER", lp.row_upper)):
+            f.write(f"{key} {nums(v)}\n")
+        f.write("COL_START " + " ".join(str(int(x)) for x in lp.col_start) + "\n")
+        f.write("ROW_INDEX " + " ".join(str(int(x)) for x in lp.row_index) + "\n")
+        f.write(f"VALUE {nums(lp.value)}\nEND\n")
+
+
 def from_dense(c, A, rl, ru, cl, cu, sense: int = 1, name: str = "") -> Lp:
     """Small models from dense rows (test fixtures)."""
     A = np.asarray(A, dtype=np.float64)

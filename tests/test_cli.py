@@ -1,0 +1,129 @@
+"""The command-line front end `ps26119-hip` (the reference CLI's `solve --gpu` and `batch`
+commands, reference apps/cli/main.cpp): argument handling, exit codes, the .lpm reader and the
+solution file, on the CPU; solves on the GPU (marked gpu)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, netlib
+
+CLI = os.path.join(ROOT, "ps26119-solver_amd", "csrc", "ps26119-hip")
+AFIRO = os.path.join(GOLDEN, "netlib_small", "afiro.lpm")
+
+
+def run(*args, timeout=120):
+    if not os.path.exists(CLI):
+        pytest.skip("ps26119-hip not built")
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def read_solution(path):
+    out = {"x": [], "y": []}
+    with open(path) as f:
+        lines = f.read().splitlines()
+    k = 0
+    while k < len(lines):
+        parts = lines[k].split()
+        if parts and parts[0] in ("COLUMNS", "ROWS"):
+            cnt = int(parts[1])
+            vals = [lines[k + 1 + i].split() for i in range(cnt)]
+            out["x" if parts[0] == "COLUMNS" else "y"] = [float(v[1 if parts[0] == "COLUMNS" else 2]) for v in vals]
+            k += cnt + 1
+            continue
+        if len(parts) >= 2 and parts[0] in ("status", "objective", "model", "engine", "check"):
+            out[parts[0]] = " ".join(parts[1:])
+        k += 1
+    return out
+
+
+def test_version_and_usage():
+    assert run("--version").returncode == 0
+    assert run().returncode == 2
+    assert run("solve").returncode == 2
+    assert run("solve", AFIRO, "--tol", "abc").returncode == 2
+    assert run("solve", AFIRO, "--bogus").returncode == 2
+    assert run("frobnicate").returncode == 2
+
+
+def test_read_errors(tmp_path):
+    assert run("solve", str(tmp_path / "missing.lpm")).returncode == 3
+    bad = tmp_path / "bad.lpm"
+    bad.write_text("LPM 1\nROWS 1\nCOLS 1\nNNZ 1\nOBJ x\nEND\n")
+    r = run("solve", str(bad))
+    assert r.returncode == 3 and "bad or missing number" in r.stderr
+    assert run("solve", str(tmp_path / "model.mps")).returncode == 3
+
+
+def test_unwritable_output_fails_before_solving(tmp_path):
+    r = run("solve", AFIRO, "--out", str(tmp_path / "no" / "such" / "dir" / "a.sol"))
+    assert r.returncode == 4
+
+
+def test_reader_matches_the_python_reader_and_writer_round_trip(tmp_path):
+    from ps26119_amd import lpgen
+    lp = lpgen.random_lp(50, seed=4)
+    p = tmp_path / "r.lpm"
+    lpgen.write_lpm(lp, str(p))
+    back = lpgen.read_lpm(str(p))
+    for a, b in ((lp.c, back.c), (lp.value, back.value), (lp.row_upper, back.row_upper), (lp.col_start, back.col_start)):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    r = run("solve", str(p), "--iteration-limit", "64")
+    assert f"rows {lp.m}  cols {lp.n}  nnz {lp.nnz}" in r.stdout
+
+
+def test_batch_rejects_a_different_matrix(tmp_path):
+    from ps26119_amd import lpgen
+    a, b = lpgen.random_lp(30, seed=1), lpgen.random_lp(30, seed=2)
+    pa, pb = tmp_path / "a.lpm", tmp_path / "b.lpm"
+    lpgen.write_lpm(a, str(pa))
+    lpgen.write_lpm(b, str(pb))
+    r = run("batch", str(pa), str(pb))
+    assert r.returncode == 2 and "differs from the base model" in r.stderr
+
+
+def test_without_a_gpu_the_solve_is_not_solved(tmp_path):
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible")
+    out = tmp_path / "a.sol"
+    r = run("solve", AFIRO, "--out", str(out))
+    assert r.returncode == 5 and "HIP backend unavailable" in r.stdout
+    sol = read_solution(out)
+    assert sol["status"] == "NotSolved"
+
+
+@pytest.mark.gpu
+def test_gpu_solve_writes_a_verified_solution_and_warm_starts(gpu_available, golden, tmp_path):
+    out = tmp_path / "afiro.sol"
+    r = run("solve", AFIRO, "--gpu", "--out", str(out))
+    assert r.returncode == 0, r.stdout + r.stderr
+    sol = read_solution(out)
+    ref = float.fromhex(golden["solves"]["afiro|presolve=1|fp64"]["objective"])
+    assert sol["status"] == "Optimal" and sol["check"].startswith("PASS")
+    assert abs(float(sol["objective"]) - ref) <= 1e-9 * (1 + abs(ref))
+    import ps26119
+    assert ps26119.check_solution(netlib("afiro"), sol["x"], sol["y"])["passed"]
+    r2 = run("solve", AFIRO, "--warm", str(out), "--warm-weight")
+    assert r2.returncode == 0 and "warm start" in r2.stdout
+    r3 = run("solve", AFIRO, "--algorithm", "pdlp", "--precision", "mixed")
+    assert r3.returncode == 0 and "engine     pdlp" in r3.stdout
+
+
+@pytest.mark.gpu
+def test_gpu_batch_command(gpu_available, tmp_path):
+    from ps26119_amd import lpgen
+    base = netlib("stocfor1")
+    files = []
+    for k, sc in enumerate(lpgen.perturbation_scenarios(base)):
+        p = tmp_path / f"s{k}.lpm"
+        lpgen.write_lpm(lpgen.with_scenario(base, sc), str(p))
+        files.append(str(p))
+    bp = tmp_path / "base.lpm"
+    lpgen.write_lpm(base, str(bp))
+    outdir = tmp_path / "sols"
+    outdir.mkdir()
+    r = run("batch", str(bp), *files, "--out-dir", str(outdir))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Optimal") == 4
+    assert len(list(outdir.glob("*.sol"))) == 4
