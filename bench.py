@@ -17,8 +17,9 @@ Other workloads (each prints one JSON line of the same shape):
       versus plain fp64, both to 1e-8; value = mixed seconds, with the fp64 seconds, the speedup
       and the objective difference between the two.
   c3  64 scenarios sharing one 100,000 × 200,000 random matrix (b, c, u scaled by lognormal(0,
-      0.05) factors), warm-started from the base solution and solved as one batch (SpMM);
-      value = scenarios per second at the tolerance, against sequential single solves.
+      0.05) factors) solved as one batch (SpMM); value = scenario-iterations per second against
+      one LP alone, plus an end-to-end run at 10,000 rows (warm start, 1e-6) against sequential
+      single solves.
   c4  serving: Poisson arrivals of random LPs with m log-uniform in [m_lo, m_hi], half at 1e-6
       and half at 1e-8, admitted under an HBM byte budget; value = solves per second, with p50 /
       p99 latency, against sequential plain fp64 solves of the same requests.
@@ -134,6 +135,11 @@ def bench_c1(args, ps26119, lpgen):
     pb, db = (delta_kernel_bytes if mixed else kernel_bytes)(lp.m, lp.n, lp.nnz)
     roof, avg_us = roofline_of(timed.stats, pb, "k_delta_primal (fp32 increments: Ãᵀδy + projection/reflection/Halpern)"
                                if mixed else "k_iter_primal (fused Ãᵀy + projection/reflection/Halpern)")
+    traffic_file = os.path.join(ROOT, "profiles", "k_iter_fp64_random1M_pmc_traffic.json")
+    if not mixed and lp.m == 1_000_000 and lp.nnz == 6_000_000 and os.path.exists(traffic_file):
+        with open(traffic_file) as f:  # HBM bytes per launch from the two rocprofv3 --pmc passes
+            roof["traffic"] = json.load(f)["traffic"].get("k_iter_primal<double>")
+        roof["traffic_source"] = "profiles/k_iter_fp64_random1M_pmc_traffic.md"
     st = res.stats
     loop_s = st["iterate_seconds"]
     per_it = loop_s / max(1, res.iterations)
@@ -200,46 +206,57 @@ def bench_c2(args, ps26119, lpgen):
 
 # ------------------------------------------------------------------ c3
 def bench_c3(args, ps26119, lpgen):
+    """Two measurements, both bounded:
+    (a) at the full size (100,000 × 200,000, 64 scenarios) the batch's rate in scenario-iterations
+        per second against one LP solved alone (the same iterations, differential timing so that
+        setup is excluded) — `value`;
+    (b) end to end at 10,000 × 20,000: 64 scenarios warm-started from the base solution, solved to
+        1e-6 as one batch, against sequential single solves (4 timed, scaled to 64)."""
     m = args.m if args.m != 1_000_000 else 100_000
     base = lpgen.random_lp(m, seed=args.seed)
     scen = lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)
     log(f"c3 base m={base.m} n={base.n} nnz={base.nnz}, {len(scen)} scenarios")
+    lo, hi = 64, args.c3_iterations
+
+    def timed(fn):
+        t = time.time()
+        fn()
+        return time.time() - t
+
+    tb = [timed(lambda k=k: ps26119.solve_model_batch(base, scen, iteration_limit=k, verbosity=1)) for k in (lo, hi)]
+    ts = [timed(lambda k=k: ps26119.solve_model(base, iteration_limit=k, presolve=False)) for k in (lo, hi)]
+    batch_rate = len(scen) * (hi - lo) / (tb[1] - tb[0])
+    single_rate = (hi - lo) / (ts[1] - ts[0])
+    log(f"batch: {batch_rate:.0f} scenario-iterations/s ({(tb[1] - tb[0]) / (hi - lo) * 1e3:.3f} ms/iteration for "
+        f"{len(scen)}); single LP: {single_rate:.0f} iterations/s")
+    # (b) end to end at 10k rows
+    small = lpgen.random_lp(10_000, seed=args.seed)
+    sscen = lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)
+    b = ps26119.solve_model(small, verbosity=1)
     t = time.time()
-    b = ps26119.solve_model(base, time_limit=args.time_limit)
-    base_s = time.time() - t
-    log(f"base solve: {b.status_name} it={b.iterations} {base_s:.2f}s")
-    results = {}
-    for tol in (1e-6, 1e-8):
-        t = time.time()
-        res = ps26119.solve_model_batch(base, scen, tolerance=tol, warm_x=b.x, warm_y=b.y, time_limit=args.time_limit)
-        wall = time.time() - t
-        ok = sum(r.status == ps26119.OPTIMAL for r in res)
-        its = [r.iterations for r in res]
-        log(f"batch tol {tol:g}: {ok}/{len(res)} optimal, iterations max {max(its)} mean {np.mean(its):.0f}, {wall:.2f}s")
-        results[tol] = (wall, ok, its, res)
-    # sequential single solves of the first few scenarios (same warm start), extrapolated
-    k_seq = min(args.seq_sample, len(scen))
-    seq = {}
-    for tol in (1e-6, 1e-8):
-        t = time.time()
-        for sc in scen[:k_seq]:
-            mk = lpgen.with_scenario(base, sc)
-            ps26119.solve_model(mk, tolerance=tol, warm_x=b.x, warm_y=b.y, presolve=False, time_limit=args.time_limit)
-        seq[tol] = (time.time() - t) / k_seq * len(scen)
-        log(f"sequential tol {tol:g}: {k_seq} solves, estimated {seq[tol]:.2f}s for {len(scen)}")
-    wall8, ok8, its8, res8 = results[1e-8]
-    value = len(scen) / wall8
-    line = base_line(args, value, "scenarios/s", True, "f64",
+    res = ps26119.solve_model_batch(small, sscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1)
+    batch_s = time.time() - t
+    ok = sum(r.status == ps26119.OPTIMAL for r in res)
+    k_seq = min(args.seq_sample, len(sscen))
+    t = time.time()
+    for sc in sscen[:k_seq]:
+        ps26119.solve_model(lpgen.with_scenario(small, sc), tolerance=1e-6, warm_x=b.x, warm_y=b.y, presolve=False,
+                            verbosity=1)
+    seq_s = (time.time() - t) / k_seq * len(sscen)
+    log(f"10k end to end: batch {batch_s:.2f}s ({ok}/{len(res)} optimal), sequential estimate {seq_s:.2f}s")
+    line = base_line(args, batch_rate, "scenario-iterations/s", True, "f64",
                      {"workload": f"{len(scen)} scenarios sharing a random {base.m}x{base.n} matrix (6 per row), b/c/u "
-                                  f"scaled by lognormal(0, 0.05), warm start from the base solution, fp64, tolerance "
-                                  f"1e-8, one batched solve (SpMM)", "m": base.m, "n": base.n, "nnz": base.nnz,
+                                  f"scaled by lognormal(0, 0.05); one batched r2HPDHG (SpMM over 64-wide block vectors), "
+                                  f"fp64; rate over iterations {lo}..{hi}", "m": base.m, "n": base.n, "nnz": base.nnz,
                       "scenarios": len(scen)})
+    line["steps"] = hi - lo
+    line["ms_per_step"] = (tb[1] - tb[0]) / (hi - lo) * 1e3
     line.update({
-        "roofline": None, "optimal": ok8, "iterations_max": max(its8), "iterations_mean": float(np.mean(its8)),
-        "batch_seconds": {"1e-6": results[1e-6][0], "1e-8": wall8},
-        "sequential_seconds_estimate": {"1e-6": seq[1e-6], "1e-8": seq[1e-8], "sample": k_seq},
-        "speedup_vs_sequential": {"1e-6": seq[1e-6] / results[1e-6][0], "1e-8": seq[1e-8] / wall8},
-        "base_solve_seconds": base_s,
+        "roofline": None, "single_lp_iterations_per_s": single_rate,
+        "speedup_vs_single_lp_iterations": batch_rate / single_rate,
+        "end_to_end_10k": {"scenarios": len(sscen), "tolerance": 1e-6, "warm_start": "base solution",
+                           "batch_seconds": batch_s, "optimal": ok, "sequential_seconds_estimate": seq_s,
+                           "sequential_sample": k_seq, "speedup": seq_s / batch_s if batch_s > 0 else None},
     })
     line["cpu_baseline"] = None
     return line
@@ -316,10 +333,12 @@ def main():
     ap.add_argument("--T", type=int, default=8760, help="c2: planning periods")
     ap.add_argument("--scenarios", type=int, default=64, help="c3")
     ap.add_argument("--seq-sample", type=int, default=4, help="c3: sequential solves timed for the comparison")
+    ap.add_argument("--c3-iterations", type=int, default=1024, help="c3: iterations of the rate measurement")
     ap.add_argument("--requests", type=int, default=48, help="c4")
     ap.add_argument("--rate", type=float, default=8.0, help="c4: arrivals per second")
     ap.add_argument("--m-lo", type=int, default=1000, help="c4")
-    ap.add_argument("--m-hi", type=int, default=200000, help="c4")
+    ap.add_argument("--m-hi", type=int, default=20000,
+                    help="c4 (at 2e5 rows a single request needs minutes: the default keeps the stream short)")
     ap.add_argument("--budget-gb", type=float, default=16.0, help="c4: HBM byte budget of the server")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -185,6 +185,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
       std::vector<double> r0(K, -1.0), r_last(K, kInf), fast_s(K, -1.0);
       std::vector<std::int64_t> inner(K, 0), fast_it(K, -1);
       std::int64_t it = 0;
+      double last_log = 0.0;
       std::vector<double> tau, sigma, stats, x, ym;
       std::vector<long long> in0;
       std::vector<int> mask;
@@ -283,8 +284,10 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
           sid.swap(ns);
           W = static_cast<int>(sid.size());
         }
-        if (eo.verbosity >= 2)
-          std::fprintf(stderr, "batch it %lld  t %.2fs  running %d/%d\n", static_cast<long long>(it), elapsed(), W, K);
+        if (eo.verbosity >= 2 || (eo.verbosity == 1 && elapsed() - last_log >= 1.0)) {  // 1: at most once a second
+          last_log = elapsed();
+          std::fprintf(stderr, "batch it %lld  t %.2fs  running %d/%d\n", static_cast<long long>(it), last_log, W, K);
+        }
       }
     }
   } catch (const std::bad_alloc&) {
