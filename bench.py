@@ -14,8 +14,8 @@ host cores) for 256 iterations of the same model, in the same unit.
 
 Other workloads (each prints one JSON line of the same shape):
   c2  refinery planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
-      versus plain fp64, both to 1e-8; value = mixed seconds, with the fp64 seconds, the speedup
-      and the objective difference between the two.
+      versus plain fp64: iteration rates at T = 8760, and both to 1e-8 end to end at a T that
+      converges in bench time; value = fp64 seconds ÷ mixed seconds, with the objective difference.
   c3  64 scenarios sharing one 100,000 × 200,000 random matrix (b, c, u scaled by lognormal(0,
       0.05) factors) solved as one batch (SpMM); value = scenario-iterations per second against
       one LP alone, plus an end-to-end run at 10,000 rows (warm start, 1e-6) against sequential
@@ -174,32 +174,47 @@ def bench_c1(args, ps26119, lpgen):
 
 # ------------------------------------------------------------------ c2
 def bench_c2(args, ps26119, lpgen):
+    """Refinery planning LP, mixed precision in increment form vs plain fp64.
+    (a) at T = --T periods (8760: 429,240 × 525,600): the iteration rate of both (loop wall time of
+        --steps iterations, setup excluded) and the HBM rate of the increment-form primal kernel;
+    (b) end to end at T = --T-small periods, both to 1e-8: seconds, iterations, and the objective
+        difference between the two answers. `value` = fp64 seconds ÷ mixed seconds of (b)."""
     t = time.time()
     lp = lpgen.refinery_lp(T=args.T, seed=3)
     log(f"refinery T={args.T}: m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t:.1f}s")
-    out = {}
-    for prec in ("mixed", "fp64"):
-        r = ps26119.solve_model(lp, precision=prec, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit,
-                                iteration_limit=args.iteration_limit, verbosity=1)
-        log(f"{prec}: {r.status_name} it={r.iterations} t={r.seconds:.2f}s obj={r.objective!r} check={r.check} "
-            f"{r.message[:160]}")
-        out[prec] = r
-    mx, fp = out["mixed"], out["fp64"]
+    iters = max(64, args.steps // 64 * 64)
+    rate = {}
+    for prec in ("fp64", "mixed"):
+        r = ps26119.solve_model(lp, precision=prec, stats=True, iteration_limit=iters, verbosity=1)
+        rate[prec] = r.iterations / r.stats["iterate_seconds"]
+        log(f"T={args.T} {prec}: {rate[prec]:.0f} iterations/s, gap {r.gap:.2e} after {r.iterations}")
+    timed = ps26119.solve_model(lp, precision="mixed", params={"hip_timing": 1}, stats=True,
+                                iteration_limit=min(iters, 2048))
     pb, db = delta_kernel_bytes(lp.m, lp.n, lp.nnz)
-    roof, avg_us = roofline_of(mx.stats, pb, "k_delta_primal (fp32 increments)")
-    line = base_line(args, mx.seconds, "s", False, "f32+f64",
-                     {"workload": f"refinery planning LP T={args.T} (49 rows x 60 columns per period), mixed precision "
-                                  f"in increment form vs fp64, tolerance 1e-8", "m": lp.m, "n": lp.n, "nnz": lp.nnz})
+    roof, avg_us = roofline_of(timed.stats, pb, "k_delta_primal (fp32 increments: Ãᵀδy + projection/reflection/Halpern)")
+    small = lpgen.refinery_lp(T=args.T_small, seed=3)
+    e2e = {}
+    for prec in ("mixed", "fp64"):
+        r = ps26119.solve_model(small, precision=prec, time_limit=args.time_limit, verbosity=1)
+        e2e[prec] = r
+        log(f"T={args.T_small} {prec}: {r.status_name} it={r.iterations} {r.seconds:.2f}s obj={r.objective!r} "
+            f"check={r.check} {r.message[:120]}")
+    mx, fp = e2e["mixed"], e2e["fp64"]
+    both = mx.status == fp.status == ps26119.OPTIMAL
+    value = fp.seconds / mx.seconds if both and mx.seconds > 0 else None
+    line = base_line(args, value, "x (fp64 s / mixed s to 1e-8)", True, "f32+f64",
+                     {"workload": f"refinery planning LP (49 rows x 60 columns per period): rates at T={args.T}, end "
+                                  f"to end to 1e-8 at T={args.T_small}; mixed = fp32 increments on fp64 anchors",
+                      "m": lp.m, "n": lp.n, "nnz": lp.nnz, "m_small": small.m})
+    line["steps"] = iters
+    line["ms_per_step"] = 1e3 / rate["mixed"]
     line.update({
-        "roofline": roof, "status": mx.status_name, "check": mx.check, "iterations": mx.iterations,
-        "fp64": {"seconds": fp.seconds, "iterations": fp.iterations, "status": fp.status_name, "check": fp.check,
-                 "ms_per_iteration": (fp.seconds - fp.stats["setup_seconds"]) / max(1, fp.iterations) * 1e3},
-        "mixed_ms_per_iteration": (mx.seconds - mx.stats["setup_seconds"]) / max(1, mx.iterations) * 1e3,
-        "speedup_mixed_vs_fp64": fp.seconds / mx.seconds if mx.seconds > 0 else None,
-        "objective_rel_diff_mixed_vs_fp64": abs(mx.objective - fp.objective) / (1 + abs(fp.objective)),
-        "message": mx.message[:200],
+        "roofline": roof, "iterations_per_s": rate, "rate_ratio_mixed_vs_fp64": rate["mixed"] / rate["fp64"],
+        "end_to_end": {p: {"status": r.status_name, "seconds": r.seconds, "iterations": r.iterations, "check": r.check}
+                       for p, r in e2e.items()},
+        "objective_rel_diff_mixed_vs_fp64": abs(mx.objective - fp.objective) / (1 + abs(fp.objective)) if both else None,
     })
-    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(lp, fp.iterations, fp.seconds,
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(lp, iters, iters / rate["fp64"],
                                                                            f"refinery T={args.T}")
     return line
 
@@ -330,7 +345,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="c1: mixed = fp32 increments on fp64 anchors (the line's dtype says which)")
-    ap.add_argument("--T", type=int, default=8760, help="c2: planning periods")
+    ap.add_argument("--T", type=int, default=8760, help="c2: planning periods of the rate measurement")
+    ap.add_argument("--T-small", type=int, default=24, help="c2: planning periods of the end-to-end comparison")
     ap.add_argument("--scenarios", type=int, default=64, help="c3")
     ap.add_argument("--seq-sample", type=int, default=4, help="c3: sequential solves timed for the comparison")
     ap.add_argument("--c3-iterations", type=int, default=1024, help="c3: iterations of the rate measurement")
