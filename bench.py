@@ -21,8 +21,9 @@ Other workloads (each prints one JSON line of the same shape):
       one LP alone, plus an end-to-end run at 10,000 rows (warm start, 1e-6) against sequential
       single solves.
   c4  serving: Poisson arrivals of random LPs with m log-uniform in [m_lo, m_hi], half at 1e-6
-      and half at 1e-8, admitted under an HBM byte budget; value = solves per second, with p50 /
-      p99 latency, against sequential plain fp64 solves of the same requests.
+      and half at 1e-8, admitted under an HBM byte budget, served in mixed precision (increment
+      form; --c4-precision fp64 for plain); value = solves per second, with p50 / p99 latency,
+      against sequential plain fp64 solves of the same requests.
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload c1|c2|c3|c4] [--m M]
                     [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
@@ -239,11 +240,14 @@ def bench_c3(args, ps26119, lpgen):
         return time.time() - t
 
     tb = [timed(lambda k=k: ps26119.solve_model_batch(base, scen, iteration_limit=k, verbosity=1)) for k in (lo, hi)]
+    tm = [timed(lambda k=k: ps26119.solve_model_batch(base, scen, iteration_limit=k, verbosity=1, precision="mixed"))
+          for k in (lo, hi)]
     ts = [timed(lambda k=k: ps26119.solve_model(base, iteration_limit=k, presolve=False)) for k in (lo, hi)]
     batch_rate = len(scen) * (hi - lo) / (tb[1] - tb[0])
+    mixed_rate = len(scen) * (hi - lo) / (tm[1] - tm[0])
     single_rate = (hi - lo) / (ts[1] - ts[0])
     log(f"batch: {batch_rate:.0f} scenario-iterations/s ({(tb[1] - tb[0]) / (hi - lo) * 1e3:.3f} ms/iteration for "
-        f"{len(scen)}); single LP: {single_rate:.0f} iterations/s")
+        f"{len(scen)}), mixed {mixed_rate:.0f}; single LP: {single_rate:.0f} iterations/s")
     # (b) end to end at 10k rows
     small = lpgen.random_lp(10_000, seed=args.seed)
     sscen = lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)
@@ -252,13 +256,20 @@ def bench_c3(args, ps26119, lpgen):
     res = ps26119.solve_model_batch(small, sscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1)
     batch_s = time.time() - t
     ok = sum(r.status == ps26119.OPTIMAL for r in res)
+    t = time.time()
+    resm = ps26119.solve_model_batch(small, sscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1,
+                                     precision="mixed")
+    batch_mixed_s = time.time() - t
+    okm = sum(r.status == ps26119.OPTIMAL for r in resm)
+    dobj = max(abs(p.objective - q.objective) / (1 + abs(q.objective)) for p, q in zip(resm, res))
     k_seq = min(args.seq_sample, len(sscen))
     t = time.time()
     for sc in sscen[:k_seq]:
         ps26119.solve_model(lpgen.with_scenario(small, sc), tolerance=1e-6, warm_x=b.x, warm_y=b.y, presolve=False,
                             verbosity=1)
     seq_s = (time.time() - t) / k_seq * len(sscen)
-    log(f"10k end to end: batch {batch_s:.2f}s ({ok}/{len(res)} optimal), sequential estimate {seq_s:.2f}s")
+    log(f"10k end to end: batch {batch_s:.2f}s ({ok}/{len(res)} optimal), mixed batch {batch_mixed_s:.2f}s "
+        f"({okm} optimal, objectives within {dobj:.1e} of fp64), sequential estimate {seq_s:.2f}s")
     line = base_line(args, batch_rate, "scenario-iterations/s", True, "f64",
                      {"workload": f"{len(scen)} scenarios sharing a random {base.m}x{base.n} matrix (6 per row), b/c/u "
                                   f"scaled by lognormal(0, 0.05); one batched r2HPDHG (SpMM over 64-wide block vectors), "
@@ -269,9 +280,12 @@ def bench_c3(args, ps26119, lpgen):
     line.update({
         "roofline": None, "single_lp_iterations_per_s": single_rate,
         "speedup_vs_single_lp_iterations": batch_rate / single_rate,
+        "mixed_scenario_iterations_per_s": mixed_rate,
         "end_to_end_10k": {"scenarios": len(sscen), "tolerance": 1e-6, "warm_start": "base solution",
                            "batch_seconds": batch_s, "optimal": ok, "sequential_seconds_estimate": seq_s,
-                           "sequential_sample": k_seq, "speedup": seq_s / batch_s if batch_s > 0 else None},
+                           "sequential_sample": k_seq, "speedup": seq_s / batch_s if batch_s > 0 else None,
+                           "mixed_batch_seconds": batch_mixed_s, "mixed_optimal": okm,
+                           "mixed_objective_rel_diff_max": dobj},
     })
     line["cpu_baseline"] = None
     return line
@@ -296,7 +310,8 @@ def bench_c4(args, ps26119, lpgen):
             d = arrivals[k] - (time.time() - t0)
             if d > 0:
                 time.sleep(d)
-            ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=args.time_limit)
+            ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=args.time_limit,
+                                precision=args.c4_precision)
 
     t = time.time()
     th = threading.Thread(target=feed)
@@ -317,10 +332,14 @@ def bench_c4(args, ps26119, lpgen):
     sequential = time.time() - t
     log(f"sequential: {sequential:.2f}s")
     value = N / served
-    line = base_line(args, value, "solves/s", True, "f64",
+    mixed = args.c4_precision == "mixed"
+    line = base_line(args, value, "solves/s", True, "f32+f64" if mixed else "f64",
                      {"workload": f"serving: {N} Poisson arrivals at {args.rate}/s of random LPs (6 per row, n=2m), m "
                                   f"log-uniform in [{args.m_lo}, {args.m_hi}], half at tolerance 1e-6 and half at 1e-8, "
-                                  f"HBM budget {args.budget_gb} GB, continuous batching", "requests": N})
+                                  f"HBM budget {args.budget_gb} GB, continuous batching, "
+                                  + ("mixed precision (fp32 increments on fp64 anchors)" if mixed else "fp64")
+                                  + "; baseline: the same requests as sequential plain fp64 solves",
+                      "requests": N})
     line.update({
         "roofline": None, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)), "served_seconds": served,
@@ -356,6 +375,7 @@ def main():
     ap.add_argument("--m-hi", type=int, default=20000,
                     help="c4 (at 2e5 rows a single request needs minutes: the default keeps the stream short)")
     ap.add_argument("--budget-gb", type=float, default=16.0, help="c4: HBM byte budget of the server")
+    ap.add_argument("--c4-precision", default="mixed", choices=["fp64", "mixed"], help="c4: precision of the served requests")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU: the r2HPDHG path here runs on a single MI355X; --gpus {args.gpus} "

@@ -14,7 +14,12 @@ extern "C" {
  * "restart_sufficient", "pid_kp", ..., plus the MI355X ones: "hip_cpu_order" (1 = every
  * reduction in the reference CPU order, bit-identical trajectory), "hip_timing" (1 = per-kernel
  * HIP event timing into ps26119_amd_stats), "hip_graphs" (1 = default: replay captured HIP
- * graphs of the iteration chunks; timing runs launch eagerly), "hip_device".
+ * graphs of the iteration chunks; timing runs launch eagerly), "hip_device", "hip_sliced"
+ * (1 = default: column-sliced dual half step for large fp64 tree-mode solves), "hip_refine"
+ * (mixed precision as fp32 increments on fp64 anchors; default 1, 0 in CPU-order mode: the
+ * reference's fp32-then-fp64 policy), "hip_refine_patience" (stalled checks before fp64 takes
+ * over, default 40), "hip_bf16_until" (> 0: the increment form starts with bf16 matrix values
+ * and moves to fp32 values at this relative KKT error or on a stall; default 0 = off).
  * presolve: 1 (default) / 0. */
 typedef struct {
   int presolve;

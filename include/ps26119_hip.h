@@ -155,7 +155,9 @@ int psh_sliced(psh_ctx* ctx, int on);
  * is read in fp32, and the anchor constants are rebuilt in fp64 at every restart, so the fp32
  * rounding is relative to the increment and each epoch refines the last. psh_chunk writes
  * x̂, x̄, ŷ, ȳ in fp64 on its hat steps, so psh_check / psh_restart work unchanged. Off
- * writes the current point into X / Y. Exclusive with psh_set_fp32(ctx, 1). */
+ * writes the current point into X / Y. Exclusive with psh_set_fp32(ctx, 1). `on` = 2 is the
+ * variant with bf16 matrix values in the increment SpMVs (fp64 products and row sums, fp32
+ * increments); 1 and 2 may be switched while on. */
 int psh_delta(psh_ctx* ctx, int on);
 int psh_check(psh_ctx* ctx, const psh_check_args* a, psh_stats* out);
 int psh_restart(psh_ctx* ctx); /* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ */
@@ -198,6 +200,14 @@ int psh_batch_chunk(psh_ctx* ctx, int iterations, const double* tau, const doubl
 int psh_batch_check(psh_ctx* ctx, int mode, double* out);
 /* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ for the slots with mask[t] != 0. */
 int psh_batch_restart(psh_ctx* ctx, const int* mask);
+/* Mixed precision in increment form for every slot (no counterpart in the reference, whose batch
+ * is fp64 only): each slot's iterate is its fp64 anchor x0 / y0 plus an fp32 increment, Ã is
+ * read in fp32, and a slot's anchor constants are rebuilt in fp64 (one fp64 SpMM pass for the
+ * re-anchored slots) at the chunk after its restart; x̂, x̄, ŷ, ȳ are written in fp64 on the
+ * hat steps, so psh_batch_check / _restart / _point work unchanged. On: every slot anchors at
+ * its x0 / y0 (call it where x = x0, y = y0: after psh_batch_set_iterate or a restart of all
+ * slots). Off writes the current points into x / y. */
+int psh_batch_delta(psh_ctx* ctx, int on);
 /* Keeps the slots keep[0..W2) in that order (finished scenarios leave the batch). */
 int psh_batch_compact(psh_ctx* ctx, const int* keep, int W2);
 /* Unscaled point of slot t: x = clip(C x̂ / bs, l, u) (n), y_min = R ŷ / os (m). */
@@ -225,9 +235,17 @@ typedef struct {
 int psh_server_create(int device, size_t budget_bytes, int max_instances, psh_server** out);
 void psh_server_destroy(psh_server* s);
 const char* psh_server_error(const psh_server* s);
-size_t psh_server_bytes(int m, int n, long long nnz);  /* arena bytes of one instance */
-size_t psh_server_free_bytes(const psh_server* s);     /* largest free block of the arena */
-int psh_server_add(psh_server* s, const psh_problem* p, int* id); /* PSH_E_OOM when it does not fit */
+/* Instance flags. PSH_SERVE_INCREMENT: also hold fp32 values and the increment-form state, and
+ * start the instance in increment form (mixed precision: fp32 matrix and increments around the
+ * fp64 anchor x0 / y0, rebuilt in fp64 at the chunk after each restart; x̂, x̄, ŷ, ȳ are still
+ * written in fp64 on the hat steps, so check / restart / download are unchanged). */
+enum { PSH_SERVE_INCREMENT = 1 };
+size_t psh_server_bytes(int m, int n, long long nnz, int flags); /* arena bytes of one instance */
+size_t psh_server_free_bytes(const psh_server* s);               /* largest free block of the arena */
+int psh_server_add(psh_server* s, const psh_problem* p, int flags, int* id); /* PSH_E_OOM when it does not fit */
+/* 1: increment form (needs PSH_SERVE_INCREMENT storage), 0: fp64, from the next chunk. Switch
+ * where x = x0 and y = y0 (after psh_server_set_iterate or a restart of the instance). */
+int psh_server_set_precision(psh_server* s, int id, int mixed);
 int psh_server_remove(psh_server* s, int id);
 int psh_server_set_iterate(psh_server* s, int id, const double* x, const double* y);
 int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count, double rho);

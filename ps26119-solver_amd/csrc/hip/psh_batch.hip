@@ -29,8 +29,9 @@ struct BArgs {
   double rho2, rho1;  // 2ρ, 1 − 2ρ
   const int *a_ptr, *a_col, *at_ptr, *at_col, *ato_col;
   const double *a_val, *at_val, *a_orig, *ato_val;
+  const float *a_valf, *at_valf;  // increment form
   const double *rs, *cs;
-  BatchState b;
+  BatchArrays b;
 };
 
 __device__ inline double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -80,6 +81,100 @@ __global__ void __launch_bounds__(kBlock) k_bdual(BArgs a) {
     a.b.yb[q] = 2 * h - yq;
   }
   a.b.y[q] = w * (a.rho2 * h + a.rho1 * yq) + (1 - w) * a.b.y0[q];
+}
+
+// ---- increment form (mixed precision): the single-solve form of psh_fused.hip per slot.
+// Anchor constants of slot t in fp64 from its anchor (x0, y0): code 2 rebuilds every constant
+// and clears δ, code 1 only KY = y0/σ (σ changed without a restart).
+__global__ void __launch_bounds__(kBlock) k_bdanchor_n(BArgs a) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long j = g / a.W;
+  if (j >= a.n) return;
+  const int t = static_cast<int>(g - j * a.W);
+  if (a.b.anchor[t] != 2) return;
+  const size_t q = static_cast<size_t>(g);
+  double s = 0.0;
+  for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) s += a.at_val[p] * a.b.y0[static_cast<size_t>(a.at_col[p]) * a.W + t];
+  const double x = a.b.x0[q];
+  a.b.G[q] = static_cast<float>(a.b.c[q] - s);
+  a.b.LX[q] = static_cast<float>(a.b.l[q] - x);
+  a.b.UX[q] = static_cast<float>(a.b.u[q] - x);
+  a.b.ix[q] = a.b.ixh[q] = a.b.ixb[q] = 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bdanchor_m(BArgs a) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long i = g / a.W;
+  if (i >= a.m) return;
+  const int t = static_cast<int>(g - i * a.W);
+  const int code = a.b.anchor[t];
+  if (code == 0) return;
+  const size_t q = static_cast<size_t>(g);
+  a.b.KY[q] = static_cast<float>(a.b.y0[q] / a.b.sigma[t]);
+  if (code != 2) return;
+  double s = 0.0;
+  for (int p = a.a_ptr[i]; p < a.a_ptr[i + 1]; ++p) s += a.a_val[p] * a.b.x0[static_cast<size_t>(a.a_col[p]) * a.W + t];
+  a.b.BL[q] = static_cast<float>(s - a.b.rl[q]);
+  a.b.BU[q] = static_cast<float>(s - a.b.ru[q]);
+  a.b.iy[q] = a.b.iyh[q] = a.b.iyb[q] = 0.0f;
+}
+
+// δx̂ = clamp(δx − τ(G − Ãᵀδy), LX, UX), δx̄ = 2δx̂ − δx, δx ← w(2ρδx̂ + (1−2ρ)δx); fp32 values and
+// increments, products and row sums in fp64 (CSR order).
+__global__ void __launch_bounds__(kBlock) k_bdprimal(BArgs a) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long j = g / a.W;
+  if (j >= a.n) return;
+  const int t = static_cast<int>(g - j * a.W);
+  double s = 0.0;
+  for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p)
+    s += static_cast<double>(a.at_valf[p]) * static_cast<double>(a.b.iy[static_cast<size_t>(a.at_col[p]) * a.W + t]);
+  const size_t q = static_cast<size_t>(g);
+  const float w = static_cast<float>(halpern_w(a.b.inner0[t], a.it));
+  const float tau = static_cast<float>(a.b.tau[t]);
+  const float xq = a.b.ix[q];
+  float h = xq - tau * (a.b.G[q] - static_cast<float>(s));
+  h = h < a.b.LX[q] ? a.b.LX[q] : (h > a.b.UX[q] ? a.b.UX[q] : h);
+  if (a.hat) a.b.ixh[q] = h;
+  a.b.ixb[q] = 2 * h - xq;
+  a.b.ix[q] = w * (static_cast<float>(a.rho2) * h + static_cast<float>(a.rho1) * xq);
+}
+
+// t = clamp(KY + δy/σ, BU + Ãδx̄, BL + Ãδx̄), δŷ = δy − σt, δȳ = 2δŷ − δy, δy ← w(2ρδŷ + (1−2ρ)δy).
+__global__ void __launch_bounds__(kBlock) k_bddual(BArgs a) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long i = g / a.W;
+  if (i >= a.m) return;
+  const int t = static_cast<int>(g - i * a.W);
+  double s = 0.0;
+  for (int p = a.a_ptr[i]; p < a.a_ptr[i + 1]; ++p)
+    s += static_cast<double>(a.a_valf[p]) * static_cast<double>(a.b.ixb[static_cast<size_t>(a.a_col[p]) * a.W + t]);
+  const size_t q = static_cast<size_t>(g);
+  const float w = static_cast<float>(halpern_w(a.b.inner0[t], a.it));
+  const float sg = static_cast<float>(a.b.sigma[t]);
+  const float ax = static_cast<float>(s);
+  const float yq = a.b.iy[q];
+  const float lo = a.b.BU[q] + ax, hi = a.b.BL[q] + ax;
+  float v = a.b.KY[q] + yq / sg;
+  v = v < lo ? lo : (v > hi ? hi : v);
+  const float h = yq - sg * v;
+  if (a.hat) {
+    a.b.iyh[q] = h;
+    a.b.iyb[q] = 2 * h - yq;
+  }
+  a.b.iy[q] = w * (static_cast<float>(a.rho2) * h + static_cast<float>(a.rho1) * yq);
+}
+
+// fp64 points from anchor + increment: ẑ and z̄ (hat steps), and z itself when `cur` is set.
+__global__ void k_bdpoint(const double* __restrict__ z0, const float* __restrict__ ih, const float* __restrict__ ib,
+                          const float* __restrict__ ic, double* __restrict__ zh, double* __restrict__ zb,
+                          double* __restrict__ zc, long long len) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= len) return;
+  const double a = z0[g];
+  zh[g] = a + static_cast<double>(ih[g]);
+  zb[g] = a + static_cast<double>(ib[g]);
+  if (zc) zc[g] = a + static_cast<double>(ic[g]);
 }
 
 // ---- residual and check passes (one thread per (row, slot))
@@ -248,7 +343,8 @@ __global__ void k_bred_seq(BRed r) {
   r.stats[static_cast<size_t>(t) * PSH_BSTAT_COUNT + J.stat] = s;
 }
 
-__global__ void k_bgather(const double* __restrict__ src, double* __restrict__ dst, long long rows, int W, int W2,
+template <class T>
+__global__ void k_bgather(const T* __restrict__ src, T* __restrict__ dst, long long rows, int W, int W2,
                           const int* __restrict__ keep) {
   const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= rows * W2) return;
@@ -294,6 +390,8 @@ BArgs make_bargs(psh_ctx* c, int it, int hat, double rho) {
   a.ato_col = c->ato_col;
   a.a_val = c->A.val;
   a.at_val = c->At.val;
+  a.a_valf = c->A.valf;
+  a.at_valf = c->At.valf;
   a.a_orig = c->a_orig;
   a.ato_val = c->ato_val;
   a.rs = c->rs;
@@ -309,7 +407,8 @@ void free_batch(psh_ctx* c) {
   if (!b) return;
   void* ptrs[] = {b->c, b->l, b->u, b->rl, b->ru, b->cmin, b->clo, b->cuo, b->rlo, b->ruo, b->bs, b->os, b->x, b->x0,
                   b->xh, b->xb, b->y, b->y0, b->yh, b->yb, b->dx, b->dy, b->adx, b->xo, b->yo, b->tau, b->sigma,
-                  b->inner0, b->mask, b->keep, b->terms, b->partials, b->stats, b->tmp};
+                  b->inner0, b->mask, b->keep, b->terms, b->partials, b->stats, b->tmp, b->ix, b->ixh, b->ixb,
+                  b->G, b->LX, b->UX, b->iy, b->iyh, b->iyb, b->BL, b->BU, b->KY, b->anchor};
   for (void* p : ptrs) dev_free(p);
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   if (b->pinned) (void)hipHostFree(b->pinned);
@@ -317,12 +416,13 @@ void free_batch(psh_ctx* c) {
   c->batch = nullptr;
 }
 
-// Pinned staging layout (bytes): params (tau, sigma, inner0) | mask | keep | stats.
+// Pinned staging layout (bytes): params (tau, sigma, inner0) | mask | keep | stats | anchor codes.
 constexpr size_t kPinParams = 0;
 constexpr size_t kPinMask = 3 * 8 * PSH_BATCH_MAX_W;
 constexpr size_t kPinKeep = kPinMask + 4 * PSH_BATCH_MAX_W;
 constexpr size_t kPinStats = kPinKeep + 4 * PSH_BATCH_MAX_W;
-constexpr size_t kPinBytes = kPinStats + 8 * PSH_BATCH_MAX_W * PSH_BSTAT_COUNT;
+constexpr size_t kPinAnchor = kPinStats + 8 * PSH_BATCH_MAX_W * PSH_BSTAT_COUNT;
+constexpr size_t kPinBytes = kPinAnchor + 4 * PSH_BATCH_MAX_W;
 
 int batch_reduce(psh_ctx* c, int mode, bool kkt, bool only_residual, int r0) {
   BatchState& b = *c->batch;
@@ -375,13 +475,52 @@ int batch_residual_pass(psh_ctx* c, bool kkt, int mode, bool r0) {
   return batch_reduce(c, mode, kkt, !kkt, r0 ? 1 : 0);
 }
 
-// Each pinned staging region (1 params, 2 mask, 4 keep) is reused by the next call of the same
-// kind: wait for a copy that may still read it.
+// Each pinned staging region (1 params, 2 mask, 4 keep, 8 anchor codes) is reused by the next
+// call of the same kind: wait for a copy that may still read it.
 int pinned_free(psh_ctx* c, unsigned region) {
   BatchState& b = *c->batch;
   if (!(b.pin_pending & region)) return PSH_OK;
   PSH_TRY(c, hipStreamSynchronize(c->stream));
   b.pin_pending = 0;
+  return PSH_OK;
+}
+
+// Increment form, before a chunk: rebuild the anchor constants of re-anchored slots (code 2) and
+// KY of slots whose σ changed (code 1).
+int batch_anchor(psh_ctx* c, const double* sigma, double rho) {
+  BatchState& b = *c->batch;
+  bool any = false, full = false;
+  int rc = pinned_free(c, 8);
+  if (rc) return rc;
+  int* pin = reinterpret_cast<int*>(b.pinned + kPinAnchor);
+  for (int t = 0; t < b.W; ++t) {
+    pin[t] = b.dirty[t] ? 2 : (sigma[t] != b.ky_sigma[t] ? 1 : 0);
+    any = any || pin[t] != 0;
+    full = full || pin[t] == 2;
+    b.dirty[t] = 0;
+    b.ky_sigma[t] = sigma[t];
+  }
+  if (!any) return PSH_OK;
+  PSH_TRY(c, hipMemcpyAsync(b.anchor, pin, sizeof(int) * b.W, hipMemcpyHostToDevice, c->stream));
+  b.pin_pending |= 8;
+  const BArgs a = make_bargs(c, 0, 0, rho);
+  if (full && c->n > 0) hipLaunchKernelGGL(k_bdanchor_n, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a);
+  if (c->m > 0) hipLaunchKernelGGL(k_bdanchor_m, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, a);
+  PSH_LAUNCHED(c, "batch anchors");
+  return PSH_OK;
+}
+
+// Increment form: x̂, x̄, ŷ, ȳ (and x, y with `cur`) of every slot in fp64.
+int batch_points(psh_ctx* c, bool cur) {
+  BatchState& b = *c->batch;
+  const long long NW = static_cast<long long>(c->n) * b.W, MW = static_cast<long long>(c->m) * b.W;
+  if (NW > 0)
+    hipLaunchKernelGGL(k_bdpoint, dim3(grid_for(NW)), dim3(kBlock), 0, c->stream, b.x0, b.ixh, b.ixb, b.ix, b.xh, b.xb,
+                       cur ? b.x : nullptr, NW);
+  if (MW > 0)
+    hipLaunchKernelGGL(k_bdpoint, dim3(grid_for(MW)), dim3(kBlock), 0, c->stream, b.y0, b.iyh, b.iyb, b.iy, b.yh, b.yb,
+                       cur ? b.y : nullptr, MW);
+  PSH_LAUNCHED(c, "batch points");
   return PSH_OK;
 }
 
@@ -428,8 +567,11 @@ int psh_batch_begin(psh_ctx* c, int W, const double* cc, const double* l, const 
       (rc = alloc_n(c, &b.mask, W)) || (rc = alloc_n(c, &b.keep, W)) || (rc = alloc_n(c, &b.terms, RW * kBTerms)) ||
       (rc = alloc_n(c, &b.partials, static_cast<long long>(kBTreeBlocks) * 20 * W)) ||
       (rc = alloc_n(c, &b.stats, static_cast<long long>(W) * PSH_BSTAT_COUNT)) ||
-      (rc = alloc_n(c, &b.tmp, std::max(RW, m + n))))  // gather staging, and one slot's x | y_min
+      (rc = alloc_n(c, &b.tmp, std::max(RW, m + n))) ||  // gather staging, and one slot's x | y_min
+      (rc = alloc_n(c, &b.anchor, W)))
     return rc;
+  b.dirty.assign(W, 1);
+  b.ky_sigma.assign(W, 0.0);
   if (hipHostMalloc(reinterpret_cast<void**>(&b.pinned), kPinBytes) != hipSuccess) return fail(c, PSH_E_OOM, "pinned staging");
   PSH_TRY(c, hipStreamSynchronize(c->stream));
   return PSH_OK;
@@ -466,6 +608,8 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
   b.pin_pending |= 1;
   bool any_new = false;
   for (int t = 0; t < W; ++t) any_new = any_new || inner0[t] == 0;
+  const bool delta = b.delta;
+  if (delta && (rc = batch_anchor(c, sigma, rho))) return rc;
   // Launch arguments depend only on (length, epoch start in any slot, width, mode): replay a
   // captured graph unless per-kernel timing is on.
   auto enqueue = [&]() -> int {
@@ -475,15 +619,21 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
       hipEvent_t ev;
       if (c->n > 0) {
         time_begin(c, 0, &ev);
-        hipLaunchKernelGGL(k_bprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
+        if (delta) hipLaunchKernelGGL(k_bdprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
+        else hipLaunchKernelGGL(k_bprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
         time_end(c, 0, ev);
       }
       if (c->m > 0) {
         time_begin(c, 1, &ev);
-        hipLaunchKernelGGL(k_bdual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
+        if (delta) hipLaunchKernelGGL(k_bddual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
+        else hipLaunchKernelGGL(k_bdual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
         time_end(c, 1, ev);
       }
       PSH_LAUNCHED(c, "batch iteration");
+      if (delta && hat) {
+        const int r = batch_points(c, false);
+        if (r) return r;
+      }
       if (s == 0 && any_new) {  // residuals of every slot after the first step (reference batch.cpp:347-354)
         const int r = batch_residual_pass(c, false, mode, true);
         if (r) return r;
@@ -494,8 +644,8 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
   if (c->timing || !c->graphs) {
     if ((rc = enqueue())) return rc;
   } else {
-    const long long key = (static_cast<long long>(iterations) << 16) | (static_cast<long long>(W) << 4) |
-                          (any_new ? 1 : 0) | (static_cast<long long>(mode & 7) << 1);
+    const long long key = (static_cast<long long>(iterations) << 17) | (static_cast<long long>(W) << 5) |
+                          (any_new ? 1 : 0) | (static_cast<long long>(mode & 7) << 1) | (delta ? 16 : 0);
     auto it = b.graphs.find(key);
     if (it == b.graphs.end()) {
       PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -554,6 +704,32 @@ int psh_batch_restart(psh_ctx* c, const int* mask) {
     hipLaunchKernelGGL(k_brestart, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, b.y, b.y0, b.yh, b.mask,
                        (long long)c->m, b.W);
   PSH_LAUNCHED(c, "batch restart");
+  for (int t = 0; t < b.W; ++t)
+    if (mask[t]) b.dirty[t] = 1;  // increment form: new anchors at the next chunk
+  return PSH_OK;
+}
+
+int psh_batch_delta(psh_ctx* c, int on) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  const bool want = on != 0;
+  if (want == b.delta) return PSH_OK;
+  if (want) {
+    if ((rc = ensure_fp32_matrix(c))) return rc;
+    if (!b.ix) {
+      const long long NW = static_cast<long long>(c->n) * b.cap, MW = static_cast<long long>(c->m) * b.cap;
+      for (float** v : {&b.ix, &b.ixh, &b.ixb, &b.G, &b.LX, &b.UX})
+        if ((rc = alloc_n(c, v, NW))) return rc;
+      for (float** v : {&b.iy, &b.iyh, &b.iyb, &b.BL, &b.BU, &b.KY})
+        if ((rc = alloc_n(c, v, MW))) return rc;
+    }
+    std::fill(b.dirty.begin(), b.dirty.end(), 1);  // every slot anchors at x0 / y0 with δ = 0
+    b.delta = true;
+    return PSH_OK;
+  }
+  if ((rc = batch_points(c, true))) return rc;  // the current points back into the fp64 arrays
+  b.delta = false;
   return PSH_OK;
 }
 
@@ -570,10 +746,13 @@ int psh_batch_compact(psh_ctx* c, const int* keep, int W2) {
   std::memcpy(pin, keep, sizeof(int) * W2);
   PSH_TRY(c, hipMemcpyAsync(b.keep, pin, sizeof(int) * std::max(W2, 1), hipMemcpyHostToDevice, c->stream));
   const long long m = c->m, n = c->n;
-  auto squeeze = [&](double* v, long long rows) -> int {
+  auto squeeze = [&](auto* v, long long rows) -> int {
+    using T = std::remove_pointer_t<decltype(v)>;
     if (rows * W2 == 0) return PSH_OK;
-    hipLaunchKernelGGL(k_bgather, dim3(grid_for(rows * W2)), dim3(kBlock), 0, c->stream, v, b.tmp, rows, b.W, W2, b.keep);
-    PSH_TRY(c, hipMemcpyAsync(v, b.tmp, sizeof(double) * rows * W2, hipMemcpyDeviceToDevice, c->stream));
+    T* tmp = reinterpret_cast<T*>(b.tmp);  // rows_max·cap doubles hold rows·W2 values of either type
+    hipLaunchKernelGGL((k_bgather<T>), dim3(grid_for(rows * W2)), dim3(kBlock), 0, c->stream, v, tmp, rows, b.W, W2,
+                       b.keep);
+    PSH_TRY(c, hipMemcpyAsync(v, tmp, sizeof(T) * rows * W2, hipMemcpyDeviceToDevice, c->stream));
     return PSH_OK;
   };
   for (double* v : {b.c, b.l, b.u, b.cmin, b.clo, b.cuo, b.x, b.x0, b.xh, b.xb})
@@ -581,7 +760,18 @@ int psh_batch_compact(psh_ctx* c, const int* keep, int W2) {
   for (double* v : {b.rl, b.ru, b.rlo, b.ruo, b.y, b.y0, b.yh, b.yb})
     if ((rc = squeeze(v, m))) return rc;
   if ((rc = squeeze(b.bs, 1)) || (rc = squeeze(b.os, 1))) return rc;
+  if (b.ix) {  // increment-form state (kept while off: it is rebuilt on the next switch anyway)
+    for (float* v : {b.ix, b.ixh, b.ixb, b.G, b.LX, b.UX})
+      if ((rc = squeeze(v, n))) return rc;
+    for (float* v : {b.iy, b.iyh, b.iyb, b.BL, b.BU, b.KY})
+      if ((rc = squeeze(v, m))) return rc;
+  }
   PSH_LAUNCHED(c, "batch compaction");
+  std::vector<char> dirty(W2);
+  std::vector<double> kys(W2);
+  for (int k = 0; k < W2; ++k) dirty[k] = b.dirty[keep[k]], kys[k] = b.ky_sigma[keep[k]];
+  b.dirty.swap(dirty);
+  b.ky_sigma.swap(kys);
   b.W = W2;
   PSH_TRY(c, hipStreamSynchronize(c->stream));
   b.pin_pending = 0;

@@ -293,6 +293,14 @@ __global__ void k_convert(const S* __restrict__ a, D* __restrict__ b, long long 
   if (i < n) b[i] = static_cast<D>(a[i]);
 }
 
+// fp64 → fp32 (round to nearest) → bf16 (round to nearest, ties to even); values are finite.
+__global__ void k_to_bf16(const double* __restrict__ a, Bf16* __restrict__ b, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned u = __float_as_uint(static_cast<float>(a[i]));
+  b[i].bits = static_cast<unsigned short>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
 template <class T>
 __global__ void k_fill(T* __restrict__ v, T a, long long n) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -439,6 +447,19 @@ int psh::ensure_fp32_matrix(psh_ctx* c) {
   return PSH_OK;
 }
 
+// bf16 copies of Ã's and Ãᵀ's values (the increment form's bf16 variant).
+int psh::ensure_bf16_matrix(psh_ctx* c) {
+  if (c->A.valh) return PSH_OK;
+  int rc;
+  if ((rc = alloc_n(c, &c->A.valh, c->nnz)) || (rc = alloc_n(c, &c->At.valh, c->nnz))) return rc;
+  if (c->nnz > 0) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->A.val, c->A.valh, c->nnz);
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->At.val, c->At.valh, c->nnz);
+  }
+  PSH_LAUNCHED(c, "bf16 matrix");
+  return PSH_OK;
+}
+
 namespace {
 
 int ensure_fp32(psh_ctx* c) {
@@ -550,7 +571,7 @@ void psh_destroy(psh_ctx* c) {
     for (void* p : dp) dev_free(p);
   }
   void* ptrs[] = {c->A.ptr, c->A.col, c->A.val, c->A.valf, c->A.tiles, c->At.ptr, c->At.col, c->At.val, c->At.valf,
-                  c->At.tiles, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
+                  c->At.tiles, c->A.valh, c->At.valh, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
                   c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->As.seg, c->As.col,

@@ -156,11 +156,13 @@ struct EpiDeltaP {
   }
 };
 
+// V = float, or Bf16 for the variant with bf16 matrix values (fp64 products and sums either way).
+template <class V>
 __global__ void __launch_bounds__(kBlock) k_delta_primal(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
-                                                         const int* __restrict__ col, const float* __restrict__ val,
+                                                         const int* __restrict__ col, const V* __restrict__ val,
                                                          const float* __restrict__ dy, DeltaP p) {
   EpiDeltaP e(p);
-  tiled_rows<float, float>(tiles, ntiles, ptr, col, val, dy, e);
+  tiled_rows<V, float>(tiles, ntiles, ptr, col, val, dy, e);
 }
 
 struct DeltaD {
@@ -191,11 +193,12 @@ struct EpiDeltaD {
   }
 };
 
+template <class V>
 __global__ void __launch_bounds__(kBlock) k_delta_dual(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
-                                                       const int* __restrict__ col, const float* __restrict__ val,
+                                                       const int* __restrict__ col, const V* __restrict__ val,
                                                        const float* __restrict__ dxb, DeltaD p) {
   EpiDeltaD e(p);
-  tiled_rows<float, float>(tiles, ntiles, ptr, col, val, dxb, e);
+  tiled_rows<V, float>(tiles, ntiles, ptr, col, val, dxb, e);
 }
 
 // Anchor constants of an epoch (fp64 arithmetic, one rounding to fp32); resets δ when asked.
@@ -818,14 +821,22 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
       hipEvent_t ev;
       if (c->At.ntiles > 0) {
         time_begin(c, 0, &ev);
-        hipLaunchKernelGGL(k_delta_primal, dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream, c->At.tiles,
-                           c->At.ntiles, c->At.ptr, c->At.col, c->At.valf, d.dy, pa);
+        if (d.bf16)
+          hipLaunchKernelGGL(k_delta_primal<Bf16>, dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
+                             c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, c->At.valh, d.dy, pa);
+        else
+          hipLaunchKernelGGL(k_delta_primal<float>, dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
+                             c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, c->At.valf, d.dy, pa);
         time_end(c, 0, ev);
       }
       if (c->A.ntiles > 0) {
         time_begin(c, 1, &ev);
-        hipLaunchKernelGGL(k_delta_dual, dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, c->A.tiles,
-                           c->A.ntiles, c->A.ptr, c->A.col, c->A.valf, d.dxb, da);
+        if (d.bf16)
+          hipLaunchKernelGGL(k_delta_dual<Bf16>, dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
+                             c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, c->A.valh, d.dxb, da);
+        else
+          hipLaunchKernelGGL(k_delta_dual<float>, dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
+                             c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, c->A.valf, d.dxb, da);
         time_end(c, 1, ev);
       }
       PSH_LAUNCHED(c, "increment-form iteration kernels");
@@ -891,9 +902,9 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
   if (c->timing || !c->graphs) {
     if ((rc = delta ? enqueue_delta() : enqueue())) return rc;
   } else {
-    const long long key = (static_cast<long long>(g->iterations) << 8) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
+    const long long key = (static_cast<long long>(g->iterations) << 9) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
                           (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3) | (delta ? 1LL << 6 : 0) |
-                          (use_slices ? 1LL << 7 : 0);
+                          (use_slices ? 1LL << 7 : 0) | (delta && c->delta.bf16 ? 1LL << 8 : 0);
     auto it = c->graph_cache.find(key);
     if (it == c->graph_cache.end()) {
       PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -932,12 +943,19 @@ int psh_sliced(psh_ctx* c, int on) {
 int psh_delta(psh_ctx* c, int on) {
   int rc = check_slots(c);
   if (rc) return rc;
+  if (on < 0 || on > 2) return fail(c, PSH_E_ARG, "psh_delta: on must be 0, 1 or 2");
   DeltaState& d = c->delta;
   const bool want = on != 0;
+  if (want && d.on) {  // switching the value type only
+    if (on == 2 && (rc = ensure_bf16_matrix(c))) return rc;
+    d.bf16 = on == 2;
+    return PSH_OK;
+  }
   if (want == d.on) return PSH_OK;
   if (want) {
     if (c->fp32) return fail(c, PSH_E_STATE, "fp32 storage and the increment form are exclusive");
-    if ((rc = ensure_fp32_matrix(c))) return rc;
+    if ((rc = on == 2 ? ensure_bf16_matrix(c) : ensure_fp32_matrix(c))) return rc;
+    d.bf16 = on == 2;
     if (!d.dx) {
       for (float** v : {&d.dx, &d.dxh, &d.dxb, &d.G, &d.LX, &d.UX})
         if ((rc = alloc_n(c, v, c->n))) return rc;

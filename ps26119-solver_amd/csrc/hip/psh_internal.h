@@ -29,12 +29,22 @@ constexpr int kTileNnz = 2048;   // products staged in LDS per tile pass (16 KiB
 constexpr int kXcd = 8;          // XCDs per MI355X
 constexpr int kTreeBlocks = 512; // fixed first-level block count of tree reductions
 
+// A bf16 matrix value: the top 16 bits of an fp32, rounded to nearest even from the fp64 value's
+// fp32 rounding. Products with it are formed in fp64 like every other value type.
+struct Bf16 {
+  unsigned short bits;
+  __device__ explicit operator double() const {
+    return static_cast<double>(__uint_as_float(static_cast<unsigned>(bits) << 16));
+  }
+};
+
 struct DevMat {
   int rows = 0;
   int* ptr = nullptr;     // rows+1
   int* col = nullptr;     // nnz
   double* val = nullptr;  // nnz, fp64
   float* valf = nullptr;  // nnz, fp32 copy (allocated on first switch to fp32)
+  Bf16* valh = nullptr;   // nnz, bf16 copy (increment form with bf16 values only)
   int ntiles = 0;
   int* tiles = nullptr;   // ntiles+1 tile start rows
 };
@@ -89,16 +99,18 @@ enum Result {
 enum Max { M_PMAX, M_DMAX, M_CINF, M_RMAX, M_DVIOL, M_DMAXRAY, M_PVIOL, M_RMAX2, M_DVIOL2, M_DMAXRAY2, M_PVIOL2, kMaxCount };
 enum Flag : unsigned { F_RES = 1, F_KKT = 2, F_RAY = 4, F_DRIFT = 8, F_PID = 16 };
 
-// Device state of a batch of scenarios (interleaved block vectors, W slots of `cap`).
 // Mixed precision in increment form: fp32 δ around the fp64 anchors in slots X0 / Y0.
 struct DeltaState {
   bool on = false, dirty = true;
+  bool bf16 = false;   // Ã's values of the increment SpMVs in bf16 (else fp32)
   double sigma = 0.0;  // σ the dual constants were built with
   float *dx = nullptr, *dxh = nullptr, *dxb = nullptr, *G = nullptr, *LX = nullptr, *UX = nullptr;
   float *dy = nullptr, *dyh = nullptr, *dyb = nullptr, *BL = nullptr, *BU = nullptr, *KY = nullptr;
 };
 
-struct BatchState {
+// Device arrays of a batch of scenarios (interleaved block vectors, W slots of `cap`); plain
+// data, passed to the batch kernels by value.
+struct BatchArrays {
   int W = 0, cap = 0;
   long long rows_max = 0;
   double *c = nullptr, *l = nullptr, *u = nullptr, *rl = nullptr, *ru = nullptr;
@@ -113,10 +125,21 @@ struct BatchState {
   double* partials = nullptr;  // tree partials
   double* stats = nullptr;     // cap · PSH_BSTAT_COUNT
   double* tmp = nullptr;       // compaction buffer, rows_max·cap
-  char* pinned = nullptr;      // host staging: params | mask | keep | stats
+  // increment form (psh_batch_delta): fp32 δ of every slot around its fp64 anchor x0 / y0, and
+  // the anchor constants G, LX, UX (n·cap) and BL, BU, KY (m·cap) of the single-solve form
+  float *ix = nullptr, *ixh = nullptr, *ixb = nullptr, *G = nullptr, *LX = nullptr, *UX = nullptr;
+  float *iy = nullptr, *iyh = nullptr, *iyb = nullptr, *BL = nullptr, *BU = nullptr, *KY = nullptr;
+  int* anchor = nullptr;       // per-slot codes of the next anchor pass (1: KY only, 2: all)
+};
+
+struct BatchState : BatchArrays {
+  char* pinned = nullptr;      // host staging: params | mask | keep | stats | anchor codes
   bool have_r0 = false;
   unsigned pin_pending = 0;                    // regions of `pinned` a queued copy may still read
   std::map<long long, hipGraphExec_t> graphs;  // captured chunks
+  bool delta = false;
+  std::vector<double> ky_sigma;  // σ each slot's KY was built with
+  std::vector<char> dirty;       // slots to re-anchor before the next chunk
 };
 
 }  // namespace psh
@@ -234,6 +257,7 @@ int fetch_results(psh_ctx* c, int count, double* out);
 
 // fp32 copies of the matrix values (psh_core.hip)
 int ensure_fp32_matrix(psh_ctx* c);
+int ensure_bf16_matrix(psh_ctx* c);
 
 // frees the batch state of a context (psh_batch.hip)
 void destroy_batch(psh_ctx* c);

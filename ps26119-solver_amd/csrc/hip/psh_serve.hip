@@ -36,6 +36,9 @@ struct Inst {
   const double *a_val, *at_val, *a_orig, *ato_val;
   const double *c, *l, *u, *rl, *ru, *rs, *cs, *clo, *cuo, *rlo, *ruo, *cmin;
   double *x, *x0, *xh, *xb, *dx, *xo, *y, *y0, *yh, *yb, *dy, *yo;
+  // increment form (PSH_SERVE_INCREMENT): fp32 values, δx, δx̄, δy and the anchor constants
+  const float *a_valf, *at_valf;
+  float *ix, *ixb, *G, *LX, *UX, *iy, *BL, *BU, *KY;
   double bs, os;
   int pt0, pt1, dt0, dt1;  // this instance's tiles in the primal / dual tables
 };
@@ -43,6 +46,8 @@ struct Inst {
 struct Param {
   double tau, sigma, inner0;
   int active, restart;
+  int mixed;   // 1: this chunk runs the instance in increment form
+  int anchor;  // anchor pass before the chunk: 1 KY only (σ changed), 2 every constant and δ = 0
 };
 
 struct Tile {
@@ -88,6 +93,47 @@ struct SrvEpiDual {
   }
 };
 
+// Increment form (psh_fused.hip's, per instance): z = z₀ + δ with the anchor z₀ in x0 / y0; on
+// hat steps the epilogue also writes x̂, x̄ (ŷ, ȳ) in fp64, so the check and restart kernels read
+// fp64 points in either form.
+struct SrvEpiDeltaP {
+  const Inst* d;
+  float tau, tw, a, b;
+  int hat;
+  __device__ void operator()(int j, bool has, double s) const {
+    if (!has) return;
+    const float xj = d->ix[j];
+    const float h = clampv<float>(xj - tau * (d->G[j] - static_cast<float>(s)), d->LX[j], d->UX[j]);
+    const float xb = 2 * h - xj;
+    d->ixb[j] = xb;
+    d->ix[j] = tw * (a * h + b * xj);
+    if (hat) {
+      const double a0 = d->x0[j];
+      d->xh[j] = a0 + static_cast<double>(h);
+      d->xb[j] = a0 + static_cast<double>(xb);
+    }
+  }
+};
+
+struct SrvEpiDeltaD {
+  const Inst* d;
+  float sg, tw, a, b;
+  int hat;
+  __device__ void operator()(int i, bool has, double s) const {
+    if (!has) return;
+    const float ax = static_cast<float>(s);
+    const float yi = d->iy[i];
+    const float t = clampv<float>(d->KY[i] + yi / sg, d->BU[i] + ax, d->BL[i] + ax);
+    const float h = yi - sg * t;
+    d->iy[i] = tw * (a * h + b * yi);
+    if (hat) {
+      const double a0 = d->y0[i];
+      d->yh[i] = a0 + static_cast<double>(h);
+      d->yb[i] = a0 + static_cast<double>(2 * h - yi);
+    }
+  }
+};
+
 __global__ void __launch_bounds__(kBlock) k_srv_primal(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
                                                        const Param* __restrict__ P, int step, int last, double a2,
                                                        double b2) {
@@ -98,6 +144,12 @@ __global__ void __launch_bounds__(kBlock) k_srv_primal(const Tile* __restrict__ 
   if (!q.active) return;
   const Inst* d = I + T.inst;
   const int hat = (step == 0 && q.inner0 == 0.0) || last ? 1 : 0;
+  if (q.mixed) {
+    SrvEpiDeltaP e{d, static_cast<float>(q.tau), static_cast<float>(halpern_w(q.inner0, step)), static_cast<float>(a2),
+                   static_cast<float>(b2), hat};
+    tile_rows<float, float>(T.r0, T.r1, d->at_ptr, d->at_col, d->at_valf, d->iy, e);
+    return;
+  }
   SrvEpiPrimal e{d, q.tau, halpern_w(q.inner0, step), a2, b2, hat};
   tile_rows<double, double>(T.r0, T.r1, d->at_ptr, d->at_col, d->at_val, d->y, e);
 }
@@ -112,8 +164,71 @@ __global__ void __launch_bounds__(kBlock) k_srv_dual(const Tile* __restrict__ ti
   if (!q.active) return;
   const Inst* d = I + T.inst;
   const int hat = (step == 0 && q.inner0 == 0.0) || last ? 1 : 0;
+  if (q.mixed) {
+    SrvEpiDeltaD e{d, static_cast<float>(q.sigma), static_cast<float>(halpern_w(q.inner0, step)),
+                   static_cast<float>(a2), static_cast<float>(b2), hat};
+    tile_rows<float, float>(T.r0, T.r1, d->a_ptr, d->a_col, d->a_valf, d->ixb, e);
+    return;
+  }
   SrvEpiDual e{d, q.sigma, halpern_w(q.inner0, step), a2, b2, hat};
   tile_rows<double, double>(T.r0, T.r1, d->a_ptr, d->a_col, d->a_val, d->xb, e);
+}
+
+// Anchor constants of the increment form in fp64 (one rounding to fp32), per instance tile:
+// primal tiles G = c̃ − Ãᵀy₀, LX = l̃ − x₀, UX = ũ − x₀ and δx = δx̄ = 0 (code 2); dual tiles
+// BL = Ãx₀ − r̃l, BU = Ãx₀ − r̃u and δy = 0 (code 2), KY = y₀/σ (codes 1 and 2).
+__global__ void __launch_bounds__(kBlock) k_srv_anchor_p(const Tile* __restrict__ tiles, int ntiles,
+                                                         const Inst* __restrict__ I, const Param* __restrict__ P) {
+  const int t = tile_of_block(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;
+  const Tile T = tiles[t];
+  const Param q = P[T.inst];
+  if (!q.active || q.anchor != 2) return;
+  const Inst* d = I + T.inst;
+  struct Epi {
+    const Inst* d;
+    __device__ void operator()(int j, bool has, double s) const {
+      if (!has) return;
+      const double x = d->x0[j];
+      d->G[j] = static_cast<float>(d->c[j] - s);
+      d->LX[j] = static_cast<float>(d->l[j] - x);
+      d->UX[j] = static_cast<float>(d->u[j] - x);
+      d->ix[j] = d->ixb[j] = 0.0f;
+    }
+  } e{d};
+  tile_rows<double, double>(T.r0, T.r1, d->at_ptr, d->at_col, d->at_val, d->y0, e);
+}
+
+__global__ void __launch_bounds__(kBlock) k_srv_anchor_d(const Tile* __restrict__ tiles, int ntiles,
+                                                         const Inst* __restrict__ I, const Param* __restrict__ P) {
+  const int t = tile_of_block(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;
+  const Tile T = tiles[t];
+  const Param q = P[T.inst];
+  if (!q.active || q.anchor == 0) return;
+  const Inst* d = I + T.inst;
+  if (q.anchor == 1) {
+    const int i = T.r0 + static_cast<int>(threadIdx.x);
+    if (i < T.r1) d->KY[i] = static_cast<float>(d->y0[i] / q.sigma);
+    return;
+  }
+  struct Epi {
+    const Inst* d;
+    double sigma;
+    __device__ void operator()(int i, bool has, double s) const {
+      if (!has) return;
+      d->BL[i] = static_cast<float>(s - d->rl[i]);
+      d->BU[i] = static_cast<float>(s - d->ru[i]);
+      d->KY[i] = static_cast<float>(d->y0[i] / sigma);
+      d->iy[i] = 0.0f;
+    }
+  } e{d, q.sigma};
+  tile_rows<double, double>(T.r0, T.r1, d->a_ptr, d->a_col, d->a_val, d->x0, e);
+}
+
+__global__ void k_srv_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = static_cast<float>(a[i]);
 }
 
 // ------------------------------------------------------------------ check kernels
@@ -373,6 +488,10 @@ struct psh_server {
     int m = 0, n = 0;
     std::vector<int> tA, tAt;  // tile row starts of Ã and Ãᵀ
     Inst inst{};
+    bool incr = false;         // increment-form storage present
+    bool mixed = false;        // running in increment form
+    bool dirty = true;         // anchor constants to rebuild before the next chunk
+    double ky_sigma = 0.0;     // σ the constant KY was built with
   };
   std::vector<Slot> slots;
   int max_inst = 0;
@@ -417,10 +536,11 @@ struct Layout {
   size_t a_val, at_val, a_orig, ato_val;
   size_t c, l, u, cs, clo, cuo, cmin, x, x0, xh, xb, dx, xo;
   size_t rl, ru, rs, rlo, ruo, y, y0, yh, yb, dy, yo;
+  size_t a_valf, at_valf, ix, ixb, G, LX, UX, iy, BL, BU, KY;  // with PSH_SERVE_INCREMENT
   size_t total;
 };
 
-Layout layout_of(long long m, long long n, long long nnz) {
+Layout layout_of(long long m, long long n, long long nnz, int flags) {
   Layout L{};
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -440,6 +560,12 @@ Layout layout_of(long long m, long long n, long long nnz) {
   for (size_t* f : {&L.c, &L.l, &L.u, &L.cs, &L.clo, &L.cuo, &L.cmin, &L.x, &L.x0, &L.xh, &L.xb, &L.dx, &L.xo})
     *f = take(8 * n);
   for (size_t* f : {&L.rl, &L.ru, &L.rs, &L.rlo, &L.ruo, &L.y, &L.y0, &L.yh, &L.yb, &L.dy, &L.yo}) *f = take(8 * m);
+  if (flags & PSH_SERVE_INCREMENT) {
+    L.a_valf = take(4 * nnz);
+    L.at_valf = take(4 * nnz);
+    for (size_t* f : {&L.ix, &L.ixb, &L.G, &L.LX, &L.UX}) *f = take(4 * n);
+    for (size_t* f : {&L.iy, &L.BL, &L.BU, &L.KY}) *f = take(4 * m);
+  }
   L.total = o;
   return L;
 }
@@ -543,7 +669,7 @@ int grid_tiles(int ntiles) { return std::max(kXcd, grid_for_tiles(ntiles)); }
 
 extern "C" {
 
-size_t psh_server_bytes(int m, int n, long long nnz) { return layout_of(m, n, nnz).total; }
+size_t psh_server_bytes(int m, int n, long long nnz, int flags) { return layout_of(m, n, nnz, flags).total; }
 
 int psh_server_create(int device, size_t budget_bytes, int max_instances, psh_server** out) {
   if (!out || max_instances < 1 || budget_bytes < kAlign) return PSH_E_ARG;
@@ -608,14 +734,15 @@ size_t psh_server_free_bytes(const psh_server* s) {
   return best;
 }
 
-int psh_server_add(psh_server* s, const psh_problem* p, int* id) {
+int psh_server_add(psh_server* s, const psh_problem* p, int flags, int* id) {
   if (!s || !p || !id) return PSH_E_ARG;
   *id = -1;
   if (p->m < 0 || p->n < 0 || p->nnz < 0 || p->nnz >= (1LL << 31)) return sfail(s, PSH_E_ARG, "bad problem sizes");
+  if (flags & ~PSH_SERVE_INCREMENT) return sfail(s, PSH_E_ARG, "unknown instance flags");
   int k = 0;
   while (k < s->max_inst && s->slots[k].used) ++k;
   if (k == s->max_inst) return sfail(s, PSH_E_OOM, "all instance slots are in use");
-  const Layout L = layout_of(p->m, p->n, p->nnz);
+  const Layout L = layout_of(p->m, p->n, p->nnz, flags);
   size_t off = 0;
   if (!arena_alloc(s, L.total, &off)) return sfail(s, PSH_E_OOM, "HBM budget exhausted");
   SRV_TRY(s, hipSetDevice(s->device));
@@ -645,6 +772,14 @@ int psh_server_add(psh_server* s, const psh_problem* p, int* id) {
   d.x = D(L.x), d.x0 = D(L.x0), d.xh = D(L.xh), d.xb = D(L.xb), d.dx = D(L.dx), d.xo = D(L.xo);
   d.rl = D(L.rl), d.ru = D(L.ru), d.rs = D(L.rs), d.rlo = D(L.rlo), d.ruo = D(L.ruo);
   d.y = D(L.y), d.y0 = D(L.y0), d.yh = D(L.yh), d.yb = D(L.yb), d.dy = D(L.dy), d.yo = D(L.yo);
+  if (flags & PSH_SERVE_INCREMENT) {
+    auto F = [&](size_t o) { return reinterpret_cast<float*>(base + o); };
+    d.a_valf = F(L.a_valf);
+    d.at_valf = F(L.at_valf);
+    d.ix = F(L.ix), d.ixb = F(L.ixb), d.G = F(L.G), d.LX = F(L.LX), d.UX = F(L.UX);
+    d.iy = F(L.iy), d.BL = F(L.BL), d.BU = F(L.BU), d.KY = F(L.KY);
+    sl.incr = sl.mixed = true;
+  }
   d.bs = p->bound_scale;
   d.os = p->obj_scale;
   const long long m = p->m, n = p->n, nnz = p->nnz;
@@ -667,6 +802,13 @@ int psh_server_add(psh_server* s, const psh_problem* p, int* id) {
     arena_free(s, sl.off, sl.bytes);
     sl = psh_server::Slot{};
     return rc;
+  }
+  if (sl.incr && nnz > 0) {
+    hipLaunchKernelGGL(k_srv_to_f32, dim3(grid_for(nnz)), dim3(kBlock), 0, s->stream, d.a_val,
+                       const_cast<float*>(d.a_valf), nnz);
+    hipLaunchKernelGGL(k_srv_to_f32, dim3(grid_for(nnz)), dim3(kBlock), 0, s->stream, d.at_val,
+                       const_cast<float*>(d.at_valf), nnz);
+    SRV_LAUNCHED(s, "fp32 values");
   }
   SRV_TRY(s, hipStreamSynchronize(s->stream));  // the caller's arrays may go away
   sl.tA = build_tiles(p->a_ptr, p->m);
@@ -697,6 +839,16 @@ int psh_server_set_iterate(psh_server* s, int id, const double* x, const double*
   for (double* dst : {d.y, d.y0, d.yh, d.yb})
     if (d.m > 0) SRV_TRY(s, hipMemcpyAsync(dst, y, 8 * static_cast<size_t>(d.m), hipMemcpyHostToDevice, s->stream));
   SRV_TRY(s, hipStreamSynchronize(s->stream));
+  s->slots[id].dirty = true;
+  return PSH_OK;
+}
+
+int psh_server_set_precision(psh_server* s, int id, int mixed) {
+  if (!s || id < 0 || id >= s->max_inst || !s->slots[id].used) return PSH_E_ARG;
+  auto& sl = s->slots[id];
+  if (mixed && !sl.incr) return sfail(s, PSH_E_STATE, "instance was added without increment-form storage");
+  if (sl.mixed != (mixed != 0)) sl.dirty = true;
+  sl.mixed = mixed != 0;
   return PSH_OK;
 }
 
@@ -716,9 +868,28 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
     q.inner0 = static_cast<double>(params[k].inner0);
     q.active = 1;
   }
+  bool any_anchor = false;
+  for (int k = 0; k < count; ++k) {
+    const int id = params[k].id;
+    auto& sl = s->slots[id];
+    Param& q = s->h_param[id];
+    q.mixed = sl.mixed ? 1 : 0;
+    q.anchor = !sl.mixed ? 0 : (sl.dirty ? 2 : (q.sigma != sl.ky_sigma ? 1 : 0));
+    any_anchor = any_anchor || q.anchor != 0;
+    if (sl.mixed) sl.dirty = false, sl.ky_sigma = q.sigma;
+  }
   if ((rc = rebuild_tables(s))) return rc;
   if ((rc = upload_params(s))) return rc;
   if (count == 0 || (s->ntp == 0 && s->ntd == 0)) return PSH_OK;
+  if (any_anchor) {
+    if (s->ntp > 0)
+      hipLaunchKernelGGL(k_srv_anchor_p, dim3(grid_tiles(s->ntp)), dim3(kBlock), 0, s->stream, s->d_ptiles, s->ntp,
+                         s->d_inst, s->d_param);
+    if (s->ntd > 0)
+      hipLaunchKernelGGL(k_srv_anchor_d, dim3(grid_tiles(s->ntd)), dim3(kBlock), 0, s->stream, s->d_dtiles, s->ntd,
+                         s->d_inst, s->d_param);
+    SRV_LAUNCHED(s, "serving anchors");
+  }
   const double a2 = 2 * rho, b2 = 1 - 2 * rho;
   auto enqueue = [&]() -> int {
     for (int st = 0; st < iterations; ++st) {
@@ -826,6 +997,7 @@ int psh_server_restart(psh_server* s, const int* ids, int count) {
     const int id = ids[k];
     if (id < 0 || id >= s->max_inst || !s->slots[id].used) return sfail(s, PSH_E_ARG, "bad instance id");
     s->h_param[id].restart = 1;
+    s->slots[id].dirty = true;  // increment form: new anchors at the next chunk
   }
   if ((rc = upload_params(s))) return rc;
   if (s->ntp > 0)

@@ -2,8 +2,11 @@
 // src/pdhg/batch.cpp:129-441; include/ps26119/batch.h:10-33). Decisions per scenario on the host
 // (restarts, PID primal weight, termination, compaction) exactly as the reference; the
 // iterations, residuals and KKT sums of all live scenarios on the device (psh_batch_*).
-// Additions the reference does not have: the GPU, and an optional warm start of every scenario
-// from Options::warm_x / warm_y / warm_primal_weight (e.g. the base model's solution).
+// Additions the reference does not have: the GPU, an optional warm start of every scenario
+// from Options::warm_x / warm_y / warm_primal_weight (e.g. the base model's solution), and
+// precision = mixed: every slot in increment form (fp32 matrix and increments around fp64
+// anchors rebuilt at its restarts); the whole batch moves to fp64 with a restart of every slot
+// when some scenario below relative KKT 1e-5 stops improving for hip_refine_patience checks.
 #include "batch.h"
 
 #include <algorithm>
@@ -43,7 +46,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
       w.row_lower = s.row_lower.empty() ? base.row_lower : s.row_lower;
       w.row_upper = s.row_upper.empty() ? base.row_upper : s.row_upper;
       out[k].engine = "r2hpdhg-batch";
-      out[k].precision = "fp64";
+      out[k].precision = options.precision == Precision::Mixed ? "mixed" : "fp64";
       out[k].model_fingerprint = w.fingerprint_hex();
     }
   }
@@ -178,6 +181,11 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
           }
       }
       dev->check(psh_batch_set_iterate(c, X.data(), Y.data()), "batch");
+      bool incr = options.precision == Precision::Mixed && eo.refine != 0;
+      if (incr) dev->check(psh_batch_delta(c, 1), "batch precision");
+      std::vector<double> best(K, kInf);
+      std::vector<int> stale(K, 0);
+      std::vector<std::int64_t> promoted_at(K, -1);
 
       const double w0 = options.warm_primal_weight > 0 ? options.warm_primal_weight : 1.0;
       std::vector<PidState> pid(K);
@@ -207,6 +215,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
         const bool over_iter = it >= eo.iteration_limit, over_time = elapsed() > eo.time_limit;
         std::vector<int> keep;
         mask.assign(W, 0);
+        bool promote = false;  // increment form stalled for some scenario: the batch goes to fp64
         for (int t = 0; t < W; ++t) {
           const int k = sid[t];
           const double* st = stats.data() + static_cast<std::size_t>(t) * SC;
@@ -251,12 +260,23 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
             s.iterations_to_fast = fast_it[k];
             s.seconds_to_fast = fast_s[k];
             s.primal_weight = pid[k].omega;
+            if (options.precision == Precision::Mixed && eo.refine != 0) {
+              s.message = s.message.empty() ? "fp32 increments on fp64 anchors" : s.message;
+              if (promoted_at[k] >= 0)
+                s.message += "; fp32 -> fp64 at iteration " + std::to_string(promoted_at[k]);
+            }
           };
           if (!kk.finite()) {
             finish(Status::NumericalError, "non-finite iterate");
             continue;
           }
           if (fast_it[k] < 0 && kk.rel_kkt() <= tol::kFirstOrderFast) fast_it[k] = it, fast_s[k] = elapsed();
+          if (incr) {
+            const double rel = kk.rel_kkt();
+            if (rel < 0.9 * best[k]) best[k] = rel, stale[k] = 0;
+            else ++stale[k];
+            promote = promote || (rel <= 1e-5 && stale[k] >= eo.refine_patience);
+          }
           if (kk.converged(eo.tolerance)) {
             finish(Status::Optimal, "");
             continue;
@@ -275,6 +295,29 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
           mask[t] = 1;
           inner[k] = 0;
           r_last[k] = kInf;
+        }
+        if (promote) {  // every live slot restarts at its x̂, ŷ in fp64 (PID as at a restart)
+          dev->check(psh_batch_delta(c, 0), "batch precision");
+          incr = false;
+          for (int t : keep) {
+            const int k = sid[t];
+            if (!mask[t]) {
+              const double* st = stats.data() + static_cast<std::size_t>(t) * SC;
+              KktStats kk;
+              kk.primal_residual = std::sqrt(st[PSH_BSTAT_RP2]);
+              kk.dual_residual = std::sqrt(st[PSH_BSTAT_RD2]);
+              kk.primal_obj = st[PSH_BSTAT_P];
+              kk.dual_obj = st[PSH_BSTAT_D];
+              kk.b_norm = std::sqrt(st[PSH_BSTAT_B2]);
+              kk.c_norm = std::sqrt(st[PSH_BSTAT_C2]);
+              pid_update(pid[k], eo, std::sqrt(st[PSH_BSTAT_PIDX]), std::sqrt(st[PSH_BSTAT_PIDY]), kk.rel_primal(),
+                         kk.rel_dual());
+              mask[t] = 1;
+              inner[k] = 0;
+              r_last[k] = kInf;
+            }
+            promoted_at[k] = it;
+          }
         }
         dev->check(psh_batch_restart(c, mask.data()), "batch restart");
         if (static_cast<int>(keep.size()) != W) {

@@ -61,6 +61,7 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_sliced") e.sliced = v != 0 ? 1 : 0;
     else if (name == "hip_refine") e.refine = v != 0 ? 1 : 0;
     else if (name == "hip_refine_patience") e.refine_patience = std::max(1, static_cast<int>(v));
+    else if (name == "hip_bf16_until") e.bf16_until = v > 0 ? v : 0.0;
     else return name;
   }
   if (e.refine < 0) e.refine = e.cpu_order ? 0 : 1;
@@ -566,13 +567,15 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
 // fp32 has done what it can, then fp64. With refine = 1 (the default outside CPU-order mode)
 // the increment form: fp32 matrix and increments around fp64 anchors that every restart
 // rebuilds, to the full tolerance; fp64 takes over only if progress stops for `refine_patience`
-// checks once the relative KKT error is below 1e-5.
+// checks once the relative KKT error is below 1e-5. The bf16 variant (bf16_until > 0) starts the
+// increment form with bf16 matrix values and moves to fp32 values at bf16_until or on a stall of
+// 10 checks.
 class PrecisionPolicy {
  public:
   PrecisionPolicy(Context& ctx, const EngineOptions& o)
       : ctx_(ctx), mixed_(o.precision == Precision::Mixed), refine_(mixed_ && o.refine != 0), target_(o.tolerance),
-        patience_(o.refine_patience) {
-    if (refine_) ctx_.dev().check(psh_delta(ctx_.ctx(), 1), "precision");
+        bf16_until_(refine_ ? o.bf16_until : 0.0), patience_(o.refine_patience) {
+    if (refine_) ctx_.dev().check(psh_delta(ctx_.ctx(), bf16_until_ > 0 ? 2 : 1), "precision");
     else ctx_.dev().check(psh_set_fp32(ctx_.ctx(), mixed_ ? 1 : 0), "precision");
   }
   bool fp32() { return psh_is_fp32(ctx_.ctx()) != 0; }
@@ -582,6 +585,13 @@ class PrecisionPolicy {
       if (left_) return false;
       if (rel < 0.9 * best_) best_ = rel, stale_ = 0;
       else ++stale_;
+      if (bf16_until_ > 0 && (rel <= bf16_until_ || stale_ >= 10)) {
+        ctx_.dev().check(psh_delta(ctx_.ctx(), 1), "precision");
+        bf16_until_ = 0;
+        stale_ = 0;
+        best_ = rel;
+        return false;
+      }
       if (rel <= 1e-5 && stale_ >= patience_) {
         ctx_.dev().check(psh_delta(ctx_.ctx(), 0), "precision");
         left_ = true;
@@ -603,7 +613,7 @@ class PrecisionPolicy {
  private:
   Context& ctx_;
   bool mixed_, refine_, left_ = false;
-  double target_, best_ = 1e300;
+  double target_, bf16_until_, best_ = 1e300;
   int stale_ = 0, patience_ = 40;
 };
 
@@ -651,7 +661,9 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
   std::int64_t it = 0, inner = 0;
   double r0 = -1.0, r_last = kInf;
   std::string msg = warm ? "warm start" : "";
-  if (policy.refining()) msg += std::string(msg.empty() ? "" : "; ") + "fp32 increments on fp64 anchors";
+  if (policy.refining())
+    msg += std::string(msg.empty() ? "" : "; ") + "fp32 increments on fp64 anchors" +
+           (opt.bf16_until > 0 ? " (bf16 matrix values first)" : "");
   ctx.loop_started();
 
   for (;;) {

@@ -44,6 +44,9 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   // reference's fp32-then-fp64 switch; CPU-order mode (hip_cpu_order = 1) defaults it to 0.
   int refine = -1;
   int refine_patience = 40;  // hip_refine_patience: stalled checks before fp64 takes over
+  // hip_bf16_until: > 0 runs the increment form's SpMVs with bf16 matrix values until the relative
+  // KKT error reaches this value or stops improving for 10 checks, then with fp32 values (0: off).
+  double bf16_until = 0.0;
 };
 
 EngineOptions engine_options_from(const Options& o);

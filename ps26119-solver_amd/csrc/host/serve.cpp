@@ -28,6 +28,9 @@ struct Server::Request {
   psh_problem prob{};
   double eta = 1.0;
   std::size_t bytes = 0;
+  bool mixed = false;  // increment form (precision = mixed, hip_refine = 1)
+  double best = kInf;  // stall guard of the increment form
+  int stale = 0;
   // live on the device
   int slot = -1;
   PidState pid;
@@ -140,7 +143,7 @@ void Server::complete(const std::shared_ptr<Request>& r) {
 void Server::prepare(Request& r) {
   Solution& s = r.result.solution;
   s.engine = "r2hpdhg-served";
-  s.precision = "fp64";
+  s.precision = r.options.precision == Precision::Mixed ? "mixed" : "fp64";
   s.model_fingerprint = r.model.fingerprint_hex();
   auto reject = [&](Status st, const std::string& why) {
     s.status = st;
@@ -167,7 +170,9 @@ void Server::prepare(Request& r) {
   const double norm = empty ? 0.0 : estimate_norm2_host(sp);
   r.eta = norm > 0 ? 0.998 / norm : 1.0;
   r.prob = problem_of(sp, r.cmin);
-  r.bytes = psh_server_bytes(sp.m, sp.n, sp.A.nnz());
+  r.mixed = r.options.precision == Precision::Mixed && r.eo.refine != 0;
+  if (r.mixed) r.message = "fp32 increments on fp64 anchors";
+  r.bytes = psh_server_bytes(sp.m, sp.n, sp.A.nnz(), r.mixed ? PSH_SERVE_INCREMENT : 0);
   if (r.bytes > capacity_) return reject(Status::NotSolved, "out of memory: the model exceeds the HBM budget");
   r.x_init.resize(sp.n);  // x = proj_X(0), y = 0 (reference r2hpdhg.cpp:31-34)
   for (int j = 0; j < sp.n; ++j) r.x_init[j] = std::min(std::max(0.0, sp.col_lower[j]), sp.col_upper[j]);
@@ -292,7 +297,7 @@ void Server::serve_loop() {
     }
     for (auto& r : admit) {
       int id = -1;
-      int rc = psh_server_add(dev_, &r->prob, &id);
+      int rc = psh_server_add(dev_, &r->prob, r->mixed ? PSH_SERVE_INCREMENT : 0, &id);
       if (rc == PSH_OK) {
         std::vector<double> y0(r->sp->m, 0.0);
         rc = psh_server_set_iterate(dev_, id, r->x_init.data(), y0.data());
@@ -334,6 +339,7 @@ void Server::serve_loop() {
     }
     // per-instance decisions (the single-LP engine's rules)
     std::vector<std::shared_ptr<Request>> keep, retire;
+    std::vector<int> to_fp64;
     restart_ids.clear();
     const double t_now = now();
     for (std::size_t q = 0; q < live.size(); ++q) {
@@ -368,7 +374,19 @@ void Server::serve_loop() {
         continue;
       }
       keep.push_back(r);
-      const bool restart = restart_due(o, rr, r->r0, r->r_last, r->inner, r->it);
+      bool promoted = false;  // increment form stalled: fp64 from the next chunk, after a restart
+      if (r->mixed) {
+        const double rel = k.rel_kkt();
+        if (rel < 0.9 * r->best) r->best = rel, r->stale = 0;
+        else ++r->stale;
+        if (rel <= 1e-5 && r->stale >= o.refine_patience) {
+          promoted = true;
+          r->mixed = false;
+          r->message += "; fp32 -> fp64 at iteration " + std::to_string(r->it);
+          to_fp64.push_back(r->slot);
+        }
+      }
+      const bool restart = promoted || restart_due(o, rr, r->r0, r->r_last, r->inner, r->it);
       r->r_last = rr;
       if (!restart) continue;
       pid_update(r->pid, o, st.dx_norm, st.dy_norm, k.rel_primal(), k.rel_dual());
@@ -377,6 +395,8 @@ void Server::serve_loop() {
       r->r_last = kInf;
     }
     rc = psh_server_restart(dev_, restart_ids.data(), static_cast<int>(restart_ids.size()));
+    for (int id : to_fp64)
+      if (rc == PSH_OK) rc = psh_server_set_precision(dev_, id, 0);
     for (auto& r : retire) {
       if (rc == PSH_OK) {
         r->xs.resize(r->sp->n);

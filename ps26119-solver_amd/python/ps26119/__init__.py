@@ -289,12 +289,14 @@ def _ext(L, presolve, check_every, params, keep):
 
 def solve_batch(c, A, row_lower, row_upper, col_lower=None, col_upper=None, scenarios=(), sense=1, offset=0.0,
                 gpu=True, tolerance=1e-8, time_limit=3600.0, iteration_limit=0, warm_x=None, warm_y=None, verbosity=0,
-                threads=1, check_every=0, params=None) -> list:
+                threads=1, check_every=0, params=None, precision="fp64") -> list:
     """Scenarios of one LP sharing A (the reference's ps26119::solve_batch, include/ps26119/batch.h).
 
     ``scenarios`` is a list of dicts with any of ``obj``, ``col_lower``, ``col_upper``,
     ``row_lower``, ``row_upper`` (missing = the base's). Returns one Result per scenario. A vector
-    of the wrong length makes only that scenario NotSolved, as in the reference."""
+    of the wrong length makes only that scenario NotSolved, as in the reference. ``precision`` =
+    "mixed" runs every scenario in increment form (fp32 increments on fp64 anchors), an addition:
+    the reference's batch is fp64 only."""
     c = np.ascontiguousarray(c, dtype=np.float64)
     n = c.size
     rl = np.ascontiguousarray(row_lower, dtype=np.float64)
@@ -307,7 +309,7 @@ def solve_batch(c, A, row_lower, row_upper, col_lower=None, col_upper=None, scen
     keep = []
     wx = None if warm_x is None else np.ascontiguousarray(warm_x, dtype=np.float64)
     wy = None if warm_y is None else np.ascontiguousarray(warm_y, dtype=np.float64)
-    opt = _options(L, "r2hpdhg", "fp64", gpu, tolerance, time_limit, iteration_limit, verbosity, threads, wx, wy)
+    opt = _options(L, "r2hpdhg", precision, gpu, tolerance, time_limit, iteration_limit, verbosity, threads, wx, wy)
     ext = _ext(L, False, check_every, params, keep)
     K = len(scenarios)
     arr = (Scenario * max(1, K))()
@@ -392,7 +394,10 @@ class Server:
         self._dims = {}
         self._keep = {}
 
-    def submit(self, lp, tolerance=1e-8, iteration_limit=0, time_limit=3600.0, verbosity=0, params=None) -> int:
+    def submit(self, lp, tolerance=1e-8, iteration_limit=0, time_limit=3600.0, verbosity=0, params=None,
+               precision="fp64") -> int:
+        """Queues one LP; ``precision`` = "mixed" serves it in increment form (fp32 increments on
+        fp64 anchors, fp64 takeover on a stall below 1e-5)."""
         L = lib()
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
                 (lp.c, lp.col_lower, lp.col_upper, lp.row_lower, lp.row_upper, lp.value)]
@@ -400,7 +405,8 @@ class Server:
         c, cl, cu, rl, ru, v = arrs
         cs, ri = ints
         keep = arrs + ints
-        opt = _options(L, "r2hpdhg", "fp64", True, tolerance, time_limit, iteration_limit, verbosity, 1, None, None)
+        opt = _options(L, "r2hpdhg", precision, True, tolerance, time_limit, iteration_limit, verbosity, 1, None,
+                       None)
         ext = _ext(L, False, 0, params, keep)
         rid = L.ps26119_amd_server_submit(self._h, lp.m, lp.n, int(lp.sense), float(lp.offset), _ptr(c), _ptr(cl),
                                           _ptr(cu), _ptr(rl), _ptr(ru), _ptr(cs, ctypes.c_int),

@@ -141,3 +141,44 @@ def test_batch_graph_replay_equals_eager_launches(gpu_available, order):
     for e, g in zip(eager, graph):
         assert e.status == g.status == ps26119.OPTIMAL
         assert e.iterations == g.iterations and fnv(e.x) == fnv(g.x) and fnv(e.y) == fnv(g.y)
+
+
+@pytest.mark.parametrize("name", CASES + ["random300_s3|scenario_set"])
+def test_mixed_batch_reaches_the_fp64_answers(gpu_available, golden, name):
+    """Extension: precision = mixed runs every scenario in increment form (fp32 matrix and
+    increments around fp64 anchors rebuilt at each of its restarts). Every scenario still ends
+    Optimal at 1e-8, passes the in-process check on its own model, and its objective is within
+    1e-7 relative of the reference batch's fp64 answer (fixtures)."""
+    import ps26119
+    base, scen = _scenarios(name)
+    exp = golden["batches"][name]
+    res = ps26119.solve_model_batch(base, scen, precision="mixed", iteration_limit=3_000_000)
+    for k, (r, e, sc) in enumerate(zip(res, exp, scen)):
+        assert r.status == e["status"], (k, r.message)
+        if e["status"] != ps26119.OPTIMAL:
+            continue
+        ref = float.fromhex(e["objective"])
+        err = abs(r.objective - ref) / (1 + abs(ref))
+        print(f"{name}[{k}]: mixed it={r.iterations} (fp64 reference {e['iterations']}) "
+              f"objective error {err:.2e}; {r.message}", flush=True)
+        assert err <= 1e-7, k
+        assert "fp32 increments" in r.message
+        ck = ps26119.check_solution(_with(base, sc), r.x, r.y)
+        assert ck["passed"], (k, ck)
+
+
+def test_mixed_batch_hands_over_to_fp64_on_a_stall(gpu_available, golden):
+    """With a one-check patience the increment form hands the batch to fp64 at the first stalled
+    check below 1e-5; the answers are still the fp64 ones."""
+    import ps26119
+    base, scen = _scenarios("stocfor1")
+    exp = golden["batches"]["stocfor1"]
+    res = ps26119.solve_model_batch(base, scen, precision="mixed", params={"hip_refine_patience": 1},
+                                    iteration_limit=3_000_000)
+    handed = 0
+    for k, (r, e) in enumerate(zip(res, exp)):
+        assert r.status == e["status"] == ps26119.OPTIMAL, (k, r.message)
+        ref = float.fromhex(e["objective"])
+        assert abs(r.objective - ref) <= 1e-7 * (1 + abs(ref)), k
+        handed += "fp32 -> fp64" in r.message
+    print(f"stocfor1 mixed, patience 1: {handed} of {len(res)} scenarios finished in fp64", flush=True)

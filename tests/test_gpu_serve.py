@@ -108,3 +108,50 @@ def test_many_small_requests_beat_sequential_solves(gpu_available):
         assert r.status == ps26119.OPTIMAL == q.status, r.message
         assert abs(r.objective - lp.optimum) <= 1e-5 * (1 + abs(lp.optimum))
     assert served < sequential
+
+
+def test_mixed_precision_requests_reach_the_optima(gpu_available, golden):
+    """Requests with precision = mixed run in increment form inside the same packed launches as
+    fp64 requests (the chunk mixes both forms); they still end Optimal at 1e-8, checked, at the
+    planted optimum, and an increment-form instance is bitwise the same alone and in a mix."""
+    import ps26119
+    lps = _mix()
+    nets = ["afiro", "sc50a", "stocfor1"]
+    with ps26119.Server(budget_bytes=1 << 30) as srv:
+        ids = [srv.submit(lp, precision="mixed" if k % 2 == 0 else "fp64") for k, lp in enumerate(lps)]
+        ids += [srv.submit(netlib(n), precision="mixed") for n in nets]
+        res = [srv.wait(i, timeout=300) for i in ids]
+    for k, (lp, r) in enumerate(zip(lps, res[:len(lps)])):
+        print(f"served m={lp.m} ({'mixed' if k % 2 == 0 else 'fp64'}): {r.status_name} it={r.iterations} "
+              f"{r.message}", flush=True)
+        assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+        assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
+        assert ("fp32 increments" in r.message) == (k % 2 == 0)
+    for name, r in zip(nets, res[len(lps):]):
+        ref = float.fromhex(golden["solves"][f"{name}|presolve=0|fp64"]["objective"])
+        assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+        assert abs(r.objective - ref) <= 1e-7 * (1 + abs(ref)), name
+    from ps26119_amd import lpgen
+    a = lpgen.random_lp(600, seed=7)
+    with ps26119.Server(budget_bytes=1 << 30) as srv:
+        alone = srv.wait(srv.submit(a, precision="mixed"), timeout=300)
+    with ps26119.Server(budget_bytes=1 << 30) as srv:
+        early = [srv.submit(lp) for lp in lps[:3]]
+        ia = srv.submit(a, precision="mixed")
+        late = [srv.submit(lp, precision="mixed") for lp in lps[3:]]
+        mixed = srv.wait(ia, timeout=300)
+        [srv.wait(i, timeout=300) for i in early + late]
+    assert alone.status == mixed.status == ps26119.OPTIMAL
+    assert alone.iterations == mixed.iterations
+    assert fnv(alone.x) == fnv(mixed.x) and fnv(alone.y) == fnv(mixed.y)
+
+
+def test_mixed_request_hands_over_to_fp64_on_a_stall(gpu_available):
+    import ps26119
+    from ps26119_amd import lpgen
+    lp = lpgen.random_lp(800, seed=2)
+    with ps26119.Server(budget_bytes=1 << 30) as srv:
+        r = srv.wait(srv.submit(lp, precision="mixed", params={"hip_refine_patience": 1}), timeout=300)
+    print(f"served mixed, patience 1: it={r.iterations} {r.message}", flush=True)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))

@@ -185,6 +185,24 @@ def test_refined_mixed_precision_reaches_the_fp64_answer(gpu_available, golden, 
     assert err <= 1e-7
 
 
+@pytest.mark.parametrize("name", ["afiro", "sc105", "adlittle", "random2000_s4"])
+def test_bf16_values_variant_reaches_the_fp64_answer(gpu_available, golden, name):
+    """The bf16 variant of the increment form (bf16 matrix values in the increment SpMVs until
+    relative KKT 1e-4, then fp32 values; fp64 anchors, checks and refinement throughout) still
+    ends Optimal, checked, within 1e-7 relative of the reference's fp64 objective."""
+    import ps26119
+    lp = _model(name)
+    exp = golden["solves"][f"{name}|presolve=1|fp64"]
+    r = ps26119.solve_model(lp, precision="mixed", params={"hip_bf16_until": 1e-4}, iteration_limit=3_000_000)
+    ref_obj = float.fromhex(exp["objective"])
+    err = abs(r.objective - ref_obj) / (1 + abs(ref_obj))
+    print(f"{name}: bf16-first increment form it={r.iterations} (fp64 reference {exp['iterations']}), "
+          f"objective error vs fp64 {err:.2e}; {r.message}", flush=True)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    assert "bf16 matrix values" in r.message
+    assert err <= 1e-7
+
+
 def test_refined_mixed_on_config0(gpu_available):
     import ps26119
     lp = _model("random10000_s1")
