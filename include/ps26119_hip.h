@@ -179,11 +179,14 @@ int psh_lanczos_advance(psh_ctx* ctx, double beta);
 enum { PSH_BATCH_MAX_W = 256 };
 /* Per-slot statistics written by psh_batch_check, PSH_BSTAT_COUNT doubles per slot:
  * residual sums, KKT sums and maxima (termination.h), PID sums, residual sums of the last
- * epoch start. */
+ * epoch start, and (want_ray) the ray-test scalars of T(z) − z in psh_ray's order. */
 enum {
   PSH_BSTAT_PX, PSH_BSTAT_PY, PSH_BSTAT_CROSS, PSH_BSTAT_RP2, PSH_BSTAT_B2, PSH_BSTAT_D, PSH_BSTAT_RD2,
   PSH_BSTAT_C2, PSH_BSTAT_P, PSH_BSTAT_PIDX, PSH_BSTAT_PIDY, PSH_BSTAT_PMAX, PSH_BSTAT_DMAX, PSH_BSTAT_CINF,
-  PSH_BSTAT_R0_PX, PSH_BSTAT_R0_PY, PSH_BSTAT_R0_CROSS, PSH_BSTAT_COUNT
+  PSH_BSTAT_R0_PX, PSH_BSTAT_R0_PY, PSH_BSTAT_R0_CROSS,
+  PSH_BSTAT_RAY_RMAX, PSH_BSTAT_RAY_DUAL_OBJ, PSH_BSTAT_RAY_DUAL_VIOL, PSH_BSTAT_RAY_DUAL_MAG,
+  PSH_BSTAT_RAY_DMAX, PSH_BSTAT_RAY_PRIMAL_CD, PSH_BSTAT_RAY_PRIMAL_VIOL, PSH_BSTAT_RAY_PRIMAL_CMAG,
+  PSH_BSTAT_COUNT
 };
 int psh_batch_begin(psh_ctx* ctx, int W, const double* c, const double* l, const double* u, const double* rl,
                     const double* ru, const double* c_min, const double* cl_o, const double* cu_o,
@@ -196,8 +199,13 @@ int psh_batch_set_iterate(psh_ctx* ctx, const double* x, const double* y);
 int psh_batch_chunk(psh_ctx* ctx, int iterations, const double* tau, const double* sigma, const long long* inner0,
                     double rho, int mode);
 /* Per-slot statistics into out (W · PSH_BSTAT_COUNT); mode PSH_RED_* (CPU order = the reference's
- * sequential per-slot loops). */
-int psh_batch_check(psh_ctx* ctx, int mode, double* out);
+ * sequential per-slot loops). want_ray: also the ray-test scalars of every slot (an addition:
+ * the reference's batch has no infeasibility detection), the single solve's tests of
+ * engine.cpp:82-184 there on the direction T(z) − z. */
+int psh_batch_check(psh_ctx* ctx, int mode, int want_ray, double* out);
+/* The unscaled dual ray (m) and primal ray (n) of slot t from the last want_ray check (either
+ * pointer may be NULL). */
+int psh_batch_ray_vectors(psh_ctx* ctx, int t, double* dual_ray, double* primal_ray);
 /* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ for the slots with mask[t] != 0. */
 int psh_batch_restart(psh_ctx* ctx, const int* mask);
 /* Mixed precision in increment form for every slot (no counterpart in the reference, whose batch

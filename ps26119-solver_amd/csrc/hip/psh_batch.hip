@@ -20,8 +20,11 @@ namespace {
 // Batch term arrays.
 enum BTerm {
   B_DX2, B_DY2, B_CROSS, B_RP2, B_B2, B_DROW, B_RD2ROW, B_C2, B_P, B_DCOL, B_RD2COL, B_PIDX, B_PIDY,
-  B_PMAXR, B_PMAXC, B_DMAXR, B_DMAXC, B_CINF, kBTerms
+  B_PMAXR, B_PMAXC, B_DMAXR, B_DMAXC, B_CINF,
+  // ray tests (engine.cpp:82-184 of the reference, per slot)
+  B_ROBJR, B_RMAGR, B_ROBJC, B_RMAGC, B_CD, B_CMAG, B_RMAX, B_DVIOL, B_DMAXRAY, B_PVIOL, kBTerms
 };
+constexpr int kBJobs = 24;
 constexpr int kBTreeBlocks = 64;
 
 struct BArgs {
@@ -178,7 +181,21 @@ __global__ void k_bdpoint(const double* __restrict__ z0, const float* __restrict
 }
 
 // ---- residual and check passes (one thread per (row, slot))
-__global__ void __launch_bounds__(kBlock) k_bcheck_n(BArgs a, int kkt) {
+// Primal and dual ray entries from a scaled direction (reference engine.cpp:132-139, 90-97).
+__device__ inline double bprimal_ray(double d, double cs, double bs, double lo, double up) {
+  double v = cs * d / bs;
+  if (isfinite(lo)) v = dmaxd(v, 0.0);
+  if (isfinite(up)) v = dmind(v, 0.0);
+  return v;
+}
+__device__ inline double bdual_ray(double d, double rs, double os, double lo, double up) {
+  double v = rs * d / os;
+  if (!isfinite(lo)) v = dmind(v, 0.0);
+  if (!isfinite(up)) v = dmaxd(v, 0.0);
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bcheck_n(BArgs a, int kkt, int ray) {
   const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long long j = g / a.W;
   if (j >= a.n) return;
@@ -192,9 +209,16 @@ __global__ void __launch_bounds__(kBlock) k_bcheck_n(BArgs a, int kkt) {
     const double e = a.b.xh[q] - a.b.x0[q];
     bterm(a, B_PIDX)[q] = e * e;
   }
+  if (ray) {  // primal ray on T(z) − z (engine.cpp:132-139): c·d and |c·d| sums, max |d|
+    const double v = bprimal_ray(d, a.cs[j], a.b.bs[t], a.b.clo[q], a.b.cuo[q]);
+    a.b.dv[q] = v;
+    bterm(a, B_DMAXRAY)[q] = fabs(v);
+    bterm(a, B_CD)[q] = a.b.cmin[q] * v;
+    bterm(a, B_CMAG)[q] = fabs(a.b.cmin[q] * v);
+  }
 }
 
-__global__ void __launch_bounds__(kBlock) k_bcheck_m(BArgs a, int kkt) {
+__global__ void __launch_bounds__(kBlock) k_bcheck_m(BArgs a, int kkt, int ray) {
   const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long long i = g / a.W;
   if (i >= a.m) return;
@@ -208,6 +232,15 @@ __global__ void __launch_bounds__(kBlock) k_bcheck_m(BArgs a, int kkt) {
     const double e = a.b.yh[q] - a.b.y0[q];
     bterm(a, B_PIDY)[q] = e * e;
   }
+  if (ray) {  // dual ray on T(z) − z (engine.cpp:90-97): row part of its objective, max |r|
+    const double lo = a.b.rlo[q], up = a.b.ruo[q];
+    const double r = bdual_ray(d, a.rs[i], a.b.os[t], lo, up);
+    a.b.rv[q] = r;
+    bterm(a, B_RMAX)[q] = fabs(r);
+    const double v = r > 0 ? lo * r : (r < 0 ? up * r : 0.0);
+    bterm(a, B_ROBJR)[q] = v;
+    bterm(a, B_RMAGR)[q] = fabs(v);
+  }
 }
 
 __device__ inline double elem_viol(double v, double lo, double up) {
@@ -216,7 +249,7 @@ __device__ inline double elem_viol(double v, double lo, double up) {
   return 0.0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_bcheck_rows(BArgs a, int kkt) {
+__global__ void __launch_bounds__(kBlock) k_bcheck_rows(BArgs a, int kkt, int ray) {
   const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long long i = g / a.W;
   if (i >= a.m) return;
@@ -227,6 +260,14 @@ __global__ void __launch_bounds__(kBlock) k_bcheck_rows(BArgs a, int kkt) {
   for (int p = b; p < e; ++p) s += a.a_val[p] * a.b.dx[static_cast<size_t>(a.a_col[p]) * a.W + t];
   a.b.adx[q] = s;
   bterm(a, B_CROSS)[q] = a.b.dy[q] * s;
+  if (ray) {  // A·d against the finite row bounds (engine.cpp:140-145)
+    double ad = 0.0;
+    for (int p = b; p < e; ++p) ad += a.a_orig[p] * a.b.dv[static_cast<size_t>(a.a_col[p]) * a.W + t];
+    double pv = 0.0;
+    if (isfinite(a.b.rlo[q])) pv = dmaxd(pv, -ad);
+    if (isfinite(a.b.ruo[q])) pv = dmaxd(pv, ad);
+    bterm(a, B_PVIOL)[q] = pv;
+  }
   if (!kkt) return;
   double ax = 0.0;
   for (int p = b; p < e; ++p) ax += a.a_orig[p] * a.b.xo[static_cast<size_t>(a.a_col[p]) * a.W + t];
@@ -250,12 +291,32 @@ __global__ void __launch_bounds__(kBlock) k_bcheck_rows(BArgs a, int kkt) {
   bterm(a, B_DMAXR)[q] = dm;
 }
 
-__global__ void __launch_bounds__(kBlock) k_bcheck_cols(BArgs a) {
+__global__ void __launch_bounds__(kBlock) k_bcheck_cols(BArgs a, int ray) {
   const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long long j = g / a.W;
   if (j >= a.n) return;
   const int t = static_cast<int>(g - j * a.W);
   const size_t q = static_cast<size_t>(g);
+  if (ray) {  // column terms of λ = −Aᵀr (engine.cpp:98-119)
+    double gs = 0.0, ga = 0.0;
+    for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) {
+      const double v = a.ato_val[p] * a.b.rv[static_cast<size_t>(a.ato_col[p]) * a.W + t];
+      gs += v;
+      ga += fabs(v);
+    }
+    const double lam = -gs, lo = a.b.clo[q], up = a.b.cuo[q];
+    double obj = 0.0, mag = 0.0, viol = 0.0;
+    if (lam > 0) {
+      if (isfinite(lo)) obj = lo * lam, mag = ga * fabs(lo);
+      else viol = lam;
+    } else if (lam < 0) {
+      if (isfinite(up)) obj = up * lam, mag = ga * fabs(up);
+      else viol = -lam;
+    }
+    bterm(a, B_ROBJC)[q] = obj;
+    bterm(a, B_RMAGC)[q] = mag;
+    bterm(a, B_DVIOL)[q] = viol;
+  }
   double s = 0.0;
   for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) s += a.ato_val[p] * a.b.yo[static_cast<size_t>(a.ato_col[p]) * a.W + t];
   const double cj = a.b.cmin[q], xj = a.b.xo[q], lo = a.b.clo[q], up = a.b.cuo[q];
@@ -283,7 +344,7 @@ struct BJob {
 };
 
 struct BRed {
-  BJob jobs[20];
+  BJob jobs[kBJobs];
   int njobs, W, mode;
   const double* terms;
   long long stride;  // rows_max · cap
@@ -341,6 +402,11 @@ __global__ void k_bred_seq(BRed r) {
   const long long R = J.ra + J.rb;
   for (long long row = 0; row < R; ++row) s += bval(r, J, row, t);
   r.stats[static_cast<size_t>(t) * PSH_BSTAT_COUNT + J.stat] = s;
+}
+
+__global__ void k_bcolumn(const double* __restrict__ src, long long rows, int W, int t, double* __restrict__ dst) {
+  const long long r = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r < rows) dst[r] = src[r * W + t];
 }
 
 template <class T>
@@ -406,7 +472,7 @@ void free_batch(psh_ctx* c) {
   BatchState* b = c->batch;
   if (!b) return;
   void* ptrs[] = {b->c, b->l, b->u, b->rl, b->ru, b->cmin, b->clo, b->cuo, b->rlo, b->ruo, b->bs, b->os, b->x, b->x0,
-                  b->xh, b->xb, b->y, b->y0, b->yh, b->yb, b->dx, b->dy, b->adx, b->xo, b->yo, b->tau, b->sigma,
+                  b->xh, b->xb, b->y, b->y0, b->yh, b->yb, b->dx, b->dy, b->adx, b->xo, b->yo, b->rv, b->dv, b->tau, b->sigma,
                   b->inner0, b->mask, b->keep, b->terms, b->partials, b->stats, b->tmp, b->ix, b->ixh, b->ixb,
                   b->G, b->LX, b->UX, b->iy, b->iyh, b->iyb, b->BL, b->BU, b->KY, b->anchor};
   for (void* p : ptrs) dev_free(p);
@@ -424,7 +490,7 @@ constexpr size_t kPinStats = kPinKeep + 4 * PSH_BATCH_MAX_W;
 constexpr size_t kPinAnchor = kPinStats + 8 * PSH_BATCH_MAX_W * PSH_BSTAT_COUNT;
 constexpr size_t kPinBytes = kPinAnchor + 4 * PSH_BATCH_MAX_W;
 
-int batch_reduce(psh_ctx* c, int mode, bool kkt, bool only_residual, int r0) {
+int batch_reduce(psh_ctx* c, int mode, bool kkt, bool only_residual, int r0, bool ray) {
   BatchState& b = *c->batch;
   BRed r{};
   r.W = b.W;
@@ -454,6 +520,16 @@ int batch_reduce(psh_ctx* c, int mode, bool kkt, bool only_residual, int r0) {
     add(B_DMAXR, m, 1, PSH_BSTAT_DMAX, B_DMAXC, n);
     add(B_CINF, n, 1, PSH_BSTAT_CINF);
   }
+  if (ray) {  // psh_ray's scalars
+    add(B_RMAX, m, 1, PSH_BSTAT_RAY_RMAX);
+    add(B_ROBJR, m, 0, PSH_BSTAT_RAY_DUAL_OBJ, B_ROBJC, n);
+    add(B_RMAGR, m, 0, PSH_BSTAT_RAY_DUAL_MAG, B_RMAGC, n);
+    add(B_DVIOL, n, 1, PSH_BSTAT_RAY_DUAL_VIOL);
+    add(B_DMAXRAY, n, 1, PSH_BSTAT_RAY_DMAX);
+    add(B_CD, n, 0, PSH_BSTAT_RAY_PRIMAL_CD);
+    add(B_CMAG, n, 0, PSH_BSTAT_RAY_PRIMAL_CMAG);
+    add(B_PVIOL, m, 1, PSH_BSTAT_RAY_PRIMAL_VIOL);
+  }
   r.njobs = k;
   hipLaunchKernelGGL(k_bred1, dim3(kBTreeBlocks, k), dim3(kBlock), 0, c->stream, r);
   hipLaunchKernelGGL(k_bred2, dim3(k), dim3(PSH_BATCH_MAX_W), 0, c->stream, r);
@@ -462,17 +538,21 @@ int batch_reduce(psh_ctx* c, int mode, bool kkt, bool only_residual, int r0) {
   return PSH_OK;
 }
 
-int batch_residual_pass(psh_ctx* c, bool kkt, int mode, bool r0) {
+int batch_residual_pass(psh_ctx* c, bool kkt, int mode, bool r0, bool ray = false) {
   BatchState& b = *c->batch;
   const BArgs a = make_bargs(c, 0, 0, 1.0);
-  if (c->n > 0) hipLaunchKernelGGL(k_bcheck_n, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a, kkt ? 1 : 0);
+  const int k = kkt ? 1 : 0, rr = ray ? 1 : 0;
+  // the dual ray (rv) must exist before the column pass reads it; the primal ray (dv) before the
+  // row pass: n-pass, m-pass, rows, cols
+  if (c->n > 0) hipLaunchKernelGGL(k_bcheck_n, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a, k, rr);
   if (c->m > 0) {
-    hipLaunchKernelGGL(k_bcheck_m, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, a, kkt ? 1 : 0);
-    hipLaunchKernelGGL(k_bcheck_rows, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, a, kkt ? 1 : 0);
+    hipLaunchKernelGGL(k_bcheck_m, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, a, k, rr);
+    hipLaunchKernelGGL(k_bcheck_rows, dim3(grid_rows(c->m, b.W)), dim3(kBlock), 0, c->stream, a, k, rr);
   }
-  if (kkt && c->n > 0) hipLaunchKernelGGL(k_bcheck_cols, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a);
+  if ((kkt || ray) && c->n > 0)
+    hipLaunchKernelGGL(k_bcheck_cols, dim3(grid_rows(c->n, b.W)), dim3(kBlock), 0, c->stream, a, rr);
   PSH_LAUNCHED(c, "batch check kernels");
-  return batch_reduce(c, mode, kkt, !kkt, r0 ? 1 : 0);
+  return batch_reduce(c, mode, kkt, !kkt, r0 ? 1 : 0, ray);
 }
 
 // Each pinned staging region (1 params, 2 mask, 4 keep, 8 anchor codes) is reused by the next
@@ -559,13 +639,13 @@ int psh_batch_begin(psh_ctx* c, int W, const double* cc, const double* l, const 
       (rc = up(&b.cuo, cu_o, NW)) || (rc = up(&b.rlo, rl_o, MW)) || (rc = up(&b.ruo, ru_o, MW)) ||
       (rc = up(&b.bs, bs, W)) || (rc = up(&b.os, os, W)))
     return rc;
-  for (double** v : {&b.x, &b.x0, &b.xh, &b.xb, &b.dx, &b.xo})
+  for (double** v : {&b.x, &b.x0, &b.xh, &b.xb, &b.dx, &b.xo, &b.dv})
     if ((rc = alloc_n(c, v, NW))) return rc;
-  for (double** v : {&b.y, &b.y0, &b.yh, &b.yb, &b.dy, &b.adx, &b.yo})
+  for (double** v : {&b.y, &b.y0, &b.yh, &b.yb, &b.dy, &b.adx, &b.yo, &b.rv})
     if ((rc = alloc_n(c, v, MW))) return rc;
   if ((rc = alloc_n(c, &b.tau, W)) || (rc = alloc_n(c, &b.sigma, W)) || (rc = alloc_n(c, &b.inner0, W)) ||
       (rc = alloc_n(c, &b.mask, W)) || (rc = alloc_n(c, &b.keep, W)) || (rc = alloc_n(c, &b.terms, RW * kBTerms)) ||
-      (rc = alloc_n(c, &b.partials, static_cast<long long>(kBTreeBlocks) * 20 * W)) ||
+      (rc = alloc_n(c, &b.partials, static_cast<long long>(kBTreeBlocks) * kBJobs * W)) ||
       (rc = alloc_n(c, &b.stats, static_cast<long long>(W) * PSH_BSTAT_COUNT)) ||
       (rc = alloc_n(c, &b.tmp, std::max(RW, m + n))) ||  // gather staging, and one slot's x | y_min
       (rc = alloc_n(c, &b.anchor, W)))
@@ -669,13 +749,13 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
   return PSH_OK;
 }
 
-int psh_batch_check(psh_ctx* c, int mode, double* out) {
+int psh_batch_check(psh_ctx* c, int mode, int want_ray, double* out) {
   int rc = need_batch(c);
   if (rc) return rc;
   BatchState& b = *c->batch;
   hipEvent_t ev;
   time_begin(c, 2, &ev);
-  rc = batch_residual_pass(c, true, mode, false);
+  rc = batch_residual_pass(c, true, mode, false, want_ray != 0);
   time_end(c, 2, ev);
   if (rc) return rc;
   double* pin = reinterpret_cast<double*>(b.pinned + kPinStats);
@@ -775,6 +855,25 @@ int psh_batch_compact(psh_ctx* c, const int* keep, int W2) {
   b.W = W2;
   PSH_TRY(c, hipStreamSynchronize(c->stream));
   b.pin_pending = 0;
+  return PSH_OK;
+}
+
+int psh_batch_ray_vectors(psh_ctx* c, int t, double* dual_ray, double* primal_ray) {
+  int rc = need_batch(c);
+  if (rc) return rc;
+  BatchState& b = *c->batch;
+  if (t < 0 || t >= b.W) return fail(c, PSH_E_ARG, "bad slot");
+  const long long m = c->m, n = c->n;
+  // column t of the interleaved rays into tmp: dual ray | primal ray
+  if (m > 0)
+    hipLaunchKernelGGL(k_bcolumn, dim3(grid_for(m)), dim3(kBlock), 0, c->stream, b.rv, m, b.W, t, b.tmp);
+  if (n > 0)
+    hipLaunchKernelGGL(k_bcolumn, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, b.dv, n, b.W, t, b.tmp + m);
+  PSH_LAUNCHED(c, "batch ray vectors");
+  if (dual_ray && m > 0) PSH_TRY(c, hipMemcpyAsync(dual_ray, b.tmp, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+  if (primal_ray && n > 0)
+    PSH_TRY(c, hipMemcpyAsync(primal_ray, b.tmp + m, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
   return PSH_OK;
 }
 

@@ -118,6 +118,7 @@ struct BatchArrays {
   double *bs = nullptr, *os = nullptr;
   double *x = nullptr, *x0 = nullptr, *xh = nullptr, *xb = nullptr, *y = nullptr, *y0 = nullptr, *yh = nullptr,
          *yb = nullptr, *dx = nullptr, *dy = nullptr, *adx = nullptr, *xo = nullptr, *yo = nullptr;
+  double *rv = nullptr, *dv = nullptr;  // unscaled dual (m·cap) and primal (n·cap) rays of the check
   double *tau = nullptr, *sigma = nullptr;
   long long* inner0 = nullptr;
   int *mask = nullptr, *keep = nullptr;

@@ -13,6 +13,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <limits>
 #include <new>
 #include <stdexcept>
 
@@ -211,7 +213,8 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
         for (int t = 0; t < W; ++t) inner[sid[t]] += Kc;
         it += Kc;
         stats.resize(static_cast<std::size_t>(W) * SC);
-        dev->check(psh_batch_check(c, mode, stats.data()), "batch check");
+        const bool want_ray = it >= 4 * Kc;  // as the single solve (reference r2hpdhg.cpp:93-111)
+        dev->check(psh_batch_check(c, mode, want_ray ? 1 : 0, stats.data()), "batch check");
         const bool over_iter = it >= eo.iteration_limit, over_time = elapsed() > eo.time_limit;
         std::vector<int> keep;
         mask.assign(W, 0);
@@ -284,6 +287,47 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
           if (over_iter || over_time) {
             finish(over_iter ? Status::IterationLimit : Status::TimeLimit, "");
             continue;
+          }
+          if (want_ray) {  // per-scenario ray tests, each certificate re-verified on its own model
+            psh_ray ry;
+            std::memcpy(&ry, st + PSH_BSTAT_RAY_RMAX, sizeof ry);
+            double robj = 0, rviol = 0;
+            const Status v = ray_decision(ry, robj, rviol);
+            if (v != Status::NotSolved) {
+              Model w = base;
+              w.obj = D[k].c;
+              w.col_lower = D[k].cl;
+              w.col_upper = D[k].cu;
+              w.row_lower = D[k].rl;
+              w.row_upper = D[k].ru;
+              std::vector<double> dr, pr;
+              if (v == Status::Infeasible) dr.assign(m, 0.0);
+              else pr.assign(n, 0.0);
+              dev->check(psh_batch_ray_vectors(c, t, v == Status::Infeasible ? dr.data() : nullptr,
+                                               v == Status::Unbounded ? pr.data() : nullptr),
+                         "batch ray vectors");
+              bool ok = false;
+              char buf[200];
+              if (v == Status::Infeasible) {
+                ok = check_infeasibility_certificate(w, dr).passed;
+                std::snprintf(buf, sizeof buf, "primal infeasible: dual ray certificate (objective %f, violation %f)",
+                              robj, rviol);
+              } else {
+                x.assign(n, 0.0);
+                ym.assign(m, 0.0);
+                dev->check(psh_batch_point(c, t, x.data(), ym.data()), "batch point");
+                ok = check_unboundedness_certificate(w, x, pr).passed;
+                std::snprintf(buf, sizeof buf, "unbounded: primal ray certificate (c'd %f, violation %f)", robj, rviol);
+              }
+              if (ok) {
+                finish(v, buf);
+                Solution& so = out[k];
+                so.objective = so.dual_objective = std::numeric_limits<double>::quiet_NaN();
+                if (v == Status::Infeasible) so.dual_ray = std::move(dr);
+                else so.primal_ray = std::move(pr);
+                continue;
+              }
+            }
           }
           keep.push_back(t);
           const double r = resid(st[PSH_BSTAT_PX], st[PSH_BSTAT_PY], st[PSH_BSTAT_CROSS]);

@@ -220,6 +220,24 @@ psh_problem problem_of(const ScaledProblem& sp, std::vector<double>& cmin) {
   return p;
 }
 
+Status ray_decision(const psh_ray& t, double& obj, double& viol) {
+  if (t.rmax > 0) {
+    const double scale = std::max(t.rmax, t.dual_viol);
+    obj = t.dual_obj / scale;
+    viol = t.dual_viol / scale;
+    if (obj > 0 && t.dual_obj > tol::kFirstOrderInfeasible * t.dual_mag && viol <= tol::kFirstOrderInfeasible * obj)
+      return Status::Infeasible;
+  }
+  if (t.dmax > 0) {
+    const double scale = std::max(t.dmax, t.primal_viol);
+    obj = t.primal_cd / scale;
+    viol = t.primal_viol / scale;
+    if (obj < 0 && -t.primal_cd > tol::kFirstOrderInfeasible * t.primal_cmag && viol <= tol::kFirstOrderInfeasible * -obj)
+      return Status::Unbounded;
+  }
+  return Status::NotSolved;
+}
+
 bool restart_due(const EngineOptions& o, double r, double r0, double r_last, std::int64_t inner, std::int64_t it) {
   const bool ratios_ok = r0 > 0 && std::isfinite(r0) && r > 0;
   return (ratios_ok && (r <= o.restart_sufficient * r0 || (r <= o.restart_necessary * r0 && r > r_last))) ||
@@ -432,31 +450,22 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   // Ray-test verdict from the device scalars (reference engine.cpp:82-184).
   Status ray_verdict(const psh_ray& t, bool drift, const KktStats& cur, std::string& msg) {
     char buf[200];
-    if (t.rmax > 0) {
-      const double scale = std::max(t.rmax, t.dual_viol);
-      const double obj = t.dual_obj / scale, viol = t.dual_viol / scale;
-      if (obj > 0 && t.dual_obj > tol::kFirstOrderInfeasible * t.dual_mag && viol <= tol::kFirstOrderInfeasible * obj) {
-        dual_ray_.assign(sp_.m, 0.0);
-        dev_->check(psh_ray_vectors(ctx(), drift ? 1 : 0, dual_ray_.data(), nullptr), "ray vectors");
-        std::snprintf(buf, sizeof buf, "primal infeasible: dual ray certificate (objective %f, violation %f)", obj, viol);
-        msg = buf;
-        return Status::Infeasible;
-      }
+    double obj = 0, viol = 0;
+    const Status st = ray_decision(t, obj, viol);
+    if (st == Status::Infeasible) {
+      dual_ray_.assign(sp_.m, 0.0);
+      dev_->check(psh_ray_vectors(ctx(), drift ? 1 : 0, dual_ray_.data(), nullptr), "ray vectors");
+      std::snprintf(buf, sizeof buf, "primal infeasible: dual ray certificate (objective %f, violation %f)", obj, viol);
+      msg = buf;
+    } else if (st == Status::Unbounded) {
+      const bool feasible = cur.rel_primal() <= tol::kVerifyPrimal && cur.primal_max_rel <= tol::kVerifyPrimal;
+      primal_ray_.assign(sp_.n, 0.0);
+      dev_->check(psh_ray_vectors(ctx(), drift ? 1 : 0, nullptr, primal_ray_.data()), "ray vectors");
+      std::snprintf(buf, sizeof buf, "unbounded: primal ray certificate (c'd %f, violation %f)%s", obj, viol,
+                    feasible ? " at a primal-feasible iterate" : "; iterate not primal-feasible yet");
+      msg = buf;
     }
-    if (t.dmax > 0) {
-      const double scale = std::max(t.dmax, t.primal_viol);
-      const double obj = t.primal_cd / scale, viol = t.primal_viol / scale;
-      if (obj < 0 && -t.primal_cd > tol::kFirstOrderInfeasible * t.primal_cmag && viol <= tol::kFirstOrderInfeasible * -obj) {
-        const bool feasible = cur.rel_primal() <= tol::kVerifyPrimal && cur.primal_max_rel <= tol::kVerifyPrimal;
-        primal_ray_.assign(sp_.n, 0.0);
-        dev_->check(psh_ray_vectors(ctx(), drift ? 1 : 0, nullptr, primal_ray_.data()), "ray vectors");
-        std::snprintf(buf, sizeof buf, "unbounded: primal ray certificate (c'd %f, violation %f)%s", obj, viol,
-                      feasible ? " at a primal-feasible iterate" : "; iterate not primal-feasible yet");
-        msg = buf;
-        return Status::Unbounded;
-      }
-    }
-    return Status::NotSolved;
+    return st;
   }
 
   // Warm start (reference engine.cpp:237-266); returns true when one was applied.

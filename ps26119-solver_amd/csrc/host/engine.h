@@ -103,6 +103,11 @@ struct PidState {
 void pid_update(PidState& p, const EngineOptions& o, double dx_norm, double dy_norm, double rel_primal,
                 double rel_dual);
 
+// Ray-test decision on the device scalars of one direction (reference engine.cpp:82-184):
+// Infeasible (dual ray), Unbounded (primal ray) or NotSolved; obj and viol are the certificate's
+// scaled objective and violation.
+Status ray_decision(const psh_ray& t, double& obj, double& viol);
+
 Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt);
 // PDLP-style restarted PDHG with iterate averaging on the same device operations (reference
 // src/pdhg/pdlp.cpp:9-126): unfused Backend ops, KKT of the current and the averaged point.

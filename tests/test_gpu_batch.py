@@ -182,3 +182,27 @@ def test_mixed_batch_hands_over_to_fp64_on_a_stall(gpu_available, golden):
         assert abs(r.objective - ref) <= 1e-7 * (1 + abs(ref)), k
         handed += "fp32 -> fp64" in r.message
     print(f"stocfor1 mixed, patience 1: {handed} of {len(res)} scenarios finished in fp64", flush=True)
+
+
+def test_infeasible_scenarios_leave_the_batch_with_certificates(gpu_available, oracle):
+    """Extension (the reference's batch has no infeasibility detection and would run such
+    scenarios to its iteration limit): scenarios of a 2,000-row random LP with b, c and u scaled
+    elementwise by lognormal(0, 0.05) factors are mostly infeasible. Each scenario's batch status
+    equals the oracle's single solve of that scenario (the reference's r²HPDHG with its ray tests);
+    an infeasible verdict is given only after the scenario's dual ray passes the certificate check
+    on the scenario's own model; optimal ones reach the oracle's objective."""
+    import ps26119
+    from ps26119_amd import lpgen
+    base = lpgen.random_lp(2000, seed=3)
+    scen = lpgen.scenario_set(base, 8, sigma=0.05, seed=7)
+    exp = [oracle.solve(lpgen.with_scenario(base, sc), presolve=False, threads=4)[0] for sc in scen]
+    for prec in ("fp64", "mixed"):
+        res = ps26119.solve_model_batch(base, scen, precision=prec, iteration_limit=3_000_000)
+        for k, (r, e) in enumerate(zip(res, exp)):
+            print(f"{prec}[{k}]: {r.status_name} it={r.iterations} (oracle {e['status']} it={e['iterations']}) "
+                  f"{r.message[:90]}", flush=True)
+            assert r.status == e["status"], (prec, k, r.message)
+            if r.status == ps26119.INFEASIBLE:
+                assert "dual ray certificate" in r.message
+            else:
+                assert abs(r.objective - e["objective"]) <= 1e-7 * (1 + abs(e["objective"])), (prec, k)
