@@ -34,7 +34,7 @@ void usage(std::FILE* f) {
                "        [--tol e] [--time-limit s] [--iteration-limit n] [--threads n] [--no-presolve]\n"
                "        [--warm file.sol [--warm-weight]] [--set name=value]... [--out file.sol] [-v|-vv]\n"
                "  %s batch <base.lpm> <scenario.lpm>... [--out-dir d] [--tol e] [--time-limit s]\n"
-               "        [--iteration-limit n] [--set name=value]... [-v|-vv]\n"
+               "        [--iteration-limit n] [--precision fp64|mixed] [--set name=value]... [-v|-vv]\n"
                "        scenarios share the base matrix; their objective and bounds may differ\n"
                "exit codes: 0 optimal, 1 infeasible/unbounded/limit, 2 usage, 3 read error,\n"
                "            4 cannot write output, 5 numerical error / not solved\n",
@@ -252,7 +252,15 @@ int cmd_batch(int argc, char** argv) {
     else if (a == "--iteration-limit") o.iteration_limit = static_cast<std::int64_t>(number(a, next(), true));
     else if (a == "--threads") o.threads = static_cast<int>(number(a, next(), false));
     else if (a == "--set") o.engine_params.push_back(name_value(next()));
-    else if (a == "--gpu") o.use_gpu = true;
+    else if (a == "--precision") {
+      const std::string v = next();
+      if (v == "fp64") o.precision = Precision::Fp64;
+      else if (v == "mixed") o.precision = Precision::Mixed;
+      else {
+        std::fprintf(stderr, "unknown precision '%s'\n", v.c_str());
+        return kUsage;
+      }
+    } else if (a == "--gpu") o.use_gpu = true;
     else if (a == "-v") o.verbosity = 1;
     else if (a == "-vv") o.verbosity = 2;
     else if (!a.empty() && a[0] == '-') {

@@ -63,11 +63,11 @@ struct EpiPrimal {
   __device__ void operator()(int j, bool has, double s) {
     if (!has) return;
     const T g = static_cast<T>(s);
-    const T xj = p.x[j];
-    const T h = clampv<T>(xj - tau * (p.c[j] - g), p.l[j], p.u[j]);
+    const T xj = ld_stream(p.x + j);
+    const T h = clampv<T>(xj - tau * (ld_stream(p.c + j) - g), ld_stream(p.l + j), ld_stream(p.u + j));
     if (p.write_hat) p.xh[j] = h;
     p.xb[j] = 2 * h - xj;
-    p.x[j] = tw * (p.a * h + p.b * xj) + (1 - tw) * p.x0[j];
+    p.x[j] = tw * (p.a * h + p.b * xj) + (1 - tw) * ld_stream(p.x0 + j);
   }
 };
 
@@ -101,14 +101,14 @@ struct EpiDual {
   __device__ void operator()(int i, bool has, double s) {
     if (!has) return;
     const T ax = static_cast<T>(s);
-    const T yi = p.y[i];
+    const T yi = ld_stream(p.y + i);
     const T q = ax - yi / sg;
-    const T h = yi - sg * ax + sg * clampv<T>(q, p.rl[i], p.ru[i]);
+    const T h = yi - sg * ax + sg * clampv<T>(q, ld_stream(p.rl + i), ld_stream(p.ru + i));
     if (p.write_hat) {
       p.yh[i] = h;
       p.yb[i] = 2 * h - yi;
     }
-    p.y[i] = tw * (p.a * h + p.b * yi) + (1 - tw) * p.y0[i];
+    p.y[i] = tw * (p.a * h + p.b * yi) + (1 - tw) * ld_stream(p.y0 + i);
   }
 };
 
@@ -148,8 +148,8 @@ struct EpiDeltaP {
   __device__ void operator()(int j, bool has, double s) {
     if (!has) return;
     const float g = static_cast<float>(s);
-    const float xj = p.dx[j];
-    const float h = clampv<float>(xj - tau * (p.G[j] - g), p.LX[j], p.UX[j]);
+    const float xj = ld_stream(p.dx + j);
+    const float h = clampv<float>(xj - tau * (ld_stream(p.G + j) - g), ld_stream(p.LX + j), ld_stream(p.UX + j));
     if (p.write_hat) p.dxh[j] = h;
     p.dxb[j] = 2 * h - xj;
     p.dx[j] = tw * (p.a * h + p.b * xj);
@@ -182,8 +182,8 @@ struct EpiDeltaD {
   __device__ void operator()(int i, bool has, double s) {
     if (!has) return;
     const float ax = static_cast<float>(s);
-    const float yi = p.dy[i];
-    const float t = clampv<float>(p.KY[i] + yi / sg, p.BU[i] + ax, p.BL[i] + ax);
+    const float yi = ld_stream(p.dy + i);
+    const float t = clampv<float>(ld_stream(p.KY + i) + yi / sg, ld_stream(p.BU + i) + ax, ld_stream(p.BL + i) + ax);
     const float h = yi - sg * t;
     if (p.write_hat) {
       p.dyh[i] = h;
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(kBlock) k_slice_partial(Sliced S, const int* _
   const int seg = tile * S.slices + k;
   const int b0 = S.seg[seg], b1 = S.seg[seg + 1];
   for (int p = b0 + static_cast<int>(threadIdx.x); p < b1; p += kBlock) {
-    prod[p - b0] = S.val[p] * x[S.col[p]];
+    prod[p - b0] = ld_stream(S.val + p) * x[ld_stream(S.col + p)];
     const int r = S.row[p];
     if (p == b0 || S.row[p - 1] != r) rs[r] = p - b0;
     if (p == b1 - 1 || S.row[p + 1] != r) re[r] = p - b0 + 1;
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(kBlock) k_slice_dual(Sliced S, int rows, DualA
   EpiDual<double> e(p);
   if (i >= rows) return;
   double s = 0.0;
-  for (int k = 0; k < S.slices; ++k) s += S.part[static_cast<size_t>(k) * rows + i];
+  for (int k = 0; k < S.slices; ++k) s += ld_stream(S.part + static_cast<size_t>(k) * rows + i);
   e(i, true, s);
 }
 
@@ -805,6 +805,9 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
   const bool delta = c->delta.on;
   // column-sliced dual half step: fp64, tree reductions (its row sums are not in CSR order)
   const bool use_slices = c->sliced && c->As.slices > 0 && !c->fp32 && !delta && g->mode == PSH_RED_TREE;
+  if (delta && c->nnz > 0 &&
+      !(c->delta.bf16 ? c->A.valh && c->At.valh : c->A.valf && c->At.valf))  // the launches' value arrays
+    return fail(c, PSH_E_STATE, "increment form without its matrix values");
   if (delta && (rc = delta_prepare(c, g->sigma))) return rc;
   hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, c->stream, c->prm, g->tau, g->sigma,
                      static_cast<double>(g->inner0));
@@ -946,8 +949,8 @@ int psh_delta(psh_ctx* c, int on) {
   if (on < 0 || on > 2) return fail(c, PSH_E_ARG, "psh_delta: on must be 0, 1 or 2");
   DeltaState& d = c->delta;
   const bool want = on != 0;
-  if (want && d.on) {  // switching the value type only
-    if (on == 2 && (rc = ensure_bf16_matrix(c))) return rc;
+  if (want && d.on) {  // switching the value type only: its copy of Ã must exist first
+    if ((rc = on == 2 ? ensure_bf16_matrix(c) : ensure_fp32_matrix(c))) return rc;
     d.bf16 = on == 2;
     return PSH_OK;
   }

@@ -287,6 +287,17 @@ __device__ inline T clampv(T v, T lo, T hi) {
 // Row-tiled sparse product with an fp64 row-sequential accumulator (CSR order) and a fused
 // epilogue epi(row, has_row, sum) run by every thread of the tile's workgroup.
 // The rows [r0, r1) of one tile (the whole workgroup calls it; r1 − r0 ≤ blockDim.x).
+// Loads of data that every launch streams once (matrix values and indices, the epilogue's
+// vectors), marked non-temporal so that the L2 keeps the randomly gathered vector.
+template <class T>
+__device__ inline T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ inline Bf16 ld_stream<Bf16>(const Bf16* p) {
+  return Bf16{__builtin_nontemporal_load(&p->bits)};
+}
+
 template <class V, class X, class Epi>
 __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
                                  const V* __restrict__ val, const X* __restrict__ x, Epi& epi) {
@@ -299,7 +310,7 @@ __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, co
   for (int cb = p0; cb < p1; cb += kTileNnz) {
     const int ce = p1 - cb > kTileNnz ? cb + kTileNnz : p1;
     for (int p = cb + static_cast<int>(threadIdx.x); p < ce; p += kBlock)
-      prod[p - cb] = static_cast<double>(val[p]) * static_cast<double>(x[col[p]]);
+      prod[p - cb] = static_cast<double>(ld_stream(val + p)) * static_cast<double>(x[ld_stream(col + p)]);
     __syncthreads();
     if (has) {
       const int b = rb > cb ? rb : cb, e = re < ce ? re : ce;
