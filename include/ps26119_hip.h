@@ -257,7 +257,10 @@ int psh_server_set_precision(psh_server* s, int id, int mixed);
 int psh_server_remove(psh_server* s, int id);
 int psh_server_set_iterate(psh_server* s, int id, const double* x, const double* y);
 int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count, double rho);
-int psh_server_check(psh_server* s, const int* ids, int count, psh_stats* out);
+/* want_ray: also psh_stats.ray (T(z) − z) of every listed instance, and its rays for
+ * psh_server_ray_vectors (unscaled; dual ray m, primal ray n). */
+int psh_server_check(psh_server* s, const int* ids, int count, int want_ray, psh_stats* out);
+int psh_server_ray_vectors(psh_server* s, int id, double* dual_ray, double* primal_ray);
 int psh_server_restart(psh_server* s, const int* ids, int count);
 int psh_server_download(psh_server* s, int id, double* xh, double* yh);
 

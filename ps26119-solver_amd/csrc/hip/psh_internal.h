@@ -321,6 +321,34 @@ __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, co
   epi(row, has, s);
 }
 
+// Row sums of one tile with the products staged in LDS (CSR order, as tile_rows): s = Σ a·x and,
+// with `abs_too`, sa = Σ |a·x|. The whole workgroup calls it.
+template <class V, class X>
+__device__ inline void tile_sums(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
+                                 const V* __restrict__ val, const X* __restrict__ x, bool abs_too, double& s,
+                                 double& sa) {
+  __shared__ double prod[kTileNnz];
+  const int p0 = ptr[r0], p1 = ptr[r1];
+  const int row = r0 + static_cast<int>(threadIdx.x);
+  const bool has = row < r1;
+  const int rb = has ? ptr[row] : 0, re = has ? ptr[row + 1] : 0;
+  s = 0.0;
+  sa = 0.0;
+  for (int cb = p0; cb < p1; cb += kTileNnz) {
+    const int ce = p1 - cb > kTileNnz ? cb + kTileNnz : p1;
+    for (int p = cb + static_cast<int>(threadIdx.x); p < ce; p += kBlock)
+      prod[p - cb] = static_cast<double>(val[p]) * static_cast<double>(x[col[p]]);
+    __syncthreads();
+    if (has) {
+      const int b = rb > cb ? rb : cb, e = re < ce ? re : ce;
+      for (int p = b; p < e; ++p) s += prod[p - cb];
+      if (abs_too)
+        for (int p = b; p < e; ++p) sa += fabs(prod[p - cb]);
+    }
+    __syncthreads();
+  }
+}
+
 template <class V, class X, class Epi>
 __device__ inline void tiled_rows(const int* __restrict__ tiles, int ntiles, const int* __restrict__ ptr,
                                   const int* __restrict__ col, const V* __restrict__ val, const X* __restrict__ x,

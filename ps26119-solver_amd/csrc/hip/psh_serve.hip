@@ -26,9 +26,16 @@ namespace {
 // Per-instance reduced quantities.
 enum SQ {
   Q_PX, Q_PY, Q_CROSS, Q_RP2, Q_B2, Q_D, Q_RD2, Q_C2, Q_P, Q_PIDX, Q_PIDY, Q_PMAX, Q_DMAX, Q_CINF, Q_R0PX, Q_R0PY,
-  Q_R0CROSS, kQ
+  Q_R0CROSS,
+  // ray tests on T(z) − z (reference engine.cpp:82-155): dual ray r, primal ray d
+  Q_RMAX, Q_ROBJ, Q_RMAG, Q_DVIOL, Q_DMAXRAY, Q_CD, Q_CMAG, Q_PVIOL, kQ
 };
-__host__ __device__ constexpr bool q_is_max(int q) { return q == Q_PMAX || q == Q_DMAX || q == Q_CINF; }
+static_assert(kQ <= 32, "quantity masks are 32-bit");
+__host__ __device__ constexpr bool q_is_max(int q) {
+  return q == Q_PMAX || q == Q_DMAX || q == Q_CINF || q == Q_RMAX || q == Q_DVIOL || q == Q_DMAXRAY || q == Q_PVIOL;
+}
+constexpr unsigned kRayMask = (1u << Q_RMAX) | (1u << Q_ROBJ) | (1u << Q_RMAG) | (1u << Q_DVIOL) | (1u << Q_DMAXRAY) |
+                              (1u << Q_CD) | (1u << Q_CMAG) | (1u << Q_PVIOL);
 
 struct Inst {
   int m, n;
@@ -36,6 +43,7 @@ struct Inst {
   const double *a_val, *at_val, *a_orig, *ato_val;
   const double *c, *l, *u, *rl, *ru, *rs, *cs, *clo, *cuo, *rlo, *ruo, *cmin;
   double *x, *x0, *xh, *xb, *dx, *xo, *y, *y0, *yh, *yb, *dy, *yo;
+  double *rv, *dv;  // unscaled dual (m) and primal (n) rays of the last ray check
   // increment form (PSH_SERVE_INCREMENT): fp32 values, δx, δx̄, δy and the anchor constants
   const float *a_valf, *at_valf;
   float *ix, *ixb, *G, *LX, *UX, *iy, *BL, *BU, *KY;
@@ -257,7 +265,7 @@ __device__ inline double elem_viol(double v, double lo, double up) {
 // Pass 1 (primal tiles, elementwise): Δx = x̄ − x̂; with kkt, the unscaled x and the PID term.
 // r0 = 1: the epoch-start residual of instances whose chunk began an epoch.
 __global__ void __launch_bounds__(kBlock) k_srv_cols1(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
-                                                      const Param* __restrict__ P, int kkt, int r0,
+                                                      const Param* __restrict__ P, int kkt, int r0, int ray,
                                                       double* __restrict__ part) {
   __shared__ double red[kBlock];
   const int t = tile_of_block(blockIdx.x, gridDim.x);
@@ -267,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_srv_cols1(const Tile* __restrict__ t
   if (!q.active || (r0 && q.inner0 != 0.0)) return;
   const Inst* d = I + T.inst;
   const int j = T.r0 + static_cast<int>(threadIdx.x);
-  double dx2 = 0, pid = 0;
+  double dx2 = 0, pid = 0, dmr = 0, cd = 0, cmag = 0;
   if (j < T.r1) {
     const double dd = d->xb[j] - d->xh[j];
     d->dx[j] = dd;
@@ -277,6 +285,15 @@ __global__ void __launch_bounds__(kBlock) k_srv_cols1(const Tile* __restrict__ t
       const double e = d->xh[j] - d->x0[j];
       pid = e * e;
     }
+    if (ray) {  // primal ray d = C·Δx/bs kept inside its sign constraints (engine.cpp:132-139)
+      double v = d->cs[j] * dd / d->bs;
+      if (isfinite(d->clo[j])) v = dmaxd(v, 0.0);
+      if (isfinite(d->cuo[j])) v = dmind(v, 0.0);
+      d->dv[j] = v;
+      dmr = fabs(v);
+      cd = d->cmin[j] * v;
+      cmag = fabs(cd);
+    }
   }
   double* out = part + static_cast<size_t>(t) * kQ;
   const double s1 = wg_reduce(dx2, false, red);
@@ -285,12 +302,20 @@ __global__ void __launch_bounds__(kBlock) k_srv_cols1(const Tile* __restrict__ t
     const double s2 = wg_reduce(pid, false, red);
     if (threadIdx.x == 0) out[Q_PIDX] = s2;
   }
+  if (ray) {
+    double v = wg_reduce(dmr, true, red);
+    if (threadIdx.x == 0) out[Q_DMAXRAY] = v;
+    v = wg_reduce(cd, false, red);
+    if (threadIdx.x == 0) out[Q_CD] = v;
+    v = wg_reduce(cmag, false, red);
+    if (threadIdx.x == 0) out[Q_CMAG] = v;
+  }
 }
 
 // Pass 2 (dual tiles): Δy, Ã·Δx and the cross term; with kkt, the unscaled y, A·x on the
 // original values and the row terms of the KKT statistics.
 __global__ void __launch_bounds__(kBlock) k_srv_rows(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
-                                                     const Param* __restrict__ P, int kkt, int r0,
+                                                     const Param* __restrict__ P, int kkt, int r0, int ray,
                                                      double* __restrict__ part) {
   __shared__ double prod[kTileNnz];
   __shared__ double prod2[kTileNnz];
@@ -322,12 +347,27 @@ __global__ void __launch_bounds__(kBlock) k_srv_rows(const Tile* __restrict__ ti
     }
     __syncthreads();
   }
+  double sdv = 0.0, sa_unused = 0.0;  // A·d of the primal ray (original values)
+  if (ray) tile_sums<double, double>(T.r0, T.r1, d->a_ptr, d->a_col, d->a_orig, d->dv, false, sdv, sa_unused);
   double dy2 = 0, cross = 0, pid = 0, rp2 = 0, b2 = 0, dt = 0, rd = 0, pmax = 0, dmax = 0;
+  double rmax = 0, robj = 0, rmag = 0, pviol = 0;
   if (has) {
     const double dd = d->yb[i] - d->yh[i];
     d->dy[i] = dd;
     dy2 = dd * dd;
     cross = dd * sdx;
+    if (ray) {  // dual ray r = R·Δy/os kept inside its sign constraints (engine.cpp:90-97), row terms
+      const double lo = d->rlo[i], up = d->ruo[i];
+      double r = d->rs[i] * dd / d->os;
+      if (!isfinite(lo)) r = dmind(r, 0.0);
+      if (!isfinite(up)) r = dmaxd(r, 0.0);
+      d->rv[i] = r;
+      rmax = fabs(r);
+      robj = r > 0 ? lo * r : (r < 0 ? up * r : 0.0);
+      rmag = fabs(robj);
+      if (isfinite(lo)) pviol = dmaxd(pviol, -sdv);  // A·d against the finite row bounds (engine.cpp:140-145)
+      if (isfinite(up)) pviol = dmaxd(pviol, sdv);
+    }
     if (kkt) {
       const double y = d->rs[i] * d->yh[i] / d->os;
       d->yo[i] = y;
@@ -354,6 +394,16 @@ __global__ void __launch_bounds__(kBlock) k_srv_rows(const Tile* __restrict__ ti
   if (threadIdx.x == 0) out[r0 ? Q_R0PY : Q_PY] = v;
   v = wg_reduce(cross, false, red);
   if (threadIdx.x == 0) out[r0 ? Q_R0CROSS : Q_CROSS] = v;
+  if (ray) {
+    v = wg_reduce(rmax, true, red);
+    if (threadIdx.x == 0) out[Q_RMAX] = v;
+    v = wg_reduce(robj, false, red);
+    if (threadIdx.x == 0) out[Q_ROBJ] = v;
+    v = wg_reduce(rmag, false, red);
+    if (threadIdx.x == 0) out[Q_RMAG] = v;
+    v = wg_reduce(pviol, true, red);
+    if (threadIdx.x == 0) out[Q_PVIOL] = v;
+  }
   if (!kkt) return;
   v = wg_reduce(pid, false, red);
   if (threadIdx.x == 0) out[Q_PIDY] = v;
@@ -373,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_srv_rows(const Tile* __restrict__ ti
 
 // Pass 3 (primal tiles): Aᵀ·y on the original values and the column terms of the KKT statistics.
 __global__ void __launch_bounds__(kBlock) k_srv_cols2(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
-                                                      const Param* __restrict__ P, double* __restrict__ part) {
+                                                      const Param* __restrict__ P, int ray, double* __restrict__ part) {
   __shared__ double red[kBlock];
   const int t = tile_of_block(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
@@ -419,6 +469,28 @@ __global__ void __launch_bounds__(kBlock) k_srv_cols2(const Tile* __restrict__ t
   if (threadIdx.x == 0) out[Q_DMAX] = v;
   v = wg_reduce(e.cinf, true, red);
   if (threadIdx.x == 0) out[Q_CINF] = v;
+  if (!ray) return;
+  // column terms of λ = −Aᵀr for the dual ray (engine.cpp:98-119)
+  double g, ga;
+  tile_sums<double, double>(T.r0, T.r1, d->at_ptr, d->ato_col, d->ato_val, d->rv, true, g, ga);
+  double obj = 0, mag = 0, viol = 0;
+  const int j = T.r0 + static_cast<int>(threadIdx.x);
+  if (j < T.r1) {
+    const double lam = -g, lo = d->clo[j], up = d->cuo[j];
+    if (lam > 0) {
+      if (isfinite(lo)) obj = lo * lam, mag = ga * fabs(lo);
+      else viol = lam;
+    } else if (lam < 0) {
+      if (isfinite(up)) obj = up * lam, mag = ga * fabs(up);
+      else viol = -lam;
+    }
+  }
+  v = wg_reduce(obj, false, red);
+  if (threadIdx.x == 0) out[Q_ROBJ] = v;
+  v = wg_reduce(mag, false, red);
+  if (threadIdx.x == 0) out[Q_RMAG] = v;
+  v = wg_reduce(viol, true, red);
+  if (threadIdx.x == 0) out[Q_DVIOL] = v;
 }
 
 // Per (instance, quantity): the instance's dual-tile partials then its primal-tile partials, in
@@ -432,9 +504,11 @@ __global__ void k_srv_reduce(const Inst* __restrict__ I, const Param* __restrict
   const Inst& d = I[k];
   const bool mx = q_is_max(q);
   const bool from_d = q == Q_PY || q == Q_CROSS || q == Q_RP2 || q == Q_B2 || q == Q_D || q == Q_RD2 ||
-                      q == Q_PIDY || q == Q_PMAX || q == Q_DMAX || q == Q_R0PY || q == Q_R0CROSS;
+                      q == Q_PIDY || q == Q_PMAX || q == Q_DMAX || q == Q_R0PY || q == Q_R0CROSS || q == Q_RMAX ||
+                      q == Q_ROBJ || q == Q_RMAG || q == Q_PVIOL;
   const bool from_p = q == Q_PX || q == Q_D || q == Q_RD2 || q == Q_C2 || q == Q_P || q == Q_PIDX || q == Q_PMAX ||
-                      q == Q_DMAX || q == Q_CINF || q == Q_R0PX;
+                      q == Q_DMAX || q == Q_CINF || q == Q_R0PX || q == Q_ROBJ || q == Q_RMAG || q == Q_DVIOL ||
+                      q == Q_DMAXRAY || q == Q_CD || q == Q_CMAG;
   double s = 0.0;
   if (from_d)
     for (int t = d.dt0; t < d.dt1; ++t) {
@@ -534,8 +608,8 @@ int shfail(psh_server* s, hipError_t e, const char* where) {
 struct Layout {
   size_t a_ptr, a_col, at_ptr, at_col, ato_col;
   size_t a_val, at_val, a_orig, ato_val;
-  size_t c, l, u, cs, clo, cuo, cmin, x, x0, xh, xb, dx, xo;
-  size_t rl, ru, rs, rlo, ruo, y, y0, yh, yb, dy, yo;
+  size_t c, l, u, cs, clo, cuo, cmin, x, x0, xh, xb, dx, xo, dv;
+  size_t rl, ru, rs, rlo, ruo, y, y0, yh, yb, dy, yo, rv;
   size_t a_valf, at_valf, ix, ixb, G, LX, UX, iy, BL, BU, KY;  // with PSH_SERVE_INCREMENT
   size_t total;
 };
@@ -557,9 +631,10 @@ Layout layout_of(long long m, long long n, long long nnz, int flags) {
   L.at_val = take(8 * nnz);
   L.a_orig = take(8 * nnz);
   L.ato_val = take(8 * nnz);
-  for (size_t* f : {&L.c, &L.l, &L.u, &L.cs, &L.clo, &L.cuo, &L.cmin, &L.x, &L.x0, &L.xh, &L.xb, &L.dx, &L.xo})
+  for (size_t* f : {&L.c, &L.l, &L.u, &L.cs, &L.clo, &L.cuo, &L.cmin, &L.x, &L.x0, &L.xh, &L.xb, &L.dx, &L.xo, &L.dv})
     *f = take(8 * n);
-  for (size_t* f : {&L.rl, &L.ru, &L.rs, &L.rlo, &L.ruo, &L.y, &L.y0, &L.yh, &L.yb, &L.dy, &L.yo}) *f = take(8 * m);
+  for (size_t* f : {&L.rl, &L.ru, &L.rs, &L.rlo, &L.ruo, &L.y, &L.y0, &L.yh, &L.yb, &L.dy, &L.yo, &L.rv})
+    *f = take(8 * m);
   if (flags & PSH_SERVE_INCREMENT) {
     L.a_valf = take(4 * nnz);
     L.at_valf = take(4 * nnz);
@@ -769,9 +844,9 @@ int psh_server_add(psh_server* s, const psh_problem* p, int flags, int* id) {
   d.a_orig = D(L.a_orig);
   d.ato_val = D(L.ato_val);
   d.c = D(L.c), d.l = D(L.l), d.u = D(L.u), d.cs = D(L.cs), d.clo = D(L.clo), d.cuo = D(L.cuo), d.cmin = D(L.cmin);
-  d.x = D(L.x), d.x0 = D(L.x0), d.xh = D(L.xh), d.xb = D(L.xb), d.dx = D(L.dx), d.xo = D(L.xo);
+  d.x = D(L.x), d.x0 = D(L.x0), d.xh = D(L.xh), d.xb = D(L.xb), d.dx = D(L.dx), d.xo = D(L.xo), d.dv = D(L.dv);
   d.rl = D(L.rl), d.ru = D(L.ru), d.rs = D(L.rs), d.rlo = D(L.rlo), d.ruo = D(L.ruo);
-  d.y = D(L.y), d.y0 = D(L.y0), d.yh = D(L.yh), d.yb = D(L.yb), d.dy = D(L.dy), d.yo = D(L.yo);
+  d.y = D(L.y), d.y0 = D(L.y0), d.yh = D(L.yh), d.yb = D(L.yb), d.dy = D(L.dy), d.yo = D(L.yo), d.rv = D(L.rv);
   if (flags & PSH_SERVE_INCREMENT) {
     auto F = [&](size_t o) { return reinterpret_cast<float*>(base + o); };
     d.a_valf = F(L.a_valf);
@@ -903,10 +978,10 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
       if (st == 0) {  // residual after the first step of instances that begin an epoch (r2hpdhg.cpp:75)
         if (s->ntp > 0)
           hipLaunchKernelGGL(k_srv_cols1, dim3(grid_tiles(s->ntp)), dim3(kBlock), 0, s->stream, s->d_ptiles, s->ntp,
-                             s->d_inst, s->d_param, 0, 1, s->d_part_p);
+                             s->d_inst, s->d_param, 0, 1, 0, s->d_part_p);
         if (s->ntd > 0)
           hipLaunchKernelGGL(k_srv_rows, dim3(grid_tiles(s->ntd)), dim3(kBlock), 0, s->stream, s->d_dtiles, s->ntd,
-                             s->d_inst, s->d_param, 0, 1, s->d_part_d);
+                             s->d_inst, s->d_param, 0, 1, 0, s->d_part_d);
         const unsigned mask = (1u << Q_R0PX) | (1u << Q_R0PY) | (1u << Q_R0CROSS);
         hipLaunchKernelGGL(k_srv_reduce, dim3((s->max_inst * kQ + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
                            s->d_inst, s->d_param, s->max_inst, mask, s->d_part_p, s->d_part_d, s->d_stats);
@@ -938,20 +1013,24 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
 }
 
 // Check of the instances active in the last chunk: out[k] for ids[k] (residual sums, r⁰ sums
-// when that chunk began the instance's epoch, KKT on the original problem, PID distances).
-int psh_server_check(psh_server* s, const int* ids, int count, psh_stats* out) {
+// when that chunk began the instance's epoch, KKT on the original problem, PID distances, and
+// with want_ray the ray-test scalars of T(z) − z).
+int psh_server_check(psh_server* s, const int* ids, int count, int want_ray, psh_stats* out) {
   if (!s || count < 0 || (count > 0 && (!ids || !out))) return PSH_E_ARG;
   if (s->tables_dirty) return sfail(s, PSH_E_STATE, "instance set changed since the last chunk");
+  const int ray = want_ray ? 1 : 0;
+  // the primal ray (cols1) is read by the row pass, the dual ray (rows) by the column pass
   if (s->ntp > 0)
     hipLaunchKernelGGL(k_srv_cols1, dim3(grid_tiles(s->ntp)), dim3(kBlock), 0, s->stream, s->d_ptiles, s->ntp, s->d_inst,
-                       s->d_param, 1, 0, s->d_part_p);
+                       s->d_param, 1, 0, ray, s->d_part_p);
   if (s->ntd > 0)
     hipLaunchKernelGGL(k_srv_rows, dim3(grid_tiles(s->ntd)), dim3(kBlock), 0, s->stream, s->d_dtiles, s->ntd, s->d_inst,
-                       s->d_param, 1, 0, s->d_part_d);
+                       s->d_param, 1, 0, ray, s->d_part_d);
   if (s->ntp > 0)
     hipLaunchKernelGGL(k_srv_cols2, dim3(grid_tiles(s->ntp)), dim3(kBlock), 0, s->stream, s->d_ptiles, s->ntp, s->d_inst,
-                       s->d_param, s->d_part_p);
-  const unsigned mask = ((1u << kQ) - 1) & ~((1u << Q_R0PX) | (1u << Q_R0PY) | (1u << Q_R0CROSS));
+                       s->d_param, ray, s->d_part_p);
+  unsigned mask = ((1u << kQ) - 1) & ~((1u << Q_R0PX) | (1u << Q_R0PY) | (1u << Q_R0CROSS));
+  if (!ray) mask &= ~kRayMask;
   hipLaunchKernelGGL(k_srv_reduce, dim3((s->max_inst * kQ + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream, s->d_inst,
                      s->d_param, s->max_inst, mask, s->d_part_p, s->d_part_d, s->d_stats);
   SRV_LAUNCHED(s, "serving check kernels");
@@ -980,8 +1059,18 @@ int psh_server_check(psh_server* s, const int* ids, int count, psh_stats* out) {
     o.kkt.dual_max_rel = q[Q_DMAX] / (1.0 + q[Q_CINF]);
     o.dx_norm = std::sqrt(q[Q_PIDX]);
     o.dy_norm = std::sqrt(q[Q_PIDY]);
+    if (ray) {
+      o.ray.rmax = q[Q_RMAX];
+      o.ray.dual_obj = q[Q_ROBJ];
+      o.ray.dual_viol = q[Q_DVIOL];
+      o.ray.dual_mag = q[Q_RMAG];
+      o.ray.dmax = q[Q_DMAXRAY];
+      o.ray.primal_cd = q[Q_CD];
+      o.ray.primal_viol = q[Q_PVIOL];
+      o.ray.primal_cmag = q[Q_CMAG];
+    }
     bool fin = true;
-    for (int j = 0; j < kQ; ++j) fin = fin && std::isfinite(q[j]);
+    for (int j = 0; j < kQ; ++j) fin = fin && (std::isfinite(q[j]) || (!ray && ((kRayMask >> j) & 1u)));
     o.finite = fin ? 1 : 0;
   }
   return PSH_OK;
@@ -1008,6 +1097,18 @@ int psh_server_restart(psh_server* s, const int* ids, int count) {
                        s->d_param, 1);
   SRV_LAUNCHED(s, "serving restart");
   return PSH_OK;  // the flags stay set in the pinned table until the next chunk clears them
+}
+
+int psh_server_ray_vectors(psh_server* s, int id, double* dual_ray, double* primal_ray) {
+  if (!s || id < 0 || id >= s->max_inst || !s->slots[id].used) return PSH_E_ARG;
+  const Inst& d = s->slots[id].inst;
+  if (dual_ray && d.m > 0)
+    SRV_TRY(s, hipMemcpyAsync(dual_ray, d.rv, 8 * static_cast<size_t>(d.m), hipMemcpyDeviceToHost, s->stream));
+  if (primal_ray && d.n > 0)
+    SRV_TRY(s, hipMemcpyAsync(primal_ray, d.dv, 8 * static_cast<size_t>(d.n), hipMemcpyDeviceToHost, s->stream));
+  SRV_TRY(s, hipStreamSynchronize(s->stream));
+  s->param_pending = false;
+  return PSH_OK;
 }
 
 // x̂ (n) and ŷ (m), scaled, of one instance.

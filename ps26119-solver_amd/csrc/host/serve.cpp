@@ -40,6 +40,7 @@ struct Server::Request {
   Status status = Status::NotSolved;
   std::string message;
   std::vector<double> xs, ys;
+  std::vector<double> dual_ray, primal_ray;  // certificate of an Infeasible / Unbounded verdict
 };
 
 namespace {
@@ -207,6 +208,8 @@ void Server::finish(Request& r) {
   sol.gap = k.rel_gap();
   sol.status = status;
   sol.message = r.message;
+  if (status == Status::Infeasible) sol.dual_ray = r.dual_ray;
+  if (status == Status::Unbounded) sol.primal_ray = r.primal_ray;
   sol.iterations = r.it;
   sol.iterations_to_fast = r.fast_it;
   sol.seconds_to_fast = r.fast_s;
@@ -270,6 +273,32 @@ void Server::serve_loop() {
   std::vector<int> ids, restart_ids;
   std::vector<psh_stats> stats;
   const int K = opt_.check_every;
+  // A ray verdict stands only when the ray passes the certificate check on the request's own
+  // model (the finishing gate checks it again).
+  auto certify = [&](Request& r, Status v, double robj, double rviol) {
+    std::vector<double> dr, pr;
+    if (v == Status::Infeasible) dr.assign(r.sp->m, 0.0);
+    else pr.assign(r.sp->n, 0.0);
+    if (psh_server_ray_vectors(dev_, r.slot, dr.empty() ? nullptr : dr.data(), pr.empty() ? nullptr : pr.data()) != PSH_OK)
+      return false;
+    char buf[200];
+    bool ok;
+    if (v == Status::Infeasible) {
+      ok = check_infeasibility_certificate(r.model, dr).passed;
+      std::snprintf(buf, sizeof buf, "primal infeasible: dual ray certificate (objective %f, violation %f)", robj, rviol);
+    } else {
+      std::vector<double> xs(r.sp->n), x;
+      if (psh_server_download(dev_, r.slot, xs.data(), nullptr) != PSH_OK) return false;
+      r.sp->unscale_primal(xs, x);
+      ok = check_unboundedness_certificate(r.model, x, pr).passed;
+      std::snprintf(buf, sizeof buf, "unbounded: primal ray certificate (c'd %f, violation %f)", robj, rviol);
+    }
+    if (!ok) return false;
+    r.message = r.message.empty() ? std::string(buf) : std::string(buf) + "; " + r.message;
+    r.dual_ray = std::move(dr);
+    r.primal_ray = std::move(pr);
+    return true;
+  };
   auto fail_live = [&](const std::string& why) {
     for (auto& r : live) {
       r->status = Status::NumericalError;
@@ -329,9 +358,11 @@ void Server::serve_loop() {
       ids.push_back(r->slot);
     }
     int rc = psh_server_chunk(dev_, K, params.data(), static_cast<int>(params.size()), 1.0);
+    bool want_ray = false;  // ray tests from an instance's fourth check on (reference r2hpdhg.cpp:93-111)
+    for (auto& r : live) want_ray = want_ray || r->it + K >= 4 * K;
     if (rc == PSH_OK) {
       stats.assign(live.size(), psh_stats{});
-      rc = psh_server_check(dev_, ids.data(), static_cast<int>(ids.size()), stats.data());
+      rc = psh_server_check(dev_, ids.data(), static_cast<int>(ids.size()), want_ray ? 1 : 0, stats.data());
     }
     if (rc != PSH_OK) {
       fail_live(std::string("serving device error: ") + psh_server_error(dev_));
@@ -368,6 +399,11 @@ void Server::serve_loop() {
       if (o.verbosity >= 2)
         std::fprintf(stderr, "served %lld it %8lld  rp %.2e  rd %.2e  gap %.2e\n", static_cast<long long>(r->id),
                      static_cast<long long>(r->it), k.rel_primal(), k.rel_dual(), k.rel_gap());
+      if (verdict == Status::NotSolved && want_ray && r->it >= 4 * K) {
+        double robj = 0, rviol = 0;
+        const Status v = ray_decision(st.ray, robj, rviol);
+        if (v != Status::NotSolved && certify(*r, v, robj, rviol)) verdict = v;
+      }
       if (verdict != Status::NotSolved) {
         r->status = verdict;
         retire.push_back(r);

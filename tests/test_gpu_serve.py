@@ -155,3 +155,31 @@ def test_mixed_request_hands_over_to_fp64_on_a_stall(gpu_available):
     print(f"served mixed, patience 1: it={r.iterations} {r.message}", flush=True)
     assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
     assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
+
+
+def test_infeasible_and_unbounded_requests_are_certified(gpu_available, oracle):
+    """Served requests are tested for a dual / primal ray from their fourth check on, like the
+    single solve (an addition to the batching loop): the reference's hand-made infeasible and
+    unbounded LPs and infeasible scenarios of a 2,000-row random LP end Infeasible / Unbounded
+    with a certificate that passes the check on their own model, while an optimal request in the
+    same stream is unaffected. Statuses equal the oracle's single solves."""
+    import ps26119
+    from ps26119_amd import lpgen
+    hm = hand_models()
+    base = lpgen.random_lp(2000, seed=3)
+    scen = lpgen.scenario_set(base, 4, sigma=0.05, seed=7)
+    lps = [hm["infeasible"], hm["unbounded"], hm["wyndor"]] + [lpgen.with_scenario(base, sc) for sc in scen]
+    exp = [oracle.solve(lp, presolve=False, threads=4)[0]["status"] for lp in lps]
+    with ps26119.Server(budget_bytes=1 << 30) as srv:
+        ids = [srv.submit(lp, iteration_limit=100_000 if k == 1 else 2_000_000, precision="mixed" if k % 2 else "fp64")
+               for k, lp in enumerate(lps)]
+        res = [srv.wait(i, timeout=300) for i in ids]
+    for k, (r, e) in enumerate(zip(res, exp)):
+        print(f"request {k}: {r.status_name} (oracle {e}) it={r.iterations} check={r.check} {r.message[:100]}",
+              flush=True)
+        if k == 1 and r.status == ps26119.ITERATION_LIMIT:
+            continue  # an unbounded verdict also needs a primal-feasible iterate, which may not come
+        assert r.status == e, (k, r.message)
+        if r.status in (ps26119.INFEASIBLE, ps26119.UNBOUNDED):
+            assert r.check == "PASS" and "ray certificate" in r.message, (k, r.message)
+    assert res[2].status == ps26119.OPTIMAL and abs(res[2].objective - 36.0) <= 1e-6
