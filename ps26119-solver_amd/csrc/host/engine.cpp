@@ -382,21 +382,28 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   Context(const Model& model, const EngineOptions& opt)
       : model_(model), opt_(opt), t0_(std::chrono::steady_clock::now()) {
     sp_ = make_scaled_problem(model, opt.scaling);
+    const double t_scale = elapsed();
     if (sp_.A.nnz() >= (1LL << 31)) {
       error_ = "matrix has 2^31 or more nonzeros (beyond the 32-bit device index range)";
       return;
     }
     dev_ = Device::open(opt.device, error_);
     if (!dev_) return;
+    const double t_open = elapsed();
     dev_->load(sp_);
     if (opt.timing) dev_->check(psh_timing(dev_->ctx(), 1), "timing");
     dev_->check(psh_graphs(dev_->ctx(), opt.graphs), "graphs");
     dev_->check(psh_sliced(dev_->ctx(), opt.sliced), "sliced");
+    dev_->check(psh_sync(dev_->ctx()), "load");
+    const double t_load = elapsed();
     const bool empty = sp_.n == 0 || sp_.A.nnz() == 0;
     const double norm = empty ? 0.0 : estimate_norm2(*dev_, sp_.n, opt.cpu_order);
     eta_ = norm > 0 ? 0.998 / norm : 1.0;
     dev_->check(psh_sync(dev_->ctx()), "setup");
     setup_seconds_ = elapsed();
+    if (opt.verbosity >= 1)
+      std::fprintf(stderr, "setup: scaling %.3fs, device open %.3fs, load %.3fs, norm %.3fs\n", t_scale,
+                   t_open - t_scale, t_load - t_open, setup_seconds_ - t_load);
   }
   bool ok() const { return dev_ != nullptr; }
   const std::string& error() const { return error_; }

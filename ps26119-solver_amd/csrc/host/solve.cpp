@@ -142,6 +142,8 @@ Solution solve_impl(const Model& model, const Options& options) {
   if (auto why = model.crossed_bounds(); !why.empty()) return trivially_infeasible(model, options, why);
   const auto t0 = std::chrono::steady_clock::now();
   PresolveResult pr = presolve(model);
+  if (options.verbosity >= 1)
+    std::fprintf(stderr, "presolve: %.3fs\n", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   if (pr.outcome == PresolveResult::Outcome::Unchanged) return solve_direct(model, options, true);
   Options inner = options;
   inner.presolve = false;

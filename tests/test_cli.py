@@ -127,3 +127,15 @@ def test_gpu_batch_command(gpu_available, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("Optimal") == 4
     assert len(list(outdir.glob("*.sol"))) == 4
+    rm = run("batch", str(bp), *files, "--precision", "mixed")
+    assert rm.returncode == 0 and rm.stdout.count("Optimal") == 4, rm.stdout + rm.stderr
+    assert "fp32 increments" in rm.stdout
+
+
+def test_batch_precision_must_be_known(tmp_path):
+    from ps26119_amd import lpgen
+    a = lpgen.random_lp(30, seed=1)
+    pa = tmp_path / "a.lpm"
+    lpgen.write_lpm(a, str(pa))
+    r = run("batch", str(pa), str(pa), "--precision", "fp16")
+    assert r.returncode == 2 and "unknown precision" in r.stderr
