@@ -252,6 +252,20 @@ def bench_c3(args, ps26119, lpgen):
     single_rate = (hi - lo) / (ts[1] - ts[0])
     log(f"batch: {batch_rate:.0f} scenario-iterations/s ({(tb[1] - tb[0]) / (hi - lo) * 1e3:.3f} ms/iteration for "
         f"{len(scen)}), mixed {mixed_rate:.0f}; single LP: {single_rate:.0f} iterations/s")
+    W, nz = len(scen), base.nnz
+    # algorithmic bytes of one batched iteration (all slots live): the matrix once, per (row, slot) the
+    # vectors read and written, and one gathered value per (nonzero, slot)
+    b64 = (12 * nz + 4 * (base.n + 1) + base.n * W * 56 + nz * W * 8) + (12 * nz + 4 * (base.m + 1) + base.m * W * 40 +
+                                                                         nz * W * 8)
+    b32 = (8 * nz + 4 * (base.n + 1) + base.n * W * 24 + nz * W * 4) + (8 * nz + 4 * (base.m + 1) + base.m * W * 20 +
+                                                                        nz * W * 4)
+    loop_gbs = {"fp64": b64 * batch_rate / W / 1e9, "mixed": b32 * mixed_rate / W / 1e9}
+    if args.c3_rate_only:
+        line = base_line(args, batch_rate, "scenario-iterations/s", True, "f64",
+                         {"workload": f"{W} scenarios sharing a random {base.m}x{base.n} matrix: batch rates only"})
+        line.update({"mixed_scenario_iterations_per_s": mixed_rate, "loop_GBs": loop_gbs, "roofline": None,
+                     "cpu_baseline": None})
+        return line
     # (b) end to end at 10k rows
     small = lpgen.random_lp(10_000, seed=args.seed)
     sscen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)]
@@ -283,7 +297,11 @@ def bench_c3(args, ps26119, lpgen):
     line["steps"] = hi - lo
     line["ms_per_step"] = (tb[1] - tb[0]) / (hi - lo) * 1e3
     line.update({
-        "roofline": None, "single_lp_iterations_per_s": single_rate,
+        "roofline": {"bound": "hbm", "achieved": loop_gbs["fp64"], "peak": 8000.0, "unit": "GB/s",
+                     "frac": loop_gbs["fp64"] / 8000.0, "traffic": None,
+                     "kernel": "batched iteration loop (k_bprimal + k_bdual + the check every 64), from the loop rate",
+                     "bytes_per_iteration": b64},
+        "loop_GBs": loop_gbs, "single_lp_iterations_per_s": single_rate,
         "speedup_vs_single_lp_iterations": batch_rate / single_rate,
         "mixed_scenario_iterations_per_s": mixed_rate,
         "end_to_end_10k": {"scenarios": len(sscen), "tolerance": 1e-6, "warm_start": "base solution",
@@ -375,6 +393,7 @@ def main():
     ap.add_argument("--scenarios", type=int, default=64, help="c3")
     ap.add_argument("--seq-sample", type=int, default=4, help="c3: sequential solves timed for the comparison")
     ap.add_argument("--c3-iterations", type=int, default=1024, help="c3: iterations of the rate measurement")
+    ap.add_argument("--c3-rate-only", action="store_true", help="c3: only the batch rates (profiling runs)")
     ap.add_argument("--requests", type=int, default=48, help="c4")
     ap.add_argument("--rate", type=float, default=8.0, help="c4: arrivals per second")
     ap.add_argument("--m-lo", type=int, default=1000, help="c4")

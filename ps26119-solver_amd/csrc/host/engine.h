@@ -44,6 +44,10 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   // reference's fp32-then-fp64 switch; CPU-order mode (hip_cpu_order = 1) defaults it to 0.
   int refine = -1;
   int refine_patience = 40;  // hip_refine_patience: stalled checks before fp64 takes over
+  // hip_refine_max_epoch: longest epoch of the increment form (0 = no limit). The Halpern term
+  // (1 − w)·δ with w = (k+1)/(k+2) falls below fp32 resolution of the increments once k nears 10^6,
+  // so a longer epoch loses its anchoring; the epoch then ends with a restart.
+  std::int64_t refine_max_epoch = 1 << 16;
   // hip_bf16_until: > 0 runs the increment form's SpMVs with bf16 matrix values until the relative
   // KKT error reaches this value or stops improving for 10 checks, then with fp32 values (0: off).
   double bf16_until = 0.0;
@@ -95,6 +99,10 @@ psh_problem problem_of(const ScaledProblem& sp, std::vector<double>& cmin);
 // Restart test of r²HPDHG (reference src/pdhg/r2hpdhg.cpp:118-122): sufficient, necessary or
 // artificial restart. r0 < 0 means no epoch-start residual yet.
 bool restart_due(const EngineOptions& o, double r, double r0, double r_last, std::int64_t inner, std::int64_t it);
+// The epoch limit of the increment form (EngineOptions::refine_max_epoch).
+inline bool increment_epoch_full(const EngineOptions& o, bool increment_form, std::int64_t inner) {
+  return increment_form && o.refine_max_epoch > 0 && inner >= o.refine_max_epoch;
+}
 
 // PID primal weight with damping and the best-balance fallback (reference r2hpdhg.cpp:127-148).
 struct PidState {
