@@ -113,6 +113,18 @@ size_t psh_device_bytes(const psh_ctx* ctx); /* device memory held by the contex
 
 /* ---- problem and precision */
 int psh_load(psh_ctx* ctx, const psh_problem* p);
+/* Scaling on the device (reference src/pdhg/scaling.cpp:37-142), for a problem loaded with the
+ * ORIGINAL values in a_val / at_val (at_val in Ãᵀ's order) and unit scales: geo_iterations
+ * geometric-mean sweeps (0 = none), ruiz_iterations ∞-norm sweeps, then one Pock–Chambolle sweep
+ * if asked, with the reference's operations in its order, so factors and values are the bits of
+ * the host scaling. Leaves Ã, Ãᵀ scaled in place and the row / column factors in the device's
+ * scale vectors, and writes them to row_scale (m) and col_scale (n). Once per context, before any
+ * precision change. */
+int psh_scale(psh_ctx* ctx, int geo_iterations, int ruiz_iterations, int pock_chambolle, double* row_scale,
+              double* col_scale);
+/* Replaces the scaled c, l, u (n), rl, ru (m) and the bound / objective scalars (after psh_scale). */
+int psh_set_vectors(psh_ctx* ctx, const double* c, const double* l, const double* u, const double* rl, const double* ru,
+                    double bound_scale, double obj_scale);
 int psh_set_fp32(psh_ctx* ctx, int on); /* working precision: converts every slot (reference set_precision) */
 int psh_is_fp32(const psh_ctx* ctx);
 

@@ -301,6 +301,62 @@ __global__ void k_to_bf16(const double* __restrict__ a, Bf16* __restrict__ b, lo
   b[i].bits = static_cast<unsigned short>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+__global__ void k_gather_vals(const double* __restrict__ a, const int* __restrict__ src, double* __restrict__ b,
+                              long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[src[i]];
+}
+
+// ---- scaling sweeps on the device (reference src/pdhg/scaling.cpp:37-142; the host restatement
+// csrc/host/scaling.cpp does the same operations in the same order, so the factors and values
+// are the same bits). One thread per row of Ã or of Ãᵀ (a column of A).
+// Geometric-mean factor of a row from |a|·f over its nonzeros: 1/(√lo·√hi), clamped.
+__global__ void k_sc_geo(const int* __restrict__ ptr, const int* __restrict__ col, const double* __restrict__ a0,
+                         const double* __restrict__ f, double* __restrict__ out, int rows) {
+  const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  double lo = HUGE_VAL, hi = 0.0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const double v = fabs(a0[k]) * f[col[k]];
+    if (v > 0 && isfinite(v)) {
+      lo = v < lo ? v : lo;
+      hi = hi < v ? v : hi;
+    }
+  }
+  if (hi > 0) {
+    const double g = 1.0 / (sqrt(lo) * sqrt(hi));
+    const double ge = g < 1e-20 ? 1e-20 : g;
+    out[i] = 1e20 < ge ? 1e20 : ge;
+  }
+}
+
+// Ruiz (∞-norm) or Pock–Chambolle (1-norm) factor of a row: 1/√‖row‖, 1 for an empty or non-finite row.
+__global__ void k_sc_norm(const int* __restrict__ ptr, const double* __restrict__ val, int inf_norm,
+                          double* __restrict__ out, int rows) {
+  const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  double s = 0.0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const double a = fabs(val[k]);
+    s = inf_norm ? (s < a ? a : s) : s + a;
+  }
+  out[i] = s > 0.0 && isfinite(s) ? 1.0 / sqrt(s) : 1.0;
+}
+
+// val ← val·(frow_i·fcol_j) for every nonzero (row i, column j) of the CSR.
+__global__ void k_sc_apply(const int* __restrict__ ptr, const int* __restrict__ col, double* __restrict__ val,
+                           const double* __restrict__ frow, const double* __restrict__ fcol, int rows) {
+  const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const double fr = frow[i];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) val[k] *= fr * fcol[col[k]];
+}
+
+__global__ void k_sc_mul(double* __restrict__ v, const double* __restrict__ f, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= f[i];
+}
+
 template <class T>
 __global__ void k_fill(T* __restrict__ v, T a, long long n) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -575,7 +631,7 @@ void psh_destroy(psh_ctx* c) {
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
                   c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->As.seg, c->As.col,
-                  c->As.val, c->As.row, c->As.part};
+                  c->As.src, c->As.val, c->As.row, c->As.part};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
@@ -646,8 +702,7 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
     S.slices = kSlices;
     S.ntp = (c->A.ntiles + per - 1) / per * per;
     const long long width = (static_cast<long long>(n) + kSlices - 1) / kSlices;
-    std::vector<int> seg(static_cast<size_t>(S.ntp) * kSlices + 1, 0), scol(nz);
-    std::vector<double> sval(nz);
+    std::vector<int> seg(static_cast<size_t>(S.ntp) * kSlices + 1, 0), scol(nz), ssrc(nz);
     std::vector<unsigned char> srow(nz);
     size_t q = 0;
     std::vector<int> cut(kSlices + 1);
@@ -660,7 +715,7 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
           while (b < e && p->a_col[b] / width < k) ++b;
           for (; b < e && p->a_col[b] / width == k; ++b, ++q) {
             scol[q] = p->a_col[b];
-            sval[q] = p->a_val[b];
+            ssrc[q] = b;
             srow[q] = static_cast<unsigned char>(r - r0);
           }
         }
@@ -668,9 +723,11 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
       }
     }
     if (static_cast<long long>(q) != nz || (rc = up(&S.seg, seg.data(), static_cast<long long>(seg.size()))) ||
-        (rc = up(&S.col, scol.data(), nz)) || (rc = up(&S.val, sval.data(), nz)) || (rc = up(&S.row, srow.data(), nz)) ||
-        (rc = alloc_n(c, &S.part, static_cast<long long>(kSlices) * m)))
+        (rc = up(&S.col, scol.data(), nz)) || (rc = up(&S.src, ssrc.data(), nz)) || (rc = alloc_n(c, &S.val, nz)) ||
+        (rc = up(&S.row, srow.data(), nz)) || (rc = alloc_n(c, &S.part, static_cast<long long>(kSlices) * m)))
       return rc ? rc : fail(c, PSH_E_ARG, "column slices do not cover the matrix");
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(nz)), dim3(kBlock), 0, c->stream, c->A.val, S.src, S.val, nz);
+    PSH_LAUNCHED(c, "sliced values");
     PSH_TRY(c, hipStreamSynchronize(c->stream));  // the staging vectors go out of scope
   }
   for (int s = 0; s < PSH_NSLOT; ++s)
@@ -699,6 +756,84 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
                          slot_len(c, s));
   PSH_LAUNCHED(c, "slot init");
   c->loaded = true;
+  return PSH_OK;
+}
+
+int psh_scale(psh_ctx* c, int geo_iterations, int ruiz_iterations, int pock_chambolle, double* row_scale,
+              double* col_scale) {
+  if (!c || !c->loaded) return c ? fail(c, PSH_E_STATE, "no problem loaded") : PSH_E_ARG;
+  if (c->scaled) return fail(c, PSH_E_STATE, "the problem is already scaled");
+  if (c->delta.on || c->fp32 || c->A.valf || c->A.valh) return fail(c, PSH_E_STATE, "scale before any precision change");
+  if (geo_iterations < 0 || ruiz_iterations < 0) return fail(c, PSH_E_ARG, "negative sweep count");
+  PSH_TRY(c, hipSetDevice(c->device));
+  const int m = c->m, n = c->n;
+  double *r = nullptr, *f = nullptr;
+  int rc;
+  if ((rc = alloc_n(c, &r, std::max(m, 1))) || (rc = alloc_n(c, &f, std::max(n, 1)))) {
+    dev_free(r);
+    return rc;
+  }
+  const int gm = grid_for(std::max(m, 1)), gn = grid_for(std::max(n, 1));
+  auto ones = [&](double* v, long long len) {
+    if (len > 0) hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(len)), dim3(kBlock), 0, c->stream, v, 1.0, len);
+  };
+  ones(c->rs, m);
+  ones(c->cs, n);
+  if (geo_iterations > 0) {
+    // factors from the ORIGINAL values every sweep (Ã's order: a_orig; Ãᵀ's order: At.val, not
+    // yet modified), rows then columns, applied once at the end; the factors are the scales
+    ones(r, m);
+    ones(f, n);
+    for (int it = 0; it < geo_iterations; ++it) {
+      if (m > 0) hipLaunchKernelGGL(k_sc_geo, dim3(gm), dim3(kBlock), 0, c->stream, c->A.ptr, c->A.col, c->a_orig, f, r, m);
+      if (n > 0) hipLaunchKernelGGL(k_sc_geo, dim3(gn), dim3(kBlock), 0, c->stream, c->At.ptr, c->At.col, c->At.val, r, f, n);
+    }
+    if (m > 0) hipLaunchKernelGGL(k_sc_apply, dim3(gm), dim3(kBlock), 0, c->stream, c->A.ptr, c->A.col, c->A.val, r, f, m);
+    if (n > 0) hipLaunchKernelGGL(k_sc_apply, dim3(gn), dim3(kBlock), 0, c->stream, c->At.ptr, c->At.col, c->At.val, f, r, n);
+    if (m > 0) hipLaunchKernelGGL(k_sc_mul, dim3(gm), dim3(kBlock), 0, c->stream, c->rs, r, static_cast<long long>(m));
+    if (n > 0) hipLaunchKernelGGL(k_sc_mul, dim3(gn), dim3(kBlock), 0, c->stream, c->cs, f, static_cast<long long>(n));
+  }
+  auto sweep = [&](int inf_norm) {
+    if (m > 0) hipLaunchKernelGGL(k_sc_norm, dim3(gm), dim3(kBlock), 0, c->stream, c->A.ptr, c->A.val, inf_norm, r, m);
+    if (n > 0) hipLaunchKernelGGL(k_sc_norm, dim3(gn), dim3(kBlock), 0, c->stream, c->At.ptr, c->At.val, inf_norm, f, n);
+    if (m > 0) hipLaunchKernelGGL(k_sc_apply, dim3(gm), dim3(kBlock), 0, c->stream, c->A.ptr, c->A.col, c->A.val, r, f, m);
+    if (n > 0) hipLaunchKernelGGL(k_sc_apply, dim3(gn), dim3(kBlock), 0, c->stream, c->At.ptr, c->At.col, c->At.val, f, r, n);
+    if (m > 0) hipLaunchKernelGGL(k_sc_mul, dim3(gm), dim3(kBlock), 0, c->stream, c->rs, r, static_cast<long long>(m));
+    if (n > 0) hipLaunchKernelGGL(k_sc_mul, dim3(gn), dim3(kBlock), 0, c->stream, c->cs, f, static_cast<long long>(n));
+  };
+  for (int it = 0; it < ruiz_iterations; ++it) sweep(1);
+  if (pock_chambolle) sweep(0);
+  if (c->As.slices > 0 && c->nnz > 0)  // the sliced copy takes the scaled values
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->A.val, c->As.src,
+                       c->As.val, c->nnz);
+  PSH_LAUNCHED(c, "scaling sweeps");
+  if (row_scale && m > 0) PSH_TRY(c, hipMemcpyAsync(row_scale, c->rs, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+  if (col_scale && n > 0) PSH_TRY(c, hipMemcpyAsync(col_scale, c->cs, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
+  dev_free(r);
+  dev_free(f);
+  c->scaled = true;
+  return PSH_OK;
+}
+
+int psh_set_vectors(psh_ctx* c, const double* cc, const double* l, const double* u, const double* rl, const double* ru,
+                    double bound_scale, double obj_scale) {
+  if (!c || !c->loaded) return c ? fail(c, PSH_E_STATE, "no problem loaded") : PSH_E_ARG;
+  if (c->fp32) return fail(c, PSH_E_STATE, "set the vectors before any precision change");
+  const size_t n = static_cast<size_t>(c->n), m = static_cast<size_t>(c->m);
+  auto put = [&](double* dst, const double* src, size_t len) -> int {
+    if (len == 0) return PSH_OK;
+    if (!src) return fail(c, PSH_E_ARG, "null vector");
+    PSH_TRY(c, hipMemcpyAsync(dst, src, sizeof(double) * len, hipMemcpyHostToDevice, c->stream));
+    return PSH_OK;
+  };
+  int rc;
+  if ((rc = put(c->c, cc, n)) || (rc = put(c->l, l, n)) || (rc = put(c->u, u, n)) || (rc = put(c->rl, rl, m)) ||
+      (rc = put(c->ru, ru, m)))
+    return rc;
+  c->bs = bound_scale;
+  c->os = obj_scale;
+  PSH_TRY(c, hipStreamSynchronize(c->stream));  // the host arrays may go away
   return PSH_OK;
 }
 

@@ -57,6 +57,7 @@ struct Sliced {
   int ntp = 0;                    // tiles padded to a multiple of 8/S
   int* seg = nullptr;             // ntp·S + 1 segment starts
   int* col = nullptr;             // nnz
+  int* src = nullptr;             // nnz: index of each entry in Ã (values are gathered on the device)
   double* val = nullptr;          // nnz
   unsigned char* row = nullptr;   // nnz: row within the tile
   double* part = nullptr;         // S·rows slice sums
@@ -152,6 +153,7 @@ struct psh_ctx {
   int m = 0, n = 0;
   long long nnz = 0;
   bool loaded = false, fp32 = false;
+  bool scaled = false;  // psh_scale ran
   size_t bytes = 0;
 
   psh::DevMat A, At;             // scaled Ã (m×n) and Ãᵀ (n×m)

@@ -98,11 +98,11 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
   try {
     ScalingOptions so = eo.scaling;
     so.bound_objective_rescaling = false;  // per scenario below (reference batch.cpp:182-221)
-    const ScaledProblem sp = make_scaled_problem(base, so);
     std::string err;
     std::unique_ptr<Device> dev = Device::open(eo.device, err);
     if (!dev) return fail_all(err);
-    dev->load(sp);
+    ScaledProblem sp;
+    scale_and_load(*dev, base, so, eo.device_scaling != 0, sp);
     psh_ctx* c = dev->ctx();
     if (eo.timing) dev->check(psh_timing(c, 1), "timing");
     dev->check(psh_graphs(c, eo.graphs), "graphs");

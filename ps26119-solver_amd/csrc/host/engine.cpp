@@ -59,6 +59,7 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_timing") e.timing = v != 0 ? 1 : 0;
     else if (name == "hip_graphs") e.graphs = v != 0 ? 1 : 0;
     else if (name == "hip_sliced") e.sliced = v != 0 ? 1 : 0;
+    else if (name == "hip_device_scaling") e.device_scaling = v != 0 ? 1 : 0;
     else if (name == "hip_refine") e.refine = v != 0 ? 1 : 0;
     else if (name == "hip_refine_patience") e.refine_patience = std::max(1, static_cast<int>(v));
     else if (name == "hip_bf16_until") e.bf16_until = v > 0 ? v : 0.0;
@@ -219,6 +220,25 @@ psh_problem problem_of(const ScaledProblem& sp, std::vector<double>& cmin) {
   p.ru_o = M.row_upper.data();
   p.c_min = cmin.data();
   return p;
+}
+
+void scale_and_load(Device& dev, const Model& model, const ScalingOptions& so, bool on_device, ScaledProblem& sp) {
+  if (!on_device) {
+    sp = make_scaled_problem(model, so);
+    dev.load(sp);
+    return;
+  }
+  sp = make_unscaled_problem(model, so);
+  dev.load(sp);
+  dev.check(psh_scale(dev.ctx(), sp.geometric_mean_applied ? so.geometric_mean_iterations : 0, so.ruiz_iterations,
+                      so.pock_chambolle ? 1 : 0, sp.row_scale.data(), sp.col_scale.data()),
+            "scaling");
+  finish_scaled_problem(sp, model, so);
+  dev.check(psh_set_vectors(dev.ctx(), sp.c.data(), sp.col_lower.data(), sp.col_upper.data(), sp.row_lower.data(),
+                            sp.row_upper.data(), sp.bound_scale, sp.obj_scale),
+            "scaling");
+  std::vector<double>().swap(sp.A.val);  // the scaled values live on the device only
+  std::vector<double>().swap(sp.At.val);
 }
 
 Status ray_decision(const psh_ray& t, double& obj, double& viol) {
@@ -382,16 +402,14 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
  public:
   Context(const Model& model, const EngineOptions& opt)
       : model_(model), opt_(opt), t0_(std::chrono::steady_clock::now()) {
-    sp_ = make_scaled_problem(model, opt.scaling);
-    const double t_scale = elapsed();
-    if (sp_.A.nnz() >= (1LL << 31)) {
+    if (static_cast<long long>(model.value.size()) >= (1LL << 31)) {
       error_ = "matrix has 2^31 or more nonzeros (beyond the 32-bit device index range)";
       return;
     }
     dev_ = Device::open(opt.device, error_);
     if (!dev_) return;
     const double t_open = elapsed();
-    dev_->load(sp_);
+    scale_and_load(*dev_, model, opt.scaling, opt.device_scaling != 0, sp_);
     if (opt.timing) dev_->check(psh_timing(dev_->ctx(), 1), "timing");
     dev_->check(psh_graphs(dev_->ctx(), opt.graphs), "graphs");
     dev_->check(psh_sliced(dev_->ctx(), opt.sliced), "sliced");
@@ -403,8 +421,8 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
     dev_->check(psh_sync(dev_->ctx()), "setup");
     setup_seconds_ = elapsed();
     if (opt.verbosity >= 1)
-      std::fprintf(stderr, "setup: scaling %.3fs, device open %.3fs, load %.3fs, norm %.3fs\n", t_scale,
-                   t_open - t_scale, t_load - t_open, setup_seconds_ - t_load);
+      std::fprintf(stderr, "setup: device open %.3fs, scaling and load (%s) %.3fs, norm %.3fs\n", t_open,
+                   opt.device_scaling ? "sweeps on the device" : "host", t_load - t_open, setup_seconds_ - t_load);
   }
   bool ok() const { return dev_ != nullptr; }
   const std::string& error() const { return error_; }

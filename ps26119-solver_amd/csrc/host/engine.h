@@ -40,6 +40,7 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   int timing = 0;  // hip_timing: per-kernel HIP event timing into the Solution
   int graphs = 1;  // hip_graphs: replay captured chunk graphs (off while timing)
   int sliced = 1;  // hip_sliced: column-sliced dual half step for large fp64 tree-mode solves
+  int device_scaling = 1;  // hip_device_scaling: the scaling sweeps on the device (same bits as the host's)
   // hip_refine: mixed precision as fp32 increments on fp64 anchors (default) instead of the
   // reference's fp32-then-fp64 switch; CPU-order mode (hip_cpu_order = 1) defaults it to 0.
   int refine = -1;
@@ -95,6 +96,12 @@ double estimate_norm2_host(const ScaledProblem& sp, int max_iter = 400, double r
 
 // The psh_problem view of a scaled problem; `cmin` receives sense·c and must outlive the view.
 psh_problem problem_of(const ScaledProblem& sp, std::vector<double>& cmin);
+
+// Builds the scaled problem of `model` and loads it on `dev`: the structure on the host and the
+// scaling sweeps on the device (psh_scale) when `on_device`, else everything on the host. On
+// return sp holds the factors and the scaled vectors; with on_device its matrix values are not
+// kept on the host (only the structure and a_orig).
+void scale_and_load(Device& dev, const Model& model, const ScalingOptions& so, bool on_device, ScaledProblem& sp);
 
 // Restart test of r²HPDHG (reference src/pdhg/r2hpdhg.cpp:118-122): sufficient, necessary or
 // artificial restart. r0 < 0 means no epoch-start residual yet.

@@ -28,9 +28,8 @@ void transpose_structure(const Csr& a, Csr& t, std::vector<int>& perm) {
     }
 }
 
-}  // namespace
-
-ScaledProblem make_scaled_problem(const Model& model, const ScalingOptions& opt) {
+// Structure, original values and the scaling decisions (no sweeps yet).
+ScaledProblem scaled_structure(const Model& model, const ScalingOptions& opt, std::vector<int>& perm) {
   ScaledProblem sp;
   const int m = model.num_rows, n = model.num_cols;
   sp.m = m;
@@ -39,14 +38,9 @@ ScaledProblem make_scaled_problem(const Model& model, const ScalingOptions& opt)
   sp.original = &model;
   sp.A = transpose(csc_as_transpose(model));  // CSR of A, columns sorted
   sp.a_orig = sp.A.val;
-  std::vector<int> perm;
   transpose_structure(sp.A, sp.At, perm);
   sp.row_scale.assign(m, 1.0);
   sp.col_scale.assign(n, 1.0);
-  const Csr& A = sp.A;
-  const Csr& At = sp.At;
-  std::vector<double>& val = sp.A.val;
-
   {
     double amin = kInf, amax = 0.0;
     for (double v : sp.a_orig)
@@ -55,6 +49,34 @@ ScaledProblem make_scaled_problem(const Model& model, const ScalingOptions& opt)
   }
   sp.geometric_mean_applied =
       opt.geometric_mean_iterations > 0 && sp.log10_range >= opt.geometric_mean_min_log10_range;
+  return sp;
+}
+
+void at_values(ScaledProblem& sp, const std::vector<int>& perm) {
+  sp.At.val.resize(perm.size());
+  parallel_for(static_cast<std::int64_t>(perm.size()), [&](std::int64_t b, std::int64_t e) {
+    for (std::int64_t q = b; q < e; ++q) sp.At.val[q] = sp.A.val[perm[q]];
+  });
+}
+
+}  // namespace
+
+ScaledProblem make_unscaled_problem(const Model& model, const ScalingOptions& opt) {
+  std::vector<int> perm;
+  ScaledProblem sp = scaled_structure(model, opt, perm);
+  at_values(sp, perm);
+  finish_scaled_problem(sp, model, ScalingOptions{opt.geometric_mean_iterations, opt.geometric_mean_min_log10_range,
+                                                  opt.ruiz_iterations, opt.pock_chambolle, false});
+  return sp;
+}
+
+ScaledProblem make_scaled_problem(const Model& model, const ScalingOptions& opt) {
+  std::vector<int> perm;
+  ScaledProblem sp = scaled_structure(model, opt, perm);
+  const int m = sp.m, n = sp.n;
+  const Csr& A = sp.A;
+  const Csr& At = sp.At;
+  std::vector<double>& val = sp.A.val;
 
   std::vector<double> rf(m), cf(n);
   if (sp.geometric_mean_applied) {
@@ -124,6 +146,13 @@ ScaledProblem make_scaled_problem(const Model& model, const ScalingOptions& opt)
   for (int it = 0; it < opt.ruiz_iterations; ++it) sweep(true);
   if (opt.pock_chambolle) sweep(false);
 
+  finish_scaled_problem(sp, model, opt);
+  at_values(sp, perm);
+  return sp;
+}
+
+void finish_scaled_problem(ScaledProblem& sp, const Model& model, const ScalingOptions& opt) {
+  const int m = sp.m, n = sp.n;
   sp.c.resize(n);
   sp.col_lower.resize(n);
   sp.col_upper.resize(n);
@@ -155,11 +184,6 @@ ScaledProblem make_scaled_problem(const Model& model, const ScalingOptions& opt)
     for (double& v : sp.col_upper) v *= sp.bound_scale;
     for (double& v : sp.c) v *= sp.obj_scale;
   }
-  sp.At.val.resize(perm.size());
-  parallel_for(static_cast<std::int64_t>(perm.size()), [&](std::int64_t b, std::int64_t e) {
-    for (std::int64_t q = b; q < e; ++q) sp.At.val[q] = val[perm[q]];
-  });
-  return sp;
 }
 
 void ScaledProblem::unscale_primal(const std::vector<double>& xs, std::vector<double>& x) const {

@@ -42,4 +42,13 @@ struct ScaledProblem {
 
 ScaledProblem make_scaled_problem(const Model& model, const ScalingOptions& opt);
 
+// The same problem with unit factors: Ã = A, Ãᵀ's values in Ãᵀ's order, log10_range and
+// geometric_mean_applied decided, vectors unscaled. The device path loads it, runs the sweeps on
+// the device (psh_scale) and then calls finish_scaled_problem with the factors it returns.
+ScaledProblem make_unscaled_problem(const Model& model, const ScalingOptions& opt);
+
+// c̃, l̃, ũ, r̃l, r̃u and the bound / objective scalars from sp.row_scale and sp.col_scale (the tail
+// of make_scaled_problem, shared so that both paths do the same arithmetic).
+void finish_scaled_problem(ScaledProblem& sp, const Model& model, const ScalingOptions& opt);
+
 }  // namespace ps26119::amd

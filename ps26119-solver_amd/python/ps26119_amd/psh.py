@@ -103,6 +103,8 @@ def lib():
             "psh_graphs": ([P, ctypes.c_int], ctypes.c_int),
             "psh_sliced": ([P, ctypes.c_int], ctypes.c_int),
             "psh_delta": ([P, ctypes.c_int], ctypes.c_int),
+            "psh_scale": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D], ctypes.c_int),
+            "psh_set_vectors": ([P, D, D, D, D, D, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -226,8 +228,28 @@ class Context:
     def sliced(self, on):
         self._ok(lib().psh_sliced(self._h, int(on)))
 
+    def scale(self, geo_iterations=12, ruiz_iterations=10, pock_chambolle=1):
+        """Scaling sweeps on the device (psh_scale) of a problem loaded with unscaled values
+        (``unscaled_problem``); returns the row and column factors."""
+        rs, cs = np.zeros(self.m), np.zeros(self.n)
+        self._ok(lib().psh_scale(self._h, geo_iterations, ruiz_iterations, pock_chambolle, _p(rs), _p(cs)))
+        return rs, cs
+
     def delta(self, on):
         self._ok(lib().psh_delta(self._h, int(on)))
+
+
+def unscaled_problem(lp) -> dict:
+    """The psh problem dict of `lp` with unit factors: the original values in Ã's and Ãᵀ's order
+    (what psh_scale starts from) and unscaled vectors."""
+    row_ptr, col, val0 = lp.csr()
+    m, n = lp.m, lp.n
+    order = np.lexsort((np.repeat(np.arange(m), np.diff(row_ptr)), col))
+    val0 = np.asarray(val0, float)
+    return scaled_problem({"a_vals": val0, "at_vals": val0[order], "c": lp.sense * np.asarray(lp.c, float),
+                           "col_lower": lp.col_lower, "col_upper": lp.col_upper, "row_lower": lp.row_lower,
+                           "row_upper": lp.row_upper, "row_scale": np.ones(m), "col_scale": np.ones(n),
+                           "bound_scale": 1.0, "obj_scale": 1.0}, lp)
 
 
 def scaled_problem(sc: dict, lp) -> dict:

@@ -197,3 +197,28 @@ def test_kkt_cpu_order_matches_oracle_on_original_point(gpu_available, oracle):
                           "primal_max_rel", "dual_max_rel")]
     assert [float(v).hex() for v in got] == [float(v).hex() for v in exp]
     ctx.close()
+
+
+@pytest.mark.parametrize("name", MODELS + ["refinery"])
+def test_device_scaling_is_bit_identical(gpu_available, oracle, name):
+    """psh_scale (geometric mean when the entries span ≥ 10^4.5, Ruiz ×10, Pock–Chambolle) on the
+    device gives the oracle's row and column factors bit for bit, and its scaled Ã, Ãᵀ give the
+    same Ã·x and Ãᵀ·y (CSR-order sums) as a context loaded with the oracle's scaled values."""
+    from ps26119_amd import lpgen, psh
+    lp = lpgen.refinery_lp(T=6, seed=3) if name == "refinery" else model(name)
+    sc = oracle.scale(lp)
+    dev = psh.Context(psh.unscaled_problem(lp))
+    rs, cs = dev.scale(12 if sc["geometric_mean_applied"] else 0, 10, 1)
+    print(f"{name}: log10 range {sc['log10_range']:.2f}, geometric mean {sc['geometric_mean_applied']}", flush=True)
+    assert np.array_equal(rs, sc["row_scale"]) and np.array_equal(cs, sc["col_scale"])
+    ref = psh.Context(psh.scaled_problem(sc, lp))
+    x, y = np.sin(1.0 + np.arange(lp.n)), np.cos(2.0 + np.arange(lp.m))
+    for ctx in (dev, ref):
+        ctx.upload(psh.X, x)
+        ctx.upload(psh.Y, y)
+        ctx.spmv(psh.X, psh.AX)
+        ctx.spmv_t(psh.Y, psh.ATY)
+    assert np.array_equal(dev.download(psh.AX), ref.download(psh.AX))
+    assert np.array_equal(dev.download(psh.ATY), ref.download(psh.ATY))
+    dev.close()
+    ref.close()
