@@ -6,7 +6,7 @@
 // from Options::warm_x / warm_y / warm_primal_weight (e.g. the base model's solution), and
 // precision = mixed: every slot in increment form (fp32 matrix and increments around fp64
 // anchors rebuilt at its restarts); the whole batch moves to fp64 with a restart of every slot
-// when some scenario below relative KKT 1e-5 stops improving for hip_refine_patience checks.
+// when some scenario's increment form stalls (RefineGuard).
 #include "batch.h"
 
 #include <algorithm>
@@ -185,8 +185,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
       dev->check(psh_batch_set_iterate(c, X.data(), Y.data()), "batch");
       bool incr = options.precision == Precision::Mixed && eo.refine != 0;
       if (incr) dev->check(psh_batch_delta(c, 1), "batch precision");
-      std::vector<double> best(K, kInf);
-      std::vector<int> stale(K, 0);
+      std::vector<RefineGuard> guard(K);
       std::vector<std::int64_t> promoted_at(K, -1);
 
       const double w0 = options.warm_primal_weight > 0 ? options.warm_primal_weight : 1.0;
@@ -274,12 +273,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
             continue;
           }
           if (fast_it[k] < 0 && kk.rel_kkt() <= tol::kFirstOrderFast) fast_it[k] = it, fast_s[k] = elapsed();
-          if (incr) {
-            const double rel = kk.rel_kkt();
-            if (rel < 0.9 * best[k]) best[k] = rel, stale[k] = 0;
-            else ++stale[k];
-            promote = promote || (rel <= 1e-5 && stale[k] >= eo.refine_patience);
-          }
+          if (incr && guard[k].stalled(kk.rel_kkt(), it, eo)) promote = true;
           if (kk.converged(eo.tolerance)) {
             finish(Status::Optimal, "");
             continue;

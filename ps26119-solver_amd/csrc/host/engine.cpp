@@ -602,32 +602,34 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
 // fp32 has done what it can, then fp64. With refine = 1 (the default outside CPU-order mode)
 // the increment form: fp32 matrix and increments around fp64 anchors that every restart
 // rebuilds, to the full tolerance; fp64 takes over only if progress stops for `refine_patience`
-// checks once the relative KKT error is below 1e-5. The bf16 variant (bf16_until > 0) starts the
+// checks once the relative KKT error is below 1e-5, or at any level for a quarter of the
+// iterations so far (RefineGuard). The bf16 variant (bf16_until > 0) starts the
 // increment form with bf16 matrix values and moves to fp32 values at bf16_until or on a stall of
 // 10 checks.
 class PrecisionPolicy {
  public:
   PrecisionPolicy(Context& ctx, const EngineOptions& o)
-      : ctx_(ctx), mixed_(o.precision == Precision::Mixed), refine_(mixed_ && o.refine != 0), target_(o.tolerance),
-        bf16_until_(refine_ ? o.bf16_until : 0.0), patience_(o.refine_patience) {
+      : ctx_(ctx), opt_(o), mixed_(o.precision == Precision::Mixed), refine_(mixed_ && o.refine != 0),
+        target_(o.tolerance), bf16_until_(refine_ ? o.bf16_until : 0.0) {
     if (refine_) ctx_.dev().check(psh_delta(ctx_.ctx(), bf16_until_ > 0 ? 2 : 1), "precision");
     else ctx_.dev().check(psh_set_fp32(ctx_.ctx(), mixed_ ? 1 : 0), "precision");
   }
   bool fp32() { return psh_is_fp32(ctx_.ctx()) != 0; }
   bool refining() const { return refine_ && !left_; }
-  bool on_check(double rel) {
+  bool on_check(double rel, std::int64_t it) {
     if (refine_) {
       if (left_) return false;
-      if (rel < 0.9 * best_) best_ = rel, stale_ = 0;
-      else ++stale_;
-      if (bf16_until_ > 0 && (rel <= bf16_until_ || stale_ >= 10)) {
-        ctx_.dev().check(psh_delta(ctx_.ctx(), 1), "precision");
-        bf16_until_ = 0;
-        stale_ = 0;
-        best_ = rel;
+      if (bf16_until_ > 0) {
+        if (rel < 0.9 * best_) best_ = rel, stale_ = 0;
+        else ++stale_;
+        if (rel <= bf16_until_ || stale_ >= 10) {
+          ctx_.dev().check(psh_delta(ctx_.ctx(), 1), "precision");
+          bf16_until_ = 0;
+          guard_ = RefineGuard{};
+        }
         return false;
       }
-      if (rel <= 1e-5 && stale_ >= patience_) {
+      if (guard_.stalled(rel, it, opt_)) {
         ctx_.dev().check(psh_delta(ctx_.ctx(), 0), "precision");
         left_ = true;
         return true;
@@ -647,9 +649,11 @@ class PrecisionPolicy {
 
  private:
   Context& ctx_;
+  const EngineOptions& opt_;
   bool mixed_, refine_, left_ = false;
   double target_, bf16_until_, best_ = 1e300;
-  int stale_ = 0, patience_ = 40;
+  int stale_ = 0;
+  RefineGuard guard_;
 };
 
 }  // namespace
@@ -741,7 +745,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
       }
       if (verdict != Status::NotSolved) return ctx.finish(verdict, PSH_XH, PSH_YH, it, why);
     }
-    const bool promoted = policy.on_check(k.rel_kkt());
+    const bool promoted = policy.on_check(k.rel_kkt(), it);
     if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
     const bool restart =
         promoted || restart_due(opt, r, r0, r_last, inner, it) || increment_epoch_full(opt, policy.refining(), inner);
@@ -858,7 +862,7 @@ Solution solve_pdlp(const Model& model, const EngineOptions& opt_in) {
       if (st != Status::NotSolved) return ctx.finish(st, x, y, it, why);
     }
     const double e = kk.rel_kkt();
-    const bool promoted = policy.on_check(e);
+    const bool promoted = policy.on_check(e, it);
     if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
     const bool restart = promoted || e <= 0.2 * restart_kkt ||
                          (e <= opt.pdlp_restart_necessary * restart_kkt && e > last_candidate) ||

@@ -3,6 +3,7 @@
 // and finish, with the iteration and the per-check reductions on the GPU.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -106,6 +107,28 @@ void scale_and_load(Device& dev, const Model& model, const ScalingOptions& so, b
 // Restart test of r²HPDHG (reference src/pdhg/r2hpdhg.cpp:118-122): sufficient, necessary or
 // artificial restart. r0 < 0 means no epoch-start residual yet.
 bool restart_due(const EngineOptions& o, double r, double r0, double r_last, std::int64_t inner, std::int64_t it);
+// When the increment form hands over to fp64: (a) the relative KKT error is below 1e-5 and has not
+// improved by 10% for refine_patience checks (fp32's floor), or (b) at any level, it has not
+// improved by 10% over the last quarter of the iterations (at least 16·patience checks): on a
+// badly conditioned LP the fp32 increments stop refining well above the tolerance.
+struct RefineGuard {
+  double best = 1e300;
+  std::int64_t it_best = 0;  // iteration of the last 10% improvement
+  int stale = 0;             // checks since then
+  bool stalled(double rel, std::int64_t it, const EngineOptions& o) {
+    if (rel < 0.9 * best) {
+      best = rel;
+      it_best = it;
+      stale = 0;
+      return false;
+    }
+    ++stale;
+    const std::int64_t window =
+        std::max<std::int64_t>(16LL * o.refine_patience * o.check_every, it / 4);
+    return (rel <= 1e-5 && stale >= o.refine_patience) || it - it_best >= window;
+  }
+};
+
 // The epoch limit of the increment form (EngineOptions::refine_max_epoch).
 inline bool increment_epoch_full(const EngineOptions& o, bool increment_form, std::int64_t inner) {
   return increment_form && o.refine_max_epoch > 0 && inner >= o.refine_max_epoch;

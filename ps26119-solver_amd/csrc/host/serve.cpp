@@ -29,8 +29,7 @@ struct Server::Request {
   double eta = 1.0;
   std::size_t bytes = 0;
   bool mixed = false;  // increment form (precision = mixed, hip_refine = 1)
-  double best = kInf;  // stall guard of the increment form
-  int stale = 0;
+  RefineGuard guard;  // stall guard of the increment form
   // live on the device
   int slot = -1;
   PidState pid;
@@ -413,9 +412,7 @@ void Server::serve_loop() {
       bool promoted = false;  // increment form stalled: fp64 from the next chunk, after a restart
       if (r->mixed) {
         const double rel = k.rel_kkt();
-        if (rel < 0.9 * r->best) r->best = rel, r->stale = 0;
-        else ++r->stale;
-        if (rel <= 1e-5 && r->stale >= o.refine_patience) {
+        if (r->guard.stalled(rel, r->it, o)) {
           promoted = true;
           r->mixed = false;
           r->message += "; fp32 -> fp64 at iteration " + std::to_string(r->it);
