@@ -567,6 +567,62 @@ int spmv_impl(psh_ctx* c, const DevMat& M, int xs, int out) {
 
 }  // namespace
 
+namespace {
+
+// A CSR's entries regrouped by (row tile, column slice) for the sliced half steps with fp64 tree
+// reductions (built when rows >= kSliceMinRows and every tile fits one LDS pass): slice k holds the
+// columns [k·w, (k+1)·w), w = ⌈cols / S⌉; tiles are padded to a multiple of 8/S so that the
+// workgroups of one slice land on the same XCDs. Values are gathered from M.val on the device.
+int build_sliced(psh_ctx* c, Sliced& S, const DevMat& M, const std::vector<int>& tiles, const int* ptr, const int* col,
+                 int rows, int cols) {
+  const long long nz = ptr[rows];
+  bool short_tiles = true;  // a slice segment must fit the LDS product buffer
+  for (int t = 0; t < M.ntiles && short_tiles; ++t) short_tiles = ptr[tiles[t + 1]] - ptr[tiles[t]] <= kTileNnz;
+  if (rows < kSliceMinRows || nz == 0 || M.ntiles == 0 || !short_tiles) return PSH_OK;
+  const int per = kXcd / kSlices;
+  S.slices = kSlices;
+  S.ntp = (M.ntiles + per - 1) / per * per;
+  const long long width = (static_cast<long long>(cols) + kSlices - 1) / kSlices;
+  std::vector<int> seg(static_cast<size_t>(S.ntp) * kSlices + 1, 0), scol(nz), ssrc(nz);
+  std::vector<unsigned char> srow(nz);
+  size_t q = 0;
+  for (int t = 0; t < S.ntp; ++t) {
+    const int r0 = t < M.ntiles ? tiles[t] : rows, r1 = t < M.ntiles ? tiles[t + 1] : rows;
+    for (int k = 0; k < kSlices; ++k) {
+      for (int r = r0; r < r1; ++r) {
+        int b = ptr[r];
+        const int e = ptr[r + 1];
+        while (b < e && col[b] / width < k) ++b;
+        for (; b < e && col[b] / width == k; ++b, ++q) {
+          scol[q] = col[b];
+          ssrc[q] = b;
+          srow[q] = static_cast<unsigned char>(r - r0);
+        }
+      }
+      seg[static_cast<size_t>(t) * kSlices + k + 1] = static_cast<int>(q);
+    }
+  }
+  if (static_cast<long long>(q) != nz) return fail(c, PSH_E_ARG, "column slices do not cover the matrix");
+  int rc;
+  auto put = [&](auto** dst, const auto* src, long long count) -> int {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+    int r = alloc_n(c, dst, count);
+    if (r) return r;
+    PSH_TRY(c, hipMemcpyAsync(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+    return PSH_OK;
+  };
+  if ((rc = put(&S.seg, seg.data(), static_cast<long long>(seg.size()))) || (rc = put(&S.col, scol.data(), nz)) ||
+      (rc = put(&S.src, ssrc.data(), nz)) || (rc = alloc_n(c, &S.val, nz)) || (rc = put(&S.row, srow.data(), nz)) ||
+      (rc = alloc_n(c, &S.part, static_cast<long long>(kSlices) * rows)))
+    return rc;
+  hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(nz)), dim3(kBlock), 0, c->stream, M.val, S.src, S.val, nz);
+  PSH_LAUNCHED(c, "sliced values");
+  PSH_TRY(c, hipStreamSynchronize(c->stream));  // the staging vectors go out of scope
+  return PSH_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int psh_device_count(void) {
@@ -631,7 +687,8 @@ void psh_destroy(psh_ctx* c) {
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
                   c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->As.seg, c->As.col,
-                  c->As.src, c->As.val, c->As.row, c->As.part};
+                  c->As.src, c->As.val, c->As.row, c->As.part, c->Ats.seg, c->Ats.col, c->Ats.src,
+                  c->Ats.val, c->Ats.row, c->Ats.part};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
@@ -694,42 +751,9 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
   if ((rc = up(&c->A.tiles, ta.data(), static_cast<long long>(ta.size()))) ||
       (rc = up(&c->At.tiles, tt.data(), static_cast<long long>(tt.size()))))
     return rc;
-  bool short_tiles = true;  // a slice segment must fit the LDS product buffer
-  for (int t = 0; t < c->A.ntiles && short_tiles; ++t) short_tiles = p->a_ptr[ta[t + 1]] - p->a_ptr[ta[t]] <= kTileNnz;
-  if (m >= kSliceMinRows && nz > 0 && c->A.ntiles > 0 && short_tiles) {  // Ã by (tile, column slice)
-    Sliced& S = c->As;
-    const int per = kXcd / kSlices;
-    S.slices = kSlices;
-    S.ntp = (c->A.ntiles + per - 1) / per * per;
-    const long long width = (static_cast<long long>(n) + kSlices - 1) / kSlices;
-    std::vector<int> seg(static_cast<size_t>(S.ntp) * kSlices + 1, 0), scol(nz), ssrc(nz);
-    std::vector<unsigned char> srow(nz);
-    size_t q = 0;
-    std::vector<int> cut(kSlices + 1);
-    for (int t = 0; t < S.ntp; ++t) {
-      const int r0 = t < c->A.ntiles ? ta[t] : m, r1 = t < c->A.ntiles ? ta[t + 1] : m;
-      for (int k = 0; k < kSlices; ++k) {
-        for (int r = r0; r < r1; ++r) {
-          int b = p->a_ptr[r];
-          const int e = p->a_ptr[r + 1];
-          while (b < e && p->a_col[b] / width < k) ++b;
-          for (; b < e && p->a_col[b] / width == k; ++b, ++q) {
-            scol[q] = p->a_col[b];
-            ssrc[q] = b;
-            srow[q] = static_cast<unsigned char>(r - r0);
-          }
-        }
-        seg[static_cast<size_t>(t) * kSlices + k + 1] = static_cast<int>(q);
-      }
-    }
-    if (static_cast<long long>(q) != nz || (rc = up(&S.seg, seg.data(), static_cast<long long>(seg.size()))) ||
-        (rc = up(&S.col, scol.data(), nz)) || (rc = up(&S.src, ssrc.data(), nz)) || (rc = alloc_n(c, &S.val, nz)) ||
-        (rc = up(&S.row, srow.data(), nz)) || (rc = alloc_n(c, &S.part, static_cast<long long>(kSlices) * m)))
-      return rc ? rc : fail(c, PSH_E_ARG, "column slices do not cover the matrix");
-    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(nz)), dim3(kBlock), 0, c->stream, c->A.val, S.src, S.val, nz);
-    PSH_LAUNCHED(c, "sliced values");
-    PSH_TRY(c, hipStreamSynchronize(c->stream));  // the staging vectors go out of scope
-  }
+  if ((rc = build_sliced(c, c->As, c->A, ta, p->a_ptr, p->a_col, m, n)) ||
+      (rc = build_sliced(c, c->Ats, c->At, tt, p->at_ptr, p->at_col, n, m)))
+    return rc;
   for (int s = 0; s < PSH_NSLOT; ++s)
     if ((rc = alloc_n(c, &c->vd[s], slot_len(c, s)))) return rc;
   const long long mx = std::max<long long>(std::max(m, n), 1);
@@ -803,9 +827,12 @@ int psh_scale(psh_ctx* c, int geo_iterations, int ruiz_iterations, int pock_cham
   };
   for (int it = 0; it < ruiz_iterations; ++it) sweep(1);
   if (pock_chambolle) sweep(0);
-  if (c->As.slices > 0 && c->nnz > 0)  // the sliced copy takes the scaled values
+  if (c->As.slices > 0 && c->nnz > 0)  // the sliced copies take the scaled values
     hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->A.val, c->As.src,
                        c->As.val, c->nnz);
+  if (c->Ats.slices > 0 && c->nnz > 0)
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->At.val, c->Ats.src,
+                       c->Ats.val, c->nnz);
   PSH_LAUNCHED(c, "scaling sweeps");
   if (row_scale && m > 0) PSH_TRY(c, hipMemcpyAsync(row_scale, c->rs, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
   if (col_scale && n > 0) PSH_TRY(c, hipMemcpyAsync(col_scale, c->cs, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));

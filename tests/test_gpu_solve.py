@@ -245,3 +245,18 @@ def test_pdlp_tree_reductions_reach_the_answer(gpu_available, golden, name):
     ref_obj = float.fromhex(exp["objective"])
     assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
     assert abs(r.objective - ref_obj) <= 1e-7 * (1 + abs(ref_obj))
+
+
+def test_stalled_increment_form_hands_over_to_fp64(gpu_available):
+    """On the refinery planning LP (entries spanning 10^6) the fp32 increments stop refining above
+    the tolerance; the stall guard hands the solve to fp64, which ends Optimal at the fp64 answer."""
+    import ps26119
+    from ps26119_amd import lpgen
+    lp = lpgen.refinery_lp(T=24, seed=3)
+    ref = ps26119.solve_model(lp, iteration_limit=5_000_000)
+    r = ps26119.solve_model(lp, precision="mixed", iteration_limit=5_000_000)
+    print(f"refinery T=24: fp64 it={ref.iterations} {ref.seconds:.2f}s; mixed it={r.iterations} {r.seconds:.2f}s "
+          f"{r.message}", flush=True)
+    assert ref.status == r.status == ps26119.OPTIMAL and r.check == "PASS"
+    assert "fp32 -> fp64" in r.message
+    assert abs(r.objective - ref.objective) <= 1e-7 * (1 + abs(ref.objective))
