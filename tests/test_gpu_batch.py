@@ -206,3 +206,19 @@ def test_infeasible_scenarios_leave_the_batch_with_certificates(gpu_available, o
                 assert "dual ray certificate" in r.message
             else:
                 assert abs(r.objective - e["objective"]) <= 1e-7 * (1 + abs(e["objective"])), (prec, k)
+
+
+def test_mixed_batch_slot_pairs_equal_single_slots(gpu_available):
+    """An even batch width runs the increment-form kernels two slots per thread (8-byte loads);
+    an odd width runs them one slot per thread. With CPU-order reductions a scenario's trajectory
+    does not depend on the width, so each scenario must come out with the same iterations and
+    bits from a batch of 3 (single slots) and a batch of 4 (slot pairs)."""
+    import ps26119
+    base, scen = _scenarios("stocfor1")
+    prm = {"hip_cpu_order": 1, "hip_refine": 1}
+    odd = ps26119.solve_model_batch(base, scen[:3], precision="mixed", params=prm, iteration_limit=3_000_000)
+    even = ps26119.solve_model_batch(base, scen[:4], precision="mixed", params=prm, iteration_limit=3_000_000)
+    for k in range(3):
+        assert odd[k].status == even[k].status == ps26119.OPTIMAL, (k, odd[k].message, even[k].message)
+        assert odd[k].iterations == even[k].iterations, k
+        assert fnv(odd[k].x) == fnv(even[k].x) and fnv(odd[k].y) == fnv(even[k].y), k
