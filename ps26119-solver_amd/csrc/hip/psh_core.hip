@@ -569,7 +569,7 @@ int spmv_impl(psh_ctx* c, const DevMat& M, int xs, int out) {
 
 namespace {
 
-// A CSR's entries regrouped by (row tile, column slice) for the sliced half steps with fp64 tree
+// A CSR's entries regrouped by (row tile, column slice) for the sliced half step with fp64 tree
 // reductions (built when rows >= kSliceMinRows and every tile fits one LDS pass): slice k holds the
 // columns [k·w, (k+1)·w), w = ⌈cols / S⌉; tiles are padded to a multiple of 8/S so that the
 // workgroups of one slice land on the same XCDs. Values are gathered from M.val on the device.
@@ -687,8 +687,7 @@ void psh_destroy(psh_ctx* c) {
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
                   c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->As.seg, c->As.col,
-                  c->As.src, c->As.val, c->As.row, c->As.part, c->Ats.seg, c->Ats.col, c->Ats.src,
-                  c->Ats.val, c->Ats.row, c->Ats.part};
+                  c->As.src, c->As.val, c->As.row, c->As.part};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
@@ -751,9 +750,7 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
   if ((rc = up(&c->A.tiles, ta.data(), static_cast<long long>(ta.size()))) ||
       (rc = up(&c->At.tiles, tt.data(), static_cast<long long>(tt.size()))))
     return rc;
-  if ((rc = build_sliced(c, c->As, c->A, ta, p->a_ptr, p->a_col, m, n)) ||
-      (rc = build_sliced(c, c->Ats, c->At, tt, p->at_ptr, p->at_col, n, m)))
-    return rc;
+  if ((rc = build_sliced(c, c->As, c->A, ta, p->a_ptr, p->a_col, m, n))) return rc;
   for (int s = 0; s < PSH_NSLOT; ++s)
     if ((rc = alloc_n(c, &c->vd[s], slot_len(c, s)))) return rc;
   const long long mx = std::max<long long>(std::max(m, n), 1);
@@ -830,9 +827,6 @@ int psh_scale(psh_ctx* c, int geo_iterations, int ruiz_iterations, int pock_cham
   if (c->As.slices > 0 && c->nnz > 0)  // the sliced copies take the scaled values
     hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->A.val, c->As.src,
                        c->As.val, c->nnz);
-  if (c->Ats.slices > 0 && c->nnz > 0)
-    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->At.val, c->Ats.src,
-                       c->Ats.val, c->nnz);
   PSH_LAUNCHED(c, "scaling sweeps");
   if (row_scale && m > 0) PSH_TRY(c, hipMemcpyAsync(row_scale, c->rs, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
   if (col_scale && n > 0) PSH_TRY(c, hipMemcpyAsync(col_scale, c->cs, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));

@@ -273,16 +273,6 @@ __global__ void __launch_bounds__(kBlock) k_slice_partial(Sliced S, const int* _
   }
 }
 
-// Slice sums added in slice order, then the fused primal update (k_iter_primal's epilogue).
-__global__ void __launch_bounds__(kBlock) k_slice_primal(Sliced S, int rows, PrimalArgs<double> p) {
-  const int j = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
-  EpiPrimal<double> e(p);
-  if (j >= rows) return;
-  double s = 0.0;
-  for (int k = 0; k < S.slices; ++k) s += ld_stream(S.part + static_cast<size_t>(k) * rows + j);
-  e(j, true, s);
-}
-
 // Slice sums added in slice order, then the fused dual update (k_iter_dual's epilogue).
 __global__ void __launch_bounds__(kBlock) k_slice_dual(Sliced S, int rows, DualArgs<double> p) {
   const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
@@ -787,7 +777,6 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
   const bool delta = c->delta.on;
   // column-sliced dual half step: fp64, tree reductions (its row sums are not in CSR order)
   const bool use_slices = c->sliced && c->As.slices > 0 && !c->fp32 && !delta && g->mode == PSH_RED_TREE;
-  const bool use_pslices = c->sliced && c->Ats.slices > 0 && !c->fp32 && !delta && g->mode == PSH_RED_TREE;
   if (delta && c->nnz > 0 &&
       !(c->delta.bf16 ? c->A.valh && c->At.valh : c->A.valf && c->At.valf))  // the launches' value arrays
     return fail(c, PSH_E_STATE, "increment form without its matrix values");
@@ -855,18 +844,8 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
         hipEvent_t ev;
         if (c->At.ntiles > 0) {
           time_begin(c, 0, &ev);
-          bool sliced = false;
-          if constexpr (sizeof(T) == 8) {
-            if (use_pslices) {
-              hipLaunchKernelGGL(k_slice_partial, dim3(c->Ats.ntp / (kXcd / c->Ats.slices) * kXcd), dim3(kBlock), 0,
-                                 c->stream, c->Ats, c->At.tiles, c->At.ntiles, c->n, sp<T>(c, PSH_Y));
-              hipLaunchKernelGGL(k_slice_primal, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->Ats, c->n, pa);
-              sliced = true;
-            }
-          }
-          if (!sliced)
-            hipLaunchKernelGGL((k_iter_primal<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
-                               c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, at_val, sp<T>(c, PSH_Y), pa);
+          hipLaunchKernelGGL((k_iter_primal<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
+                             c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, at_val, sp<T>(c, PSH_Y), pa);
           time_end(c, 0, ev);
         }
         if (c->A.ntiles > 0) {
@@ -898,10 +877,9 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
   if (c->timing || !c->graphs) {
     if ((rc = delta ? enqueue_delta() : enqueue())) return rc;
   } else {
-    const long long key = (static_cast<long long>(g->iterations) << 10) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
+    const long long key = (static_cast<long long>(g->iterations) << 9) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
                           (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3) | (delta ? 1LL << 6 : 0) |
-                          (use_slices ? 1LL << 7 : 0) | (delta && c->delta.bf16 ? 1LL << 8 : 0) |
-                          (use_pslices ? 1LL << 9 : 0);
+                          (use_slices ? 1LL << 7 : 0) | (delta && c->delta.bf16 ? 1LL << 8 : 0);
     auto it = c->graph_cache.find(key);
     if (it == c->graph_cache.end()) {
       PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

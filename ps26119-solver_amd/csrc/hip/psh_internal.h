@@ -49,9 +49,9 @@ struct DevMat {
   int* tiles = nullptr;   // ntiles+1 tile start rows
 };
 
-// Ã's (Ãᵀ's) entries regrouped by (row tile, column slice) for the half steps with fp64 tree
-// reductions: workgroup b gathers only from column slice (b mod 8)·S/8 of x̄ (of y), so the slice
-// stays in its XCD's L2; each row's slice sums are combined in slice order by the fused epilogue.
+// Ã's entries regrouped by (row tile, column slice) for the dual half step with fp64 tree
+// reductions: workgroup b gathers only from column slice (b mod 8)·S/8 of x̄, so the slice stays in
+// its XCD's L2; each row's slice sums are combined in slice order by the fused epilogue kernel.
 struct Sliced {
   int slices = 0;                 // S (0: not built)
   int ntp = 0;                    // tiles padded to a multiple of 8/S
@@ -158,7 +158,6 @@ struct psh_ctx {
 
   psh::DevMat A, At;             // scaled Ã (m×n) and Ãᵀ (n×m)
   psh::Sliced As;                // Ã by (tile, column slice), built for large m
-  psh::Sliced Ats;               // Ãᵀ by (tile, column slice), built for large n
   bool sliced = true;            // use it for fp64 tree-mode chunks
   double* a_orig = nullptr;      // original A values, Ã's structure
   int* ato_col = nullptr;        // original Aᵀ (CSC) row indices (aliases At.col when equal)
