@@ -567,7 +567,10 @@ void free_batch(psh_ctx* c) {
                   b->G, b->LX, b->UX, b->iy, b->iyh, b->iyb, b->BL, b->BU, b->KY, b->anchor};
   for (void* p : ptrs) dev_free(p);
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  if (b->pinned) (void)hipHostFree(b->pinned);
+  if (b->pinned) {
+    std::lock_guard<std::recursive_mutex> lock(api_mutex());
+    (void)hipHostFree(b->pinned);
+  }
   delete b;
   c->batch = nullptr;
 }
@@ -742,7 +745,11 @@ int psh_batch_begin(psh_ctx* c, int W, const double* cc, const double* l, const 
     return rc;
   b.dirty.assign(W, 1);
   b.ky_sigma.assign(W, 0.0);
-  if (hipHostMalloc(reinterpret_cast<void**>(&b.pinned), kPinBytes) != hipSuccess) return fail(c, PSH_E_OOM, "pinned staging");
+  {
+    std::lock_guard<std::recursive_mutex> lock(api_mutex());
+    if (hipHostMalloc(reinterpret_cast<void**>(&b.pinned), kPinBytes) != hipSuccess)
+      return fail(c, PSH_E_OOM, "pinned staging");
+  }
   PSH_TRY(c, hipStreamSynchronize(c->stream));
   return PSH_OK;
 }
@@ -824,6 +831,7 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
                           (any_new ? 1 : 0) | (static_cast<long long>(mode & 7) << 1) | (delta ? 16 : 0);
     auto it = b.graphs.find(key);
     if (it == b.graphs.end()) {
+      std::lock_guard<std::recursive_mutex> capture_lock(api_mutex());  // no device-wide calls meanwhile
       PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       const int erc = enqueue();
       hipGraph_t graph = nullptr;

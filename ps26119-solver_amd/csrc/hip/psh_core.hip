@@ -21,7 +21,13 @@ int hip_fail(psh_ctx* c, hipError_t e, const char* where) {
   return fail(c, code, std::string(where) + ": " + hipGetErrorString(e));
 }
 
+std::recursive_mutex& api_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 int dev_alloc(psh_ctx* c, void** p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -33,7 +39,9 @@ int dev_alloc(psh_ctx* c, void** p, size_t bytes) {
 }
 
 void dev_free(void* p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
+  (void)hipFree(p);
 }
 
 // Greedy row tiles: ≤ kBlock rows and ≤ kTileNnz nonzeros, or one row that alone exceeds kTileNnz.
@@ -272,8 +280,10 @@ const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err) {
     *err = PSH_E_OOM;
     return nullptr;
   }
-  if (hipMemcpy(t.dev, t.host.data(), sizeof(RedJob) * t.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
-    *err = hip_fail(c, hipGetLastError(), "reduction table upload");
+  hipError_t e = hipMemcpyAsync(t.dev, t.host.data(), sizeof(RedJob) * t.host.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // t.host moves into the map below
+  if (e != hipSuccess) {
+    *err = hip_fail(c, e, "reduction table upload");
     return nullptr;
   }
   return &c->tables.emplace(key, std::move(t)).first->second;
@@ -651,12 +661,15 @@ int psh_create(int device, psh_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return PSH_E_NO_DEVICE;
   auto* c = new psh_ctx;
   c->device = device;
+  std::unique_lock<std::recursive_mutex> lock(api_mutex());
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return PSH_E_HIP;
   }
-  if (hipHostMalloc(reinterpret_cast<void**>(&c->host_results), sizeof(double) * 128) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&c->jobs_host), sizeof(RedJob) * kMaxJobs) != hipSuccess) {
+  const bool pinned_ok = hipHostMalloc(reinterpret_cast<void**>(&c->host_results), sizeof(double) * 128) == hipSuccess &&
+                         hipHostMalloc(reinterpret_cast<void**>(&c->jobs_host), sizeof(RedJob) * kMaxJobs) == hipSuccess;
+  lock.unlock();
+  if (!pinned_ok) {
     psh_destroy(c);
     return PSH_E_OOM;
   }
@@ -691,11 +704,17 @@ void psh_destroy(psh_ctx* c) {
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
-  if (c->host_results) (void)hipHostFree(c->host_results);
-  if (c->jobs_host) (void)hipHostFree(c->jobs_host);
+  {
+    std::lock_guard<std::recursive_mutex> lock(api_mutex());
+    if (c->host_results) (void)hipHostFree(c->host_results);
+    if (c->jobs_host) (void)hipHostFree(c->jobs_host);
+  }
   for (auto& p : c->pending) c->ev_pool.push_back(p.a), c->ev_pool.push_back(p.b);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream) {
+    std::lock_guard<std::recursive_mutex> lock(api_mutex());
+    (void)hipStreamDestroy(c->stream);
+  }
   delete c;
 }
 

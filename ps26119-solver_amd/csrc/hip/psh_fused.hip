@@ -882,6 +882,7 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
                           (use_slices ? 1LL << 7 : 0) | (delta && c->delta.bf16 ? 1LL << 8 : 0);
     auto it = c->graph_cache.find(key);
     if (it == c->graph_cache.end()) {
+      std::lock_guard<std::recursive_mutex> capture_lock(api_mutex());  // no device-wide calls meanwhile
       PSH_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       const int erc = delta ? enqueue_delta() : enqueue();
       hipGraph_t graph = nullptr;

@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -231,6 +232,12 @@ int hip_fail(psh_ctx* c, hipError_t e, const char* where);
     hipError_t e_ = hipGetLastError();                             \
     if (e_ != hipSuccess) return psh::hip_fail(ctx, e_, where);    \
   } while (0)
+
+// Several host threads may drive their own contexts at once. Graph capture (even thread-local)
+// must not overlap allocation and release calls of other threads, which synchronise the device:
+// captures and every hipMalloc / hipFree / hipHostMalloc / hipHostFree of this library take this
+// lock. Copies always go through a context's own stream, never the legacy stream.
+std::recursive_mutex& api_mutex();
 
 int dev_alloc(psh_ctx* c, void** p, size_t bytes);
 template <class T>

@@ -711,6 +711,7 @@ int rebuild_tables(psh_server* s) {
   auto grow = [&](Tile** dt, double** part, int* cap, size_t need) -> int {
     if (static_cast<int>(need) <= *cap) return PSH_OK;
     const int nc = std::max<int>(static_cast<int>(need), std::max(1024, 2 * *cap));
+    std::lock_guard<std::recursive_mutex> lock(psh::api_mutex());
     if (*dt) (void)hipFree(*dt);
     if (*part) (void)hipFree(*part);
     *dt = nullptr;
@@ -725,12 +726,15 @@ int rebuild_tables(psh_server* s) {
   if ((rc = grow(&s->d_ptiles, &s->d_part_p, &s->cap_tiles_p, tp.size())) ||
       (rc = grow(&s->d_dtiles, &s->d_part_d, &s->cap_tiles_d, td.size())))
     return rc;
-  if (!tp.empty()) SRV_TRY(s, hipMemcpy(s->d_ptiles, tp.data(), sizeof(Tile) * tp.size(), hipMemcpyHostToDevice));
-  if (!td.empty()) SRV_TRY(s, hipMemcpy(s->d_dtiles, td.data(), sizeof(Tile) * td.size(), hipMemcpyHostToDevice));
+  if (!tp.empty())
+    SRV_TRY(s, hipMemcpyAsync(s->d_ptiles, tp.data(), sizeof(Tile) * tp.size(), hipMemcpyHostToDevice, s->stream));
+  if (!td.empty())
+    SRV_TRY(s, hipMemcpyAsync(s->d_dtiles, td.data(), sizeof(Tile) * td.size(), hipMemcpyHostToDevice, s->stream));
   std::vector<Inst> desc(s->max_inst);
   for (int k = 0; k < s->max_inst; ++k)
     if (s->slots[k].used) desc[k] = s->slots[k].inst;
-  SRV_TRY(s, hipMemcpy(s->d_inst, desc.data(), sizeof(Inst) * s->max_inst, hipMemcpyHostToDevice));
+  SRV_TRY(s, hipMemcpyAsync(s->d_inst, desc.data(), sizeof(Inst) * s->max_inst, hipMemcpyHostToDevice, s->stream));
+  SRV_TRY(s, hipStreamSynchronize(s->stream));  // the host tables go out of scope
   s->ntp = static_cast<int>(tp.size());
   s->ntd = static_cast<int>(td.size());
   s->tables_dirty = false;
@@ -767,6 +771,7 @@ int psh_server_create(int device, size_t budget_bytes, int max_instances, psh_se
     return rc;
   };
   hipError_t e;
+  std::unique_lock<std::recursive_mutex> lock(psh::api_mutex());
   if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e);
   if ((e = hipMalloc(reinterpret_cast<void**>(&s->arena), s->budget)) != hipSuccess) return bad(e);
   if ((e = hipMalloc(reinterpret_cast<void**>(&s->d_inst), sizeof(Inst) * max_instances)) != hipSuccess) return bad(e);
@@ -778,7 +783,10 @@ int psh_server_create(int device, size_t budget_bytes, int max_instances, psh_se
   if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_stats), sizeof(double) * kQ * max_instances)) != hipSuccess)
     return bad(e);
   std::memset(s->h_param, 0, sizeof(Param) * max_instances);
-  if ((e = hipMemcpy(s->d_param, s->h_param, sizeof(Param) * max_instances, hipMemcpyHostToDevice)) != hipSuccess)
+  lock.unlock();
+  if ((e = hipMemcpyAsync(s->d_param, s->h_param, sizeof(Param) * max_instances, hipMemcpyHostToDevice, s->stream)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(s->stream)) != hipSuccess)
     return bad(e);
   s->free_blocks[0] = s->budget;
   *out = s;
@@ -790,6 +798,7 @@ void psh_server_destroy(psh_server* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   drop_graphs(s);
+  std::lock_guard<std::recursive_mutex> lock(psh::api_mutex());
   for (void* p : {static_cast<void*>(s->arena), static_cast<void*>(s->d_inst), static_cast<void*>(s->d_param),
                   static_cast<void*>(s->d_stats), static_cast<void*>(s->d_ptiles), static_cast<void*>(s->d_dtiles),
                   static_cast<void*>(s->d_part_p), static_cast<void*>(s->d_part_d)})
@@ -993,6 +1002,7 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
   const long long key = iterations;
   auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {
+    std::lock_guard<std::recursive_mutex> capture_lock(psh::api_mutex());  // no device-wide calls meanwhile
     SRV_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     const int erc = enqueue();
     hipGraph_t graph = nullptr;

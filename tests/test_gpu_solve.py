@@ -260,3 +260,27 @@ def test_stalled_increment_form_hands_over_to_fp64(gpu_available):
     assert ref.status == r.status == ps26119.OPTIMAL and r.check == "PASS"
     assert "fp32 -> fp64" in r.message
     assert abs(r.objective - ref.objective) <= 1e-7 * (1 + abs(ref.objective))
+
+
+def test_concurrent_solves_from_host_threads(gpu_available):
+    """solve() may be called from several host threads at once (reference README.md:75-76,
+    test_certificates.cpp:384-422): each call owns its device context and stream, so four
+    concurrent solves give the same bits as the same solves one after another."""
+    import threading
+    import ps26119
+    from ps26119_amd import lpgen
+    lps = [lpgen.random_lp(m, seed=s) for m, s in ((400, 1), (900, 2), (300, 3), (1200, 4))]
+    seq = [ps26119.solve_model(lp) for lp in lps]
+    out = [None] * len(lps)
+
+    def run(k):
+        out[k] = ps26119.solve_model(lps[k])
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(len(lps))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=300)
+    for k, (a, b) in enumerate(zip(seq, out)):
+        assert b is not None and a.status == b.status == ps26119.OPTIMAL, k
+        assert a.iterations == b.iterations and fnv(a.x) == fnv(b.x) and fnv(a.y) == fnv(b.y), k
