@@ -86,6 +86,72 @@ __global__ void __launch_bounds__(kBlock) k_bdual(BArgs a) {
   a.b.y[q] = w * (a.rho2 * h + a.rho1 * yq) + (1 - w) * a.b.y0[q];
 }
 
+// fp64 steps for an even batch width, two adjacent slots per thread with 16-byte loads (same
+// arithmetic per slot as k_bprimal / k_bdual, so the same bits).
+__global__ void __launch_bounds__(kBlock) k_bprimal2(BArgs a) {
+  const long long g = 2 * (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+  const long long j = g / a.W;
+  if (j >= a.n) return;
+  const int t = static_cast<int>(g - j * a.W);
+  double s0 = 0.0, s1 = 0.0;
+  for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) {
+    const double v = a.at_val[p];
+    const double2 y = *reinterpret_cast<const double2*>(a.b.y + static_cast<size_t>(a.at_col[p]) * a.W + t);
+    s0 += v * y.x;
+    s1 += v * y.y;
+  }
+  const size_t q = static_cast<size_t>(g);
+  const double2 xq = *reinterpret_cast<const double2*>(a.b.x + q);
+  const double2 c = *reinterpret_cast<const double2*>(a.b.c + q);
+  const double2 l = *reinterpret_cast<const double2*>(a.b.l + q);
+  const double2 u = *reinterpret_cast<const double2*>(a.b.u + q);
+  const double2 x0 = *reinterpret_cast<const double2*>(a.b.x0 + q);
+  const double w0 = halpern_w(a.b.inner0[t], a.it), w1 = halpern_w(a.b.inner0[t + 1], a.it);
+  double2 h, xb, xn;
+  h.x = clampd(xq.x - a.b.tau[t] * (c.x - s0), l.x, u.x);
+  h.y = clampd(xq.y - a.b.tau[t + 1] * (c.y - s1), l.y, u.y);
+  xb.x = 2 * h.x - xq.x;
+  xb.y = 2 * h.y - xq.y;
+  xn.x = w0 * (a.rho2 * h.x + a.rho1 * xq.x) + (1 - w0) * x0.x;
+  xn.y = w1 * (a.rho2 * h.y + a.rho1 * xq.y) + (1 - w1) * x0.y;
+  if (a.hat) *reinterpret_cast<double2*>(a.b.xh + q) = h;
+  *reinterpret_cast<double2*>(a.b.xb + q) = xb;
+  *reinterpret_cast<double2*>(a.b.x + q) = xn;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bdual2(BArgs a) {
+  const long long g = 2 * (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+  const long long i = g / a.W;
+  if (i >= a.m) return;
+  const int t = static_cast<int>(g - i * a.W);
+  double s0 = 0.0, s1 = 0.0;
+  for (int p = a.a_ptr[i]; p < a.a_ptr[i + 1]; ++p) {
+    const double v = a.a_val[p];
+    const double2 x = *reinterpret_cast<const double2*>(a.b.xb + static_cast<size_t>(a.a_col[p]) * a.W + t);
+    s0 += v * x.x;
+    s1 += v * x.y;
+  }
+  const size_t q = static_cast<size_t>(g);
+  const double2 yq = *reinterpret_cast<const double2*>(a.b.y + q);
+  const double2 rl = *reinterpret_cast<const double2*>(a.b.rl + q);
+  const double2 ru = *reinterpret_cast<const double2*>(a.b.ru + q);
+  const double2 y0 = *reinterpret_cast<const double2*>(a.b.y0 + q);
+  const double w0 = halpern_w(a.b.inner0[t], a.it), w1 = halpern_w(a.b.inner0[t + 1], a.it);
+  const double sg0 = a.b.sigma[t], sg1 = a.b.sigma[t + 1];
+  double2 h, yb, yn;
+  h.x = yq.x - sg0 * s0 + sg0 * clampd(s0 - yq.x / sg0, rl.x, ru.x);
+  h.y = yq.y - sg1 * s1 + sg1 * clampd(s1 - yq.y / sg1, rl.y, ru.y);
+  yb.x = 2 * h.x - yq.x;
+  yb.y = 2 * h.y - yq.y;
+  yn.x = w0 * (a.rho2 * h.x + a.rho1 * yq.x) + (1 - w0) * y0.x;
+  yn.y = w1 * (a.rho2 * h.y + a.rho1 * yq.y) + (1 - w1) * y0.y;
+  if (a.hat) {
+    *reinterpret_cast<double2*>(a.b.yh + q) = h;
+    *reinterpret_cast<double2*>(a.b.yb + q) = yb;
+  }
+  *reinterpret_cast<double2*>(a.b.y + q) = yn;
+}
+
 // ---- increment form (mixed precision): the single-solve form of psh_fused.hip per slot.
 // Anchor constants of slot t in fp64 from its anchor (x0, y0): code 2 rebuilds every constant
 // and clears δ, code 1 only KY = y0/σ (σ changed without a restart).
@@ -800,6 +866,9 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
           hipLaunchKernelGGL(k_bdprimal2, dim3(grid_for(c->n * static_cast<long long>(W / 2))), dim3(kBlock), 0,
                              c->stream, a);
         else if (delta) hipLaunchKernelGGL(k_bdprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
+        else if (W % 2 == 0)
+          hipLaunchKernelGGL(k_bprimal2, dim3(grid_for(c->n * static_cast<long long>(W / 2))), dim3(kBlock), 0,
+                             c->stream, a);
         else hipLaunchKernelGGL(k_bprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
         time_end(c, 0, ev);
       }
@@ -809,6 +878,9 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
           hipLaunchKernelGGL(k_bddual2, dim3(grid_for(c->m * static_cast<long long>(W / 2))), dim3(kBlock), 0,
                              c->stream, a);
         else if (delta) hipLaunchKernelGGL(k_bddual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
+        else if (W % 2 == 0)
+          hipLaunchKernelGGL(k_bdual2, dim3(grid_for(c->m * static_cast<long long>(W / 2))), dim3(kBlock), 0,
+                             c->stream, a);
         else hipLaunchKernelGGL(k_bdual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
         time_end(c, 1, ev);
       }
