@@ -16,10 +16,10 @@ Other workloads (each prints one JSON line of the same shape):
   c2  refinery planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
       versus plain fp64: iteration rates at T = 8760, and both to 1e-8 end to end at a T that
       converges in bench time; value = fp64 seconds ÷ mixed seconds, with the objective difference.
-  c3  64 scenarios sharing one 100,000 × 200,000 random matrix (b, c, u scaled by lognormal(0,
-      0.05) factors) solved as one batch (SpMM); value = scenario-iterations per second against
-      one LP alone, plus an end-to-end run at 10,000 rows (warm start, 1e-6) against sequential
-      single solves.
+  c3  64 scenarios sharing one 100,000 × 200,000 random matrix solved as one batch (SpMM);
+      value = scenario-iterations per second (fp64; mixed beside it) against one LP alone, plus an
+      end-to-end run at 10,000 rows with b, c and u scaled by lognormal(0, 0.05) factors (warm
+      start, 1e-6) against sequential single solves.
   c4  serving: Poisson arrivals of random LPs with m log-uniform in [m_lo, m_hi], half at 1e-6
       and half at 1e-8, admitted under an HBM byte budget, served in mixed precision (increment
       form; --c4-precision fp64 for plain); value = solves per second, with p50 / p99 latency,
@@ -224,18 +224,18 @@ def bench_c2(args, ps26119, lpgen):
 def bench_c3(args, ps26119, lpgen):
     """Two measurements, both bounded:
     (a) at the full size (100,000 × 200,000, 64 scenarios) the batch's rate in scenario-iterations
-        per second against one LP solved alone (the same iterations, differential timing so that
-        setup is excluded) — `value`;
-    (b) end to end at 10,000 × 20,000: 64 cost scenarios (c scaled by lognormal(0, 0.05) factors)
-        warm-started from the base solution, solved to 1e-6 as one batch in fp64 and in mixed
-        precision, against sequential single solves (4 timed, scaled to 64). Scenarios that also
-        scale b and u elementwise are mostly infeasible on this construction (the oracle certifies
-        it with a Farkas ray), and the batch engine, like the reference's, has no ray test: such a
-        batch would run to its iteration limit, so (b) varies the costs only."""
+        per second, fp64 and mixed, against one LP alone (its loop time) — `value`. The rate runs
+        use cost scenarios (c scaled by lognormal(0, 0.05) factors), which all stay live: scaling b,
+        c and u together (below) makes most scenarios infeasible, and those leave the batch after a
+        few checks with a certificate, which would change the width being timed;
+    (b) end to end at 10,000 × 20,000 with b, c and u scaled by lognormal(0, 0.05) factors,
+        warm-started from the base solution, to 1e-6: fp64 batch,
+        mixed batch and sequential single solves (4 timed, scaled to 64), with the verdicts of
+        each (Optimal, or Infeasible with a checked dual-ray certificate)."""
     m = args.m if args.m != 1_000_000 else 100_000
     base = lpgen.random_lp(m, seed=args.seed)
-    scen = lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)
-    log(f"c3 base m={base.m} n={base.n} nnz={base.nnz}, {len(scen)} scenarios")
+    scen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)]
+    log(f"c3 base m={base.m} n={base.n} nnz={base.nnz}, {len(scen)} cost scenarios")
     lo, hi = 64, args.c3_iterations
 
     def timed(fn):
@@ -246,10 +246,10 @@ def bench_c3(args, ps26119, lpgen):
     tb = [timed(lambda k=k: ps26119.solve_model_batch(base, scen, iteration_limit=k, verbosity=1)) for k in (lo, hi)]
     tm = [timed(lambda k=k: ps26119.solve_model_batch(base, scen, iteration_limit=k, verbosity=1, precision="mixed"))
           for k in (lo, hi)]
-    ts = [timed(lambda k=k: ps26119.solve_model(base, iteration_limit=k, presolve=False)) for k in (lo, hi)]
+    one = ps26119.solve_model(base, iteration_limit=hi, presolve=False, stats=True)
     batch_rate = len(scen) * (hi - lo) / (tb[1] - tb[0])
     mixed_rate = len(scen) * (hi - lo) / (tm[1] - tm[0])
-    single_rate = (hi - lo) / (ts[1] - ts[0])
+    single_rate = one.iterations / one.stats["iterate_seconds"]
     log(f"batch: {batch_rate:.0f} scenario-iterations/s ({(tb[1] - tb[0]) / (hi - lo) * 1e3:.3f} ms/iteration for "
         f"{len(scen)}), mixed {mixed_rate:.0f}; single LP: {single_rate:.0f} iterations/s")
     W, nz = len(scen), base.nnz
@@ -262,54 +262,59 @@ def bench_c3(args, ps26119, lpgen):
     loop_gbs = {"fp64": b64 * batch_rate / W / 1e9, "mixed": b32 * mixed_rate / W / 1e9}
     if args.c3_rate_only:
         line = base_line(args, batch_rate, "scenario-iterations/s", True, "f64",
-                         {"workload": f"{W} scenarios sharing a random {base.m}x{base.n} matrix: batch rates only"})
+                         {"workload": f"{W} cost scenarios sharing a random {base.m}x{base.n} matrix: batch rates only"})
         line.update({"mixed_scenario_iterations_per_s": mixed_rate, "loop_GBs": loop_gbs, "roofline": None,
                      "cpu_baseline": None})
         return line
-    # (b) end to end at 10k rows
+    # (b) end to end at 10k rows, b, c and u scaled
     small = lpgen.random_lp(10_000, seed=args.seed)
-    sscen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)]
-    b = ps26119.solve_model(small, verbosity=1)
-    t = time.time()
+    sscen = lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)
+    b = ps26119.solve_model(small, tolerance=1e-6)
     cap = {"iteration_limit": 3_000_000, "time_limit": 150.0}
-    res = ps26119.solve_model_batch(small, sscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1, **cap)
-    batch_s = time.time() - t
-    ok = sum(r.status == ps26119.OPTIMAL for r in res)
-    t = time.time()
-    resm = ps26119.solve_model_batch(small, sscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1,
-                                     precision="mixed", **cap)
-    batch_mixed_s = time.time() - t
-    okm = sum(r.status == ps26119.OPTIMAL for r in resm)
-    dobj = max(abs(p.objective - q.objective) / (1 + abs(q.objective)) for p, q in zip(resm, res))
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        t = time.time()
+        res = ps26119.solve_model_batch(small, sscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1,
+                                        precision=prec, **cap)
+        runs[prec] = (time.time() - t, res)
     k_seq = min(args.seq_sample, len(sscen))
     t = time.time()
-    for sc in sscen[:k_seq]:
-        ps26119.solve_model(lpgen.with_scenario(small, sc), tolerance=1e-6, warm_x=b.x, warm_y=b.y, presolve=False,
-                            verbosity=1, **cap)
+    seq = [ps26119.solve_model(lpgen.with_scenario(small, sc), tolerance=1e-6, warm_x=b.x, warm_y=b.y, presolve=False,
+                               **cap) for sc in sscen[:k_seq]]
     seq_s = (time.time() - t) / k_seq * len(sscen)
-    log(f"10k end to end: batch {batch_s:.2f}s ({ok}/{len(res)} optimal), mixed batch {batch_mixed_s:.2f}s "
-        f"({okm} optimal, objectives within {dobj:.1e} of fp64), sequential estimate {seq_s:.2f}s")
+
+    def verdicts(res):
+        out = {}
+        for r in res:
+            out[r.status_name] = out.get(r.status_name, 0) + 1
+        return out
+
+    agree = sum(r.status == q.status for r, q in zip(runs["fp64"][1], seq))
+    log(f"10k end to end: fp64 batch {runs['fp64'][0]:.2f}s {verdicts(runs['fp64'][1])}, mixed batch "
+        f"{runs['mixed'][0]:.2f}s {verdicts(runs['mixed'][1])}, sequential estimate {seq_s:.2f}s "
+        f"(sample {verdicts(seq)}, {agree}/{k_seq} verdicts equal to the batch's)")
     line = base_line(args, batch_rate, "scenario-iterations/s", True, "f64",
-                     {"workload": f"{len(scen)} scenarios sharing a random {base.m}x{base.n} matrix (6 per row), b/c/u "
-                                  f"scaled by lognormal(0, 0.05); one batched r2HPDHG (SpMM over 64-wide block vectors), "
-                                  f"fp64; rate over iterations {lo}..{hi}", "m": base.m, "n": base.n, "nnz": base.nnz,
-                      "scenarios": len(scen)})
+                     {"workload": f"{W} scenarios sharing a random {base.m}x{base.n} matrix (6 per row), c scaled by "
+                                  f"lognormal(0, 0.05); one batched r2HPDHG (SpMM over 64-wide block vectors), fp64; "
+                                  f"rate over iterations {lo}..{hi}", "m": base.m, "n": base.n, "nnz": base.nnz,
+                      "scenarios": W})
     line["steps"] = hi - lo
     line["ms_per_step"] = (tb[1] - tb[0]) / (hi - lo) * 1e3
     line.update({
         "roofline": {"bound": "hbm", "achieved": loop_gbs["fp64"], "peak": 8000.0, "unit": "GB/s",
                      "frac": loop_gbs["fp64"] / 8000.0, "traffic": None,
-                     "kernel": "batched iteration loop (k_bprimal + k_bdual + the check every 64), from the loop rate",
+                     "kernel": "batched iteration loop (k_bprimal2 + k_bdual2 + the check every 64), from the loop rate",
                      "bytes_per_iteration": b64},
         "loop_GBs": loop_gbs, "single_lp_iterations_per_s": single_rate,
         "speedup_vs_single_lp_iterations": batch_rate / single_rate,
         "mixed_scenario_iterations_per_s": mixed_rate,
         "end_to_end_10k": {"scenarios": len(sscen), "tolerance": 1e-6, "warm_start": "base solution",
-                           "perturbation": "c scaled by lognormal(0, 0.05) factors",
-                           "batch_seconds": batch_s, "optimal": ok, "sequential_seconds_estimate": seq_s,
-                           "sequential_sample": k_seq, "speedup": seq_s / batch_s if batch_s > 0 else None,
-                           "mixed_batch_seconds": batch_mixed_s, "mixed_optimal": okm,
-                           "mixed_objective_rel_diff_max": dobj},
+                           "perturbation": "b, c and u scaled by lognormal(0, 0.05) factors",
+                           "fp64_batch_seconds": runs["fp64"][0], "fp64_verdicts": verdicts(runs["fp64"][1]),
+                           "mixed_batch_seconds": runs["mixed"][0], "mixed_verdicts": verdicts(runs["mixed"][1]),
+                           "sequential_seconds_estimate": seq_s, "sequential_sample": k_seq,
+                           "sequential_sample_verdicts": verdicts(seq),
+                           "speedup_fp64_batch": seq_s / runs["fp64"][0] if runs["fp64"][0] > 0 else None},
     })
     line["cpu_baseline"] = None
     return line

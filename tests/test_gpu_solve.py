@@ -83,7 +83,7 @@ def test_long_solves_follow_the_reference(gpu_available, golden, key):
     assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
 
 
-def test_default_reductions_on_config0(gpu_available, golden):
+def test_default_reductions_on_random_10k(gpu_available, golden):
     import ps26119
     lp = _model("random10000_s1")
     exp = golden["solves"]["random10000_s1|presolve=1|fp64"]
@@ -203,11 +203,11 @@ def test_bf16_values_variant_reaches_the_fp64_answer(gpu_available, golden, name
     assert err <= 1e-7
 
 
-def test_refined_mixed_on_config0(gpu_available):
+def test_refined_mixed_on_random_10k(gpu_available):
     import ps26119
     lp = _model("random10000_s1")
     r = ps26119.solve_model(lp, precision="mixed")
-    print(f"C0 increment-form mixed: it={r.iterations} t={r.seconds:.2f}s obj={r.objective!r} "
+    print(f"random 10k increment-form mixed: it={r.iterations} t={r.seconds:.2f}s obj={r.objective!r} "
           f"planted={lp.optimum!r}; {r.message}", flush=True)
     assert r.status == ps26119.OPTIMAL and r.check == "PASS"
     assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
