@@ -25,7 +25,7 @@ enum BTerm {
   B_ROBJR, B_RMAGR, B_ROBJC, B_RMAGC, B_CD, B_CMAG, B_RMAX, B_DVIOL, B_DMAXRAY, B_PVIOL, kBTerms
 };
 constexpr int kBJobs = 24;
-constexpr int kBTreeBlocks = 64;
+constexpr int kBTreeBlocks = 512;  // level-1 blocks per job: enough workgroups for 200k rows × 64 slots
 
 struct BArgs {
   int m, n, W, it, hat;
