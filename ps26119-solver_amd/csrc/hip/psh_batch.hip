@@ -234,94 +234,104 @@ __global__ void __launch_bounds__(kBlock) k_bddual(BArgs a) {
   a.b.iy[q] = w * (static_cast<float>(a.rho2) * h + static_cast<float>(a.rho1) * yq);
 }
 
-// The same two steps for an even batch width, two adjacent slots per thread with 8-byte loads
-// (a wave then covers two rows of 64 slots): same arithmetic per slot, so the same bits.
-__global__ void __launch_bounds__(kBlock) k_bdprimal2(BArgs a) {
-  const long long g = 2 * (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+// The same two steps for a batch width divisible by P, P adjacent slots per thread with 4P-byte
+// loads (a wave then covers P rows of 64 slots): same arithmetic per slot, so the same bits.
+template <int P>
+struct Floats;
+template <>
+struct Floats<2> {
+  using V = float2;
+};
+template <>
+struct Floats<4> {
+  using V = float4;
+};
+template <int P>
+__device__ inline void ldf(const float* p, float (&v)[P]) {
+  *reinterpret_cast<typename Floats<P>::V*>(v) = *reinterpret_cast<const typename Floats<P>::V*>(p);
+}
+template <int P>
+__device__ inline void stf(float* p, const float (&v)[P]) {
+  *reinterpret_cast<typename Floats<P>::V*>(p) = *reinterpret_cast<const typename Floats<P>::V*>(v);
+}
+
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_bdprimalP(BArgs a) {
+  const long long g = P * (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
   const long long j = g / a.W;
   if (j >= a.n) return;
   const int t = static_cast<int>(g - j * a.W);
-  double s0 = 0.0, s1 = 0.0;
+  double s[P];
+#pragma unroll
+  for (int e = 0; e < P; ++e) s[e] = 0.0;
   for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) {
     const double v = static_cast<double>(a.at_valf[p]);
-    const float2 y = *reinterpret_cast<const float2*>(a.b.iy + static_cast<size_t>(a.at_col[p]) * a.W + t);
-    s0 += v * static_cast<double>(y.x);
-    s1 += v * static_cast<double>(y.y);
+    float y[P];
+    ldf<P>(a.b.iy + static_cast<size_t>(a.at_col[p]) * a.W + t, y);
+#pragma unroll
+    for (int e = 0; e < P; ++e) s[e] += v * static_cast<double>(y[e]);
   }
   const size_t q = static_cast<size_t>(g);
-  const float2 xq = *reinterpret_cast<const float2*>(a.b.ix + q);
-  const float2 G = *reinterpret_cast<const float2*>(a.b.G + q);
-  const float2 LX = *reinterpret_cast<const float2*>(a.b.LX + q);
-  const float2 UX = *reinterpret_cast<const float2*>(a.b.UX + q);
+  float xq[P], G[P], LX[P], UX[P], h[P], xb[P], xn[P];
+  ldf<P>(a.b.ix + q, xq);
+  ldf<P>(a.b.G + q, G);
+  ldf<P>(a.b.LX + q, LX);
+  ldf<P>(a.b.UX + q, UX);
   const float r2 = static_cast<float>(a.rho2), r1 = static_cast<float>(a.rho1);
-  float2 h, xb, xn;
-  {
-    const float w = static_cast<float>(halpern_w(a.b.inner0[t], a.it));
-    const float tau = static_cast<float>(a.b.tau[t]);
-    float v = xq.x - tau * (G.x - static_cast<float>(s0));
-    h.x = v < LX.x ? LX.x : (v > UX.x ? UX.x : v);
-    xb.x = 2 * h.x - xq.x;
-    xn.x = w * (r2 * h.x + r1 * xq.x);
+#pragma unroll
+  for (int e = 0; e < P; ++e) {
+    const float w = static_cast<float>(halpern_w(a.b.inner0[t + e], a.it));
+    const float tau = static_cast<float>(a.b.tau[t + e]);
+    const float v = xq[e] - tau * (G[e] - static_cast<float>(s[e]));
+    h[e] = v < LX[e] ? LX[e] : (v > UX[e] ? UX[e] : v);
+    xb[e] = 2 * h[e] - xq[e];
+    xn[e] = w * (r2 * h[e] + r1 * xq[e]);
   }
-  {
-    const float w = static_cast<float>(halpern_w(a.b.inner0[t + 1], a.it));
-    const float tau = static_cast<float>(a.b.tau[t + 1]);
-    float v = xq.y - tau * (G.y - static_cast<float>(s1));
-    h.y = v < LX.y ? LX.y : (v > UX.y ? UX.y : v);
-    xb.y = 2 * h.y - xq.y;
-    xn.y = w * (r2 * h.y + r1 * xq.y);
-  }
-  if (a.hat) *reinterpret_cast<float2*>(a.b.ixh + q) = h;
-  *reinterpret_cast<float2*>(a.b.ixb + q) = xb;
-  *reinterpret_cast<float2*>(a.b.ix + q) = xn;
+  if (a.hat) stf<P>(a.b.ixh + q, h);
+  stf<P>(a.b.ixb + q, xb);
+  stf<P>(a.b.ix + q, xn);
 }
 
-__global__ void __launch_bounds__(kBlock) k_bddual2(BArgs a) {
-  const long long g = 2 * (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_bddualP(BArgs a) {
+  const long long g = P * (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
   const long long i = g / a.W;
   if (i >= a.m) return;
   const int t = static_cast<int>(g - i * a.W);
-  double s0 = 0.0, s1 = 0.0;
+  double s[P];
+#pragma unroll
+  for (int e = 0; e < P; ++e) s[e] = 0.0;
   for (int p = a.a_ptr[i]; p < a.a_ptr[i + 1]; ++p) {
     const double v = static_cast<double>(a.a_valf[p]);
-    const float2 x = *reinterpret_cast<const float2*>(a.b.ixb + static_cast<size_t>(a.a_col[p]) * a.W + t);
-    s0 += v * static_cast<double>(x.x);
-    s1 += v * static_cast<double>(x.y);
+    float x[P];
+    ldf<P>(a.b.ixb + static_cast<size_t>(a.a_col[p]) * a.W + t, x);
+#pragma unroll
+    for (int e = 0; e < P; ++e) s[e] += v * static_cast<double>(x[e]);
   }
   const size_t q = static_cast<size_t>(g);
-  const float2 yq = *reinterpret_cast<const float2*>(a.b.iy + q);
-  const float2 KY = *reinterpret_cast<const float2*>(a.b.KY + q);
-  const float2 BL = *reinterpret_cast<const float2*>(a.b.BL + q);
-  const float2 BU = *reinterpret_cast<const float2*>(a.b.BU + q);
+  float yq[P], KY[P], BL[P], BU[P], h[P], yb[P], yn[P];
+  ldf<P>(a.b.iy + q, yq);
+  ldf<P>(a.b.KY + q, KY);
+  ldf<P>(a.b.BL + q, BL);
+  ldf<P>(a.b.BU + q, BU);
   const float r2 = static_cast<float>(a.rho2), r1 = static_cast<float>(a.rho1);
-  float2 h, yb, yn;
-  {
-    const float w = static_cast<float>(halpern_w(a.b.inner0[t], a.it));
-    const float sg = static_cast<float>(a.b.sigma[t]);
-    const float ax = static_cast<float>(s0);
-    const float lo = BU.x + ax, hi = BL.x + ax;
-    float v = KY.x + yq.x / sg;
+#pragma unroll
+  for (int e = 0; e < P; ++e) {
+    const float w = static_cast<float>(halpern_w(a.b.inner0[t + e], a.it));
+    const float sg = static_cast<float>(a.b.sigma[t + e]);
+    const float ax = static_cast<float>(s[e]);
+    const float lo = BU[e] + ax, hi = BL[e] + ax;
+    float v = KY[e] + yq[e] / sg;
     v = v < lo ? lo : (v > hi ? hi : v);
-    h.x = yq.x - sg * v;
-    yb.x = 2 * h.x - yq.x;
-    yn.x = w * (r2 * h.x + r1 * yq.x);
-  }
-  {
-    const float w = static_cast<float>(halpern_w(a.b.inner0[t + 1], a.it));
-    const float sg = static_cast<float>(a.b.sigma[t + 1]);
-    const float ax = static_cast<float>(s1);
-    const float lo = BU.y + ax, hi = BL.y + ax;
-    float v = KY.y + yq.y / sg;
-    v = v < lo ? lo : (v > hi ? hi : v);
-    h.y = yq.y - sg * v;
-    yb.y = 2 * h.y - yq.y;
-    yn.y = w * (r2 * h.y + r1 * yq.y);
+    h[e] = yq[e] - sg * v;
+    yb[e] = 2 * h[e] - yq[e];
+    yn[e] = w * (r2 * h[e] + r1 * yq[e]);
   }
   if (a.hat) {
-    *reinterpret_cast<float2*>(a.b.iyh + q) = h;
-    *reinterpret_cast<float2*>(a.b.iyb + q) = yb;
+    stf<P>(a.b.iyh + q, h);
+    stf<P>(a.b.iyb + q, yb);
   }
-  *reinterpret_cast<float2*>(a.b.iy + q) = yn;
+  stf<P>(a.b.iy + q, yn);
 }
 
 // fp64 points from anchor + increment: ẑ and z̄ (hat steps), and z itself when `cur` is set.
@@ -862,8 +872,11 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
       hipEvent_t ev;
       if (c->n > 0) {
         time_begin(c, 0, &ev);
-        if (delta && W % 2 == 0)
-          hipLaunchKernelGGL(k_bdprimal2, dim3(grid_for(c->n * static_cast<long long>(W / 2))), dim3(kBlock), 0,
+        if (delta && W % 4 == 0)
+          hipLaunchKernelGGL(k_bdprimalP<4>, dim3(grid_for(c->n * static_cast<long long>(W / 4))), dim3(kBlock), 0,
+                             c->stream, a);
+        else if (delta && W % 2 == 0)
+          hipLaunchKernelGGL(k_bdprimalP<2>, dim3(grid_for(c->n * static_cast<long long>(W / 2))), dim3(kBlock), 0,
                              c->stream, a);
         else if (delta) hipLaunchKernelGGL(k_bdprimal, dim3(grid_rows(c->n, W)), dim3(kBlock), 0, c->stream, a);
         else if (W % 2 == 0)
@@ -874,8 +887,11 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
       }
       if (c->m > 0) {
         time_begin(c, 1, &ev);
-        if (delta && W % 2 == 0)
-          hipLaunchKernelGGL(k_bddual2, dim3(grid_for(c->m * static_cast<long long>(W / 2))), dim3(kBlock), 0,
+        if (delta && W % 4 == 0)
+          hipLaunchKernelGGL(k_bddualP<4>, dim3(grid_for(c->m * static_cast<long long>(W / 4))), dim3(kBlock), 0,
+                             c->stream, a);
+        else if (delta && W % 2 == 0)
+          hipLaunchKernelGGL(k_bddualP<2>, dim3(grid_for(c->m * static_cast<long long>(W / 2))), dim3(kBlock), 0,
                              c->stream, a);
         else if (delta) hipLaunchKernelGGL(k_bddual, dim3(grid_rows(c->m, W)), dim3(kBlock), 0, c->stream, a);
         else if (W % 2 == 0)

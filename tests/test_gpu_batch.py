@@ -208,17 +208,21 @@ def test_infeasible_scenarios_leave_the_batch_with_certificates(gpu_available, o
                 assert abs(r.objective - e["objective"]) <= 1e-7 * (1 + abs(e["objective"])), (prec, k)
 
 
-def test_mixed_batch_slot_pairs_equal_single_slots(gpu_available):
-    """An even batch width runs the increment-form kernels two slots per thread (8-byte loads);
-    an odd width runs them one slot per thread. With CPU-order reductions a scenario's trajectory
-    does not depend on the width, so each scenario must come out with the same iterations and
-    bits from a batch of 3 (single slots) and a batch of 4 (slot pairs)."""
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_slot_groups_equal_single_slots(gpu_available, prec):
+    """Widths divisible by 4 (mixed) or 2 run the batch kernels on 4 or 2 adjacent slots per thread
+    (wider loads); an odd width runs them one slot per thread. With CPU-order reductions a
+    scenario's trajectory does not depend on the width, so each scenario must come out with the
+    same iterations and bits from batches of 3 (single slots), 2 (pairs) and 4 (groups of 4 in
+    mixed precision, pairs in fp64)."""
     import ps26119
     base, scen = _scenarios("stocfor1")
     prm = {"hip_cpu_order": 1, "hip_refine": 1}
-    odd = ps26119.solve_model_batch(base, scen[:3], precision="mixed", params=prm, iteration_limit=3_000_000)
-    even = ps26119.solve_model_batch(base, scen[:4], precision="mixed", params=prm, iteration_limit=3_000_000)
-    for k in range(3):
-        assert odd[k].status == even[k].status == ps26119.OPTIMAL, (k, odd[k].message, even[k].message)
-        assert odd[k].iterations == even[k].iterations, k
-        assert fnv(odd[k].x) == fnv(even[k].x) and fnv(odd[k].y) == fnv(even[k].y), k
+    runs = {w: ps26119.solve_model_batch(base, scen[:w], precision=prec, params=prm, iteration_limit=3_000_000)
+            for w in (3, 2, 4)}
+    for w in (2, 4):
+        for k in range(min(w, 3)):
+            a, b = runs[3][k], runs[w][k]
+            assert a.status == b.status == ps26119.OPTIMAL, (w, k, a.message, b.message)
+            assert a.iterations == b.iterations, (w, k)
+            assert fnv(a.x) == fnv(b.x) and fnv(a.y) == fnv(b.y), (w, k)
