@@ -222,7 +222,7 @@ def bench_c2(args, ps26119, lpgen):
 
 # ------------------------------------------------------------------ c3
 def bench_c3(args, ps26119, lpgen):
-    """Two measurements, both bounded:
+    """Three measurements, all bounded:
     (a) at the full size (100,000 × 200,000, 64 scenarios) the batch's rate in scenario-iterations
         per second, fp64 and mixed, against one LP alone (its loop time) — `value`. The rate runs
         use cost scenarios (c scaled by lognormal(0, 0.05) factors), which all stay live: scaling b,
@@ -231,7 +231,9 @@ def bench_c3(args, ps26119, lpgen):
     (b) end to end at 10,000 × 20,000 with b, c and u scaled by lognormal(0, 0.05) factors,
         warm-started from the base solution, to 1e-6: fp64 batch,
         mixed batch and sequential single solves (4 timed, scaled to 64), with the verdicts of
-        each (Optimal, or Infeasible with a checked dual-ray certificate)."""
+        each (Optimal, or Infeasible with a checked dual-ray certificate);
+    (c) at 10,000 × 20,000 the 64 cost scenarios warm-started to 1e-6, fp64 batch against mixed
+        batch (capped at 3M iterations or 120 s)."""
     m = args.m if args.m != 1_000_000 else 100_000
     base = lpgen.random_lp(m, seed=args.seed)
     scen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)]
@@ -291,6 +293,16 @@ def bench_c3(args, ps26119, lpgen):
         return out
 
     agree = sum(r.status == q.status for r, q in zip(runs["fp64"][1], seq))
+    # (c) feasible scenarios at 10k rows (costs scaled only), fp64 batch vs mixed batch to 1e-6
+    cscen = [{"obj": sc["obj"]} for sc in sscen]
+    cost = {}
+    for prec in ("fp64", "mixed"):
+        t = time.time()
+        res = ps26119.solve_model_batch(small, cscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1,
+                                        precision=prec, iteration_limit=3_000_000, time_limit=120.0)
+        cost[prec] = {"seconds": time.time() - t, "verdicts": verdicts(res),
+                      "iterations_max": max(r.iterations for r in res)}
+    log(f"10k cost scenarios to 1e-6: fp64 batch {cost['fp64']}, mixed batch {cost['mixed']}")
     log(f"10k end to end: fp64 batch {runs['fp64'][0]:.2f}s {verdicts(runs['fp64'][1])}, mixed batch "
         f"{runs['mixed'][0]:.2f}s {verdicts(runs['mixed'][1])}, sequential estimate {seq_s:.2f}s "
         f"(sample {verdicts(seq)}, {agree}/{k_seq} verdicts equal to the batch's)")
@@ -316,6 +328,8 @@ def bench_c3(args, ps26119, lpgen):
                            "sequential_seconds_estimate": seq_s, "sequential_sample": k_seq,
                            "sequential_sample_verdicts": verdicts(seq),
                            "speedup_fp64_batch": seq_s / runs["fp64"][0] if runs["fp64"][0] > 0 else None},
+        "cost_scenarios_10k": {"perturbation": "c scaled by lognormal(0, 0.05) factors", "tolerance": 1e-6,
+                               "warm_start": "base solution", "caps": "3M iterations, 120 s", **cost},
     })
     line["cpu_baseline"] = None
     return line
