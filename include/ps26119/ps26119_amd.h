@@ -19,7 +19,10 @@ extern "C" {
  * (mixed precision as fp32 increments on fp64 anchors; default 1, 0 in CPU-order mode: the
  * reference's fp32-then-fp64 policy), "hip_refine_patience" (stalled checks before fp64 takes
  * over, default 40), "hip_refine_max_epoch" (longest epoch of the increment form, default
- * 65536; 0 = no limit), "hip_bf16_until" (> 0: the increment form starts with bf16 matrix values
+ * 65536; 0 = no limit), "hip_max_epoch" (longest epoch in any precision: a restart to the
+ * averaged point is forced when the epoch reaches it; default 0 = the reference's restart schedule
+ * alone), "hip_max_epoch_gap_only" (1 = hip_max_epoch applies only while the relative gap is the
+ * largest KKT term), "hip_bf16_until" (> 0: the increment form starts with bf16 matrix values
  * and moves to fp32 values at this relative KKT error or on a stall; default 0 = off),
  * "hip_device_scaling" (1 = default: the scaling sweeps run on the device, same bits as the host).
  * presolve: 1 (default) / 0. */

@@ -325,7 +325,8 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
           }
           keep.push_back(t);
           const double r = resid(st[PSH_BSTAT_PX], st[PSH_BSTAT_PY], st[PSH_BSTAT_CROSS]);
-          const bool restart = restart_due(eo, r, r0[k], r_last[k], inner[k], it) || increment_epoch_full(eo, incr, inner[k]);
+          const bool restart = restart_due(eo, r, r0[k], r_last[k], inner[k], it) || increment_epoch_full(eo, incr, inner[k]) ||
+                               epoch_cap_reached(eo, inner[k], kk);
           r_last[k] = r;
           if (!restart) continue;
           pid_update(pid[k], eo, std::sqrt(st[PSH_BSTAT_PIDX]), std::sqrt(st[PSH_BSTAT_PIDY]), kk.rel_primal(),

@@ -64,6 +64,8 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_refine_patience") e.refine_patience = std::max(1, static_cast<int>(v));
     else if (name == "hip_bf16_until") e.bf16_until = v > 0 ? v : 0.0;
     else if (name == "hip_refine_max_epoch") e.refine_max_epoch = v > 0 ? static_cast<std::int64_t>(v) : 0;
+    else if (name == "hip_max_epoch") e.max_epoch = v > 0 ? static_cast<std::int64_t>(v) : 0;
+    else if (name == "hip_max_epoch_gap_only") e.max_epoch_gap_only = v != 0 ? 1 : 0;
     else return name;
   }
   if (e.refine < 0) e.refine = e.cpu_order ? 0 : 1;
@@ -747,8 +749,8 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
     }
     const bool promoted = policy.on_check(k.rel_kkt(), it);
     if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
-    const bool restart =
-        promoted || restart_due(opt, r, r0, r_last, inner, it) || increment_epoch_full(opt, policy.refining(), inner);
+    const bool restart = promoted || restart_due(opt, r, r0, r_last, inner, it) ||
+                         increment_epoch_full(opt, policy.refining(), inner) || epoch_cap_reached(opt, inner, k);
     r_last = r;
     if (!restart) continue;
 

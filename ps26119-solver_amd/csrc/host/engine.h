@@ -50,6 +50,13 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   // (1 − w)·δ with w = (k+1)/(k+2) falls below fp32 resolution of the increments once k nears 10^6,
   // so a longer epoch loses its anchoring; the epoch then ends with a restart.
   std::int64_t refine_max_epoch = 1 << 16;
+  // hip_max_epoch: longest epoch in any precision (0 = the reference's schedule alone). Not in the
+  // reference: on the random LPs with a planted optimum the artificial restart (inner ≥ 0.36·it)
+  // lets epochs grow with the iteration count and the relative gap stalls near 3e-7 at 1M rows;
+  // shorter epochs keep closing it. hip_max_epoch_gap_only = 1 applies the cap only while the
+  // relative gap is the largest KKT term, so long epochs return once feasibility is what is left.
+  std::int64_t max_epoch = 0;
+  int max_epoch_gap_only = 0;
   // hip_bf16_until: > 0 runs the increment form's SpMVs with bf16 matrix values until the relative
   // KKT error reaches this value or stops improving for 10 checks, then with fp32 values (0: off).
   double bf16_until = 0.0;
@@ -129,9 +136,14 @@ struct RefineGuard {
   }
 };
 
-// The epoch limit of the increment form (EngineOptions::refine_max_epoch).
+// The epoch limits: the increment form's (refine_max_epoch) and the optional one of any
+// precision (max_epoch, gated on the gap leading the KKT terms when max_epoch_gap_only is set).
 inline bool increment_epoch_full(const EngineOptions& o, bool increment_form, std::int64_t inner) {
   return increment_form && o.refine_max_epoch > 0 && inner >= o.refine_max_epoch;
+}
+inline bool epoch_cap_reached(const EngineOptions& o, std::int64_t inner, const KktStats& k) {
+  if (o.max_epoch <= 0 || inner < o.max_epoch) return false;
+  return !o.max_epoch_gap_only || k.rel_gap() >= std::max(k.rel_primal(), k.rel_dual());
 }
 
 // PID primal weight with damping and the best-balance fallback (reference r2hpdhg.cpp:127-148).

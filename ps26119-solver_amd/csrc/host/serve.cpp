@@ -420,7 +420,8 @@ void Server::serve_loop() {
         }
       }
       const bool restart =
-          promoted || restart_due(o, rr, r->r0, r->r_last, r->inner, r->it) || increment_epoch_full(o, r->mixed, r->inner);
+          promoted || restart_due(o, rr, r->r0, r->r_last, r->inner, r->it) || increment_epoch_full(o, r->mixed, r->inner) ||
+          epoch_cap_reached(o, r->inner, k);
       r->r_last = rr;
       if (!restart) continue;
       pid_update(r->pid, o, st.dx_norm, st.dy_norm, k.rel_primal(), k.rel_dual());

@@ -213,6 +213,22 @@ def test_refined_mixed_on_random_10k(gpu_available):
     assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_epoch_cap_reaches_the_planted_optimum(gpu_available, golden, prec):
+    """hip_max_epoch = 8192 with hip_max_epoch_gap_only ends every epoch that reaches 8192
+    iterations while the relative gap is the largest KKT term: the same verified optimum, in fewer
+    iterations than the reference schedule takes on this LP (343,104 in the reference's CPU order)."""
+    import ps26119
+    lp = _model("random10000_s1")
+    exp = golden["solves"]["random10000_s1|presolve=1|fp64"]
+    r = ps26119.solve_model(lp, precision=prec, params={"hip_max_epoch": 8192, "hip_max_epoch_gap_only": 1})
+    print(f"random 10k {prec}, epoch cap 8192 while the gap leads: it={r.iterations} t={r.seconds:.2f}s obj={r.objective!r} "
+          f"planted={lp.optimum!r}; {r.message}", flush=True)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    assert abs(r.objective - lp.optimum) <= (1e-9 if prec == "fp64" else 2e-8) * (1 + abs(lp.optimum))
+    assert r.iterations < exp["iterations"]
+
+
 PDLP_CASES = ["afiro", "sc50a", "sc105", "stocfor1", "wyndor", "random300_s3"]
 
 
