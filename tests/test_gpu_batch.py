@@ -226,3 +226,24 @@ def test_batch_slot_groups_equal_single_slots(gpu_available, prec):
             assert a.status == b.status == ps26119.OPTIMAL, (w, k, a.message, b.message)
             assert a.iterations == b.iterations, (w, k)
             assert fnv(a.x) == fnv(b.x) and fnv(a.y) == fnv(b.y), (w, k)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_epoch_capped_batch_matches_single_solves(gpu_available, prec):
+    """With hip_max_epoch and hip_max_epoch_gap_only every scenario of a batch decides its forced
+    restarts on its own KKT terms: cost scenarios of a 2,000-row random LP come out Optimal, pass
+    the check on their own models and agree with capped single solves to 1e-6."""
+    import ps26119
+    from ps26119_amd import lpgen
+    base = lpgen.random_lp(2000, seed=4)
+    scen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(base, 4)]
+    knobs = {"hip_max_epoch": 2048, "hip_max_epoch_gap_only": 1}
+    res = ps26119.solve_model_batch(base, scen, precision=prec, params=knobs, iteration_limit=3_000_000)
+    for k, (r, sc) in enumerate(zip(res, scen)):
+        mk = _with(base, sc)
+        single = ps26119.solve_model(mk, precision=prec, params=knobs, iteration_limit=3_000_000)
+        print(f"capped batch {prec} scenario {k}: it={r.iterations} single it={single.iterations}", flush=True)
+        assert r.status == ps26119.OPTIMAL == single.status, (k, r.message)
+        assert abs(r.objective - single.objective) <= 1e-6 * (1 + abs(single.objective)), k
+        ck = ps26119.check_solution(mk, r.x, r.y)
+        assert ck["passed"], (k, ck)
