@@ -5,9 +5,10 @@ rows, n = 2,000,000 columns, exactly 6 nonzeros per row (6,000,000 nnz), seed 1,
 through the C API toward relative KKT 1e-8 (presolve, scaling, ‖Ã‖₂, r²HPDHG on the GPU). A
 step is one r²HPDHG iteration of the solver's loop (two fused launches, the check every 64
 iterations, restarts); `value` is the HBM GB/s of the iteration loop (algorithmic bytes per
-iteration × iterations ÷ the loop's wall time, inputs resident in HBM). The time to 1e-8 is
-reported when reached: on this instance r²HPDHG needs millions of iterations (its relative gap
-plateaus near 3e-7), far beyond a bench run. Per-kernel durations come from HIP events on the
+iteration × iterations ÷ the loop's wall time, inputs resident in HBM). The default run times
+16,384 iterations; `--to-tolerance` runs the solve to 1e-8 and reports `time_to_1e-8_s` (about
+2.4M iterations, ≈ 410 s, with the default `--epoch-cap 8192`: the reference's restart schedule
+alone stalls near a relative gap of 3e-7 on this instance). Per-kernel durations come from HIP events on the
 solve's stream in a second, timed run; `roofline` reports the fused Ãᵀy + primal-update kernel
 against the 8 TB/s HBM peak. `cpu_baseline` times the reference CPU solver (oracle/_ref, all
 host cores) for 256 iterations of the same model, in the same unit.
@@ -19,7 +20,8 @@ Other workloads (each prints one JSON line of the same shape):
   c3  64 scenarios sharing one 100,000 × 200,000 random matrix solved as one batch (SpMM);
       value = scenario-iterations per second (fp64; mixed beside it) against one LP alone, plus an
       end-to-end run at 10,000 rows with b, c and u scaled by lognormal(0, 0.05) factors (warm
-      start, 1e-6) against sequential single solves.
+      start, 1e-6) against sequential single solves, and the 64 cost scenarios at 10,000 rows to
+      1e-6, fp64 batch against mixed batch.
   c4  serving: Poisson arrivals of random LPs with m log-uniform in [m_lo, m_hi], half at 1e-6
       and half at 1e-8, admitted under an HBM byte budget, served in mixed precision (increment
       form; --c4-precision fp64 for plain); value = solves per second, with p50 / p99 latency,
@@ -27,6 +29,7 @@ Other workloads (each prints one JSON line of the same shape):
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload c1|c2|c3|c4] [--m M]
                     [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
+                    [--epoch-cap N] [--to-tolerance]
                     [--iteration-limit N]   (profiling: same per-launch work, N iterations)
 """
 import argparse
