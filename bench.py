@@ -209,6 +209,16 @@ def bench_c2(args, ps26119, lpgen):
                                 iteration_limit=min(iters, 2048))
     pb, db = delta_kernel_bytes(lp.m, lp.n, lp.nnz)
     roof, avg_us = roofline_of(timed.stats, pb, "k_delta_primal (fp32 increments: Ãᵀδy + projection/reflection/Halpern)")
+    traffic_file = os.path.join(ROOT, "profiles", "k_delta_refinery8760_pmc_traffic.json")
+    if args.T == 8760 and os.path.exists(traffic_file):
+        with open(traffic_file) as f:  # HBM bytes per launch from the two rocprofv3 --pmc passes
+            roof["traffic"] = json.load(f)["traffic"].get("k_delta_primal<float>")
+        roof["traffic_source"] = "profiles/k_delta_refinery8760_pmc_traffic.md"
+    if args.c2_rate_only:
+        line = base_line(args, rate["mixed"], "iterations/s", True, "f32+f64",
+                         {"workload": f"refinery planning LP at T={args.T}: iteration rates only"})
+        line.update({"roofline": roof, "iterations_per_s": rate, "cpu_baseline": None})
+        return line
     small = lpgen.refinery_lp(T=args.T_small, seed=3)
     e2e = {}
     for prec in ("mixed", "fp64"):
@@ -435,6 +445,7 @@ def main():
                     help="c1: mixed = fp32 increments on fp64 anchors (the line's dtype says which)")
     ap.add_argument("--T", type=int, default=8760, help="c2: planning periods of the rate measurement")
     ap.add_argument("--T-small", type=int, default=24, help="c2: planning periods of the end-to-end comparison")
+    ap.add_argument("--c2-rate-only", action="store_true", help="c2: only the rates at --T (profiling runs)")
     ap.add_argument("--scenarios", type=int, default=64, help="c3")
     ap.add_argument("--seq-sample", type=int, default=4, help="c3: sequential solves timed for the comparison")
     ap.add_argument("--c3-iterations", type=int, default=16384, help="c3: iterations of the rate measurement")
