@@ -152,10 +152,13 @@ def bench_c1(args, ps26119, lpgen):
     roof, avg_us = roofline_of(timed.stats, pb, "k_delta_primal (fp32 increments: Ãᵀδy + projection/reflection/Halpern)"
                                if mixed else "k_iter_primal (fused Ãᵀy + projection/reflection/Halpern)")
     traffic_file = os.path.join(ROOT, "profiles", "k_iter_fp64_random1M_pmc_traffic.json")
+    phase_traffic = None
     if not mixed and lp.m == 1_000_000 and lp.nnz == 6_000_000 and os.path.exists(traffic_file):
         with open(traffic_file) as f:  # HBM bytes per launch from the two rocprofv3 --pmc passes
-            roof["traffic"] = json.load(f)["traffic"].get("k_iter_primal<double>")
+            tr = json.load(f)["traffic"]
+        roof["traffic"] = tr.get("k_iter_primal<double>")
         roof["traffic_source"] = "profiles/k_iter_fp64_random1M_pmc_traffic.md"
+        phase_traffic = tr["k_iter_primal<double>"] + tr["k_slice_partial"] + tr["k_slice_dual"]
     st = res.stats
     loop_s = st["iterate_seconds"]
     per_it = loop_s / max(1, res.iterations)
@@ -175,6 +178,10 @@ def bench_c1(args, ps26119, lpgen):
         "dual_kernel": {"achieved_GBs": db / (avg_us[1] * 1e-6) / 1e9, "avg_launch_us": avg_us[1], "bytes_per_launch": db},
         "check_pass": {"launches": timed.stats["kernel_launches"][2], "avg_us": avg_us[2]},
         "spmv_phase_GBs": (pb + db) / ((avg_us[0] + avg_us[1]) * 1e-6) / 1e9,
+        # bytes the L2 exchanged with memory per iteration (counters, scaled by the copy probe)
+        # over the same launch times: the whole lines the random gathers bring in count here
+        "spmv_phase_traffic_GBs": (phase_traffic / ((avg_us[0] + avg_us[1]) * 1e-6) / 1e9
+                                   if phase_traffic else None),
         "device_bytes": st["device_bytes"], "message": res.message[:200],
     })
     cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP")
@@ -258,8 +265,8 @@ def bench_c3(args, ps26119, lpgen):
         warm-started from the base solution, to 1e-6: fp64 batch,
         mixed batch and sequential single solves (4 timed, scaled to 64), with the verdicts of
         each (Optimal, or Infeasible with a checked dual-ray certificate);
-    (c) at 10,000 × 20,000 the 64 cost scenarios warm-started to 1e-6, fp64 batch against mixed
-        batch (capped at 3M iterations or 120 s), both with the --epoch-cap knobs."""
+    (c) at 10,000 × 20,000 the 64 cost scenarios warm-started to 1e-6 and to 1e-8, fp64 batch
+        against mixed batch (capped at 6M iterations or 150 s), both with the --epoch-cap knobs."""
     m = args.m if args.m != 1_000_000 else 100_000
     base = lpgen.random_lp(m, seed=args.seed)
     scen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)]
@@ -324,14 +331,18 @@ def bench_c3(args, ps26119, lpgen):
     # (c) feasible scenarios at 10k rows (costs scaled only), fp64 batch vs mixed batch to 1e-6
     cscen = [{"obj": sc["obj"]} for sc in sscen]
     cost = {}
-    for prec in ("fp64", "mixed"):
-        t = time.time()
-        res = ps26119.solve_model_batch(small, cscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1,
-                                        precision=prec, iteration_limit=3_000_000, time_limit=120.0,
-                                        params=epoch_knobs(args))
-        cost[prec] = {"seconds": time.time() - t, "verdicts": verdicts(res),
-                      "iterations_max": max(r.iterations for r in res)}
-    log(f"10k cost scenarios to 1e-6: fp64 batch {cost['fp64']}, mixed batch {cost['mixed']}")
+    for tol in (1e-6, 1e-8):
+        for prec in ("fp64", "mixed"):
+            t = time.time()
+            res = ps26119.solve_model_batch(small, cscen, tolerance=tol, warm_x=b.x, warm_y=b.y, verbosity=1,
+                                            precision=prec, iteration_limit=6_000_000, time_limit=150.0,
+                                            params=epoch_knobs(args))
+            cost[f"{prec}_{tol:.0e}"] = {"seconds": time.time() - t, "verdicts": verdicts(res),
+                                         "iterations_max": max(r.iterations for r in res)}
+            log(f"10k cost scenarios to {tol:.0e}, {prec} batch: {cost[f'{prec}_{tol:.0e}']}")
+        fp, mx = cost[f"fp64_{tol:.0e}"], cost[f"mixed_{tol:.0e}"]
+        if fp["verdicts"].get("Optimal") == mx["verdicts"].get("Optimal") == len(cscen):
+            cost[f"mixed_speedup_{tol:.0e}"] = fp["seconds"] / mx["seconds"]
     log(f"10k end to end: fp64 batch {runs['fp64'][0]:.2f}s {verdicts(runs['fp64'][1])}, mixed batch "
         f"{runs['mixed'][0]:.2f}s {verdicts(runs['mixed'][1])}, sequential estimate {seq_s:.2f}s "
         f"(sample {verdicts(seq)}, {agree}/{k_seq} verdicts equal to the batch's)")
@@ -357,8 +368,8 @@ def bench_c3(args, ps26119, lpgen):
                            "sequential_seconds_estimate": seq_s, "sequential_sample": k_seq,
                            "sequential_sample_verdicts": verdicts(seq),
                            "speedup_fp64_batch": seq_s / runs["fp64"][0] if runs["fp64"][0] > 0 else None},
-        "cost_scenarios_10k": {"perturbation": "c scaled by lognormal(0, 0.05) factors", "tolerance": 1e-6,
-                               "warm_start": "base solution", "caps": "3M iterations, 120 s",
+        "cost_scenarios_10k": {"perturbation": "c scaled by lognormal(0, 0.05) factors", "tolerances": [1e-6, 1e-8],
+                               "warm_start": "base solution", "caps": "6M iterations, 150 s",
                                "epoch_cap": args.epoch_cap, **cost},
     })
     line["cpu_baseline"] = None
