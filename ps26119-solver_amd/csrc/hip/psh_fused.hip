@@ -54,20 +54,31 @@ struct PrimalArgs {
   int write_hat;
 };
 
+// The epilogues load their row's operands in prefetch(), which tile_rows calls before the
+// product loop, so those loads are in flight while the gathers are; operator() then only computes
+// and stores (the same expressions, so the same bits).
 template <class T>
 struct EpiPrimal {
   PrimalArgs<T> p;
   T tau, tw;
+  T xj{}, cj{}, lj{}, uj{}, x0j{};
   __device__ explicit EpiPrimal(const PrimalArgs<T>& a)
       : p(a), tau(static_cast<T>(a.prm[0])), tw(static_cast<T>(halpern_weight(a.prm, a.step))) {}
+  __device__ void prefetch(int j, bool has) {
+    if (!has) return;
+    xj = ld_stream(p.x + j);
+    cj = ld_stream(p.c + j);
+    lj = ld_stream(p.l + j);
+    uj = ld_stream(p.u + j);
+    x0j = ld_stream(p.x0 + j);
+  }
   __device__ void operator()(int j, bool has, double s) {
     if (!has) return;
     const T g = static_cast<T>(s);
-    const T xj = ld_stream(p.x + j);
-    const T h = clampv<T>(xj - tau * (ld_stream(p.c + j) - g), ld_stream(p.l + j), ld_stream(p.u + j));
+    const T h = clampv<T>(xj - tau * (cj - g), lj, uj);
     if (p.write_hat) p.xh[j] = h;
     p.xb[j] = 2 * h - xj;
-    p.x[j] = tw * (p.a * h + p.b * xj) + (1 - tw) * ld_stream(p.x0 + j);
+    p.x[j] = tw * (p.a * h + p.b * xj) + (1 - tw) * x0j;
   }
 };
 
@@ -96,19 +107,26 @@ template <class T>
 struct EpiDual {
   DualArgs<T> p;
   T sg, tw;
+  T yi{}, rli{}, rui{}, y0i{};
   __device__ explicit EpiDual(const DualArgs<T>& a)
       : p(a), sg(static_cast<T>(a.prm[1])), tw(static_cast<T>(halpern_weight(a.prm, a.step))) {}
+  __device__ void prefetch(int i, bool has) {
+    if (!has) return;
+    yi = ld_stream(p.y + i);
+    rli = ld_stream(p.rl + i);
+    rui = ld_stream(p.ru + i);
+    y0i = ld_stream(p.y0 + i);
+  }
   __device__ void operator()(int i, bool has, double s) {
     if (!has) return;
     const T ax = static_cast<T>(s);
-    const T yi = ld_stream(p.y + i);
     const T q = ax - yi / sg;
-    const T h = yi - sg * ax + sg * clampv<T>(q, ld_stream(p.rl + i), ld_stream(p.ru + i));
+    const T h = yi - sg * ax + sg * clampv<T>(q, rli, rui);
     if (p.write_hat) {
       p.yh[i] = h;
       p.yb[i] = 2 * h - yi;
     }
-    p.y[i] = tw * (p.a * h + p.b * yi) + (1 - tw) * ld_stream(p.y0 + i);
+    p.y[i] = tw * (p.a * h + p.b * yi) + (1 - tw) * y0i;
   }
 };
 
@@ -143,13 +161,20 @@ struct DeltaP {
 struct EpiDeltaP {
   DeltaP p;
   float tau, tw;
+  float xj = 0.0f, gj = 0.0f, lj = 0.0f, uj = 0.0f;
   __device__ explicit EpiDeltaP(const DeltaP& a)
       : p(a), tau(static_cast<float>(a.prm[0])), tw(static_cast<float>(halpern_weight(a.prm, a.step))) {}
+  __device__ void prefetch(int j, bool has) {
+    if (!has) return;
+    xj = ld_stream(p.dx + j);
+    gj = ld_stream(p.G + j);
+    lj = ld_stream(p.LX + j);
+    uj = ld_stream(p.UX + j);
+  }
   __device__ void operator()(int j, bool has, double s) {
     if (!has) return;
     const float g = static_cast<float>(s);
-    const float xj = ld_stream(p.dx + j);
-    const float h = clampv<float>(xj - tau * (ld_stream(p.G + j) - g), ld_stream(p.LX + j), ld_stream(p.UX + j));
+    const float h = clampv<float>(xj - tau * (gj - g), lj, uj);
     if (p.write_hat) p.dxh[j] = h;
     p.dxb[j] = 2 * h - xj;
     p.dx[j] = tw * (p.a * h + p.b * xj);
@@ -177,13 +202,20 @@ struct DeltaD {
 struct EpiDeltaD {
   DeltaD p;
   float sg, tw;
+  float yi = 0.0f, kyi = 0.0f, bui = 0.0f, bli = 0.0f;
   __device__ explicit EpiDeltaD(const DeltaD& a)
       : p(a), sg(static_cast<float>(a.prm[1])), tw(static_cast<float>(halpern_weight(a.prm, a.step))) {}
+  __device__ void prefetch(int i, bool has) {
+    if (!has) return;
+    yi = ld_stream(p.dy + i);
+    kyi = ld_stream(p.KY + i);
+    bui = ld_stream(p.BU + i);
+    bli = ld_stream(p.BL + i);
+  }
   __device__ void operator()(int i, bool has, double s) {
     if (!has) return;
     const float ax = static_cast<float>(s);
-    const float yi = ld_stream(p.dy + i);
-    const float t = clampv<float>(ld_stream(p.KY + i) + yi / sg, ld_stream(p.BU + i) + ax, ld_stream(p.BL + i) + ax);
+    const float t = clampv<float>(kyi + yi / sg, bui + ax, bli + ax);
     const float h = yi - sg * t;
     if (p.write_hat) {
       p.dyh[i] = h;
@@ -278,6 +310,7 @@ __global__ void __launch_bounds__(kBlock) k_slice_dual(Sliced S, int rows, DualA
   const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
   EpiDual<double> e(p);
   if (i >= rows) return;
+  e.prefetch(i, true);
   double s = 0.0;
   for (int k = 0; k < S.slices; ++k) s += ld_stream(S.part + static_cast<size_t>(k) * rows + i);
   e(i, true, s);

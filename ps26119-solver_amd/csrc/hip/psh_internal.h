@@ -13,6 +13,9 @@
 //    compiled with -ffp-contract=off (no FMA contraction), so it matches the CPU bits.
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -307,6 +310,12 @@ __device__ inline Bf16 ld_stream<Bf16>(const Bf16* p) {
   return Bf16{__builtin_nontemporal_load(&p->bits)};
 }
 
+// An epilogue may load its row's operands ahead of the product loop: prefetch(row, has).
+template <class E, class = void>
+struct has_prefetch : std::false_type {};
+template <class E>
+struct has_prefetch<E, std::void_t<decltype(std::declval<E&>().prefetch(0, true))>> : std::true_type {};
+
 template <class V, class X, class Epi>
 __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
                                  const V* __restrict__ val, const X* __restrict__ x, Epi& epi) {
@@ -314,6 +323,7 @@ __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, co
   const int p0 = ptr[r0], p1 = ptr[r1];
   const int row = r0 + static_cast<int>(threadIdx.x);
   const bool has = row < r1;
+  if constexpr (has_prefetch<Epi>::value) epi.prefetch(row, has);
   const int rb = has ? ptr[row] : 0, re = has ? ptr[row + 1] : 0;
   double s = 0.0;
   for (int cb = p0; cb < p1; cb += kTileNnz) {

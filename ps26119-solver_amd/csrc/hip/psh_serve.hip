@@ -70,17 +70,27 @@ __device__ inline double dmaxd(double a, double b) { return a < b ? b : a; }
 __device__ inline double dmind(double a, double b) { return b < a ? b : a; }
 
 // ------------------------------------------------------------------ iteration kernels
+// Each epilogue loads its row's operands in prefetch(), which tile_rows calls before the product
+// loop, so the loads overlap the gathers; operator() computes the same expressions.
 struct SrvEpiPrimal {
   const Inst* d;
   double tau, tw, a, b;
   int hat;
+  double xj = 0.0, cj = 0.0, lj = 0.0, uj = 0.0, x0j = 0.0;
+  __device__ void prefetch(int j, bool has) {
+    if (!has) return;
+    xj = d->x[j];
+    cj = d->c[j];
+    lj = d->l[j];
+    uj = d->u[j];
+    x0j = d->x0[j];
+  }
   __device__ void operator()(int j, bool has, double s) const {
     if (!has) return;
-    const double xj = d->x[j];
-    const double h = clampv<double>(xj - tau * (d->c[j] - s), d->l[j], d->u[j]);
+    const double h = clampv<double>(xj - tau * (cj - s), lj, uj);
     if (hat) d->xh[j] = h;
     d->xb[j] = 2 * h - xj;
-    d->x[j] = tw * (a * h + b * xj) + (1 - tw) * d->x0[j];
+    d->x[j] = tw * (a * h + b * xj) + (1 - tw) * x0j;
   }
 };
 
@@ -88,16 +98,23 @@ struct SrvEpiDual {
   const Inst* d;
   double sg, tw, a, b;
   int hat;
+  double yi = 0.0, rli = 0.0, rui = 0.0, y0i = 0.0;
+  __device__ void prefetch(int i, bool has) {
+    if (!has) return;
+    yi = d->y[i];
+    rli = d->rl[i];
+    rui = d->ru[i];
+    y0i = d->y0[i];
+  }
   __device__ void operator()(int i, bool has, double s) const {
     if (!has) return;
-    const double yi = d->y[i];
     const double q = s - yi / sg;
-    const double h = yi - sg * s + sg * clampv<double>(q, d->rl[i], d->ru[i]);
+    const double h = yi - sg * s + sg * clampv<double>(q, rli, rui);
     if (hat) {
       d->yh[i] = h;
       d->yb[i] = 2 * h - yi;
     }
-    d->y[i] = tw * (a * h + b * yi) + (1 - tw) * d->y0[i];
+    d->y[i] = tw * (a * h + b * yi) + (1 - tw) * y0i;
   }
 };
 
@@ -108,10 +125,17 @@ struct SrvEpiDeltaP {
   const Inst* d;
   float tau, tw, a, b;
   int hat;
+  float xj = 0.0f, gj = 0.0f, lj = 0.0f, uj = 0.0f;
+  __device__ void prefetch(int j, bool has) {
+    if (!has) return;
+    xj = d->ix[j];
+    gj = d->G[j];
+    lj = d->LX[j];
+    uj = d->UX[j];
+  }
   __device__ void operator()(int j, bool has, double s) const {
     if (!has) return;
-    const float xj = d->ix[j];
-    const float h = clampv<float>(xj - tau * (d->G[j] - static_cast<float>(s)), d->LX[j], d->UX[j]);
+    const float h = clampv<float>(xj - tau * (gj - static_cast<float>(s)), lj, uj);
     const float xb = 2 * h - xj;
     d->ixb[j] = xb;
     d->ix[j] = tw * (a * h + b * xj);
@@ -127,11 +151,18 @@ struct SrvEpiDeltaD {
   const Inst* d;
   float sg, tw, a, b;
   int hat;
+  float yi = 0.0f, kyi = 0.0f, bui = 0.0f, bli = 0.0f;
+  __device__ void prefetch(int i, bool has) {
+    if (!has) return;
+    yi = d->iy[i];
+    kyi = d->KY[i];
+    bui = d->BU[i];
+    bli = d->BL[i];
+  }
   __device__ void operator()(int i, bool has, double s) const {
     if (!has) return;
     const float ax = static_cast<float>(s);
-    const float yi = d->iy[i];
-    const float t = clampv<float>(d->KY[i] + yi / sg, d->BU[i] + ax, d->BL[i] + ax);
+    const float t = clampv<float>(kyi + yi / sg, bui + ax, bli + ax);
     const float h = yi - sg * t;
     d->iy[i] = tw * (a * h + b * yi);
     if (hat) {
