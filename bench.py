@@ -408,6 +408,12 @@ def bench_c4(args, ps26119, lpgen):
     lat = np.array([r.stats["latency"] for r in res])
     ok = sum(r.status == ps26119.OPTIMAL for r in res)
     log(f"served: {ok}/{N} optimal in {served:.2f}s, p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+    if args.c4_served_only:
+        line = base_line(args, N / served, "solves/s", True, "f32+f64" if args.c4_precision == "mixed" else "f64",
+                         {"workload": f"serving: {N} requests, served part only", "requests": N})
+        line.update({"roofline": None, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
+                     "p99_latency_s": float(np.percentile(lat, 99)), "cpu_baseline": None})
+        return line
     # sequential plain fp64 solves of the same requests, back to back
     t = time.time()
     seq_lat = []
@@ -468,6 +474,7 @@ def main():
                     help="c4 (at 2e5 rows a single request needs minutes: the default keeps the stream short)")
     ap.add_argument("--budget-gb", type=float, default=16.0, help="c4: HBM byte budget of the server")
     ap.add_argument("--c4-precision", default="mixed", choices=["fp64", "mixed"], help="c4: precision of the served requests")
+    ap.add_argument("--c4-served-only", action="store_true", help="c4: skip the sequential comparison (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU: the r2HPDHG path here runs on a single MI355X; --gpus {args.gpus} "
