@@ -610,7 +610,8 @@ struct psh_server {
   double* h_stats = nullptr;  // pinned
   bool tables_dirty = true;
   bool param_pending = false;
-  std::map<long long, hipGraphExec_t> graphs;
+  std::map<long long, hipGraphExec_t> graphs;  // by chunk length, all captured with graph_rho
+  double graph_rho = 1.0;
 };
 
 namespace {
@@ -1030,6 +1031,11 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
     SRV_LAUNCHED(s, "serving iteration kernels");
     return PSH_OK;
   };
+  if (rho != s->graph_rho) {  // the captured launches carry 2ρ and 1 − 2ρ
+    SRV_TRY(s, hipStreamSynchronize(s->stream));
+    drop_graphs(s);
+    s->graph_rho = rho;
+  }
   const long long key = iterations;
   auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {

@@ -163,6 +163,8 @@ void Server::prepare(Request& r) {
   r.eo = engine_options_from(r.options);
   if (auto bad = apply_engine_params(r.options, r.eo); !bad.empty())
     return reject(Status::NotSolved, "unknown engine parameter '" + bad + "'");
+  if (r.eo.reflection != 1.0)  // the shared chunk runs every instance with one reflection
+    return reject(Status::NotSolved, "the server runs r2HPDHG with reflection 1; solve this request alone");
   r.sp = std::make_unique<ScaledProblem>(make_scaled_problem(r.model, r.eo.scaling));
   const ScaledProblem& sp = *r.sp;
   if (sp.A.nnz() >= (1LL << 31)) return reject(Status::NotSolved, "matrix beyond the 32-bit device index range");

@@ -84,8 +84,10 @@ def test_rejections_finish_at_once(gpu_available):
         r1 = srv.wait(srv.submit(crossed), timeout=60)
         r2 = srv.wait(srv.submit(big), timeout=60)
         r3 = srv.wait(srv.submit(hm["wyndor"]), timeout=60)
+        r4 = srv.wait(srv.submit(hm["wyndor"], params={"reflection": 0.5}), timeout=60)
     assert r1.status == ps26119.INFEASIBLE and "bounds cross" in r1.message
     assert r2.status == ps26119.NOT_SOLVED and "budget" in r2.message
+    assert r4.status == ps26119.NOT_SOLVED and "reflection" in r4.message
     assert r3.status == ps26119.OPTIMAL and abs(r3.objective - 36.0) <= 1e-6
 
 
