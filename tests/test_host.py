@@ -182,6 +182,24 @@ def test_no_device_is_not_solved_with_reason():
     assert r.status == ps26119.NOT_SOLVED and "HIP backend unavailable" in r.message
 
 
+def test_engine_parameters_are_checked_before_the_device():
+    """Every engine knob the extension header documents is accepted by the parser, which runs
+    before the device is opened (so this holds without a GPU); an unknown name is refused with
+    that name in the message, as the reference's solve() does."""
+    import ps26119
+    text = open(os.path.join(ROOT, "include", "ps26119", "ps26119_amd.h")).read()
+    block = text[text.index("Named engine knobs"):text.index("typedef struct")]
+    names = sorted(set(re.findall(r'"([a-z_0-9]+)"', block)))
+    assert {"hip_max_epoch", "hip_max_epoch_gap_only", "hip_refine_max_epoch", "hip_cpu_order"} <= set(names)
+    lp = hand_models()["wyndor"]
+    for name in names:
+        value = 0.0 if name == "hip_device" else 1.0
+        r = ps26119.solve_model(lp, params={name: value}, iteration_limit=64)
+        assert "unknown engine parameter" not in r.message, (name, r.message)
+    r = ps26119.solve_model(lp, params={"hip_max_epochs": 8192.0})
+    assert r.status == ps26119.NOT_SOLVED and "hip_max_epochs" in r.message
+
+
 def test_version_and_defaults():
     import ps26119
     assert ps26119.version()
