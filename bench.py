@@ -1,6 +1,6 @@
 """Benchmark of the MI355X r²HPDHG backend.
 
-Default workload (`--workload c1`, one GPU): the random LP with a planted optimum, m = 1,000,000
+Default workload (`--workload lp1m`, one GPU): the random LP with a planted optimum, m = 1,000,000
 rows, n = 2,000,000 columns, exactly 6 nonzeros per row (6,000,000 nnz), seed 1, fp64, solved
 through the C API toward relative KKT 1e-8 (presolve, scaling, ‖Ã‖₂, r²HPDHG on the GPU). A
 step is one r²HPDHG iteration of the solver's loop (two fused launches, the check every 64
@@ -14,20 +14,20 @@ against the 8 TB/s HBM peak. `cpu_baseline` times the reference CPU solver (orac
 host cores) for 256 iterations of the same model, in the same unit.
 
 Other workloads (each prints one JSON line of the same shape):
-  c2  refinery planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
+  refinery   planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
       versus plain fp64: iteration rates at T = 8760, and both to 1e-8 end to end at a T that
       converges in bench time; value = fp64 seconds ÷ mixed seconds, with the objective difference.
-  c3  64 scenarios sharing one 100,000 × 200,000 random matrix solved as one batch (SpMM);
+  scenarios  64 scenarios sharing one 100,000 × 200,000 random matrix solved as one batch (SpMM);
       value = scenario-iterations per second (fp64; mixed beside it) against one LP alone, plus an
       end-to-end run at 10,000 rows with b, c and u scaled by lognormal(0, 0.05) factors (warm
       start, 1e-6) against sequential single solves, and the 64 cost scenarios at 10,000 rows to
-      1e-6, fp64 batch against mixed batch.
-  c4  serving: Poisson arrivals of random LPs with m log-uniform in [m_lo, m_hi], half at 1e-6
+      1e-6 and to 1e-8, fp64 batch against mixed batch.
+  serving    Poisson arrivals of random LPs with m log-uniform in [m_lo, m_hi], half at 1e-6
       and half at 1e-8, admitted under an HBM byte budget, served in mixed precision (increment
-      form; --c4-precision fp64 for plain); value = solves per second, with p50 / p99 latency,
+      form; --serving-precision fp64 for plain); value = solves per second, with p50 / p99 latency,
       against sequential plain fp64 solves of the same requests.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload c1|c2|c3|c4] [--m M]
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload lp1m|refinery|scenarios|serving] [--m M]
                     [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
                     [--epoch-cap N] [--to-tolerance]
                     [--iteration-limit N]   (profiling: same per-launch work, N iterations)
@@ -120,8 +120,8 @@ def epoch_knobs(args):
     return {"hip_max_epoch": args.epoch_cap, "hip_max_epoch_gap_only": 1} if args.epoch_cap > 0 else {}
 
 
-# ------------------------------------------------------------------ c1
-def bench_c1(args, ps26119, lpgen):
+# ------------------------------------------------------------------ lp1m
+def bench_lp1m(args, ps26119, lpgen):
     """One step = one r²HPDHG iteration of the solver's own loop on the 1M-row LP (two fused
     launches, plus the check every 64 iterations and the restarts, exactly as a solve runs them).
     `value` = algorithmic HBM bytes of the iterations ÷ the wall time of the iteration loop
@@ -196,8 +196,8 @@ def bench_c1(args, ps26119, lpgen):
     return line
 
 
-# ------------------------------------------------------------------ c2
-def bench_c2(args, ps26119, lpgen):
+# ------------------------------------------------------------------ refinery
+def bench_refinery(args, ps26119, lpgen):
     """Refinery planning LP, mixed precision in increment form vs plain fp64.
     (a) at T = --T periods (8760: 429,240 × 525,600): the iteration rate of both (loop wall time of
         --steps iterations, setup excluded) and the HBM rate of the increment-form primal kernel;
@@ -221,7 +221,7 @@ def bench_c2(args, ps26119, lpgen):
         with open(traffic_file) as f:  # HBM bytes per launch from the two rocprofv3 --pmc passes
             roof["traffic"] = json.load(f)["traffic"].get("k_delta_primal<float>")
         roof["traffic_source"] = "profiles/k_delta_refinery8760_pmc_traffic.md"
-    if args.c2_rate_only:
+    if args.refinery_rate_only:
         line = base_line(args, rate["mixed"], "iterations/s", True, "f32+f64",
                          {"workload": f"refinery planning LP at T={args.T}: iteration rates only"})
         line.update({"roofline": roof, "iterations_per_s": rate, "cpu_baseline": None})
@@ -253,8 +253,8 @@ def bench_c2(args, ps26119, lpgen):
     return line
 
 
-# ------------------------------------------------------------------ c3
-def bench_c3(args, ps26119, lpgen):
+# ------------------------------------------------------------------ scenarios
+def bench_scenarios(args, ps26119, lpgen):
     """Three measurements, all bounded:
     (a) at the full size (100,000 × 200,000, 64 scenarios) the batch's rate in scenario-iterations
         per second, fp64 and mixed, against one LP alone (its loop time) — `value`. The rate runs
@@ -270,8 +270,8 @@ def bench_c3(args, ps26119, lpgen):
     m = args.m if args.m != 1_000_000 else 100_000
     base = lpgen.random_lp(m, seed=args.seed)
     scen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)]
-    log(f"c3 base m={base.m} n={base.n} nnz={base.nnz}, {len(scen)} cost scenarios")
-    lo, hi = 64, args.c3_iterations
+    log(f"scenarios: base m={base.m} n={base.n} nnz={base.nnz}, {len(scen)} cost scenarios")
+    lo, hi = 64, args.scenario_iterations
 
     def timed(fn):
         t = time.time()
@@ -298,7 +298,7 @@ def bench_c3(args, ps26119, lpgen):
     b32 = (8 * nz + 4 * (base.n + 1) + base.n * W * 24 + nz * W * 4) + (8 * nz + 4 * (base.m + 1) + base.m * W * 20 +
                                                                         nz * W * 4)
     loop_gbs = {"fp64": b64 * batch_rate / W / 1e9, "mixed": b32 * mixed_rate / W / 1e9}
-    if args.c3_rate_only:
+    if args.scenario_rate_only:
         line = base_line(args, batch_rate, "scenario-iterations/s", True, "f64",
                          {"workload": f"{W} cost scenarios sharing a random {base.m}x{base.n} matrix: batch rates only"})
         line.update({"mixed_scenario_iterations_per_s": mixed_rate, "loop_GBs": loop_gbs, "roofline": None,
@@ -376,8 +376,8 @@ def bench_c3(args, ps26119, lpgen):
     return line
 
 
-# ------------------------------------------------------------------ c4
-def bench_c4(args, ps26119, lpgen):
+# ------------------------------------------------------------------ serving
+def bench_serving(args, ps26119, lpgen):
     rng = np.random.default_rng(args.seed)
     N = args.requests
     ms = np.exp(rng.uniform(np.log(args.m_lo), np.log(args.m_hi), N)).astype(int)
@@ -385,7 +385,7 @@ def bench_c4(args, ps26119, lpgen):
     gaps = rng.exponential(1.0 / args.rate, N)
     arrivals = np.cumsum(gaps) - gaps[0]
     lps = [lpgen.random_lp(int(m), seed=1000 + k) for k, m in enumerate(ms)]
-    log(f"c4: {N} requests, m in [{ms.min()}, {ms.max()}], rate {args.rate}/s, budget {args.budget_gb} GB")
+    log(f"serving: {N} requests, m in [{ms.min()}, {ms.max()}], rate {args.rate}/s, budget {args.budget_gb} GB")
     srv = ps26119.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_instances=256, setup_threads=8)
     ids = [None] * N
 
@@ -396,7 +396,7 @@ def bench_c4(args, ps26119, lpgen):
             if d > 0:
                 time.sleep(d)
             ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=args.time_limit,
-                                precision=args.c4_precision)
+                                precision=args.serving_precision)
 
     t = time.time()
     th = threading.Thread(target=feed)
@@ -408,8 +408,8 @@ def bench_c4(args, ps26119, lpgen):
     lat = np.array([r.stats["latency"] for r in res])
     ok = sum(r.status == ps26119.OPTIMAL for r in res)
     log(f"served: {ok}/{N} optimal in {served:.2f}s, p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
-    if args.c4_served_only:
-        line = base_line(args, N / served, "solves/s", True, "f32+f64" if args.c4_precision == "mixed" else "f64",
+    if args.serving_only:
+        line = base_line(args, N / served, "solves/s", True, "f32+f64" if args.serving_precision == "mixed" else "f64",
                          {"workload": f"serving: {N} requests, served part only", "requests": N})
         line.update({"roofline": None, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
                      "p99_latency_s": float(np.percentile(lat, 99)), "cpu_baseline": None})
@@ -423,7 +423,7 @@ def bench_c4(args, ps26119, lpgen):
     sequential = time.time() - t
     log(f"sequential: {sequential:.2f}s")
     value = N / served
-    mixed = args.c4_precision == "mixed"
+    mixed = args.serving_precision == "mixed"
     line = base_line(args, value, "solves/s", True, "f32+f64" if mixed else "f64",
                      {"workload": f"serving: {N} Poisson arrivals at {args.rate}/s of random LPs (6 per row, n=2m), m "
                                   f"log-uniform in [{args.m_lo}, {args.m_hi}], half at tolerance 1e-6 and half at 1e-8, "
@@ -444,9 +444,9 @@ def bench_c4(args, ps26119, lpgen):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16384, help="c1: r2HPDHG iterations timed (multiple of 64)")
+    ap.add_argument("--steps", type=int, default=16384, help="lp1m: r2HPDHG iterations timed (multiple of 64)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="lp1m", choices=["lp1m", "refinery", "scenarios", "serving"])
     ap.add_argument("--m", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--time-limit", type=float, default=900.0)
@@ -454,27 +454,27 @@ def main():
                     help="stop each timed solve after this many iterations (profiling runs; 0 = to 1e-8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--epoch-cap", type=int, default=8192,
-                    help="c1 and c3's cost scenarios: hip_max_epoch with hip_max_epoch_gap_only "
+                    help="lp1m and the cost scenarios of scenarios: hip_max_epoch with hip_max_epoch_gap_only "
                          "(0 = the reference's restart schedule alone)")
     ap.add_argument("--to-tolerance", action="store_true",
-                    help="c1: run the timed solve to 1e-8 (bounded by --time-limit) and report time_to_1e-8_s")
+                    help="lp1m: run the timed solve to 1e-8 (bounded by --time-limit) and report time_to_1e-8_s")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
-                    help="c1: mixed = fp32 increments on fp64 anchors (the line's dtype says which)")
-    ap.add_argument("--T", type=int, default=8760, help="c2: planning periods of the rate measurement")
-    ap.add_argument("--T-small", type=int, default=24, help="c2: planning periods of the end-to-end comparison")
-    ap.add_argument("--c2-rate-only", action="store_true", help="c2: only the rates at --T (profiling runs)")
-    ap.add_argument("--scenarios", type=int, default=64, help="c3")
-    ap.add_argument("--seq-sample", type=int, default=4, help="c3: sequential solves timed for the comparison")
-    ap.add_argument("--c3-iterations", type=int, default=16384, help="c3: iterations of the rate measurement")
-    ap.add_argument("--c3-rate-only", action="store_true", help="c3: only the batch rates (profiling runs)")
-    ap.add_argument("--requests", type=int, default=48, help="c4")
-    ap.add_argument("--rate", type=float, default=8.0, help="c4: arrivals per second")
-    ap.add_argument("--m-lo", type=int, default=1000, help="c4")
+                    help="lp1m: mixed = fp32 increments on fp64 anchors (the line's dtype says which)")
+    ap.add_argument("--T", type=int, default=8760, help="refinery: planning periods of the rate measurement")
+    ap.add_argument("--T-small", type=int, default=24, help="refinery: planning periods of the end-to-end comparison")
+    ap.add_argument("--refinery-rate-only", action="store_true", help="refinery: only the rates at --T (profiling runs)")
+    ap.add_argument("--scenarios", type=int, default=64, help="scenarios")
+    ap.add_argument("--seq-sample", type=int, default=4, help="scenarios: sequential solves timed for the comparison")
+    ap.add_argument("--scenario-iterations", type=int, default=16384, help="scenarios: iterations of the rate measurement")
+    ap.add_argument("--scenario-rate-only", action="store_true", help="scenarios: only the batch rates (profiling runs)")
+    ap.add_argument("--requests", type=int, default=48, help="serving")
+    ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
+    ap.add_argument("--m-lo", type=int, default=1000, help="serving")
     ap.add_argument("--m-hi", type=int, default=20000,
-                    help="c4 (at 2e5 rows a single request needs minutes: the default keeps the stream short)")
-    ap.add_argument("--budget-gb", type=float, default=16.0, help="c4: HBM byte budget of the server")
-    ap.add_argument("--c4-precision", default="mixed", choices=["fp64", "mixed"], help="c4: precision of the served requests")
-    ap.add_argument("--c4-served-only", action="store_true", help="c4: skip the sequential comparison (profiling runs)")
+                    help="serving (at 2e5 rows a single request needs minutes: the default keeps the stream short)")
+    ap.add_argument("--budget-gb", type=float, default=16.0, help="serving: HBM byte budget of the server")
+    ap.add_argument("--serving-precision", default="mixed", choices=["fp64", "mixed"], help="serving: precision of the served requests")
+    ap.add_argument("--serving-only", action="store_true", help="serving: skip the sequential comparison (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU: the r2HPDHG path here runs on a single MI355X; --gpus {args.gpus} "
@@ -486,7 +486,7 @@ def main():
     if ps26119.device_count() <= 0:
         log("no GPU visible to the HIP backend")
         sys.exit(1)
-    fn = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload]
+    fn = {"lp1m": bench_lp1m, "refinery": bench_refinery, "scenarios": bench_scenarios, "serving": bench_serving}[args.workload]
     print(json.dumps(fn(args, ps26119, lpgen)), flush=True)
 
 

@@ -147,7 +147,7 @@ Solution solve_impl(const Model& model, const Options& options) {
   if (pr.outcome == PresolveResult::Outcome::Unchanged) return solve_direct(model, options, true);
   Options inner = options;
   inner.presolve = false;
-  {  // judge the reduced solve with the ORIGINAL model's norms (reference solve.cpp:282-294)
+  {  // assess the reduced solve with the ORIGINAL model's norms (reference solve.cpp:282-294)
     double b2 = 0, c2 = 0;
     for (int i = 0; i < model.num_rows; ++i) {
       const double lo = std::isfinite(model.row_lower[i]) ? std::fabs(model.row_lower[i]) : 0.0;
