@@ -251,6 +251,7 @@ typedef struct {
   int id;
   double tau, sigma;
   long long inner0;  /* Halpern index of the chunk's first iteration; 0 = the chunk begins an epoch */
+  double rho;        /* reflection ρ of the instance (reference src/pdhg/engine.h:40, 1 = full) */
 } psh_serve_param;
 int psh_server_create(int device, size_t budget_bytes, int max_instances, psh_server** out);
 void psh_server_destroy(psh_server* s);
@@ -268,7 +269,7 @@ int psh_server_add(psh_server* s, const psh_problem* p, int flags, int* id); /* 
 int psh_server_set_precision(psh_server* s, int id, int mixed);
 int psh_server_remove(psh_server* s, int id);
 int psh_server_set_iterate(psh_server* s, int id, const double* x, const double* y);
-int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count, double rho);
+int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count);
 /* want_ray: also psh_stats.ray (T(z) − z) of every listed instance, and its rays for
  * psh_server_ray_vectors (unscaled; dual ray m, primal ray n). */
 int psh_server_check(psh_server* s, const int* ids, int count, int want_ray, psh_stats* out);

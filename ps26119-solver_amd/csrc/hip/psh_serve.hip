@@ -53,6 +53,7 @@ struct Inst {
 
 struct Param {
   double tau, sigma, inner0;
+  double a2, b2;  // 2ρ and 1 − 2ρ of the instance's reflection ρ
   int active, restart;
   int mixed;   // 1: this chunk runs the instance in increment form
   int anchor;  // anchor pass before the chunk: 1 KY only (σ changed), 2 every constant and δ = 0
@@ -174,8 +175,7 @@ struct SrvEpiDeltaD {
 };
 
 __global__ void __launch_bounds__(kBlock) k_srv_primal(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
-                                                       const Param* __restrict__ P, int step, int last, double a2,
-                                                       double b2) {
+                                                       const Param* __restrict__ P, int step, int last) {
   const int t = tile_of_block(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const Tile T = tiles[t];
@@ -184,18 +184,17 @@ __global__ void __launch_bounds__(kBlock) k_srv_primal(const Tile* __restrict__ 
   const Inst* d = I + T.inst;
   const int hat = (step == 0 && q.inner0 == 0.0) || last ? 1 : 0;
   if (q.mixed) {
-    SrvEpiDeltaP e{d, static_cast<float>(q.tau), static_cast<float>(halpern_w(q.inner0, step)), static_cast<float>(a2),
-                   static_cast<float>(b2), hat};
+    SrvEpiDeltaP e{d, static_cast<float>(q.tau), static_cast<float>(halpern_w(q.inner0, step)), static_cast<float>(q.a2),
+                   static_cast<float>(q.b2), hat};
     tile_rows<float, float>(T.r0, T.r1, d->at_ptr, d->at_col, d->at_valf, d->iy, e);
     return;
   }
-  SrvEpiPrimal e{d, q.tau, halpern_w(q.inner0, step), a2, b2, hat};
+  SrvEpiPrimal e{d, q.tau, halpern_w(q.inner0, step), q.a2, q.b2, hat};
   tile_rows<double, double>(T.r0, T.r1, d->at_ptr, d->at_col, d->at_val, d->y, e);
 }
 
 __global__ void __launch_bounds__(kBlock) k_srv_dual(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
-                                                     const Param* __restrict__ P, int step, int last, double a2,
-                                                     double b2) {
+                                                     const Param* __restrict__ P, int step, int last) {
   const int t = tile_of_block(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const Tile T = tiles[t];
@@ -205,11 +204,11 @@ __global__ void __launch_bounds__(kBlock) k_srv_dual(const Tile* __restrict__ ti
   const int hat = (step == 0 && q.inner0 == 0.0) || last ? 1 : 0;
   if (q.mixed) {
     SrvEpiDeltaD e{d, static_cast<float>(q.sigma), static_cast<float>(halpern_w(q.inner0, step)),
-                   static_cast<float>(a2), static_cast<float>(b2), hat};
+                   static_cast<float>(q.a2), static_cast<float>(q.b2), hat};
     tile_rows<float, float>(T.r0, T.r1, d->a_ptr, d->a_col, d->a_valf, d->ixb, e);
     return;
   }
-  SrvEpiDual e{d, q.sigma, halpern_w(q.inner0, step), a2, b2, hat};
+  SrvEpiDual e{d, q.sigma, halpern_w(q.inner0, step), q.a2, q.b2, hat};
   tile_rows<double, double>(T.r0, T.r1, d->a_ptr, d->a_col, d->a_val, d->xb, e);
 }
 
@@ -610,8 +609,7 @@ struct psh_server {
   double* h_stats = nullptr;  // pinned
   bool tables_dirty = true;
   bool param_pending = false;
-  std::map<long long, hipGraphExec_t> graphs;  // by chunk length, all captured with graph_rho
-  double graph_rho = 1.0;
+  std::map<long long, hipGraphExec_t> graphs;  // by chunk length
 };
 
 namespace {
@@ -969,7 +967,7 @@ int psh_server_set_precision(psh_server* s, int id, int mixed) {
 }
 
 // `count` iterations for the listed instances (others pause). params[k] = {id, τ, σ, inner0}.
-int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count, double rho) {
+int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count) {
   if (!s || iterations < 1 || count < 0 || (count > 0 && !params)) return PSH_E_ARG;
   int rc;
   if ((rc = wait_param(s))) return rc;
@@ -978,10 +976,13 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
     const int id = params[k].id;
     if (id < 0 || id >= s->max_inst || !s->slots[id].used) return sfail(s, PSH_E_ARG, "bad instance id");
     if (params[k].inner0 < 0 || params[k].inner0 > (1LL << 52)) return sfail(s, PSH_E_ARG, "epoch counter out of range");
+    if (!std::isfinite(params[k].rho)) return sfail(s, PSH_E_ARG, "reflection is not finite");
     Param& q = s->h_param[id];
     q.tau = params[k].tau;
     q.sigma = params[k].sigma;
     q.inner0 = static_cast<double>(params[k].inner0);
+    q.a2 = 2 * params[k].rho;
+    q.b2 = 1 - 2 * params[k].rho;
     q.active = 1;
   }
   bool any_anchor = false;
@@ -1006,16 +1007,15 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
                          s->d_inst, s->d_param);
     SRV_LAUNCHED(s, "serving anchors");
   }
-  const double a2 = 2 * rho, b2 = 1 - 2 * rho;
   auto enqueue = [&]() -> int {
     for (int st = 0; st < iterations; ++st) {
       const int last = st == iterations - 1 ? 1 : 0;
       if (s->ntp > 0)
         hipLaunchKernelGGL(k_srv_primal, dim3(grid_tiles(s->ntp)), dim3(kBlock), 0, s->stream, s->d_ptiles, s->ntp,
-                           s->d_inst, s->d_param, st, last, a2, b2);
+                           s->d_inst, s->d_param, st, last);
       if (s->ntd > 0)
         hipLaunchKernelGGL(k_srv_dual, dim3(grid_tiles(s->ntd)), dim3(kBlock), 0, s->stream, s->d_dtiles, s->ntd,
-                           s->d_inst, s->d_param, st, last, a2, b2);
+                           s->d_inst, s->d_param, st, last);
       if (st == 0) {  // residual after the first step of instances that begin an epoch (r2hpdhg.cpp:75)
         if (s->ntp > 0)
           hipLaunchKernelGGL(k_srv_cols1, dim3(grid_tiles(s->ntp)), dim3(kBlock), 0, s->stream, s->d_ptiles, s->ntp,
@@ -1031,11 +1031,6 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
     SRV_LAUNCHED(s, "serving iteration kernels");
     return PSH_OK;
   };
-  if (rho != s->graph_rho) {  // the captured launches carry 2ρ and 1 − 2ρ
-    SRV_TRY(s, hipStreamSynchronize(s->stream));
-    drop_graphs(s);
-    s->graph_rho = rho;
-  }
   const long long key = iterations;
   auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {

@@ -163,8 +163,6 @@ void Server::prepare(Request& r) {
   r.eo = engine_options_from(r.options);
   if (auto bad = apply_engine_params(r.options, r.eo); !bad.empty())
     return reject(Status::NotSolved, "unknown engine parameter '" + bad + "'");
-  if (r.eo.reflection != 1.0)  // the shared chunk runs every instance with one reflection
-    return reject(Status::NotSolved, "the server runs r2HPDHG with reflection 1; solve this request alone");
   r.sp = std::make_unique<ScaledProblem>(make_scaled_problem(r.model, r.eo.scaling));
   const ScaledProblem& sp = *r.sp;
   if (sp.A.nnz() >= (1LL << 31)) return reject(Status::NotSolved, "matrix beyond the 32-bit device index range");
@@ -355,10 +353,10 @@ void Server::serve_loop() {
     params.clear();
     ids.clear();
     for (auto& r : live) {
-      params.push_back(psh_serve_param{r->slot, r->eta / r->pid.omega, r->eta * r->pid.omega, r->inner});
+      params.push_back(psh_serve_param{r->slot, r->eta / r->pid.omega, r->eta * r->pid.omega, r->inner, r->eo.reflection});
       ids.push_back(r->slot);
     }
-    int rc = psh_server_chunk(dev_, K, params.data(), static_cast<int>(params.size()), 1.0);
+    int rc = psh_server_chunk(dev_, K, params.data(), static_cast<int>(params.size()));
     bool want_ray = false;  // ray tests from an instance's fourth check on (reference r2hpdhg.cpp:93-111)
     for (auto& r : live) want_ray = want_ray || r->it + K >= 4 * K;
     if (rc == PSH_OK) {

@@ -84,10 +84,8 @@ def test_rejections_finish_at_once(gpu_available):
         r1 = srv.wait(srv.submit(crossed), timeout=60)
         r2 = srv.wait(srv.submit(big), timeout=60)
         r3 = srv.wait(srv.submit(hm["wyndor"]), timeout=60)
-        r4 = srv.wait(srv.submit(hm["wyndor"], params={"reflection": 0.5}), timeout=60)
     assert r1.status == ps26119.INFEASIBLE and "bounds cross" in r1.message
     assert r2.status == ps26119.NOT_SOLVED and "budget" in r2.message
-    assert r4.status == ps26119.NOT_SOLVED and "reflection" in r4.message
     assert r3.status == ps26119.OPTIMAL and abs(r3.objective - 36.0) <= 1e-6
 
 
@@ -157,6 +155,28 @@ def test_mixed_request_hands_over_to_fp64_on_a_stall(gpu_available):
     print(f"served mixed, patience 1: it={r.iterations} {r.message}", flush=True)
     assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
     assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
+
+
+def test_reflection_is_per_request(gpu_available):
+    """Each request runs with its own reflection ρ (engine parameter "reflection"), carried per
+    instance into the shared launches: a ρ = 0.5 request and a default one in the same stream both
+    end Optimal at their planted optimum, each with the bits it gives alone, and the ρ = 0.5
+    trajectory differs from the default one."""
+    import ps26119
+    from ps26119_amd import lpgen
+    lp = lpgen.random_lp(1200, seed=8)
+    half = {"reflection": 0.5}
+    with ps26119.Server(budget_bytes=1 << 30) as srv:
+        alone_half = srv.wait(srv.submit(lp, params=half), timeout=300)
+        alone_full = srv.wait(srv.submit(lp), timeout=300)
+        ids = [srv.submit(lp, params=half), srv.submit(lp)]
+        mix_half, mix_full = [srv.wait(i, timeout=300) for i in ids]
+    for r in (alone_half, alone_full, mix_half, mix_full):
+        assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+        assert abs(r.objective - lp.optimum) <= 2e-8 * (1 + abs(lp.optimum))
+    assert alone_half.iterations == mix_half.iterations and fnv(alone_half.x) == fnv(mix_half.x)
+    assert alone_full.iterations == mix_full.iterations and fnv(alone_full.x) == fnv(mix_full.x)
+    assert fnv(alone_half.x) != fnv(alone_full.x)
 
 
 def test_epoch_capped_requests_reach_the_optima(gpu_available):
