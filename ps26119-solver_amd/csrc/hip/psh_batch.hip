@@ -261,6 +261,18 @@ __global__ void __launch_bounds__(kBlock) k_bdprimalP(BArgs a) {
   const long long j = g / a.W;
   if (j >= a.n) return;
   const int t = static_cast<int>(g - j * a.W);
+  // the row's operands first, so their loads are in flight during the gathers (same values)
+  const size_t q = static_cast<size_t>(g);
+  float xq[P], G[P], LX[P], UX[P], h[P], xb[P], xn[P], w[P], tau[P];
+  ldf<P>(a.b.ix + q, xq);
+  ldf<P>(a.b.G + q, G);
+  ldf<P>(a.b.LX + q, LX);
+  ldf<P>(a.b.UX + q, UX);
+#pragma unroll
+  for (int e = 0; e < P; ++e) {
+    w[e] = static_cast<float>(halpern_w(a.b.inner0[t + e], a.it));
+    tau[e] = static_cast<float>(a.b.tau[t + e]);
+  }
   double s[P];
 #pragma unroll
   for (int e = 0; e < P; ++e) s[e] = 0.0;
@@ -271,21 +283,13 @@ __global__ void __launch_bounds__(kBlock) k_bdprimalP(BArgs a) {
 #pragma unroll
     for (int e = 0; e < P; ++e) s[e] += v * static_cast<double>(y[e]);
   }
-  const size_t q = static_cast<size_t>(g);
-  float xq[P], G[P], LX[P], UX[P], h[P], xb[P], xn[P];
-  ldf<P>(a.b.ix + q, xq);
-  ldf<P>(a.b.G + q, G);
-  ldf<P>(a.b.LX + q, LX);
-  ldf<P>(a.b.UX + q, UX);
   const float r2 = static_cast<float>(a.rho2), r1 = static_cast<float>(a.rho1);
 #pragma unroll
   for (int e = 0; e < P; ++e) {
-    const float w = static_cast<float>(halpern_w(a.b.inner0[t + e], a.it));
-    const float tau = static_cast<float>(a.b.tau[t + e]);
-    const float v = xq[e] - tau * (G[e] - static_cast<float>(s[e]));
+    const float v = xq[e] - tau[e] * (G[e] - static_cast<float>(s[e]));
     h[e] = v < LX[e] ? LX[e] : (v > UX[e] ? UX[e] : v);
     xb[e] = 2 * h[e] - xq[e];
-    xn[e] = w * (r2 * h[e] + r1 * xq[e]);
+    xn[e] = w[e] * (r2 * h[e] + r1 * xq[e]);
   }
   if (a.hat) stf<P>(a.b.ixh + q, h);
   stf<P>(a.b.ixb + q, xb);
@@ -298,6 +302,18 @@ __global__ void __launch_bounds__(kBlock) k_bddualP(BArgs a) {
   const long long i = g / a.W;
   if (i >= a.m) return;
   const int t = static_cast<int>(g - i * a.W);
+  // the row's operands first, so their loads are in flight during the gathers (same values)
+  const size_t q = static_cast<size_t>(g);
+  float yq[P], KY[P], BL[P], BU[P], h[P], yb[P], yn[P], w[P], sgs[P];
+  ldf<P>(a.b.iy + q, yq);
+  ldf<P>(a.b.KY + q, KY);
+  ldf<P>(a.b.BL + q, BL);
+  ldf<P>(a.b.BU + q, BU);
+#pragma unroll
+  for (int e = 0; e < P; ++e) {
+    w[e] = static_cast<float>(halpern_w(a.b.inner0[t + e], a.it));
+    sgs[e] = static_cast<float>(a.b.sigma[t + e]);
+  }
   double s[P];
 #pragma unroll
   for (int e = 0; e < P; ++e) s[e] = 0.0;
@@ -308,24 +324,17 @@ __global__ void __launch_bounds__(kBlock) k_bddualP(BArgs a) {
 #pragma unroll
     for (int e = 0; e < P; ++e) s[e] += v * static_cast<double>(x[e]);
   }
-  const size_t q = static_cast<size_t>(g);
-  float yq[P], KY[P], BL[P], BU[P], h[P], yb[P], yn[P];
-  ldf<P>(a.b.iy + q, yq);
-  ldf<P>(a.b.KY + q, KY);
-  ldf<P>(a.b.BL + q, BL);
-  ldf<P>(a.b.BU + q, BU);
   const float r2 = static_cast<float>(a.rho2), r1 = static_cast<float>(a.rho1);
 #pragma unroll
   for (int e = 0; e < P; ++e) {
-    const float w = static_cast<float>(halpern_w(a.b.inner0[t + e], a.it));
-    const float sg = static_cast<float>(a.b.sigma[t + e]);
+    const float sg = sgs[e];
     const float ax = static_cast<float>(s[e]);
     const float lo = BU[e] + ax, hi = BL[e] + ax;
     float v = KY[e] + yq[e] / sg;
     v = v < lo ? lo : (v > hi ? hi : v);
     h[e] = yq[e] - sg * v;
     yb[e] = 2 * h[e] - yq[e];
-    yn[e] = w * (r2 * h[e] + r1 * yq[e]);
+    yn[e] = w[e] * (r2 * h[e] + r1 * yq[e]);
   }
   if (a.hat) {
     stf<P>(a.b.iyh + q, h);
