@@ -93,13 +93,7 @@ __global__ void __launch_bounds__(kBlock) k_bprimal2(BArgs a) {
   const long long j = g / a.W;
   if (j >= a.n) return;
   const int t = static_cast<int>(g - j * a.W);
-  double s0 = 0.0, s1 = 0.0;
-  for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) {
-    const double v = a.at_val[p];
-    const double2 y = *reinterpret_cast<const double2*>(a.b.y + static_cast<size_t>(a.at_col[p]) * a.W + t);
-    s0 += v * y.x;
-    s1 += v * y.y;
-  }
+  // the row's operands first, so their loads are in flight during the gathers (same values)
   const size_t q = static_cast<size_t>(g);
   const double2 xq = *reinterpret_cast<const double2*>(a.b.x + q);
   const double2 c = *reinterpret_cast<const double2*>(a.b.c + q);
@@ -107,9 +101,17 @@ __global__ void __launch_bounds__(kBlock) k_bprimal2(BArgs a) {
   const double2 u = *reinterpret_cast<const double2*>(a.b.u + q);
   const double2 x0 = *reinterpret_cast<const double2*>(a.b.x0 + q);
   const double w0 = halpern_w(a.b.inner0[t], a.it), w1 = halpern_w(a.b.inner0[t + 1], a.it);
+  const double tau0 = a.b.tau[t], tau1 = a.b.tau[t + 1];
+  double s0 = 0.0, s1 = 0.0;
+  for (int p = a.at_ptr[j]; p < a.at_ptr[j + 1]; ++p) {
+    const double v = a.at_val[p];
+    const double2 y = *reinterpret_cast<const double2*>(a.b.y + static_cast<size_t>(a.at_col[p]) * a.W + t);
+    s0 += v * y.x;
+    s1 += v * y.y;
+  }
   double2 h, xb, xn;
-  h.x = clampd(xq.x - a.b.tau[t] * (c.x - s0), l.x, u.x);
-  h.y = clampd(xq.y - a.b.tau[t + 1] * (c.y - s1), l.y, u.y);
+  h.x = clampd(xq.x - tau0 * (c.x - s0), l.x, u.x);
+  h.y = clampd(xq.y - tau1 * (c.y - s1), l.y, u.y);
   xb.x = 2 * h.x - xq.x;
   xb.y = 2 * h.y - xq.y;
   xn.x = w0 * (a.rho2 * h.x + a.rho1 * xq.x) + (1 - w0) * x0.x;
@@ -124,13 +126,7 @@ __global__ void __launch_bounds__(kBlock) k_bdual2(BArgs a) {
   const long long i = g / a.W;
   if (i >= a.m) return;
   const int t = static_cast<int>(g - i * a.W);
-  double s0 = 0.0, s1 = 0.0;
-  for (int p = a.a_ptr[i]; p < a.a_ptr[i + 1]; ++p) {
-    const double v = a.a_val[p];
-    const double2 x = *reinterpret_cast<const double2*>(a.b.xb + static_cast<size_t>(a.a_col[p]) * a.W + t);
-    s0 += v * x.x;
-    s1 += v * x.y;
-  }
+  // the row's operands first, so their loads are in flight during the gathers (same values)
   const size_t q = static_cast<size_t>(g);
   const double2 yq = *reinterpret_cast<const double2*>(a.b.y + q);
   const double2 rl = *reinterpret_cast<const double2*>(a.b.rl + q);
@@ -138,6 +134,13 @@ __global__ void __launch_bounds__(kBlock) k_bdual2(BArgs a) {
   const double2 y0 = *reinterpret_cast<const double2*>(a.b.y0 + q);
   const double w0 = halpern_w(a.b.inner0[t], a.it), w1 = halpern_w(a.b.inner0[t + 1], a.it);
   const double sg0 = a.b.sigma[t], sg1 = a.b.sigma[t + 1];
+  double s0 = 0.0, s1 = 0.0;
+  for (int p = a.a_ptr[i]; p < a.a_ptr[i + 1]; ++p) {
+    const double v = a.a_val[p];
+    const double2 x = *reinterpret_cast<const double2*>(a.b.xb + static_cast<size_t>(a.a_col[p]) * a.W + t);
+    s0 += v * x.x;
+    s1 += v * x.y;
+  }
   double2 h, yb, yn;
   h.x = yq.x - sg0 * s0 + sg0 * clampd(s0 - yq.x / sg0, rl.x, ru.x);
   h.y = yq.y - sg1 * s1 + sg1 * clampd(s1 - yq.y / sg1, rl.y, ru.y);
