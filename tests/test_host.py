@@ -206,3 +206,17 @@ def test_version_and_defaults():
     opt = ps26119.Options()
     ps26119.lib().ps26119_default_options(ctypes.byref(opt))
     assert (opt.algorithm, opt.precision, opt.use_gpu, opt.tolerance, opt.threads) == (0, 0, 0, 1e-8, 1)
+
+
+def test_bench_refuses_more_than_one_gpu_and_lists_its_workloads():
+    """bench.py measures one GPU: --gpus 2 exits with a message before touching a device; --help
+    names the four workloads."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
+    h = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
+                       timeout=120)
+    assert h.returncode == 0
+    for name in ("lp1m", "refinery", "scenarios", "serving", "--epoch-cap", "--to-tolerance"):
+        assert name in h.stdout, name
