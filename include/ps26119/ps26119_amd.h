@@ -22,7 +22,8 @@ extern "C" {
  * 65536; 0 = no limit), "hip_max_epoch" (longest epoch in any precision: a restart to the
  * averaged point is forced when the epoch reaches it; default 0 = the reference's restart schedule
  * alone), "hip_max_epoch_gap_only" (1 = hip_max_epoch applies only while the relative gap is the
- * largest KKT term), "hip_bf16_until" (> 0: the increment form starts with bf16 matrix values
+ * largest KKT term; 2 = also only once the gap has not improved by 10% for hip_max_epoch
+ * iterations, single solves only, experimental), "hip_bf16_until" (> 0: the increment form starts with bf16 matrix values
  * and moves to fp32 values at this relative KKT error or on a stall; default 0 = off),
  * "hip_device_scaling" (1 = default: the scaling sweeps run on the device, same bits as the host).
  * presolve: 1 (default) / 0. */

@@ -65,7 +65,7 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_bf16_until") e.bf16_until = v > 0 ? v : 0.0;
     else if (name == "hip_refine_max_epoch") e.refine_max_epoch = v > 0 ? static_cast<std::int64_t>(v) : 0;
     else if (name == "hip_max_epoch") e.max_epoch = v > 0 ? static_cast<std::int64_t>(v) : 0;
-    else if (name == "hip_max_epoch_gap_only") e.max_epoch_gap_only = v != 0 ? 1 : 0;
+    else if (name == "hip_max_epoch_gap_only") e.max_epoch_gap_only = v >= 2 ? 2 : (v != 0 ? 1 : 0);
     else return name;
   }
   if (e.refine < 0) e.refine = e.cpu_order ? 0 : 1;
@@ -701,6 +701,8 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
   const int K = opt.check_every;
   std::int64_t it = 0, inner = 0;
   double r0 = -1.0, r_last = kInf;
+  double gap_best = kInf;  // hip_max_epoch_gap_only = 2: the cap waits for the gap to stall
+  std::int64_t gap_best_it = 0;
   std::string msg = warm ? "warm start" : "";
   if (policy.refining())
     msg += std::string(msg.empty() ? "" : "; ") + "fp32 increments on fp64 anchors" +
@@ -749,8 +751,10 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
     }
     const bool promoted = policy.on_check(k.rel_kkt(), it);
     if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
+    if (k.rel_gap() < 0.9 * gap_best) gap_best = k.rel_gap(), gap_best_it = it;
+    const bool cap_ok = opt.max_epoch_gap_only != 2 || it - gap_best_it >= opt.max_epoch;
     const bool restart = promoted || restart_due(opt, r, r0, r_last, inner, it) ||
-                         increment_epoch_full(opt, policy.refining(), inner) || epoch_cap_reached(opt, inner, k);
+                         increment_epoch_full(opt, policy.refining(), inner) || (cap_ok && epoch_cap_reached(opt, inner, k));
     r_last = r;
     if (!restart) continue;
 

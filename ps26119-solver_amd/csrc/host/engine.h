@@ -54,7 +54,8 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   // reference: on the random LPs with a planted optimum the artificial restart (inner ≥ 0.36·it)
   // lets epochs grow with the iteration count and the relative gap stalls near 3e-7 at 1M rows;
   // shorter epochs keep closing it. hip_max_epoch_gap_only = 1 applies the cap only while the
-  // relative gap is the largest KKT term, so long epochs return once feasibility is what is left.
+  // relative gap is the largest KKT term, so long epochs return once feasibility is what is left;
+  // 2 (single solves) also waits until the gap has not improved by 10% for max_epoch iterations.
   std::int64_t max_epoch = 0;
   int max_epoch_gap_only = 0;
   // hip_bf16_until: > 0 runs the increment form's SpMVs with bf16 matrix values until the relative
