@@ -12,10 +12,13 @@ extern "C" {
 
 /* Named engine knobs (the reference's Options::engine_params, options.h:70-82 there), e.g.
  * "restart_sufficient", "pid_kp", ..., plus the MI355X ones: "hip_cpu_order" (1 = every
- * reduction in the reference CPU order, bit-identical trajectory), "hip_timing" (1 = per-kernel
+ * reduction in the reference CPU order, bit-identical trajectory and KKT figures; by default the
+ * sums that restart and primal-weight decisions read follow the reference order and the KKT
+ * sums are device trees, which keeps the reference's iteration counts), "hip_tree_reductions"
+ * (1 = every sum a device tree: a deterministic trajectory of its own), "hip_timing" (1 = per-kernel
  * HIP event timing into ps26119_amd_stats), "hip_graphs" (1 = default: replay captured HIP
  * graphs of the iteration chunks; timing runs launch eagerly), "hip_device", "hip_sliced"
- * (1 = default: column-sliced dual half step for large fp64 tree-mode solves), "hip_refine"
+ * (1 = default: column-sliced dual half step for large fp64 solves with hip_tree_reductions), "hip_refine"
  * (mixed precision as fp32 increments on fp64 anchors; default 1, 0 in CPU-order mode: the
  * reference's fp32-then-fp64 policy), "hip_refine_patience" (stalled checks before fp64 takes
  * over, default 40), "hip_refine_max_epoch" (longest epoch of the increment form, default

@@ -16,7 +16,11 @@
  *  - Reductions accumulate in fp64. mode PSH_RED_TREE = fixed-shape tree (deterministic, any
  *    size); PSH_RED_CPU = the reference CPU order (64 fixed chunks for dot / norms, exactly
  *    src/la/parallel.h:169-188; single sequential loops for the KKT sums,
- *    src/pdhg/cpu_backend.cpp:240-274), bit-identical to the CPU backend.
+ *    src/pdhg/cpu_backend.cpp:240-274, and the Lanczos dots, src/la/csr.cpp:117-123),
+ *    bit-identical to the CPU backend; PSH_RED_DECISION = the reference order for every sum a
+ *    restart or primal-weight decision reads (residual dots, PID distances, Lanczos dots) and
+ *    trees for the KKT and ray-test sums, so the trajectory is the CPU backend's while a check
+ *    costs about what a tree check costs.
  */
 #ifndef PS26119_HIP_H
 #define PS26119_HIP_H
@@ -36,7 +40,7 @@ enum {
   PSH_E_STATE = -5        /* call out of order (e.g. before psh_load) */
 };
 
-enum { PSH_RED_TREE = 0, PSH_RED_CPU = 1 };
+enum { PSH_RED_TREE = 0, PSH_RED_CPU = 1, PSH_RED_DECISION = 2 };
 
 /* Vector slots. Primal (size n): X, X0, XH, XB, DX, ATY. Dual (size m): Y, Y0, YH, YB, DY, AX. */
 enum { PSH_X = 0, PSH_X0, PSH_XH, PSH_XB, PSH_DX, PSH_ATY, PSH_Y, PSH_Y0, PSH_YH, PSH_YB, PSH_DY, PSH_AX, PSH_NSLOT };
@@ -182,6 +186,10 @@ int psh_restart(psh_ctx* ctx); /* x ← x̂, x0 ← x̂, y ← ŷ, y0 ← ŷ */
 int psh_lanczos_begin(psh_ctx* ctx, const double* v0);
 int psh_lanczos_step(psh_ctx* ctx, double beta_prev, int mode, double* alpha, double* beta_sq);
 int psh_lanczos_advance(psh_ctx* ctx, double beta);
+/* w = ÃᵀÃ v for a host vector v (n), returned in w (n): the two products of one Lanczos step
+ * (CSR order, the bits of the host products), for drivers that take the step's dots on the host
+ * in the reference's sequential order. */
+int psh_lanczos_product(psh_ctx* ctx, const double* v, double* w);
 
 /* ---- batched scenarios sharing Ã (reference src/pdhg/batch.cpp:129-441), on a context that has
  * the shared scaled problem loaded (psh_load). Block vectors are interleaved: entry (row r,

@@ -158,53 +158,77 @@ __global__ void __launch_bounds__(kBlock) k_tree2(const RedJob* __restrict__ job
   if (threadIdx.x == 0) results[first + blockIdx.x] = red[0];
 }
 
-// s + a[b] + a[b+1] + ... + a[e-1], strictly left to right; eight loads are issued ahead of
-// the dependent additions so one lane keeps several cache lines in flight.
-__device__ inline double seq_sum(const double* __restrict__ a, long long b, long long e, double s) {
-  long long i = b;
-  for (; i + 8 <= e; i += 8) {
-    const double v0 = a[i], v1 = a[i + 1], v2 = a[i + 2], v3 = a[i + 3], v4 = a[i + 4], v5 = a[i + 5], v6 = a[i + 6],
-                 v7 = a[i + 7];
-    s += v0;
-    s += v1;
-    s += v2;
-    s += v3;
-    s += v4;
-    s += v5;
-    s += v6;
-    s += v7;
+// s + a[b] + a[b+1] + ... + a[e-1], strictly left to right, by one wavefront: the 64 lanes load
+// 512 consecutive values (coalesced) into LDS, issue the loads of the next 512, and meanwhile every
+// lane adds the staged values in order (a uniform dependent chain, so the lanes agree). The last
+// batch is padded with +0.0: the running sum is never −0.0 (it starts at +0.0 or at a sum that
+// started there, and x + (−x) rounds to +0.0), so adding +0.0 leaves it unchanged bit for bit.
+constexpr int kSeqBatch = 512;
+__device__ inline double wave_seq_sum(const double* __restrict__ a, long long b, long long e, double s,
+                                      double* __restrict__ stage) {
+  const int lane = static_cast<int>(threadIdx.x);
+  double v[kSeqBatch / 64];
+#pragma unroll
+  for (int u = 0; u < kSeqBatch / 64; ++u) {
+    const long long i = b + u * 64 + lane;
+    v[u] = i < e ? a[i] : 0.0;
   }
-  for (; i < e; ++i) s += a[i];
+  for (long long base = b; base < e; base += kSeqBatch) {
+#pragma unroll
+    for (int u = 0; u < kSeqBatch / 64; ++u) stage[u * 64 + lane] = v[u];
+    __syncthreads();
+    const long long next = base + kSeqBatch;
+#pragma unroll
+    for (int u = 0; u < kSeqBatch / 64; ++u) {
+      const long long i = next + u * 64 + lane;
+      v[u] = i < e ? a[i] : 0.0;
+    }
+#pragma unroll 16
+    for (int k = 0; k < kSeqBatch; ++k) s += stage[k];
+    __syncthreads();
+  }
   return s;
 }
 
 // CPU-order sums: mode 1 = 64 fixed chunks (la/parallel.h:169-188), mode 2 = one sequential loop
-// over a then b (cpu_backend.cpp:240-274). One 64-thread workgroup per job.
-__global__ void __launch_bounds__(64) k_ordered(const RedJob* __restrict__ jobs, double* __restrict__ results, int first) {
-  const RedJob J = jobs[blockIdx.x];
+// over a then b (cpu_backend.cpp:240-274, csr.cpp:117-123). Grid (64, jobs), one wavefront per
+// (chunk, job); chunk sums go to partials[job][chunk] and k_ordered_final adds them in order.
+__global__ void __launch_bounds__(64) k_ordered(const RedJob* __restrict__ jobs, double* __restrict__ partials) {
+  const RedJob J = jobs[blockIdx.y];
   if (J.op != 0 || J.mode == 0) return;
+  __shared__ double stage[kSeqBatch];
+  double* out = partials + static_cast<size_t>(blockIdx.y) * kTreeBlocks + blockIdx.x;
   if (J.mode == 2) {
-    if (threadIdx.x == 0) {
-      double s = seq_sum(J.a, 0, J.na, 0.0);
-      s = seq_sum(J.b, 0, J.nb, s);
-      results[first + blockIdx.x] = s;
-    }
+    if (blockIdx.x != 0) return;
+    double s = wave_seq_sum(J.a, 0, J.na, 0.0, stage);
+    s = wave_seq_sum(J.b, 0, J.nb, s, stage);
+    if (threadIdx.x == 0) *out = s;
     return;
   }
-  __shared__ double part[64];
   const long long n = J.na;
   const int chunks = static_cast<int>(n < 1 ? 1 : (n > 64 ? 64 : n));
-  const int c = threadIdx.x;
-  if (c < chunks) {
-    const long long b = n * c / chunks, e = n * (c + 1) / chunks;
-    part[c] = seq_sum(J.a, b, e, 0.0);
+  const int c = static_cast<int>(blockIdx.x);
+  if (c >= chunks) return;
+  const long long b = n * c / chunks, e = n * (c + 1) / chunks;
+  const double s = wave_seq_sum(J.a, b, e, 0.0, stage);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void __launch_bounds__(64) k_ordered_final(const RedJob* __restrict__ jobs,
+                                                      const double* __restrict__ partials, double* __restrict__ results,
+                                                      int first) {
+  const RedJob J = jobs[blockIdx.x];
+  if (J.op != 0 || J.mode == 0 || threadIdx.x != 0) return;
+  const double* p = partials + static_cast<size_t>(blockIdx.x) * kTreeBlocks;
+  if (J.mode == 2) {
+    results[first + blockIdx.x] = p[0];
+    return;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < chunks; ++k) s += part[k];
-    results[first + blockIdx.x] = s;
-  }
+  const long long n = J.na;
+  const int chunks = static_cast<int>(n < 1 ? 1 : (n > 64 ? 64 : n));
+  double s = 0.0;
+  for (int k = 0; k < chunks; ++k) s += p[k];
+  results[first + blockIdx.x] = s;
 }
 
 static int launch_reduce(psh_ctx* c, const RedJob* dev, const RedJob* host, int njobs, int first) {
@@ -217,7 +241,11 @@ static int launch_reduce(psh_ctx* c, const RedJob* dev, const RedJob* host, int 
     hipLaunchKernelGGL(k_tree1, dim3(kTreeBlocks, njobs), dim3(kBlock), 0, c->stream, dev, c->partials);
     hipLaunchKernelGGL(k_tree2, dim3(njobs), dim3(kBlock), 0, c->stream, dev, c->partials, c->results, first);
   }
-  if (ordered) hipLaunchKernelGGL(k_ordered, dim3(njobs), dim3(64), 0, c->stream, dev, c->results, first);
+  // ordered jobs use their own rows of `partials` (a job is either a tree job or an ordered one)
+  if (ordered) {
+    hipLaunchKernelGGL(k_ordered, dim3(64, njobs), dim3(64), 0, c->stream, dev, c->partials);
+    hipLaunchKernelGGL(k_ordered_final, dim3(njobs), dim3(64), 0, c->stream, dev, c->partials, c->results, first);
+  }
   PSH_LAUNCHED(c, "reduction kernels");
   return PSH_OK;
 }
@@ -241,8 +269,9 @@ int reduce_table(psh_ctx* c, const JobTable& t, int first) {
 
 // Sum jobs of a check pass. CPU-order mode follows the reference: 64-chunk sums for the
 // residual dots and the PID distances (parallel_sum), single loops for the KKT sums with the
-// row part followed by the column part in one accumulator (cpu_backend.cpp:240-274). The ray
-// sums are trees in both modes.
+// row part followed by the column part in one accumulator (cpu_backend.cpp:240-274). Decision
+// mode keeps the 64-chunk order for the sums the restart and primal-weight decisions read and
+// takes the KKT sums as trees. The ray sums are trees in every mode.
 const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err) {
   const int key = (mode << 8) | static_cast<int>(flags);
   auto it = c->tables.find(key);
@@ -251,7 +280,7 @@ const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err) {
   const long long m = c->m, n = c->n, L = c->term_len;
   auto T = [&](int k) { return c->terms + static_cast<size_t>(k) * L; };
   const bool cpu = mode == PSH_RED_CPU;
-  const int chunk = cpu ? 1 : 0, seq = cpu ? 2 : 0;
+  const int chunk = mode != PSH_RED_TREE ? 1 : 0, seq = cpu ? 2 : 0;
   // Every slot R_PX..R_CMAG2 gets a job (unused ones sum nothing) so results keep their index.
   auto job = [&](int k, long long na, int md, int k2 = -1, long long nb = 0) {
     t.host.push_back(RedJob{T(k), k2 >= 0 ? T(k2) : nullptr, na, nb, 0, md});
@@ -1005,7 +1034,7 @@ static int dot_like(psh_ctx* c, int a, int b, int mode, int diff, double* out) {
     return PSH_OK;
   });
   if (rc) return rc;
-  RedJob j{c->terms, nullptr, len, 0, 0, mode == PSH_RED_CPU ? 1 : 0};
+  RedJob j{c->terms, nullptr, len, 0, 0, mode != PSH_RED_TREE ? 1 : 0};
   if ((rc = reduce(c, &j, 1, 0))) return rc;
   return fetch_results(c, 1, out);
 }
@@ -1112,14 +1141,39 @@ int psh_lanczos_step(psh_ctx* c, double beta_prev, int mode, double* alpha, doub
                        c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, c->At.val, c->lt, c->lw);
   hipLaunchKernelGGL((k_dot_terms<double>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->lw, c->lv, c->terms, n, 0);
   PSH_LAUNCHED(c, "lanczos products");
-  // α = wᵀv as one sequential loop in CPU mode (csr.cpp:117), tree otherwise
-  RedJob j{c->terms, nullptr, n, 0, 0, mode == PSH_RED_CPU ? 2 : 0};
+  // α = wᵀv as one sequential loop in CPU and decision modes (csr.cpp:117), tree otherwise
+  RedJob j{c->terms, nullptr, n, 0, 0, mode != PSH_RED_TREE ? 2 : 0};
   int rc;
   if ((rc = reduce(c, &j, 1, 0)) || (rc = fetch_results(c, 1, alpha))) return rc;
   hipLaunchKernelGGL(k_lanczos_update, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->lw, c->lv, c->lvp, *alpha,
                      beta_prev, c->terms, n);
   PSH_LAUNCHED(c, "lanczos update");
   if ((rc = reduce(c, &j, 1, 0)) || (rc = fetch_results(c, 1, beta_sq))) return rc;
+  return PSH_OK;
+}
+
+int psh_lanczos_product(psh_ctx* c, const double* v, double* w) {
+  if (!c || !v || !w) return PSH_E_ARG;
+  if (!c->loaded) return fail(c, PSH_E_STATE, "no problem loaded");
+  int rc;
+  if (!c->lv && ((rc = alloc_n(c, &c->lv, c->n)) || (rc = alloc_n(c, &c->lvp, c->n)) || (rc = alloc_n(c, &c->lw, c->n)) ||
+                 (rc = alloc_n(c, &c->lt, c->m))))
+    return rc;
+  if (c->n == 0) return PSH_OK;
+  PSH_TRY(c, hipMemcpyAsync(c->lv, v, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  if (c->m > 0 && c->A.ntiles > 0)
+    hipLaunchKernelGGL((k_spmv<double, double>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
+                       c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, c->A.val, c->lv, c->lt);
+  else if (c->m > 0)
+    hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, c->lt, 0.0, (long long)c->m);
+  if (c->At.ntiles > 0)
+    hipLaunchKernelGGL((k_spmv<double, double>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
+                       c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, c->At.val, c->lt, c->lw);
+  else
+    hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->lw, 0.0, (long long)c->n);
+  PSH_LAUNCHED(c, "lanczos products");
+  PSH_TRY(c, hipMemcpyAsync(w, c->lw, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+  PSH_TRY(c, hipStreamSynchronize(c->stream));
   return PSH_OK;
 }
 

@@ -107,7 +107,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
     if (eo.timing) dev->check(psh_timing(c, 1), "timing");
     dev->check(psh_graphs(c, eo.graphs), "graphs");
     const bool empty = n == 0 || sp.A.nnz() == 0;
-    const double norm = empty ? 0.0 : estimate_norm2(*dev, n, eo.cpu_order);
+    const double norm = empty ? 0.0 : estimate_norm2(*dev, n, eo.cpu_order ? PSH_RED_CPU : PSH_RED_TREE);
     const double eta = norm > 0 ? 0.998 / norm : 1.0;
     const double setup_shared = elapsed();
     const int mode = eo.cpu_order ? PSH_RED_CPU : PSH_RED_TREE;

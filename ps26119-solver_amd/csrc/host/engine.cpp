@@ -55,6 +55,7 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "ruiz_iterations") e.scaling.ruiz_iterations = static_cast<int>(v);
     else if (name == "pock_chambolle") e.scaling.pock_chambolle = v != 0;
     else if (name == "hip_cpu_order") e.cpu_order = v != 0 ? 1 : 0;
+    else if (name == "hip_tree_reductions") e.tree = v != 0 ? 1 : 0;
     else if (name == "hip_device") e.device = static_cast<int>(v);
     else if (name == "hip_timing") e.timing = v != 0 ? 1 : 0;
     else if (name == "hip_graphs") e.graphs = v != 0 ? 1 : 0;
@@ -70,6 +71,10 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
   }
   if (e.refine < 0) e.refine = e.cpu_order ? 0 : 1;
   return {};
+}
+
+int reduction_mode(const EngineOptions& o) {
+  return o.cpu_order ? PSH_RED_CPU : (o.tree ? PSH_RED_TREE : PSH_RED_DECISION);
 }
 
 // ------------------------------------------------------------------ KKT
@@ -364,9 +369,31 @@ double lanczos_norm(int n, int max_iter, double rel_tol, Begin&& begin, Step&& s
 
 }  // namespace
 
-double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter, double rel_tol) {
+double estimate_norm2(Device& dev, int n, int mode, int max_iter, double rel_tol) {
   psh_ctx* c = dev.ctx();
-  const int mode = cpu_order ? PSH_RED_CPU : PSH_RED_TREE;
+  if (mode != PSH_RED_TREE) {
+    // The reference's sequential dots (csr.cpp:117-123) on a host core, which runs a dependent
+    // fp64 add chain about 4x faster than one device lane; the products stay on the device.
+    std::vector<double> v, vp(n, 0.0), w(n);
+    return lanczos_norm(
+        n, max_iter, rel_tol, [&](const std::vector<double>& v0) { v = v0; },
+        [&](double bp, double& al, double& bsq) {
+          dev.check(psh_lanczos_product(c, v.data(), w.data()), "lanczos");
+          al = 0.0;
+          for (int j = 0; j < n; ++j) al += w[j] * v[j];
+          bsq = 0.0;
+          for (int j = 0; j < n; ++j) {
+            w[j] -= al * v[j] + bp * vp[j];
+            bsq += w[j] * w[j];
+          }
+        },
+        [&](double bn) {
+          for (int j = 0; j < n; ++j) {
+            vp[j] = v[j];
+            v[j] = w[j] / bn;
+          }
+        });
+  }
   return lanczos_norm(
       n, max_iter, rel_tol, [&](const std::vector<double>& v) { dev.check(psh_lanczos_begin(c, v.data()), "lanczos"); },
       [&](double bp, double& al, double& bsq) { dev.check(psh_lanczos_step(c, bp, mode, &al, &bsq), "lanczos"); },
@@ -418,7 +445,7 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
     dev_->check(psh_sync(dev_->ctx()), "load");
     const double t_load = elapsed();
     const bool empty = sp_.n == 0 || sp_.A.nnz() == 0;
-    const double norm = empty ? 0.0 : estimate_norm2(*dev_, sp_.n, opt.cpu_order);
+    const double norm = empty ? 0.0 : estimate_norm2(*dev_, sp_.n, reduction_mode(opt));
     eta_ = norm > 0 ? 0.998 / norm : 1.0;
     dev_->check(psh_sync(dev_->ctx()), "setup");
     setup_seconds_ = elapsed();
@@ -432,7 +459,7 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
   psh_ctx* ctx() { return dev_->ctx(); }
   const ScaledProblem& sp() const { return sp_; }
   double eta() const { return eta_; }
-  int mode() const { return opt_.cpu_order ? PSH_RED_CPU : PSH_RED_TREE; }
+  int mode() const { return reduction_mode(opt_); }
   double elapsed() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count(); }
   bool out_of_time() const { return elapsed() > opt_.time_limit; }
   double initial_primal_weight() const {
@@ -760,7 +787,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
 
     // PID primal weight (reference r2hpdhg.cpp:127-148)
     double dxn = st.dx_norm, dyn = st.dy_norm;
-    if (mode == PSH_RED_CPU || promoted) {
+    if (mode != PSH_RED_TREE || promoted) {
       dev.check(psh_diff_norm2(c, PSH_XH, PSH_X0, mode, &dxn), "pid");
       dev.check(psh_diff_norm2(c, PSH_YH, PSH_Y0, mode, &dyn), "pid");
     }

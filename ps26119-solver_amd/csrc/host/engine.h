@@ -32,11 +32,16 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   double pid_max_log_step = 2.302585092994046;
   double pdlp_restart_necessary = 0.8;        // PDLP engine (reference engine.h:46-47)
   double pdlp_primal_weight_smoothing = 0.5;  // θ
-  // MI355X knobs (engine_params names hip_cpu_order, hip_device):
-  //   cpu_order = 1: every reduction in the reference CPU order (bit-identical trajectory to the
-  //   reference CPU backend, slower checks); 0: fixed-shape device trees for the KKT sums and the
-  //   residual / PID norms. The Lanczos dots follow the same switch.
+  // MI355X knobs (engine_params names hip_cpu_order, hip_tree_reductions, hip_device):
+  //   cpu_order = 1: every reduction in the reference CPU order (bit-identical trajectory and KKT
+  //   figures to the reference CPU backend; the KKT sums are long sequential loops).
+  //   Default (both 0): the sums that restart and primal-weight decisions read (residual dots,
+  //   PID distances, Lanczos dots) in the reference order and the KKT sums as device trees: the
+  //   same η, restarts, ω and iteration counts as the reference; KKT figures within rounding.
+  //   tree = 1: every sum a fixed-shape device tree (and, at large m, the column-sliced dual
+  //   half step): a deterministic trajectory of its own.
   int cpu_order = 0;
+  int tree = 0;
   int device = 0;
   int timing = 0;  // hip_timing: per-kernel HIP event timing into the Solution
   int graphs = 1;  // hip_graphs: replay captured chunk graphs (off while timing)
@@ -64,6 +69,8 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
 };
 
 EngineOptions engine_options_from(const Options& o);
+// The psh reduction mode (PSH_RED_*) the options ask for.
+int reduction_mode(const EngineOptions& o);
 // Applies Options::engine_params; returns the first unknown name or "".
 std::string apply_engine_params(const Options& o, EngineOptions& e);
 
@@ -99,7 +106,7 @@ class Device {
 };
 
 // ‖Ã‖₂ by Lanczos on ÃᵀÃ with the device products (reference src/la/csr.cpp:86-132).
-double estimate_norm2(Device& dev, int n, int cpu_order, int max_iter = 400, double rel_tol = 1e-12);
+double estimate_norm2(Device& dev, int n, int mode, int max_iter = 400, double rel_tol = 1e-12);
 // The same iteration with host products and sequential dots (serving setup, on host threads).
 double estimate_norm2_host(const ScaledProblem& sp, int max_iter = 400, double rel_tol = 1e-12);
 

@@ -232,7 +232,10 @@ def test_batch_slot_groups_equal_single_slots(gpu_available, prec):
 def test_epoch_capped_batch_matches_single_solves(gpu_available, prec):
     """With hip_max_epoch and hip_max_epoch_gap_only every scenario of a batch decides its forced
     restarts on its own KKT terms: cost scenarios of a 2,000-row random LP come out Optimal, pass
-    the check on their own models and agree with capped single solves to 1e-6."""
+    the check on their own models and agree with capped single solves to 1e-6. The batch's sums
+    are device trees, so the single solves run with tree reductions too: the forced restarts
+    follow the trajectory, and on scenario 0 the cap stalls on the decision-mode trajectory
+    (3M iterations) while it converges on the tree-mode one."""
     import ps26119
     from ps26119_amd import lpgen
     base = lpgen.random_lp(2000, seed=4)
@@ -241,7 +244,8 @@ def test_epoch_capped_batch_matches_single_solves(gpu_available, prec):
     res = ps26119.solve_model_batch(base, scen, precision=prec, params=knobs, iteration_limit=3_000_000)
     for k, (r, sc) in enumerate(zip(res, scen)):
         mk = _with(base, sc)
-        single = ps26119.solve_model(mk, precision=prec, params=knobs, iteration_limit=3_000_000)
+        single = ps26119.solve_model(mk, precision=prec, params={**knobs, "hip_tree_reductions": 1},
+                                     iteration_limit=3_000_000)
         print(f"capped batch {prec} scenario {k}: it={r.iterations} single it={single.iterations}", flush=True)
         assert r.status == ps26119.OPTIMAL == single.status, (k, r.message)
         assert abs(r.objective - single.objective) <= 1e-6 * (1 + abs(single.objective)), k
