@@ -162,9 +162,12 @@ int psh_ray_vectors(psh_ctx* ctx, int drift, double* dual_ray, double* primal_ra
  * HIP graph and replayed. */
 int psh_chunk(psh_ctx* ctx, const psh_chunk_args* a);
 int psh_graphs(psh_ctx* ctx, int on); /* 1 (default): replay captured chunk graphs */
-/* 1 (default): for m >= 65536, fp64 chunks with tree reductions run the dual half step over
- * column slices of Ã (each XCD gathers from one L2-sized slice of x̄; a row's slice sums are added
- * in slice order, so its sum is rounded differently from the CSR-order sum of CPU-order mode). */
+/* Phased products for fp64 chunks: persistent workgroups sweep column slices of Ã (Ãᵀ) in order,
+ * so the workgroups of an XCD gather from one L2-sized slice of x̄ (y) at a time while every row's
+ * running sum keeps the CSR order: the same bits as the row-tiled kernels, in every reduction
+ * mode. 1 (default): built at psh_load for a matrix whose gathered vector exceeds 4 MiB and used
+ * by fp64 chunks; 0: row tiles only; 2: built at any size (tests). Call before psh_load for 0 and
+ * 2 to govern the build; afterwards 0 / nonzero only switches the use of what was built. */
 int psh_sliced(psh_ctx* ctx, int on);
 /* Mixed precision in increment form (no counterpart in the reference; its mixed mode is
  * psh_set_fp32): the iterate is the fp64 anchor in X0 / Y0 plus an fp32 increment, the matrix

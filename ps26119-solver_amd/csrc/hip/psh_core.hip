@@ -608,40 +608,54 @@ int spmv_impl(psh_ctx* c, const DevMat& M, int xs, int out) {
 
 namespace {
 
-// A CSR's entries regrouped by (row tile, column slice) for the sliced half step with fp64 tree
-// reductions (built when rows >= kSliceMinRows and every tile fits one LDS pass): slice k holds the
-// columns [k·w, (k+1)·w), w = ⌈cols / S⌉; tiles are padded to a multiple of 8/S so that the
-// workgroups of one slice land on the same XCDs. Values are gathered from M.val on the device.
-int build_sliced(psh_ctx* c, Sliced& S, const DevMat& M, const std::vector<int>& tiles, const int* ptr, const int* col,
-                 int rows, int cols) {
-  const long long nz = ptr[rows];
-  bool short_tiles = true;  // a slice segment must fit the LDS product buffer
-  for (int t = 0; t < M.ntiles && short_tiles; ++t) short_tiles = ptr[tiles[t + 1]] - ptr[tiles[t]] <= kTileNnz;
-  if (rows < kSliceMinRows || nz == 0 || M.ntiles == 0 || !short_tiles) return PSH_OK;
-  const int per = kXcd / kSlices;
-  S.slices = kSlices;
-  S.ntp = (M.ntiles + per - 1) / per * per;
-  const long long width = (static_cast<long long>(cols) + kSlices - 1) / kSlices;
-  std::vector<int> seg(static_cast<size_t>(S.ntp) * kSlices + 1, 0), scol(nz), ssrc(nz);
-  std::vector<unsigned char> srow(nz);
-  size_t q = 0;
-  for (int t = 0; t < S.ntp; ++t) {
-    const int r0 = t < M.ntiles ? tiles[t] : rows, r1 = t < M.ntiles ? tiles[t + 1] : rows;
-    for (int k = 0; k < kSlices; ++k) {
+// The phased layout of a CSR (struct Phased), built when the gathered vector (cols fp64 values)
+// is larger than an XCD's L2 share (kPhMinBytes) or when `force` is set: S phases (slices of
+// about 2 MiB, at least 8), K rows per thread so that the workgroups stay resident together
+// (about 1024 of 512 threads), J product slots per thread for the largest (workgroup, phase)
+// block. Left unbuilt (the row tiles serve) when a block or a row's entries in one phase do not
+// fit (more than kPhMaxJ·kPhBlock entries, or 255 in one row).
+int build_phased(psh_ctx* c, Phased& P, const int* ptr, const int* col, const double* vals_dev, int rows, int cols,
+                 bool force) {
+  const long long nz = rows > 0 ? ptr[rows] : 0;
+  const long long bytes = static_cast<long long>(cols) * 8;
+  if (nz == 0 || rows == 0 || cols == 0 || (!force && bytes <= kPhMinBytes)) return PSH_OK;
+  for (int r = 0; r < rows; ++r)
+    for (int q = ptr[r] + 1; q < ptr[r + 1]; ++q)
+      if (col[q] < col[q - 1]) return PSH_OK;  // a row with unsorted columns: row tiles only
+  const int S = bytes <= (16LL << 20) ? 8 : static_cast<int>(std::min<long long>(32, (bytes + (2LL << 20) - 1) >> 21));
+  const long long per_k = 1024LL * kPhBlock;
+  const int K = rows <= per_k ? 1 : (rows <= 2 * per_k ? 2 : 4);
+  const int R = K * kPhBlock;
+  const int G = (rows + R - 1) / R;
+  const long long width = (static_cast<long long>(cols) + S - 1) / S;
+  std::vector<int> blk(static_cast<size_t>(G) * S + 1, 0), pcol, psrc;
+  std::vector<unsigned char> cnt(static_cast<size_t>(G) * S * R, 0);
+  pcol.reserve(static_cast<size_t>(nz));
+  psrc.reserve(static_cast<size_t>(nz));
+  std::vector<int> cur(ptr, ptr + rows);
+  int maxblk = 0;
+  for (int g = 0; g < G; ++g) {
+    const int r0 = g * R, r1 = std::min(rows, r0 + R);
+    for (int p = 0; p < S; ++p) {
+      const size_t bp = static_cast<size_t>(g) * S + p;
+      const int start = static_cast<int>(pcol.size());
       for (int r = r0; r < r1; ++r) {
-        int b = ptr[r];
-        const int e = ptr[r + 1];
-        while (b < e && col[b] / width < k) ++b;
-        for (; b < e && col[b] / width == k; ++b, ++q) {
-          scol[q] = col[b];
-          ssrc[q] = b;
-          srow[q] = static_cast<unsigned char>(r - r0);
+        int k = 0;
+        for (int q = cur[r]; q < ptr[r + 1] && col[q] / width == p; ++q, ++k) {
+          pcol.push_back(col[q]);
+          psrc.push_back(q);
         }
+        cur[r] += k;
+        if (k > 255) return PSH_OK;
+        cnt[bp * R + (r - r0)] = static_cast<unsigned char>(k);
       }
-      seg[static_cast<size_t>(t) * kSlices + k + 1] = static_cast<int>(q);
+      blk[bp + 1] = static_cast<int>(pcol.size());
+      maxblk = std::max(maxblk, blk[bp + 1] - start);
+      if (maxblk > kPhMaxJ * kPhBlock) return PSH_OK;
     }
   }
-  if (static_cast<long long>(q) != nz) return fail(c, PSH_E_ARG, "column slices do not cover the matrix");
+  if (static_cast<long long>(pcol.size()) != nz) return fail(c, PSH_E_ARG, "phased layout does not cover the matrix");
+  const int J = maxblk <= 2 * kPhBlock ? 2 : (maxblk <= 4 * kPhBlock ? 4 : 8);
   int rc;
   auto put = [&](auto** dst, const auto* src, long long count) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
@@ -650,13 +664,30 @@ int build_sliced(psh_ctx* c, Sliced& S, const DevMat& M, const std::vector<int>&
     PSH_TRY(c, hipMemcpyAsync(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
     return PSH_OK;
   };
-  if ((rc = put(&S.seg, seg.data(), static_cast<long long>(seg.size()))) || (rc = put(&S.col, scol.data(), nz)) ||
-      (rc = put(&S.src, ssrc.data(), nz)) || (rc = alloc_n(c, &S.val, nz)) || (rc = put(&S.row, srow.data(), nz)) ||
-      (rc = alloc_n(c, &S.part, static_cast<long long>(kSlices) * rows)))
+  if ((rc = put(&P.blk, blk.data(), static_cast<long long>(blk.size()))) ||
+      (rc = put(&P.cnt, cnt.data(), static_cast<long long>(cnt.size()))) || (rc = put(&P.col, pcol.data(), nz)) ||
+      (rc = put(&P.src, psrc.data(), nz)) || (rc = alloc_n(c, &P.val, nz)))
     return rc;
-  hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(nz)), dim3(kBlock), 0, c->stream, M.val, S.src, S.val, nz);
-  PSH_LAUNCHED(c, "sliced values");
+  hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(nz)), dim3(kBlock), 0, c->stream, vals_dev, P.src, P.val, nz);
+  PSH_LAUNCHED(c, "phased values");
   PSH_TRY(c, hipStreamSynchronize(c->stream));  // the staging vectors go out of scope
+  P.S = S;
+  P.K = K;
+  P.J = J;
+  P.G = G;
+  P.R = R;
+  return PSH_OK;
+}
+
+// The phased copies take the current (scaled) values of Ã and Ãᵀ.
+int regather_phased(psh_ctx* c) {
+  if (c->Pa.S > 0)
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->A.val, c->Pa.src, c->Pa.val,
+                       c->nnz);
+  if (c->Pat.S > 0)
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->At.val, c->Pat.src,
+                       c->Pat.val, c->nnz);
+  PSH_LAUNCHED(c, "phased values");
   return PSH_OK;
 }
 
@@ -728,8 +759,8 @@ void psh_destroy(psh_ctx* c) {
                   c->At.tiles, c->A.valh, c->At.valh, c->a_orig, c->ato_alias ? nullptr : c->ato_col, c->ato_val, c->c, c->l, c->u, c->rl,
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
-                  c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->As.seg, c->As.col,
-                  c->As.src, c->As.val, c->As.row, c->As.part};
+                  c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->Pa.blk, c->Pa.cnt,
+                  c->Pa.col, c->Pa.src, c->Pa.val, c->Pat.blk, c->Pat.cnt, c->Pat.col, c->Pat.src, c->Pat.val};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
@@ -798,7 +829,9 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
   if ((rc = up(&c->A.tiles, ta.data(), static_cast<long long>(ta.size()))) ||
       (rc = up(&c->At.tiles, tt.data(), static_cast<long long>(tt.size()))))
     return rc;
-  if ((rc = build_sliced(c, c->As, c->A, ta, p->a_ptr, p->a_col, m, n))) return rc;
+  if (c->sliced && ((rc = build_phased(c, c->Pa, p->a_ptr, p->a_col, c->A.val, m, n, c->sliced == 2)) ||
+                    (rc = build_phased(c, c->Pat, p->at_ptr, p->at_col, c->At.val, n, m, c->sliced == 2))))
+    return rc;
   for (int s = 0; s < PSH_NSLOT; ++s)
     if ((rc = alloc_n(c, &c->vd[s], slot_len(c, s)))) return rc;
   const long long mx = std::max<long long>(std::max(m, n), 1);
@@ -872,10 +905,8 @@ int psh_scale(psh_ctx* c, int geo_iterations, int ruiz_iterations, int pock_cham
   };
   for (int it = 0; it < ruiz_iterations; ++it) sweep(1);
   if (pock_chambolle) sweep(0);
-  if (c->As.slices > 0 && c->nnz > 0)  // the sliced copies take the scaled values
-    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream, c->A.val, c->As.src,
-                       c->As.val, c->nnz);
   PSH_LAUNCHED(c, "scaling sweeps");
+  if ((rc = regather_phased(c))) return rc;
   if (row_scale && m > 0) PSH_TRY(c, hipMemcpyAsync(row_scale, c->rs, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
   if (col_scale && n > 0) PSH_TRY(c, hipMemcpyAsync(col_scale, c->cs, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   PSH_TRY(c, hipStreamSynchronize(c->stream));

@@ -138,6 +138,21 @@ __global__ void __launch_bounds__(kBlock) k_iter_dual(const int* __restrict__ ti
   tiled_rows<T, T>(tiles, ntiles, ptr, col, val, xb, e);
 }
 
+// The same two half steps over the phased layouts (large problems; same sums, same epilogues).
+template <class T, int K, int J>
+__global__ void __launch_bounds__(kPhBlock) k_ph_primal(Phased P, int rows, const T* __restrict__ val,
+                                                        const T* __restrict__ y, PrimalArgs<T> p) {
+  EpiPrimal<T> e(p);
+  phased_rows<K, J, T, T>(P, rows, val, y, e);
+}
+
+template <class T, int K, int J>
+__global__ void __launch_bounds__(kPhBlock) k_ph_dual(Phased P, int rows, const T* __restrict__ val,
+                                                      const T* __restrict__ xb, DualArgs<T> p) {
+  EpiDual<T> e(p);
+  phased_rows<K, J, T, T>(P, rows, val, xb, e);
+}
+
 // ------------------------------------------------------------------ increment form (mixed precision)
 // The iterate is held as an fp64 anchor plus an fp32 increment, z = z₀ + δ, where z₀ is the
 // epoch's restart point — which is also the Halpern anchor, so the anchor term of the averaging
@@ -269,51 +284,6 @@ __global__ void k_delta_point(const double* __restrict__ z0, const float* __rest
   zh[j] = a + static_cast<double>(dh[j]);
   zb[j] = a + static_cast<double>(db[j]);
   if (zc) zc[j] = a + static_cast<double>(dc[j]);
-}
-
-// Dual half step over column slices (fp64, tree-reduction mode): workgroup b handles slice
-// k = (b mod 8)·S/8 of tile (b / 8)·(8/S) + (b mod 8) − k·8/S, so the XCD that runs it only gathers
-// from x̄'s k-th slice; products of the (tile, slice) segment go through LDS and each thread writes
-// its row's slice sum (CSR order within the slice).
-__global__ void __launch_bounds__(kBlock) k_slice_partial(Sliced S, const int* __restrict__ tiles, int ntiles, int rows,
-                                                          const double* __restrict__ x) {
-  __shared__ double prod[kTileNnz];
-  __shared__ int rs[kBlock], re[kBlock];
-  const int xcd = static_cast<int>(blockIdx.x) % kXcd;
-  const int per = kXcd / S.slices;
-  const int k = xcd / per;
-  const int tile = static_cast<int>(blockIdx.x) / kXcd * per + (xcd - k * per);
-  rs[threadIdx.x] = 0;
-  re[threadIdx.x] = 0;
-  __syncthreads();
-  const int seg = tile * S.slices + k;
-  const int b0 = S.seg[seg], b1 = S.seg[seg + 1];
-  for (int p = b0 + static_cast<int>(threadIdx.x); p < b1; p += kBlock) {
-    prod[p - b0] = ld_stream(S.val + p) * x[ld_stream(S.col + p)];
-    const int r = S.row[p];
-    if (p == b0 || S.row[p - 1] != r) rs[r] = p - b0;
-    if (p == b1 - 1 || S.row[p + 1] != r) re[r] = p - b0 + 1;
-  }
-  __syncthreads();
-  if (tile >= ntiles) return;
-  const int r0 = tiles[tile], r1 = tiles[tile + 1];
-  const int row = r0 + static_cast<int>(threadIdx.x);
-  if (row < r1) {
-    double v = 0.0;
-    for (int q = rs[threadIdx.x]; q < re[threadIdx.x]; ++q) v += prod[q];
-    S.part[static_cast<size_t>(k) * rows + row] = v;
-  }
-}
-
-// Slice sums added in slice order, then the fused dual update (k_iter_dual's epilogue).
-__global__ void __launch_bounds__(kBlock) k_slice_dual(Sliced S, int rows, DualArgs<double> p) {
-  const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
-  EpiDual<double> e(p);
-  if (i >= rows) return;
-  e.prefetch(i, true);
-  double s = 0.0;
-  for (int k = 0; k < S.slices; ++k) s += ld_stream(S.part + static_cast<size_t>(k) * rows + i);
-  e(i, true, s);
 }
 
 template <class T>
@@ -808,8 +778,9 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
     if (!check_table(c, g->mode, F_RES, &err)) return err;
   }
   const bool delta = c->delta.on;
-  // column-sliced dual half step: fp64, tree reductions (its row sums are not in CSR order)
-  const bool use_slices = c->sliced && c->As.slices > 0 && !c->fp32 && !delta && g->mode == PSH_RED_TREE;
+  // phased products for fp64 chunks of large problems (the same sums as the row tiles)
+  const bool ph_primal = c->sliced && c->Pat.S > 0 && !c->fp32 && !delta;
+  const bool ph_dual = c->sliced && c->Pa.S > 0 && !c->fp32 && !delta;
   if (delta && c->nnz > 0 &&
       !(c->delta.bf16 ? c->A.valh && c->At.valh : c->A.valf && c->At.valf))  // the launches' value arrays
     return fail(c, PSH_E_STATE, "increment form without its matrix values");
@@ -877,25 +848,36 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
         hipEvent_t ev;
         if (c->At.ntiles > 0) {
           time_begin(c, 0, &ev);
-          hipLaunchKernelGGL((k_iter_primal<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
-                             c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, at_val, sp<T>(c, PSH_Y), pa);
+          bool done = false;
+          if constexpr (sizeof(T) == 8) {
+            if (ph_primal) {
+              phased_dispatch(c->Pat, [&](auto kc, auto jc) {
+                hipLaunchKernelGGL((k_ph_primal<T, decltype(kc)::value, decltype(jc)::value>), dim3(c->Pat.G),
+                                   dim3(kPhBlock), 0, c->stream, c->Pat, c->n, c->Pat.val, sp<T>(c, PSH_Y), pa);
+              });
+              done = true;
+            }
+          }
+          if (!done)
+            hipLaunchKernelGGL((k_iter_primal<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream,
+                               c->At.tiles, c->At.ntiles, c->At.ptr, c->At.col, at_val, sp<T>(c, PSH_Y), pa);
           time_end(c, 0, ev);
         }
         if (c->A.ntiles > 0) {
           time_begin(c, 1, &ev);
+          bool done = false;
           if constexpr (sizeof(T) == 8) {
-            if (use_slices) {
-              hipLaunchKernelGGL(k_slice_partial, dim3(c->As.ntp / (kXcd / c->As.slices) * kXcd), dim3(kBlock), 0,
-                                 c->stream, c->As, c->A.tiles, c->A.ntiles, c->m, sp<T>(c, PSH_XB));
-              hipLaunchKernelGGL(k_slice_dual, dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, c->As, c->m, da);
-            } else {
-              hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
-                                 c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
+            if (ph_dual) {
+              phased_dispatch(c->Pa, [&](auto kc, auto jc) {
+                hipLaunchKernelGGL((k_ph_dual<T, decltype(kc)::value, decltype(jc)::value>), dim3(c->Pa.G),
+                                   dim3(kPhBlock), 0, c->stream, c->Pa, c->m, c->Pa.val, sp<T>(c, PSH_XB), da);
+              });
+              done = true;
             }
-          } else {
+          }
+          if (!done)
             hipLaunchKernelGGL((k_iter_dual<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream,
                                c->A.tiles, c->A.ntiles, c->A.ptr, c->A.col, a_val, sp<T>(c, PSH_XB), da);
-          }
           time_end(c, 1, ev);
         }
         PSH_LAUNCHED(c, "iteration kernels");
@@ -912,7 +894,7 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
   } else {
     const long long key = (static_cast<long long>(g->iterations) << 9) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
                           (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3) | (delta ? 1LL << 6 : 0) |
-                          (use_slices ? 1LL << 7 : 0) | (delta && c->delta.bf16 ? 1LL << 8 : 0);
+                          (ph_primal || ph_dual ? 1LL << 7 : 0) | (delta && c->delta.bf16 ? 1LL << 8 : 0);
     auto it = c->graph_cache.find(key);
     if (it == c->graph_cache.end()) {
       std::lock_guard<std::recursive_mutex> capture_lock(api_mutex());  // no device-wide calls meanwhile
@@ -945,7 +927,8 @@ int psh_graphs(psh_ctx* c, int on) {
 
 int psh_sliced(psh_ctx* c, int on) {
   if (!c) return PSH_E_ARG;
-  c->sliced = on != 0;
+  if (on < 0 || on > 2) return fail(c, PSH_E_ARG, "psh_sliced: on must be 0, 1 or 2");
+  c->sliced = on;
   return PSH_OK;
 }
 

@@ -53,21 +53,25 @@ struct DevMat {
   int* tiles = nullptr;   // ntiles+1 tile start rows
 };
 
-// Ã's entries regrouped by (row tile, column slice) for the dual half step with fp64 tree
-// reductions: workgroup b gathers only from column slice (b mod 8)·S/8 of x̄, so the slice stays in
-// its XCD's L2; each row's slice sums are combined in slice order by the fused epilogue kernel.
-struct Sliced {
-  int slices = 0;                 // S (0: not built)
-  int ntp = 0;                    // tiles padded to a multiple of 8/S
-  int* seg = nullptr;             // ntp·S + 1 segment starts
+// A CSR's entries regrouped for the phased product (fp64 iteration kernels of large problems).
+// Persistent workgroup g owns the rows [g·R, g·R + R), R = K·kPhBlock; phase p holds the entries
+// of those rows whose column lies in slice p ([p·w, (p+1)·w), w = ⌈cols / S⌉), rows in order and
+// each row's entries in CSR order. Sweeping the phases in order keeps every row's running sum
+// in CSR order (columns are sorted in each row), so the sums are the bits of the row-tiled
+// product, while the workgroups of an XCD gather from one slice of the vector at a time, which
+// stays in that XCD's L2. Values are gathered from the CSR's values on the device (`src`).
+struct Phased {
+  int S = 0;                      // phases (0: not built)
+  int K = 0, J = 0, G = 0, R = 0; // rows per thread, product slots per thread, workgroups, rows per workgroup
+  int* blk = nullptr;             // G·S + 1 entry starts of the (workgroup, phase) blocks
+  unsigned char* cnt = nullptr;   // G·S·R: entries of each row in each phase, thread-major rows
   int* col = nullptr;             // nnz
-  int* src = nullptr;             // nnz: index of each entry in Ã (values are gathered on the device)
+  int* src = nullptr;             // nnz: index of each entry in the CSR
   double* val = nullptr;          // nnz
-  unsigned char* row = nullptr;   // nnz: row within the tile
-  double* part = nullptr;         // S·rows slice sums
 };
-constexpr int kSlices = 4;
-constexpr int kSliceMinRows = 1 << 16;  // below this the extra launch costs more than it saves
+constexpr int kPhBlock = 512;                    // threads per phased workgroup
+constexpr long long kPhMinBytes = 4LL << 20;     // gathered vectors up to this size stay in L2 anyway
+constexpr int kPhMaxJ = 8;                       // ≤ kPhMaxJ·kPhBlock entries per (workgroup, phase)
 
 // A reduction job: Σ or max over t[0..na) followed by u[0..nb) (u optional).
 struct RedJob {
@@ -161,8 +165,8 @@ struct psh_ctx {
   size_t bytes = 0;
 
   psh::DevMat A, At;             // scaled Ã (m×n) and Ãᵀ (n×m)
-  psh::Sliced As;                // Ã by (tile, column slice), built for large m
-  bool sliced = true;            // use it for fp64 tree-mode chunks
+  psh::Phased Pa, Pat;           // Ã and Ãᵀ regrouped for the phased product (large problems)
+  int sliced = 1;                // 0: row tiles only; 1: phased where built (auto); 2: build at any size
   double* a_orig = nullptr;      // original A values, Ã's structure
   int* ato_col = nullptr;        // original Aᵀ (CSC) row indices (aliases At.col when equal)
   bool ato_alias = false;
@@ -366,6 +370,86 @@ __device__ inline void tile_sums(int r0, int r1, const int* __restrict__ ptr, co
     }
     __syncthreads();
   }
+}
+
+// The phased product (struct Phased) with the epilogue of the row-tiled kernels. Thread t
+// accumulates the rows t·K .. t·K + K − 1 of its workgroup in registers across the phases
+// (products staged in LDS, each row's products added in order), then the sums go through LDS
+// and the epilogue runs on rows k·kPhBlock + t (coalesced). The whole workgroup calls it.
+template <int K, int J, class V, class X, class Epi>
+__device__ inline void phased_rows(const Phased& P, int rows, const V* __restrict__ val, const X* __restrict__ x,
+                                   Epi& epi) {
+  __shared__ double prod[J * kPhBlock];
+  __shared__ double part[K * kPhBlock];
+  __shared__ int wtot[kPhBlock / 64];
+  const int t = static_cast<int>(threadIdx.x), g = static_cast<int>(blockIdx.x), lane = t & 63, w = t >> 6;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int p = 0; p < P.S; ++p) {
+    const int bp = g * P.S + p;
+    const int b0 = P.blk[bp], ne = P.blk[bp + 1] - b0;
+    unsigned cn[K];
+    const unsigned char* cp = P.cnt + static_cast<size_t>(bp) * P.R + t * K;
+    if constexpr (K == 4) {
+      const unsigned v = *reinterpret_cast<const unsigned*>(cp);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) cn[k] = (v >> (8 * k)) & 255u;
+    } else if constexpr (K == 2) {
+      const unsigned short v = *reinterpret_cast<const unsigned short*>(cp);
+      cn[0] = v & 255u;
+      cn[1] = static_cast<unsigned>(v) >> 8;
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) cn[k] = cp[k];
+    }
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) mine += static_cast<int>(cn[k]);
+    int inc = mine;  // inclusive scan of the entry counts over the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(inc, o);
+      if (lane >= o) inc += v;
+    }
+    if (lane == 63) wtot[w] = inc;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int q = t + j * kPhBlock;
+      if (q < ne) prod[q] = static_cast<double>(ld_stream(val + b0 + q)) * static_cast<double>(x[ld_stream(P.col + b0 + q)]);
+    }
+    __syncthreads();
+    int off = inc - mine;
+    for (int k = 0; k < w; ++k) off += wtot[k];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      for (unsigned c = 0; c < cn[k]; ++c) acc[k] += prod[off++];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) part[t * K + k] = acc[k];
+  __syncthreads();
+  const int r0 = g * P.R;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int row = r0 + k * kPhBlock + t;
+    const bool has = row < rows;
+    if constexpr (has_prefetch<Epi>::value) epi.prefetch(row, has);
+    epi(row, has, part[k * kPhBlock + t]);
+  }
+}
+
+// Calls f(K, J) with K, J as compile-time constants for a built Phased (K ∈ {1,2,4}, J ∈ {2,4,8}).
+template <class F>
+inline void phased_dispatch(const Phased& P, F&& f) {
+  auto by_j = [&](auto kc) {
+    if (P.J <= 2) f(kc, std::integral_constant<int, 2>{});
+    else if (P.J <= 4) f(kc, std::integral_constant<int, 4>{});
+    else f(kc, std::integral_constant<int, 8>{});
+  };
+  if (P.K == 1) by_j(std::integral_constant<int, 1>{});
+  else if (P.K == 2) by_j(std::integral_constant<int, 2>{});
+  else by_j(std::integral_constant<int, 4>{});
 }
 
 template <class V, class X, class Epi>

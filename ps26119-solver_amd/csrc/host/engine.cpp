@@ -59,7 +59,7 @@ std::string apply_engine_params(const Options& o, EngineOptions& e) {  // refere
     else if (name == "hip_device") e.device = static_cast<int>(v);
     else if (name == "hip_timing") e.timing = v != 0 ? 1 : 0;
     else if (name == "hip_graphs") e.graphs = v != 0 ? 1 : 0;
-    else if (name == "hip_sliced") e.sliced = v != 0 ? 1 : 0;
+    else if (name == "hip_sliced") e.sliced = v >= 2 ? 2 : (v != 0 ? 1 : 0);
     else if (name == "hip_device_scaling") e.device_scaling = v != 0 ? 1 : 0;
     else if (name == "hip_refine") e.refine = v != 0 ? 1 : 0;
     else if (name == "hip_refine_patience") e.refine_patience = std::max(1, static_cast<int>(v));
@@ -438,10 +438,10 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
     dev_ = Device::open(opt.device, error_);
     if (!dev_) return;
     const double t_open = elapsed();
+    dev_->check(psh_sliced(dev_->ctx(), opt.sliced), "sliced");  // governs the phased build at load
     scale_and_load(*dev_, model, opt.scaling, opt.device_scaling != 0, sp_);
     if (opt.timing) dev_->check(psh_timing(dev_->ctx(), 1), "timing");
     dev_->check(psh_graphs(dev_->ctx(), opt.graphs), "graphs");
-    dev_->check(psh_sliced(dev_->ctx(), opt.sliced), "sliced");
     dev_->check(psh_sync(dev_->ctx()), "load");
     const double t_load = elapsed();
     const bool empty = sp_.n == 0 || sp_.A.nnz() == 0;

@@ -45,7 +45,7 @@ struct EngineOptions {  // reference src/pdhg/engine.h:27-48
   int device = 0;
   int timing = 0;  // hip_timing: per-kernel HIP event timing into the Solution
   int graphs = 1;  // hip_graphs: replay captured chunk graphs (off while timing)
-  int sliced = 1;  // hip_sliced: column-sliced dual half step for large fp64 tree-mode solves
+  int sliced = 1;  // hip_sliced: phased products for large fp64 solves (0 off, 1 auto, 2 at any size)
   int device_scaling = 1;  // hip_device_scaling: the scaling sweeps on the device (same bits as the host's)
   // hip_refine: mixed precision as fp32 increments on fp64 anchors (default) instead of the
   // reference's fp32-then-fp64 switch; CPU-order mode (hip_cpu_order = 1) defaults it to 0.

@@ -18,7 +18,7 @@ D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
 
 X, X0, XH, XB, DX, ATY, Y, Y0, YH, YB, DY, AX = range(12)
-RED_TREE, RED_CPU = 0, 1
+RED_TREE, RED_CPU, RED_DECISION = 0, 1, 2
 
 
 class Problem(ctypes.Structure):
@@ -123,12 +123,15 @@ class Context:
 
     ``scaled`` is a dict with the scaled CSR data (as produced by ``scaled_problem``)."""
 
-    def __init__(self, scaled: dict, device: int = 0):
+    def __init__(self, scaled: dict, device: int = 0, sliced: int = 1):
+        """sliced: phased products (0 row tiles only, 1 where the gathered vector exceeds 4 MiB,
+        2 at any size), set before the problem is loaded."""
         L = lib()
         self._h = ctypes.c_void_p()
         rc = L.psh_create(device, ctypes.byref(self._h))
         if rc != 0:
             raise RuntimeError(f"psh_create failed ({rc})")
+        self._ok(L.psh_sliced(self._h, sliced))
         self.m, self.n = scaled["m"], scaled["n"]
         self._keep = dict(scaled)
         s = self._keep

@@ -7,7 +7,8 @@ Fixtures are data only: per case the status, iteration count, objective (exact, 
 float), certified bound, in-process check verdict, final primal weight, iterations to 1e-4
 and FNV-1a hashes of the bits of x and y; per kernel-level case the hash of every output vector
 of a fixed backend op sequence and the reductions. --big adds the 10,000-row random LP (about
-five minutes of reference CPU time).
+five minutes of reference CPU time). --refinery only adds the refinery planning LPs (T = 6 and 24
+periods, seed 3; fp64 and mixed, with and without presolve; a few minutes) to the existing file.
 """
 from __future__ import annotations
 
@@ -76,9 +77,35 @@ def batch_case(ref, lp, scenarios):
     return out
 
 
+REFINERY = {"refinery_T6_s3": (6, 3), "refinery_T24_s3": (24, 3)}
+
+
+def refinery_only(ref, out_path):
+    with open(out_path) as f:
+        data = json.load(f)
+    for name, (T, seed) in REFINERY.items():
+        lp = lpgen.refinery_lp(T=T, seed=seed)
+        for pre in (True, False):
+            for prec in (0, 1):
+                key = f"{name}|presolve={int(pre)}|{'mixed' if prec else 'fp64'}"
+                data["solves"][key] = solve_case(ref, lp, presolve=pre, precision=prec, iteration_limit=3_000_000)
+                print(key, data["solves"][key]["status"], data["solves"][key]["iterations"], flush=True)
+        sc = ref.scale(lp)
+        data["kernels"][name] = {"norm2": hexf(ref.norm2(lp)), "row_scale": fnv(sc["row_scale"]),
+                                 "col_scale": fnv(sc["col_scale"]), "a_vals": fnv(sc["a_vals"]),
+                                 "bound_scale": hexf(sc["bound_scale"]), "obj_scale": hexf(sc["obj_scale"]),
+                                 "geometric_mean_applied": int(sc["geometric_mean_applied"])}
+    with open(out_path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    print("wrote", out_path)
+
+
 def main():
     big = "--big" in sys.argv
     ref = orc.load("ref")
+    if "--refinery" in sys.argv:
+        refinery_only(ref, os.path.join(HERE, "reference_runs.json"))
+        return
     models = {name: lpgen.read_lpm(os.path.join(HERE, "netlib_small", name + ".lpm")) for name in NETLIB}
     models.update(hand_models())
     models["random300_s3"] = lpgen.random_lp(300, seed=3)
@@ -135,6 +162,12 @@ def main():
         for k, v in old.get("solves", {}).items():
             if k.startswith("random10000"):
                 solves[k] = v
+    for k, v in old.get("solves", {}).items():  # and the refinery cases (--refinery)
+        if k.startswith("refinery"):
+            solves[k] = v
+    for k, v in old.get("kernels", {}).items():
+        if k.startswith("refinery"):
+            kernels[k] = v
     with open(out_path, "w") as f:
         json.dump({"generator": "tests/golden/make_golden.py", "solves": solves, "kernels": kernels,
                    "batches": batches, "pdlp": pdlp}, f, indent=1,

@@ -124,18 +124,19 @@ def test_fused_chunk_equals_unfused_ops(gpu_available, oracle, name, prec):
     b.close()
 
 
-def test_sliced_dual_half_step(gpu_available, oracle):
-    """Above 65,536 rows the fp64 tree-mode chunk runs the dual half step over column slices of Ã
-    (x̄'s slice stays in an XCD's L2). Its row sums add the slice sums in slice order, so they may
-    differ from the CSR-order sums in the last bits only; CPU-order chunks never use slices and stay
-    bit-identical."""
-    from ps26119_amd import lpgen, psh
-    lp = lpgen.random_lp(70000, seed=9)
+@pytest.mark.parametrize("name", ["rand300", "long_rows", "rand20k"])
+def test_phased_products_equal_row_tiles(gpu_available, oracle, name):
+    """The phased half steps (persistent workgroups sweeping column slices in order, each row's
+    running sum kept in CSR order) give the bits of the row-tiled kernels in every reduction mode;
+    hip_sliced = 2 builds them at any size (the long-rows model has rows beyond the phased layout's
+    limits and falls back to the tiles)."""
+    from ps26119_amd import psh
+    lp = model(name)
     K, tau, sigma, rho = 7, 0.3, 0.4, 1.0
-    a = context(oracle, lp, 0)
-    b = context(oracle, lp, 0)
-    b.sliced(0)
-    for mode in (psh.RED_TREE, psh.RED_CPU):
+    sc = psh.scaled_problem(oracle.scale(lp), lp)
+    a = psh.Context(sc, sliced=2)
+    b = psh.Context(sc, sliced=0)
+    for mode in (psh.RED_TREE, psh.RED_CPU, psh.RED_DECISION):
         for ctx in (a, b):
             ctx.upload(psh.X, np.sin(1.0 + np.arange(lp.n)))
             ctx.upload(psh.Y, np.cos(2.0 + np.arange(lp.m)))
@@ -145,13 +146,66 @@ def test_sliced_dual_half_step(gpu_available, oracle):
             a.chunk(K, inner0, tau, sigma, rho, mode=mode)
             b.chunk(K, inner0, tau, sigma, rho, mode=mode)
         for slot in (psh.X, psh.XH, psh.XB, psh.Y, psh.YH, psh.YB):
-            va, vb = a.download(slot), b.download(slot)
-            if mode == psh.RED_CPU:
-                assert np.array_equal(va, vb), slot
-            else:
-                assert np.allclose(va, vb, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(vb).max())), slot
+            assert np.array_equal(a.download(slot), b.download(slot)), (mode, slot)
     a.close()
     b.close()
+
+
+_big = {}
+
+
+def random_1m():
+    from ps26119_amd import lpgen
+    if "lp" not in _big:
+        _big["lp"] = lpgen.random_lp(1_000_000, seed=1)
+    return _big["lp"]
+
+
+def test_full_size_chunk_is_the_oracle(gpu_available, oracle):
+    """The 1,000,000-row random LP (6M nonzeros; phased products on both half steps): 64 fused
+    iterations are the oracle's op sequence bit for bit at full size (x, x̂, y, ŷ, and the CPU-order
+    reductions and KKT terms of that point) in CPU-order, tree and decision modes (the reductions
+    only feed r⁰ inside a chunk)."""
+    from ps26119_amd import psh
+    lp = random_1m()
+    sc = psh.scaled_problem(oracle.scale(lp), lp)
+    exp, ered = oracle.backend_ops(lp, precision=0, iters=64, tau=0.3, sigma=0.4, rho=1.0)
+    for mode in (psh.RED_CPU, psh.RED_TREE, psh.RED_DECISION):
+        ctx = psh.Context(sc)
+        ctx.upload(psh.X, np.sin(1.0 + np.arange(lp.n)))
+        ctx.upload(psh.Y, np.cos(2.0 + np.arange(lp.m)))
+        ctx.copy(psh.X0, psh.X)
+        ctx.copy(psh.Y0, psh.Y)
+        ctx.chunk(64, 0, 0.3, 0.4, 1.0, mode=mode)
+        got = {"x": ctx.download(psh.X), "xh": ctx.download(psh.XH), "y": ctx.download(psh.Y),
+               "yh": ctx.download(psh.YH)}
+        for k in got:
+            assert np.array_equal(got[k], exp[k]), (mode, k, float(np.max(np.abs(got[k] - exp[k]))))
+        red = [ctx.dot(psh.X, psh.XH, psh.RED_CPU), np.sqrt(ctx.dot(psh.Y, psh.Y, psh.RED_CPU)),
+               ctx.diff_norm2(psh.X, psh.X0, psh.RED_CPU)] + ctx.kkt(psh.XH, psh.YH, psh.RED_CPU)
+        assert [float(v).hex() for v in red] == [float(v).hex() for v in ered]
+        ctx.close()
+
+
+def test_full_size_scaling_and_solve_prefix_are_the_oracle(gpu_available, oracle):
+    """At 1,000,000 rows: psh_scale's factors are the oracle's bits, and a 128-iteration solve
+    through the C API (presolve, device scaling, ‖Ã‖₂ with the reference's sequential Lanczos dots,
+    two checks) ends on the oracle's x and y bits in CPU-order and default modes, so η, the scaled
+    problem and the phased iterations are the reference's at full size."""
+    import ps26119
+    from ps26119_amd import psh
+    lp = random_1m()
+    sc = oracle.scale(lp)
+    dev = psh.Context(psh.unscaled_problem(lp))
+    rs, cs = dev.scale(12 if sc["geometric_mean_applied"] else 0, 10, 1)
+    assert np.array_equal(rs, sc["row_scale"]) and np.array_equal(cs, sc["col_scale"])
+    dev.close()
+    e, v = oracle.solve(lp, presolve=True, iteration_limit=128)
+    for prm in ({"hip_cpu_order": 1}, {}):
+        r = ps26119.solve_model(lp, iteration_limit=128, params=prm)
+        print(f"1M rows, {prm or 'default'}: {r.status_name} it={r.iterations} oracle it={e['iterations']}", flush=True)
+        assert r.iterations == e["iterations"] == 128
+        assert np.array_equal(r.x, v["x"]) and np.array_equal(r.y, v["y"]), prm
 
 
 @pytest.mark.parametrize("name", ["stocfor1", "rand300", "rand20k"])
