@@ -244,6 +244,21 @@ int psh_batch_compact(psh_ctx* ctx, const int* keep, int W2);
 /* Unscaled point of slot t: x = clip(C x̂ / bs, l, u) (n), y_min = R ŷ / os (m). */
 int psh_batch_point(psh_ctx* ctx, int t, double* x, double* y_min);
 
+/* ---- allocation-free dot product. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). out[0] ← Σ a[i]·b[i] for device arrays a, b of n doubles and a device output out, on
+ * `stream` (a hipStream_t; NULL = the null stream), with psh_dot's reduction in `mode` (tree, or
+ * the 64 fixed chunks of the CPU order for PSH_RED_CPU / PSH_RED_DECISION), so the result has
+ * psh_dot's bits. Allocates nothing: its scratch (512 partial sums) is a static device array of
+ * this library, one per device, whose address is looked up on the first call on a device. Every
+ * call on a device shares that array and the last call leaves its partials there: calls that
+ * overlap on one device (different streams, or host threads) race. 0 <= n <= PSH_NOALLOC_MAX_N,
+ * else PSH_E_ARG; errors are described by psh_noalloc_error() (the calling thread's last one).
+ * Asynchronous: out is written when the stream reaches the call. */
+enum { PSH_NOALLOC_MAX_N = 2147483647 };
+int psh_dot_noalloc(const double* a, const double* b, long long n, int mode, double* out, void* stream);
+const char* psh_noalloc_error(void);
+
 /* ---- per-kernel timing (HIP events on the context's stream), for the benchmark. */
 int psh_timing(psh_ctx* ctx, int on);
 /* kernel classes: 0 fused primal (Ãᵀy + projection/reflection/Halpern), 1 fused dual (Ãx̄ + ...),

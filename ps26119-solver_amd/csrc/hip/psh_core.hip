@@ -129,15 +129,8 @@ __global__ void __launch_bounds__(kBlock) k_tree1(const RedJob* __restrict__ job
     if (J.op == 0) acc += v;
     else acc = fmax(acc, v);
   }
-  __shared__ double red[kBlock];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w)
-      red[threadIdx.x] = J.op == 0 ? red[threadIdx.x] + red[threadIdx.x + w] : fmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
+  const double r = J.op == 0 ? block_tree_sum(acc) : block_tree_max(acc);
+  if (threadIdx.x == 0) *out = r;
 }
 
 __global__ void __launch_bounds__(kBlock) k_tree2(const RedJob* __restrict__ jobs, const double* __restrict__ partials,
@@ -147,47 +140,8 @@ __global__ void __launch_bounds__(kBlock) k_tree2(const RedJob* __restrict__ job
   const double* p = partials + static_cast<size_t>(blockIdx.x) * kTreeBlocks;
   double acc = 0.0;
   for (int i = threadIdx.x; i < kTreeBlocks; i += kBlock) acc = J.op == 0 ? acc + p[i] : fmax(acc, p[i]);
-  __shared__ double red[kBlock];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w)
-      red[threadIdx.x] = J.op == 0 ? red[threadIdx.x] + red[threadIdx.x + w] : fmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) results[first + blockIdx.x] = red[0];
-}
-
-// s + a[b] + a[b+1] + ... + a[e-1], strictly left to right, by one wavefront: the 64 lanes load
-// 512 consecutive values (coalesced) into LDS, issue the loads of the next 512, and meanwhile every
-// lane adds the staged values in order (a uniform dependent chain, so the lanes agree). The last
-// batch is padded with +0.0: the running sum is never −0.0 (it starts at +0.0 or at a sum that
-// started there, and x + (−x) rounds to +0.0), so adding +0.0 leaves it unchanged bit for bit.
-constexpr int kSeqBatch = 512;
-__device__ inline double wave_seq_sum(const double* __restrict__ a, long long b, long long e, double s,
-                                      double* __restrict__ stage) {
-  const int lane = static_cast<int>(threadIdx.x);
-  double v[kSeqBatch / 64];
-#pragma unroll
-  for (int u = 0; u < kSeqBatch / 64; ++u) {
-    const long long i = b + u * 64 + lane;
-    v[u] = i < e ? a[i] : 0.0;
-  }
-  for (long long base = b; base < e; base += kSeqBatch) {
-#pragma unroll
-    for (int u = 0; u < kSeqBatch / 64; ++u) stage[u * 64 + lane] = v[u];
-    __syncthreads();
-    const long long next = base + kSeqBatch;
-#pragma unroll
-    for (int u = 0; u < kSeqBatch / 64; ++u) {
-      const long long i = next + u * 64 + lane;
-      v[u] = i < e ? a[i] : 0.0;
-    }
-#pragma unroll 16
-    for (int k = 0; k < kSeqBatch; ++k) s += stage[k];
-    __syncthreads();
-  }
-  return s;
+  const double r = J.op == 0 ? block_tree_sum(acc) : block_tree_max(acc);
+  if (threadIdx.x == 0) results[first + blockIdx.x] = r;
 }
 
 // CPU-order sums: mode 1 = 64 fixed chunks (la/parallel.h:169-188), mode 2 = one sequential loop
@@ -198,10 +152,12 @@ __global__ void __launch_bounds__(64) k_ordered(const RedJob* __restrict__ jobs,
   if (J.op != 0 || J.mode == 0) return;
   __shared__ double stage[kSeqBatch];
   double* out = partials + static_cast<size_t>(blockIdx.y) * kTreeBlocks + blockIdx.x;
+  auto ga = [&](long long i) { return J.a[i]; };
+  auto gb = [&](long long i) { return J.b[i]; };
   if (J.mode == 2) {
     if (blockIdx.x != 0) return;
-    double s = wave_seq_sum(J.a, 0, J.na, 0.0, stage);
-    s = wave_seq_sum(J.b, 0, J.nb, s, stage);
+    double s = wave_seq_sum(ga, 0, J.na, 0.0, stage);
+    s = wave_seq_sum(gb, 0, J.nb, s, stage);
     if (threadIdx.x == 0) *out = s;
     return;
   }
@@ -210,7 +166,7 @@ __global__ void __launch_bounds__(64) k_ordered(const RedJob* __restrict__ jobs,
   const int c = static_cast<int>(blockIdx.x);
   if (c >= chunks) return;
   const long long b = n * c / chunks, e = n * (c + 1) / chunks;
-  const double s = wave_seq_sum(J.a, b, e, 0.0, stage);
+  const double s = wave_seq_sum(ga, b, e, 0.0, stage);
   if (threadIdx.x == 0) *out = s;
 }
 

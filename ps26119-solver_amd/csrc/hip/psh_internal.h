@@ -372,6 +372,63 @@ __device__ inline void tile_sums(int r0, int r1, const int* __restrict__ ptr, co
   }
 }
 
+// ---- reduction shapes shared by the context reductions and the allocation-free entry point
+// Fixed-shape sum (max) of one value per thread over a kBlock-thread workgroup: pairwise in LDS,
+// halving strides. Every thread calls it; the result is valid in thread 0.
+__device__ inline double block_tree_sum(double acc) {
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+__device__ inline double block_tree_max(double acc) {
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// s + t(b) + t(b+1) + ... + t(e−1), strictly left to right, by one wavefront (64 threads): the
+// lanes form 512 consecutive terms (coalesced loads) into an LDS stage, issue the loads of the
+// next 512, and meanwhile every lane adds the staged terms in order (a uniform dependent chain,
+// so the lanes agree). The last batch is padded with +0.0: the running sum is never −0.0 (it
+// starts at +0.0 or at a sum that started there, and x + (−x) rounds to +0.0), so adding +0.0
+// leaves it unchanged bit for bit. `stage` holds kSeqBatch doubles of LDS.
+constexpr int kSeqBatch = 512;
+template <class Get>
+__device__ inline double wave_seq_sum(Get get, long long b, long long e, double s, double* __restrict__ stage) {
+  const int lane = static_cast<int>(threadIdx.x);
+  double v[kSeqBatch / 64];
+#pragma unroll
+  for (int u = 0; u < kSeqBatch / 64; ++u) {
+    const long long i = b + u * 64 + lane;
+    v[u] = i < e ? get(i) : 0.0;
+  }
+  for (long long base = b; base < e; base += kSeqBatch) {
+#pragma unroll
+    for (int u = 0; u < kSeqBatch / 64; ++u) stage[u * 64 + lane] = v[u];
+    __syncthreads();
+    const long long next = base + kSeqBatch;
+#pragma unroll
+    for (int u = 0; u < kSeqBatch / 64; ++u) {
+      const long long i = next + u * 64 + lane;
+      v[u] = i < e ? get(i) : 0.0;
+    }
+#pragma unroll 16
+    for (int k = 0; k < kSeqBatch; ++k) s += stage[k];
+    __syncthreads();
+  }
+  return s;
+}
+
 // The phased product (struct Phased) with the epilogue of the row-tiled kernels. Thread t
 // accumulates the rows t·K .. t·K + K − 1 of its workgroup in registers across the phases
 // (products staged in LDS, each row's products added in order), then the sums go through LDS

@@ -100,6 +100,9 @@ def lib():
             "psh_lanczos_begin": ([P, D], ctypes.c_int),
             "psh_lanczos_step": ([P, ctypes.c_double, ctypes.c_int, D, D], ctypes.c_int),
             "psh_lanczos_advance": ([P, ctypes.c_double], ctypes.c_int),
+            "psh_lanczos_product": ([P, D, D], ctypes.c_int),
+            "psh_dot_noalloc": ([P, P, ctypes.c_longlong, ctypes.c_int, P, P], ctypes.c_int),
+            "psh_noalloc_error": ([], ctypes.c_char_p),
             "psh_graphs": ([P, ctypes.c_int], ctypes.c_int),
             "psh_sliced": ([P, ctypes.c_int], ctypes.c_int),
             "psh_delta": ([P, ctypes.c_int], ctypes.c_int),
