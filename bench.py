@@ -2,16 +2,18 @@
 
 Default workload (`--workload lp1m`, one GPU): the random LP with a planted optimum, m = 1,000,000
 rows, n = 2,000,000 columns, exactly 6 nonzeros per row (6,000,000 nnz), seed 1, fp64, solved
-through the C API toward relative KKT 1e-8 (presolve, scaling, ‖Ã‖₂, r²HPDHG on the GPU). A
-step is one r²HPDHG iteration of the solver's loop (two fused launches, the check every 64
-iterations, restarts); `value` is the HBM GB/s of the iteration loop (algorithmic bytes per
-iteration × iterations ÷ the loop's wall time, inputs resident in HBM). The default run times
-16,384 iterations; `--to-tolerance` runs the solve to 1e-8 and reports `time_to_1e-8_s` (about
-2.4M iterations, ≈ 410 s, with the default `--epoch-cap 8192`: the reference's restart schedule
-alone stalls near a relative gap of 3e-7 on this instance). Per-kernel durations come from HIP events on the
-solve's stream in a second, timed run; `roofline` reports the fused Ãᵀy + primal-update kernel
-against the 8 TB/s HBM peak. `cpu_baseline` times the reference CPU solver (oracle/_ref, all
-host cores) for 256 iterations of the same model, in the same unit.
+through the C API to relative KKT 1e-8 and verified (presolve, scaling, ‖Ã‖₂, r²HPDHG on the GPU,
+finish and check). `time_to_1e-8_s` is the whole solve (the reference's definition); a step is
+one r²HPDHG iteration of the solver's loop (two fused launches, the check every 64 iterations,
+restarts), `steps` is the number the solve ran; `value` is the HBM GB/s of the iteration loop
+(algorithmic bytes per iteration × iterations ÷ the loop's wall time, inputs resident in HBM).
+The reference's restart schedule alone stalls near a relative gap of 3e-7 on this instance, so
+epochs are capped at `--epoch-cap 8192` iterations while the gap leads (said in `config`); both
+schedules also run for `--compare-seconds` and report the gap they reach. Per-kernel durations
+come from HIP events on the solve's stream in a second, short run; `roofline` reports the fused
+Ãᵀy + primal-update kernel against the 8 TB/s HBM peak, with the HBM bytes per launch from the
+committed rocprofv3 counter summary. `cpu_baseline` times the reference CPU solver (oracle/_ref,
+all host cores) for 256 iterations of the same model, in the same unit.
 
 Other workloads (each prints one JSON line of the same shape):
   refinery   planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
@@ -27,9 +29,11 @@ Other workloads (each prints one JSON line of the same shape):
       form; --serving-precision fp64 for plain); value = solves per second, with p50 / p99 latency,
       against sequential plain fp64 solves of the same requests.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload lp1m|refinery|scenarios|serving] [--m M]
-                    [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
-                    [--epoch-cap N] [--to-tolerance]
+  copy       a device copy of known size (torch), the calibration kernel of the counter summaries.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lp1m|refinery|scenarios|serving|copy]
+                    [--m M] [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
+                    [--epoch-cap N] [--compare-seconds S]
                     [--iteration-limit N]   (profiling: same per-launch work, N iterations)
 """
 import argparse
@@ -71,6 +75,17 @@ def delta_kernel_bytes(m, n, nnz):
     primal = nnz * 8 + (n + 1) * 4 + 4 * m + 24 * n
     dual = nnz * 8 + (m + 1) * 4 + 4 * n + 20 * m
     return primal, dual
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def host_cores():
@@ -121,79 +136,121 @@ def epoch_knobs(args):
 
 
 # ------------------------------------------------------------------ lp1m
+PH_PRIMAL = "k_ph_primal (phased Ãᵀy + projection / reflection / Halpern update)"
+
+
+def load_traffic(name):
+    """HBM bytes per launch of the iteration kernels from the committed rocprofv3 counter summary
+    profiles/<name>.json (read / write requests counted per request size), or {}."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get("traffic", {})
+
+
 def bench_lp1m(args, ps26119, lpgen):
-    """One step = one r²HPDHG iteration of the solver's own loop on the 1M-row LP (two fused
-    launches, plus the check every 64 iterations and the restarts, exactly as a solve runs them).
-    `value` = algorithmic HBM bytes of the iterations ÷ the wall time of the iteration loop
-    (scaling, ‖Ã‖₂, upload and finish excluded: the inputs are resident when the loop starts).
-    Time to 1e-8 is reported when the run reaches it (--to-tolerance): on this instance r²HPDHG
-    needs millions of iterations. With the reference's restart schedule alone the relative gap
-    plateaus near 3e-7; --epoch-cap (default 8192) adds the engine knobs hip_max_epoch and
-    hip_max_epoch_gap_only, which end an epoch at that length while the gap is the largest KKT
-    term. The default run measures the rate over --steps iterations of that solve."""
+    """The 1M-row random LP solved to relative KKT 1e-8 through the C API, verified (presolve,
+    device scaling, ‖Ã‖₂, r²HPDHG, finish and the in-process check, all inside `seconds`, the
+    reference's time-to-1e-8 definition). A step is one r²HPDHG iteration of that solve (two
+    fused launches, the check every 64 iterations, restarts): `steps` = the iterations it ran,
+    `value` = algorithmic bytes per iteration × iterations ÷ the wall time of the iteration loop
+    (inputs resident in HBM; setup and finish excluded). The reference's restart schedule alone
+    plateaus near a relative gap of 3e-7 on this instance, so the solve ends an epoch at
+    --epoch-cap iterations while the gap is the largest KKT term (engine knobs hip_max_epoch,
+    hip_max_epoch_gap_only; in `config`); beside it both schedules run for --compare-seconds and
+    report the gap they reach. --iteration-limit N instead stops the timed solve after N
+    iterations (profiling runs)."""
     t = time.time()
     lp = lpgen.random_lp(args.m, seed=args.seed)
     log(f"generated m={lp.m} n={lp.n} nnz={lp.nnz} in {time.time() - t:.1f}s")
     mixed = args.precision == "mixed"
-    iters = args.iteration_limit if args.iteration_limit > 0 else max(64, args.steps // 64 * 64)
-    if args.to_tolerance:
-        iters = 0
     knobs = epoch_knobs(args)
-    for w in range(args.warmup):
-        r = ps26119.solve_model(lp, iteration_limit=64, precision=args.precision)
-        log(f"warmup {w}: {r.status_name} {r.seconds:.2f}s")
-    res = ps26119.solve_model(lp, stats=True, time_limit=args.time_limit, iteration_limit=iters, verbosity=1,
-                              precision=args.precision, params=knobs)
-    log(f"timed: {res.status_name} it={res.iterations} solve={res.seconds:.2f}s loop={res.stats['iterate_seconds']:.3f}s "
-        f"obj={res.objective!r} planted={lp.optimum!r} gap={res.gap:.2e} {res.message[:160]}")
-    timed = ps26119.solve_model(lp, params={"hip_timing": 1}, stats=True, time_limit=args.time_limit,
-                                iteration_limit=min(iters or 2048, 2048), precision=args.precision)
+    profiling = args.iteration_limit > 0
+    if args.warmup > 0:  # one untimed solve: --warmup iterations rounded up to whole 64-iteration blocks
+        w_it = 64 * ((args.warmup + 63) // 64)
+        r = ps26119.solve_model(lp, iteration_limit=w_it, precision=args.precision)
+        log(f"warmup: {r.status_name} it={r.iterations} {r.seconds:.2f}s")
+    res = ps26119.solve_model(lp, stats=True, time_limit=args.time_limit, iteration_limit=args.iteration_limit,
+                              verbosity=1, precision=args.precision, params=knobs)
+    st = res.stats
+    log(f"timed: {res.status_name} it={res.iterations} solve={res.seconds:.2f}s setup={st['setup_seconds']:.2f}s "
+        f"loop={st['iterate_seconds']:.3f}s obj={res.objective!r} planted={lp.optimum!r} gap={res.gap:.2e} "
+        f"check={res.check} {res.message[:160]}")
+    timed = ps26119.solve_model(lp, params={"hip_timing": 1, **knobs}, stats=True, iteration_limit=256,
+                                precision=args.precision)
     pb, db = (delta_kernel_bytes if mixed else kernel_bytes)(lp.m, lp.n, lp.nnz)
     roof, avg_us = roofline_of(timed.stats, pb, "k_delta_primal (fp32 increments: Ãᵀδy + projection/reflection/Halpern)"
-                               if mixed else "k_iter_primal (fused Ãᵀy + projection/reflection/Halpern)")
-    traffic_file = os.path.join(ROOT, "profiles", "k_iter_fp64_random1M_pmc_traffic.json")
+                               if mixed else PH_PRIMAL)
+    traffic = load_traffic("k_ph_fp64_random1M_pmc_traffic")
     phase_traffic = None
-    if not mixed and lp.m == 1_000_000 and lp.nnz == 6_000_000 and os.path.exists(traffic_file):
-        with open(traffic_file) as f:  # HBM bytes per launch from the two rocprofv3 --pmc passes
-            tr = json.load(f)["traffic"]
-        roof["traffic"] = tr.get("k_iter_primal<double>")
-        roof["traffic_source"] = "profiles/k_iter_fp64_random1M_pmc_traffic.md"
-        phase_traffic = tr["k_iter_primal<double>"] + tr["k_slice_partial"] + tr["k_slice_dual"]
-    st = res.stats
+    if not mixed and lp.m == 1_000_000 and lp.nnz == 6_000_000 and traffic:
+        roof["traffic"] = traffic.get("primal")
+        roof["traffic_source"] = "profiles/k_ph_fp64_random1M_pmc_traffic.md"
+        phase_traffic = traffic.get("primal", 0) + traffic.get("dual", 0)
     loop_s = st["iterate_seconds"]
     per_it = loop_s / max(1, res.iterations)
     value = (pb + db) / per_it / 1e9
+    reached = res.status == ps26119.OPTIMAL and res.check == "PASS"
     line = base_line(args, value, "GB/s", True, "f32+f64" if mixed else "f64",
                      {"workload": f"random LP with planted optimum, m={lp.m} n={lp.n} nnz={lp.nnz} (6 per row), seed "
-                                  f"{args.seed}, {args.precision}, r2HPDHG iterations of a presolved solve to 1e-8"
-                                  + (f", epochs capped at {args.epoch_cap} while the gap leads" if knobs else ""),
-                      "m": lp.m, "n": lp.n, "nnz": lp.nnz, "epoch_cap": args.epoch_cap})
+                                  f"{args.seed}, {args.precision}, r2HPDHG solve to relative KKT 1e-8 through the C API"
+                                  + (f", epochs capped at {args.epoch_cap} iterations while the gap leads" if knobs else "")
+                                  + (f"; profiling run stopped at {args.iteration_limit} iterations" if profiling else ""),
+                      "m": lp.m, "n": lp.n, "nnz": lp.nnz, "tolerance": 1e-8,
+                      "engine_knobs": knobs, "reductions": "reference order for decisions, KKT sums as device trees"})
     line["steps"] = res.iterations
     line["ms_per_step"] = per_it * 1e3
     line.update({
-        "roofline": roof, "status": res.status_name, "iterations": res.iterations,
-        "bytes_per_iteration": pb + db, "iterations_per_s": 1.0 / per_it,
-        "relative_gap_reached": res.gap, "time_to_1e-8_s": res.seconds if res.status == ps26119.OPTIMAL else None,
-        "setup_seconds": st["setup_seconds"], "loop_seconds": loop_s,
+        "time_to_1e-8_s": res.seconds if reached else None,
+        "status": res.status_name, "check": res.check, "iterations": res.iterations,
+        "objective_error_vs_planted": abs(res.objective - lp.optimum) / (1 + abs(lp.optimum)),
+        "relative_gap_reached": res.gap, "setup_seconds": st["setup_seconds"], "loop_seconds": loop_s,
+        "roofline": roof, "bytes_per_iteration": pb + db, "iterations_per_s": 1.0 / per_it,
         "dual_kernel": {"achieved_GBs": db / (avg_us[1] * 1e-6) / 1e9, "avg_launch_us": avg_us[1], "bytes_per_launch": db},
         "check_pass": {"launches": timed.stats["kernel_launches"][2], "avg_us": avg_us[2]},
         "spmv_phase_GBs": (pb + db) / ((avg_us[0] + avg_us[1]) * 1e-6) / 1e9,
-        # bytes the L2 exchanged with memory per iteration (counters, scaled by the copy probe)
-        # over the same launch times: the whole lines the random gathers bring in count here
+        # HBM bytes the two iteration kernels move per launch (counters) over the same launch times
         "spmv_phase_traffic_GBs": (phase_traffic / ((avg_us[0] + avg_us[1]) * 1e-6) / 1e9
                                    if phase_traffic else None),
         "device_bytes": st["device_bytes"], "message": res.message[:200],
     })
+    if not profiling and args.compare_seconds > 0 and knobs:
+        cmp = {}
+        for tag, prm in (("reference_schedule", {}), ("epoch_cap", knobs)):
+            r = ps26119.solve_model(lp, time_limit=args.compare_seconds, params=prm, precision=args.precision)
+            cmp[tag] = {"status": r.status_name, "iterations": r.iterations, "relative_gap": r.gap,
+                        "relative_primal": r.primal_residual, "seconds": r.seconds}
+            log(f"{tag} for {args.compare_seconds:.0f}s: {r.status_name} it={r.iterations} gap={r.gap:.2e}")
+        line["equal_time_comparison"] = cmp
     cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP")
     if cb:
         cpu_gbs = (pb + db) / (cb["ms_per_iteration"] * 1e-3) / 1e9
         sample = (f"reference r2HPDHG (oracle/_ref, threads=all) on the same 1M-row LP: 256 iterations after its setup, "
-                  f"{cb['ms_per_iteration']:.2f} ms/iteration; value = the same algorithmic bytes per iteration ÷ that time")
+                  f"{cb['ms_per_iteration']:.2f} ms/iteration; value = the same algorithmic bytes per iteration ÷ that time; "
+                  f"time to 1e-8 extrapolated to the GPU run's iterations: {cb['value']:.0f} s")
         cb = {"value": cpu_gbs, "unit": "GB/s", "cores": cb["cores"], "kind": "reference", "sample": sample,
-              "ms_per_iteration": cb["ms_per_iteration"], "setup_seconds": cb["setup_seconds"],
-              "speedup_per_iteration": cb["ms_per_iteration"] / (per_it * 1e3)}
+              "cpu_model": cpu_model(), "ms_per_iteration": cb["ms_per_iteration"], "setup_seconds": cb["setup_seconds"],
+              "time_to_1e-8_s_extrapolated": cb["value"], "speedup_per_iteration": cb["ms_per_iteration"] / (per_it * 1e3)}
     line["cpu_baseline"] = cb
     return line
+
+
+# ------------------------------------------------------------------ copy (counter calibration)
+def bench_copy(args):
+    """A device-to-device copy of --copy-mib MiB, 11 times (torch): the kernel of known read and
+    write bytes whose rocprofv3 request counters calibrate the traffic summaries in profiles/."""
+    import torch
+    n = args.copy_mib << 17  # doubles
+    a = torch.arange(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(11):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"metric": "copy calibration", "value": args.copy_mib, "unit": "MiB", "n_gpus": 1, "steps": 11,
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic", "config": {"workload": f"device copy of {args.copy_mib} MiB"},
+            "roofline": None, "cpu_baseline": None}
 
 
 # ------------------------------------------------------------------ refinery
@@ -444,20 +501,23 @@ def bench_serving(args, ps26119, lpgen):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16384, help="lp1m: r2HPDHG iterations timed (multiple of 64)")
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lp1m", choices=["lp1m", "refinery", "scenarios", "serving"])
+    ap.add_argument("--steps", type=int, default=16384,
+                    help="refinery / scenarios: iterations of the rate measurements; lp1m runs to 1e-8 and "
+                         "reports the iterations it ran as steps")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed warm-up iterations (whole 64-iteration blocks)")
+    ap.add_argument("--workload", default="lp1m", choices=["lp1m", "refinery", "scenarios", "serving", "copy"])
     ap.add_argument("--m", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--time-limit", type=float, default=900.0)
+    ap.add_argument("--time-limit", type=float, default=450.0)
+    ap.add_argument("--compare-seconds", type=float, default=30.0,
+                    help="lp1m: both restart schedules run this long and report the gap they reach (0 = skip)")
+    ap.add_argument("--copy-mib", type=int, default=64, help="copy: buffer size of the calibration copy")
     ap.add_argument("--iteration-limit", type=int, default=0,
                     help="stop each timed solve after this many iterations (profiling runs; 0 = to 1e-8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--epoch-cap", type=int, default=8192,
                     help="lp1m and the cost scenarios of scenarios: hip_max_epoch with hip_max_epoch_gap_only "
                          "(0 = the reference's restart schedule alone)")
-    ap.add_argument("--to-tolerance", action="store_true",
-                    help="lp1m: run the timed solve to 1e-8 (bounded by --time-limit) and report time_to_1e-8_s")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="lp1m: mixed = fp32 increments on fp64 anchors (the line's dtype says which)")
     ap.add_argument("--T", type=int, default=8760, help="refinery: planning periods of the rate measurement")
@@ -481,6 +541,9 @@ def main():
               "is not supported (multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
+        return
     import ps26119
     from ps26119_amd import lpgen
     if ps26119.device_count() <= 0:

@@ -210,7 +210,7 @@ def test_version_and_defaults():
 
 def test_bench_refuses_more_than_one_gpu_and_lists_its_workloads():
     """bench.py measures one GPU: --gpus 2 exits with a message before touching a device; --help
-    names the four workloads."""
+    names the workloads and the lp1m knobs."""
     import sys
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
@@ -218,5 +218,5 @@ def test_bench_refuses_more_than_one_gpu_and_lists_its_workloads():
     h = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert h.returncode == 0
-    for name in ("lp1m", "refinery", "scenarios", "serving", "--epoch-cap", "--to-tolerance"):
+    for name in ("lp1m", "refinery", "scenarios", "serving", "copy", "--epoch-cap", "--compare-seconds"):
         assert name in h.stdout, name
