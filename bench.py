@@ -443,7 +443,10 @@ def bench_serving(args, ps26119, lpgen):
     arrivals = np.cumsum(gaps) - gaps[0]
     lps = [lpgen.random_lp(int(m), seed=1000 + k) for k, m in enumerate(ms)]
     log(f"serving: {N} requests, m in [{ms.min()}, {ms.max()}], rate {args.rate}/s, budget {args.budget_gb} GB")
-    srv = ps26119.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_instances=256, setup_threads=8)
+    # HIP events around every chunk give the device time of the served iterations (the roofline
+    # below); --serving-eager launches the kernels directly instead of replaying graphs (profiling)
+    srv = ps26119.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_instances=256, setup_threads=8,
+                         eager=args.serving_eager, timing=True)
     ids = [None] * N
 
     def feed():
@@ -461,14 +464,24 @@ def bench_serving(args, ps26119, lpgen):
     th.join()
     res = [srv.wait(i) for i in ids]
     served = time.time() - t
+    kt = srv.kernel_time()
     srv.close()
+    roof = None
+    if kt and kt["iterations"]:
+        avg_us = kt["ms"] / kt["iterations"] * 1e3
+        gbs = kt["bytes"] / (kt["ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                "traffic": None, "kernel": "k_srv_primal + k_srv_dual (one served iteration of every live instance)",
+                "bytes_per_launch": kt["bytes"] / kt["iterations"], "avg_launch_us": avg_us,
+                "launches": kt["iterations"], "timing": "HIP events around every chunk on the serving stream"}
+        log(f"served iterations: {kt['iterations']}, avg {avg_us:.1f} us, {gbs:.0f} GB/s algorithmic")
     lat = np.array([r.stats["latency"] for r in res])
     ok = sum(r.status == ps26119.OPTIMAL for r in res)
     log(f"served: {ok}/{N} optimal in {served:.2f}s, p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
     if args.serving_only:
         line = base_line(args, N / served, "solves/s", True, "f32+f64" if args.serving_precision == "mixed" else "f64",
                          {"workload": f"serving: {N} requests, served part only", "requests": N})
-        line.update({"roofline": None, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
+        line.update({"roofline": roof, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
                      "p99_latency_s": float(np.percentile(lat, 99)), "cpu_baseline": None})
         return line
     # sequential plain fp64 solves of the same requests, back to back
@@ -489,7 +502,7 @@ def bench_serving(args, ps26119, lpgen):
                                   + "; baseline: the same requests as sequential plain fp64 solves",
                       "requests": N})
     line.update({
-        "roofline": None, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
+        "roofline": roof, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)), "served_seconds": served,
         "sequential": {"seconds": sequential, "solves_per_s": N / sequential},
         "speedup_vs_sequential": sequential / served,
@@ -535,6 +548,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=16.0, help="serving: HBM byte budget of the server")
     ap.add_argument("--serving-precision", default="mixed", choices=["fp64", "mixed"], help="serving: precision of the served requests")
     ap.add_argument("--serving-only", action="store_true", help="serving: skip the sequential comparison (profiling runs)")
+    ap.add_argument("--serving-eager", action="store_true",
+                    help="serving: launch the kernels directly and time every primal launch (the line's roofline)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU: the r2HPDHG path here runs on a single MI355X; --gpus {args.gpus} "

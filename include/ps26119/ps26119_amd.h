@@ -95,6 +95,8 @@ typedef struct {
   int max_instances;     /* live at once (0 = 256) */
   int check_every;       /* 0 = 64 */
   int setup_threads;     /* host workers for preparation and finish (0 = 4) */
+  int eager_launches;    /* 1: launch each chunk's kernels directly instead of replaying captured graphs */
+  int timing;            /* 1: time every chunk with HIP events (ps26119_amd_server_kernel_time) */
 } ps26119_amd_server_options;
 
 /* NULL on failure, with the reason in error (may be NULL). */
@@ -114,6 +116,11 @@ long long ps26119_amd_server_submit(ps26119_amd_server* s, int num_rows, int num
 int ps26119_amd_server_wait(ps26119_amd_server* s, long long id, double timeout_s, ps26119_result* result, double* x,
                             double* y, double* z, double* times);
 int ps26119_amd_server_in_flight(ps26119_amd_server* s);
+/* With timing = 1: the served iterations so far (each one primal and one dual half step over the
+ * live instances), their total device milliseconds (HIP events around every chunk on the serving
+ * stream) and the algorithmic bytes they moved. Returns 0, or -1 when the server was not created
+ * with timing. */
+int ps26119_amd_server_kernel_time(ps26119_amd_server* s, long long* iterations, double* total_ms, double* bytes);
 
 /* Number of visible GPUs the backend can use (0 when none). */
 int ps26119_amd_device_count(void);

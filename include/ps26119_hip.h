@@ -296,6 +296,15 @@ int psh_server_set_precision(psh_server* s, int id, int mixed);
 int psh_server_remove(psh_server* s, int id);
 int psh_server_set_iterate(psh_server* s, int id, const double* x, const double* y);
 int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* params, int count);
+/* 1 (default): a chunk replays a graph captured while the instance set is unchanged; 0: every
+ * chunk launches its kernels directly (per-kernel profiling, or callers that avoid graphs). */
+int psh_server_graphs(psh_server* s, int on);
+/* 1: HIP events on the server's stream around every chunk (a replayed graph has no host gaps
+ * between its launches); psh_server_timing_get returns the iterations timed, their total
+ * milliseconds and the algorithmic bytes of their two half steps (per iteration and active
+ * instance: 24·nnz + 4(m+n+2) + 48m + 64n in fp64, 16·nnz + 4(m+n+2) + 24m + 28n in increment form). */
+int psh_server_timing(psh_server* s, int on);
+int psh_server_timing_get(psh_server* s, long long* iterations, double* total_ms, double* bytes);
 /* want_ray: also psh_stats.ray (T(z) − z) of every listed instance, and its rays for
  * psh_server_ray_vectors (unscaled; dual ray m, primal ray n). */
 int psh_server_check(psh_server* s, const int* ids, int count, int want_ray, psh_stats* out);

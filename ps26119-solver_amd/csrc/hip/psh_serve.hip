@@ -174,8 +174,8 @@ struct SrvEpiDeltaD {
   }
 };
 
-__global__ void __launch_bounds__(kBlock) k_srv_primal(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
-                                                       const Param* __restrict__ P, int step, int last) {
+__device__ inline void srv_primal(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
+                                  const Param* __restrict__ P, int step, int last) {
   const int t = tile_of_block(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const Tile T = tiles[t];
@@ -191,6 +191,11 @@ __global__ void __launch_bounds__(kBlock) k_srv_primal(const Tile* __restrict__ 
   }
   SrvEpiPrimal e{d, q.tau, halpern_w(q.inner0, step), q.a2, q.b2, hat};
   tile_rows<double, double>(T.r0, T.r1, d->at_ptr, d->at_col, d->at_val, d->y, e);
+}
+
+__global__ void __launch_bounds__(kBlock) k_srv_primal(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
+                                                       const Param* __restrict__ P, int step, int last) {
+  srv_primal(tiles, ntiles, I, P, step, last);
 }
 
 __global__ void __launch_bounds__(kBlock) k_srv_dual(const Tile* __restrict__ tiles, int ntiles, const Inst* __restrict__ I,
@@ -590,6 +595,7 @@ struct psh_server {
     bool used = false;
     size_t off = 0, bytes = 0;
     int m = 0, n = 0;
+    long long nnz = 0;
     std::vector<int> tA, tAt;  // tile row starts of Ã and Ãᵀ
     Inst inst{};
     bool incr = false;         // increment-form storage present
@@ -610,6 +616,15 @@ struct psh_server {
   bool tables_dirty = true;
   bool param_pending = false;
   std::map<long long, hipGraphExec_t> graphs;  // by chunk length
+  bool use_graphs = true;
+  // timing of whole chunks (HIP events around each chunk's launches on the serving stream, which
+  // with graph replay carry no host gaps) with the chunk's iterations and algorithmic bytes
+  bool timing = false;
+  struct Timed { hipEvent_t a, b; int iterations; double bytes; };
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> ev_pool;
+  long long t_launches = 0;
+  double t_ms = 0.0, t_bytes = 0.0;
 };
 
 namespace {
@@ -774,6 +789,34 @@ int rebuild_tables(psh_server* s) {
 
 int grid_tiles(int ntiles) { return std::max(kXcd, grid_for_tiles(ntiles)); }
 
+hipEvent_t take_event(psh_server* s) {
+  if (!s->ev_pool.empty()) {
+    hipEvent_t e = s->ev_pool.back();
+    s->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+// Adds the timed chunks to the totals (synchronises).
+int collect_timing(psh_server* s) {
+  if (s->timed.empty()) return PSH_OK;
+  SRV_TRY(s, hipStreamSynchronize(s->stream));
+  for (auto& t : s->timed) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      s->t_ms += ms;
+      s->t_launches += t.iterations;
+      s->t_bytes += t.bytes;
+    }
+    s->ev_pool.push_back(t.a);
+    s->ev_pool.push_back(t.b);
+  }
+  s->timed.clear();
+  return PSH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -828,6 +871,8 @@ void psh_server_destroy(psh_server* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   drop_graphs(s);
+  for (auto& t : s->timed) s->ev_pool.push_back(t.a), s->ev_pool.push_back(t.b);
+  for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
   std::lock_guard<std::recursive_mutex> lock(psh::api_mutex());
   for (void* p : {static_cast<void*>(s->arena), static_cast<void*>(s->d_inst), static_cast<void*>(s->d_param),
                   static_cast<void*>(s->d_stats), static_cast<void*>(s->d_ptiles), static_cast<void*>(s->d_dtiles),
@@ -867,6 +912,7 @@ int psh_server_add(psh_server* s, const psh_problem* p, int flags, int* id) {
   sl.bytes = L.total;
   sl.m = p->m;
   sl.n = p->n;
+  sl.nnz = p->nnz;
   char* base = s->arena + off;
   Inst& d = sl.inst;
   d.m = p->m;
@@ -1031,6 +1077,33 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
     SRV_LAUNCHED(s, "serving iteration kernels");
     return PSH_OK;
   };
+  psh_server::Timed tm{nullptr, nullptr, iterations, 0.0};
+  if (s->timing) {  // events around the chunk's launches; bytes of its iterations over the active instances
+    for (int k = 0; k < s->max_inst; ++k) {
+      if (!s->h_param[k].active) continue;
+      const auto& sl = s->slots[k];
+      const double m = sl.m, n = sl.n, nz = static_cast<double>(sl.nnz);
+      const double pb = sl.mixed ? 8 * nz + 4 * (n + 1) + 4 * m + 24 * n : 12 * nz + 4 * (n + 1) + 8 * m + 56 * n;
+      const double db = sl.mixed ? 8 * nz + 4 * (m + 1) + 4 * n + 20 * m : 12 * nz + 4 * (m + 1) + 8 * n + 40 * m;
+      tm.bytes += iterations * (pb + db);
+    }
+    if (s->timed.size() >= 4096 && (rc = collect_timing(s))) return rc;
+    tm.a = take_event(s);
+    tm.b = take_event(s);
+    if (!tm.a || !tm.b) return sfail(s, PSH_E_HIP, "timing events could not be created");
+    SRV_TRY(s, hipEventRecord(tm.a, s->stream));
+  }
+  auto timed_end = [&]() -> int {
+    if (!s->timing) return PSH_OK;
+    SRV_TRY(s, hipEventRecord(tm.b, s->stream));
+    s->timed.push_back(tm);
+    return PSH_OK;
+  };
+  if (!s->use_graphs) {
+    if ((rc = enqueue())) return rc;
+    return timed_end();
+  }
+
   const long long key = iterations;
   auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {
@@ -1051,6 +1124,28 @@ int psh_server_chunk(psh_server* s, int iterations, const psh_serve_param* param
     it = s->graphs.emplace(key, exec).first;
   }
   SRV_TRY(s, hipGraphLaunch(it->second, s->stream));
+  return timed_end();
+}
+
+int psh_server_graphs(psh_server* s, int on) {
+  if (!s) return PSH_E_ARG;
+  s->use_graphs = on != 0;
+  return PSH_OK;
+}
+
+int psh_server_timing(psh_server* s, int on) {
+  if (!s) return PSH_E_ARG;
+  s->timing = on != 0;
+  return PSH_OK;
+}
+
+int psh_server_timing_get(psh_server* s, long long* iterations, double* total_ms, double* bytes) {
+  if (!s) return PSH_E_ARG;
+  int rc;
+  if ((rc = collect_timing(s))) return rc;
+  if (iterations) *iterations = s->t_launches;
+  if (total_ms) *total_ms = s->t_ms;
+  if (bytes) *bytes = s->t_bytes;
   return PSH_OK;
 }
 

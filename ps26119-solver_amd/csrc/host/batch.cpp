@@ -60,6 +60,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
   if (auto err = base.validate(); !err.empty()) return fail_all("invalid base model: " + err);
   EngineOptions eo = engine_options_from(options);
   if (auto bad = apply_engine_params(options, eo); !bad.empty()) return fail_all("unknown engine parameter '" + bad + "'");
+  if (eo.max_epoch_gap_only == 2) return fail_all("hip_max_epoch_gap_only = 2 is implemented for single solves only");
   if (!options.use_gpu) return fail_all("the MI355X backend serves GPU solves: set use_gpu = 1");
   const std::vector<double>& warm_x = options.warm_x;
   const std::vector<double>& warm_y = options.warm_y;

@@ -317,6 +317,8 @@ ps26119_amd_server* ps26119_amd_server_create(const ps26119_amd_server_options* 
       if (o->max_instances > 0) so.max_instances = o->max_instances;
       if (o->check_every > 0) so.check_every = o->check_every;
       if (o->setup_threads > 0) so.setup_threads = o->setup_threads;
+      so.graphs = o->eager_launches == 0;
+      so.timing = o->timing != 0;
     }
     auto s = std::make_unique<ps26119_amd_server>();
     s->srv = std::make_unique<amd::Server>(so);
@@ -403,6 +405,16 @@ int ps26119_amd_server_wait(ps26119_amd_server* s, long long id, double timeout_
 }
 
 int ps26119_amd_server_in_flight(ps26119_amd_server* s) { return s ? s->srv->in_flight() : 0; }
+
+int ps26119_amd_server_kernel_time(ps26119_amd_server* s, long long* launches, double* total_ms, double* bytes) {
+  long long l = 0;
+  double ms = 0.0, b = 0.0;
+  if (!s || !s->srv->kernel_time(l, ms, b)) return -1;
+  if (launches) *launches = l;
+  if (total_ms) *total_ms = ms;
+  if (bytes) *bytes = b;
+  return 0;
+}
 
 int ps26119_amd_device_count(void) { return psh_device_count(); }
 

@@ -73,6 +73,8 @@ Server::Server(const ServerOptions& o) : opt_(o), t0_(clock_s()) {
     dev_ = nullptr;
     return;
   }
+  psh_server_graphs(dev_, opt_.graphs ? 1 : 0);
+  psh_server_timing(dev_, opt_.timing ? 1 : 0);
   capacity_ = psh_server_free_bytes(dev_);
   for (int k = 0; k < std::max(1, opt_.setup_threads); ++k) workers_.emplace_back([this] { setup_worker(); });
   loop_ = std::thread([this] { serve_loop(); });
@@ -91,6 +93,12 @@ Server::~Server() {
 }
 
 double Server::now() const { return clock_s() - t0_; }
+
+bool Server::kernel_time(long long& iterations, double& total_ms, double& bytes) {
+  if (!dev_ || !opt_.timing) return false;
+  std::lock_guard<std::mutex> g(dev_mu_);  // between two rounds of the serving thread
+  return psh_server_timing_get(dev_, &iterations, &total_ms, &bytes) == PSH_OK;
+}
 
 int Server::in_flight() const {
   std::lock_guard<std::mutex> g(mu_);
@@ -163,6 +171,8 @@ void Server::prepare(Request& r) {
   r.eo = engine_options_from(r.options);
   if (auto bad = apply_engine_params(r.options, r.eo); !bad.empty())
     return reject(Status::NotSolved, "unknown engine parameter '" + bad + "'");
+  if (r.eo.max_epoch_gap_only == 2)
+    return reject(Status::NotSolved, "hip_max_epoch_gap_only = 2 is implemented for single solves only");
   r.sp = std::make_unique<ScaledProblem>(make_scaled_problem(r.model, r.eo.scaling));
   const ScaledProblem& sp = *r.sp;
   if (sp.A.nnz() >= (1LL << 31)) return reject(Status::NotSolved, "matrix beyond the 32-bit device index range");
@@ -298,22 +308,31 @@ void Server::serve_loop() {
     r.primal_ray = std::move(pr);
     return true;
   };
+  // A device error ends every live request; their slots and arena space are released so that
+  // later requests can still be admitted.
   auto fail_live = [&](const std::string& why) {
     for (auto& r : live) {
+      (void)psh_server_remove(dev_, r->slot);
       r->status = Status::NumericalError;
       r->result.solution.status = Status::NumericalError;
       r->result.solution.message = why;
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        --live_;
+      }
       complete(r);
     }
     live.clear();
   };
   for (;;) {
+    std::unique_lock<std::mutex> round;  // this round's device calls (released at the end of the round)
     // admission: prepared requests in arrival order while the budget has room
     std::vector<std::shared_ptr<Request>> admit;
     {
       std::unique_lock<std::mutex> lk(mu_);
       if (live.empty()) cv_serve_.wait(lk, [&] { return stop_ || !ready_.empty(); });
       if (stop_) break;  // destruction abandons the requests still in flight
+      round = std::unique_lock<std::mutex>(dev_mu_);
       std::size_t free_bytes = psh_server_free_bytes(dev_);
       while (!ready_.empty() && static_cast<int>(live.size() + admit.size()) < opt_.max_instances) {
         auto& r = ready_.front();
@@ -331,7 +350,7 @@ void Server::serve_loop() {
         rc = psh_server_set_iterate(dev_, id, r->x_init.data(), y0.data());
       }
       if (rc != PSH_OK) {
-        if (rc == PSH_E_OOM) {  // fragmentation: back to the queue head
+        if (rc == PSH_E_OOM && !live.empty()) {  // fragmentation: back to the queue head until a retirement
           std::lock_guard<std::mutex> g(mu_);
           ready_.push_front(r);
           continue;

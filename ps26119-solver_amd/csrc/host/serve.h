@@ -33,6 +33,8 @@ struct ServerOptions {
   int max_instances = 256;                          // live instances at once
   int check_every = 64;                             // K
   int setup_threads = 4;                            // host workers for setup and finish
+  bool graphs = true;                               // replay captured chunk graphs (false: eager launches)
+  bool timing = false;                              // HIP-event timing of every chunk
 };
 
 struct ServedResult {
@@ -61,6 +63,8 @@ class Server {
   bool wait(std::int64_t id, ServedResult& out, double timeout_s = -1.0);
   // Requests queued, being prepared or live on the device.
   int in_flight() const;
+  // timed serving (ServerOptions::timing): iterations, their device milliseconds and bytes
+  bool kernel_time(long long& iterations, double& total_ms, double& bytes);
   double now() const;
 
  private:
@@ -78,6 +82,7 @@ class Server {
   double t0_ = 0;
 
   mutable std::mutex mu_;
+  std::mutex dev_mu_;  // held by the serving thread for each round of device calls
   std::condition_variable cv_work_, cv_done_, cv_serve_;
   std::deque<std::shared_ptr<Request>> to_prepare_, to_finish_, ready_;
   std::map<std::int64_t, std::shared_ptr<Request>> all_;

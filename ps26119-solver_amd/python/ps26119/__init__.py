@@ -82,7 +82,8 @@ class Scenario(ctypes.Structure):  # ps26119_amd_scenario
 
 class ServerOptions(ctypes.Structure):  # ps26119_amd_server_options
     _fields_ = [("device", ctypes.c_int), ("budget_bytes", ctypes.c_double), ("max_instances", ctypes.c_int),
-                ("check_every", ctypes.c_int), ("setup_threads", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("setup_threads", ctypes.c_int), ("eager_launches", ctypes.c_int),
+                ("timing", ctypes.c_int)]
 
 
 def _load():
@@ -112,6 +113,9 @@ def _load():
     lib.ps26119_amd_server_wait.restype = ctypes.c_int
     lib.ps26119_amd_server_in_flight.argtypes = [ctypes.c_void_p]
     lib.ps26119_amd_server_in_flight.restype = ctypes.c_int
+    lib.ps26119_amd_server_kernel_time.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                                                   D, D]
+    lib.ps26119_amd_server_kernel_time.restype = ctypes.c_int
     lib.ps26119_default_options.argtypes = [ctypes.POINTER(Options)]
     lib.ps26119_amd_default_options.argtypes = [ctypes.POINTER(AmdOptions)]
     lib.ps26119_solve_mps.argtypes = [ctypes.c_char_p, ctypes.POINTER(Options), ctypes.POINTER(CResult),
@@ -384,9 +388,13 @@ class Server:
     Requests are prepared on host threads, admitted while the HBM budget has room, advanced
     together as one packed block-diagonal system and finished on their own schedule."""
 
-    def __init__(self, budget_bytes=8 << 30, max_instances=256, check_every=0, setup_threads=4, device=0):
+    def __init__(self, budget_bytes=8 << 30, max_instances=256, check_every=0, setup_threads=4, device=0,
+                 eager=False, timing=False):
+        """eager: launch every chunk's kernels directly instead of replaying captured graphs;
+        timing: HIP events around every chunk (kernel_time())."""
         L = lib()
-        o = ServerOptions(device, float(budget_bytes), max_instances, check_every, setup_threads)
+        o = ServerOptions(device, float(budget_bytes), max_instances, check_every, setup_threads, int(bool(eager)),
+                          int(bool(timing)))
         err = ctypes.create_string_buffer(512)
         self._h = L.ps26119_amd_server_create(ctypes.byref(o), err, 512)
         if not self._h:
@@ -440,6 +448,15 @@ class Server:
 
     def in_flight(self) -> int:
         return lib().ps26119_amd_server_in_flight(self._h)
+
+    def kernel_time(self):
+        """With timing=True: {iterations, ms, bytes} of the served iterations so far (device time
+        of both half steps over the live instances, and their algorithmic bytes)."""
+        n = ctypes.c_longlong(0)
+        ms, b = ctypes.c_double(0), ctypes.c_double(0)
+        if lib().ps26119_amd_server_kernel_time(self._h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)) != 0:
+            return None
+        return {"iterations": n.value, "ms": ms.value, "bytes": b.value}
 
     def close(self):
         if self._h:
