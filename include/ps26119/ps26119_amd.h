@@ -155,6 +155,17 @@ int ps26119_amd_presolve_info(int num_rows, int num_cols, int sense, double obj_
                               const double* row_upper, const int* col_start, const int* row_index,
                               const double* value, double* out);
 
+/* The reduced model itself: info as ps26119_presolve_info's out; when info[0] = 0 (reduced) the
+ * kept rows and columns (reduced → original index), the reduced row and column bounds and the
+ * reduced CSC (col_start: reduced_cols + 1, row_index / value: reduced_nnz). Arrays sized for the
+ * original model suffice; any may be NULL. */
+int ps26119_amd_presolve_model(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                               const double* col_lower, const double* col_upper, const double* row_lower,
+                               const double* row_upper, const int* col_start, const int* row_index,
+                               const double* value, double* info, int* kept_rows, int* kept_cols, double* rl,
+                               double* ru, double* cl, double* cu, int* r_col_start, int* r_row_index,
+                               double* r_value);
+
 #ifdef __cplusplus
 }
 #endif

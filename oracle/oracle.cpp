@@ -2142,6 +2142,34 @@ int orc_solve(const orc_lp* lp, const orc_opts* o, orc_result* r, double* x, dou
   return r->status;
 }
 
+// The oracle's presolve result, in ps26119_amd_presolve_model's layout (test comparison).
+int orc_presolve(const orc_lp* lp, double* info, int* kept_rows, int* kept_cols, double* rl, double* ru, double* cl,
+                 double* cu, int* r_col_start, int* r_row_index, double* r_value) {
+  const Lp l = from_abi(lp);
+  const Pre r = presolve(l);
+  info[0] = r.outcome;
+  info[1] = r.rows_removed;
+  info[2] = r.cols_removed;
+  info[3] = r.red.m;
+  info[4] = r.red.n;
+  info[5] = static_cast<double>(r.red.v.size());
+  info[6] = r.red.offset;
+  if (r.outcome != 0) return 0;
+  auto put = [](auto* dst, const auto& src) {
+    if (dst && !src.empty()) std::copy(src.begin(), src.end(), dst);
+  };
+  put(kept_rows, r.row_map);
+  put(kept_cols, r.col_map);
+  put(rl, r.red.rl);
+  put(ru, r.red.ru);
+  put(cl, r.red.cl);
+  put(cu, r.red.cu);
+  put(r_col_start, r.red.cs);
+  put(r_row_index, r.red.ri);
+  put(r_value, r.red.v);
+  return 0;
+}
+
 int orc_scale(const orc_lp* lp, int geo_iters, double geo_min_range, int ruiz, int pc, int bor, double* row_scale,
               double* col_scale, double* scalars, double* a_vals, double* at_vals, double* c_scaled, double* bounds) {
   const Lp l = from_abi(lp);

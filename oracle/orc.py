@@ -99,6 +99,9 @@ class Checker:
         self._batch = fn("solve_batch", [L, ctypes.c_int, ctypes.POINTER(OrcScenario), ctypes.POINTER(OrcOpts),
                                          ctypes.POINTER(OrcResult), D, D])
         self._check = fn("check_solution", [L, D, D, D])
+        I = ctypes.POINTER(ctypes.c_int)
+        self._presolve = (fn("presolve", [L, D, I, I, D, D, D, D, I, I, D])
+                          if hasattr(self.lib, PREFIX[kind] + "presolve") else None)
 
     def solve(self, lp, precision=0, presolve=True, tolerance=1e-8, iteration_limit=0, time_limit=0.0,
               threads=1, check_every=0, warm_x=None, warm_y=None, warm_primal_weight=0.0, params=None, algorithm=3):
@@ -156,6 +159,21 @@ class Checker:
                 "col_lower": bnd[:lp.n], "col_upper": bnd[lp.n:2 * lp.n], "row_lower": bnd[2 * lp.n:2 * lp.n + lp.m],
                 "row_upper": bnd[2 * lp.n + lp.m:]}
 
+    def presolve(self, lp):
+        """The reduced model as arrays (the oracle build only): info = {outcome (0 reduced,
+        1 infeasible, 2 unchanged), rows removed, cols removed, rows, cols, nnz, offset}."""
+        h = LpHandle(lp)
+        info = np.zeros(7)
+        I = ctypes.c_int
+        out = {"kept_rows": np.zeros(lp.m, np.int32), "kept_cols": np.zeros(lp.n, np.int32), "rl": np.zeros(lp.m),
+               "ru": np.zeros(lp.m), "cl": np.zeros(lp.n), "cu": np.zeros(lp.n),
+               "col_start": np.zeros(lp.n + 1, np.int32), "row_index": np.zeros(max(1, lp.nnz), np.int32),
+               "value": np.zeros(max(1, lp.nnz))}
+        self._presolve(ctypes.byref(h.s), _p(info), _p(out["kept_rows"], I), _p(out["kept_cols"], I), _p(out["rl"]),
+                       _p(out["ru"]), _p(out["cl"]), _p(out["cu"]), _p(out["col_start"], I), _p(out["row_index"], I),
+                       _p(out["value"]))
+        return info, _trim(info, out)
+
     def norm2(self, lp) -> float:
         h = LpHandle(lp)
         return self._norm2(ctypes.byref(h.s))
@@ -190,6 +208,16 @@ class Checker:
         self._check(ctypes.byref(h.s), _p(x), _p(y), _p(out))
         return dict(zip(["max_primal_violation", "max_dual_violation", "relative_gap", "primal_objective",
                          "dual_objective", "passed"], out))
+
+
+def _trim(info, out):
+    """The reduced model's arrays cut to its sizes (nothing when presolve did not reduce)."""
+    if info[0] != 0:
+        return {}
+    m, n, nz = int(info[3]), int(info[4]), int(info[5])
+    return {"kept_rows": out["kept_rows"][:m], "kept_cols": out["kept_cols"][:n], "rl": out["rl"][:m],
+            "ru": out["ru"][:m], "cl": out["cl"][:n], "cu": out["cu"][:n], "col_start": out["col_start"][:n + 1],
+            "row_index": out["row_index"][:nz], "value": out["value"][:nz]}
 
 
 def load(kind: str) -> Checker:

@@ -929,6 +929,12 @@ int psh_batch_chunk(psh_ctx* c, int iterations, const double* tau, const double*
   } else {
     const long long key = (static_cast<long long>(iterations) << 17) | (static_cast<long long>(W) << 5) |
                           (any_new ? 1 : 0) | (static_cast<long long>(mode & 7) << 1) | (delta ? 16 : 0);
+    if (rho != b.graphs_rho) {  // ρ is an argument of the captured launches: recapture
+      PSH_TRY(c, hipStreamSynchronize(c->stream));
+      for (auto& kv : b.graphs) (void)hipGraphExecDestroy(kv.second);
+      b.graphs.clear();
+      b.graphs_rho = rho;
+    }
     auto it = b.graphs.find(key);
     if (it == b.graphs.end()) {
       std::lock_guard<std::recursive_mutex> capture_lock(api_mutex());  // no device-wide calls meanwhile

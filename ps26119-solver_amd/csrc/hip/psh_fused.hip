@@ -895,6 +895,12 @@ int psh_chunk(psh_ctx* c, const psh_chunk_args* g) {
     const long long key = (static_cast<long long>(g->iterations) << 9) | (c->fp32 ? 1 : 0) | (epoch_start ? 2 : 0) |
                           (hat_last ? 4 : 0) | (static_cast<long long>(g->mode & 7) << 3) | (delta ? 1LL << 6 : 0) |
                           (ph_primal || ph_dual ? 1LL << 7 : 0) | (delta && c->delta.bf16 ? 1LL << 8 : 0);
+    if (g->rho != c->graph_rho) {  // ρ is an argument of the captured launches: recapture
+      PSH_TRY(c, hipStreamSynchronize(c->stream));
+      for (auto& kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
+      c->graph_cache.clear();
+      c->graph_rho = g->rho;
+    }
     auto it = c->graph_cache.find(key);
     if (it == c->graph_cache.end()) {
       std::lock_guard<std::recursive_mutex> capture_lock(api_mutex());  // no device-wide calls meanwhile

@@ -147,6 +147,7 @@ struct BatchState : BatchArrays {
   bool have_r0 = false;
   unsigned pin_pending = 0;                    // regions of `pinned` a queued copy may still read
   std::map<long long, hipGraphExec_t> graphs;  // captured chunks
+  double graphs_rho = 1.0;                     // the reflection they were captured with
   bool delta = false;
   std::vector<double> ky_sigma;  // σ each slot's KY was built with
   std::vector<char> dirty;       // slots to re-anchor before the next chunk
@@ -212,6 +213,7 @@ struct psh_ctx {
   double* prm = nullptr;
   bool graphs = true;
   std::map<long long, hipGraphExec_t> graph_cache;
+  double graph_rho = 1.0;  // the reflection the cached chunk graphs were captured with
 
   // timing
   bool timing = false;

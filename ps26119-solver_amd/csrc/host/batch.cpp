@@ -26,9 +26,68 @@ namespace ps26119::amd {
 
 namespace {
 
-struct ScenarioData {  // original space, original sense
+// A scenario's five vectors (original space and sense), the base model's where the scenario
+// leaves one empty, with what keeps it from being solved: `defect` (a vector of the wrong
+// length, a non-finite cost, a NaN bound) or `crossing` (a bound range that is empty, so the
+// scenario is infeasible as given).
+struct ScenarioVectors {
   std::vector<double> c, cl, cu, rl, ru;
+  std::string defect, crossing;
 };
+
+ScenarioVectors overlay(const Model& base, const Scenario& s) {
+  ScenarioVectors v;
+  const std::size_t n = static_cast<std::size_t>(base.num_cols), m = static_cast<std::size_t>(base.num_rows);
+  const struct {
+    const std::vector<double>& given;
+    const std::vector<double>& fallback;
+    std::vector<double>& into;
+    std::size_t len;
+    const char* name;
+  } parts[] = {{s.obj, base.obj, v.c, n, "objective"},
+               {s.col_lower, base.col_lower, v.cl, n, "column lower bounds"},
+               {s.col_upper, base.col_upper, v.cu, n, "column upper bounds"},
+               {s.row_lower, base.row_lower, v.rl, m, "row lower bounds"},
+               {s.row_upper, base.row_upper, v.ru, m, "row upper bounds"}};
+  for (const auto& p : parts) {
+    if (!p.given.empty() && p.given.size() != p.len) {
+      v.defect = std::string("scenario ") + p.name + " have " + std::to_string(p.given.size()) + " entries, not " +
+                 std::to_string(p.len);
+      return v;
+    }
+    p.into = p.given.empty() ? p.fallback : p.given;
+  }
+  char buf[160];
+  for (std::size_t j = 0; j < n; ++j)
+    if (!std::isfinite(v.c[j]) || std::isnan(v.cl[j]) || std::isnan(v.cu[j])) {
+      std::snprintf(buf, sizeof buf, "scenario column %zu has a non-finite cost or a NaN bound", j);
+      v.defect = buf;
+      return v;
+    }
+  for (std::size_t i = 0; i < m; ++i)
+    if (std::isnan(v.rl[i]) || std::isnan(v.ru[i])) {
+      std::snprintf(buf, sizeof buf, "scenario row %zu has a NaN bound", i);
+      v.defect = buf;
+      return v;
+    }
+  for (std::size_t j = 0; j < n && v.crossing.empty(); ++j)
+    if (v.cl[j] > v.cu[j]) std::snprintf(buf, sizeof buf, "column %zu of the scenario", j), v.crossing = buf;
+  for (std::size_t i = 0; i < m && v.crossing.empty(); ++i)
+    if (v.rl[i] > v.ru[i]) std::snprintf(buf, sizeof buf, "row %zu of the scenario", i), v.crossing = buf;
+  return v;
+}
+
+// Why the whole batch cannot run (every scenario then gets NotSolved with it), or "".
+std::string batch_refusal(const Model& base, const Options& options, EngineOptions& eo) {
+  if (auto err = base.validate(); !err.empty()) return "the shared model is invalid: " + err;
+  if (auto bad = apply_engine_params(options, eo); !bad.empty()) return "unknown engine parameter '" + bad + "'";
+  if (eo.max_epoch_gap_only == 2) return "hip_max_epoch_gap_only = 2 is implemented for single solves only";
+  if (!options.use_gpu) return "the MI355X backend serves GPU solves: set use_gpu = 1";
+  if ((!options.warm_x.empty() && static_cast<int>(options.warm_x.size()) != base.num_cols) ||
+      (!options.warm_y.empty() && static_cast<int>(options.warm_y.size()) != base.num_rows))
+    return "a warm start vector does not match the shared model's size";
+  return {};
+}
 
 }  // namespace
 
@@ -52,47 +111,25 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
       out[k].model_fingerprint = w.fingerprint_hex();
     }
   }
-  auto fail_all = [&](const std::string& why) {
+  if (K == 0) return out;
+  EngineOptions eo = engine_options_from(options);
+  if (const std::string why = batch_refusal(base, options, eo); !why.empty()) {
     for (auto& s : out) s.status = Status::NotSolved, s.message = why;
     return out;
-  };
-  if (K == 0) return out;
-  if (auto err = base.validate(); !err.empty()) return fail_all("invalid base model: " + err);
-  EngineOptions eo = engine_options_from(options);
-  if (auto bad = apply_engine_params(options, eo); !bad.empty()) return fail_all("unknown engine parameter '" + bad + "'");
-  if (eo.max_epoch_gap_only == 2) return fail_all("hip_max_epoch_gap_only = 2 is implemented for single solves only");
-  if (!options.use_gpu) return fail_all("the MI355X backend serves GPU solves: set use_gpu = 1");
+  }
   const std::vector<double>& warm_x = options.warm_x;
   const std::vector<double>& warm_y = options.warm_y;
-  if ((!warm_x.empty() && static_cast<int>(warm_x.size()) != n) || (!warm_y.empty() && static_cast<int>(warm_y.size()) != m))
-    return fail_all("warm start vectors have the wrong size");
   set_host_threads(options.threads);
 
-  std::vector<ScenarioData> D(K);
-  for (int k = 0; k < K; ++k) {  // reference batch.cpp:152-179
-    const Scenario& s = scenarios[k];
-    auto pick = [](const std::vector<double>& v, const std::vector<double>& d, std::size_t len, bool& ok) {
-      if (v.empty()) return d;
-      ok = ok && v.size() == len;
-      return v;
-    };
-    bool ok = true;
-    D[k].c = pick(s.obj, base.obj, n, ok);
-    D[k].cl = pick(s.col_lower, base.col_lower, n, ok);
-    D[k].cu = pick(s.col_upper, base.col_upper, n, ok);
-    D[k].rl = pick(s.row_lower, base.row_lower, m, ok);
-    D[k].ru = pick(s.row_upper, base.row_upper, m, ok);
-    for (int j = 0; ok && j < n; ++j) ok = std::isfinite(D[k].c[j]) && !std::isnan(D[k].cl[j]) && !std::isnan(D[k].cu[j]);
-    for (int i = 0; ok && i < m; ++i) ok = !std::isnan(D[k].rl[i]) && !std::isnan(D[k].ru[i]);
-    bool crossed = false;
-    for (int j = 0; ok && !crossed && j < n; ++j) crossed = D[k].cl[j] > D[k].cu[j];
-    for (int i = 0; ok && !crossed && i < m; ++i) crossed = D[k].rl[i] > D[k].ru[i];
-    if (!ok) {
+  std::vector<ScenarioVectors> D(K);
+  for (int k = 0; k < K; ++k) {
+    D[k] = overlay(base, scenarios[k]);
+    if (!D[k].defect.empty()) {
       out[k].status = Status::NotSolved;
-      out[k].message = "scenario has wrong vector sizes or non-finite data";
-    } else if (crossed) {
+      out[k].message = D[k].defect;
+    } else if (!D[k].crossing.empty()) {
       out[k].status = Status::Infeasible;
-      out[k].message = "bounds cross: scenario has a lower bound above its upper bound";
+      out[k].message = "bounds cross: " + D[k].crossing + " has a lower bound above its upper bound";
     }
   }
 
@@ -101,7 +138,10 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
     so.bound_objective_rescaling = false;  // per scenario below (reference batch.cpp:182-221)
     std::string err;
     std::unique_ptr<Device> dev = Device::open(eo.device, err);
-    if (!dev) return fail_all(err);
+    if (!dev) {
+      for (auto& r : out) r.status = Status::NotSolved, r.message = err;
+      return out;
+    }
     ScaledProblem sp;
     scale_and_load(*dev, base, so, eo.device_scaling != 0, sp);
     psh_ctx* c = dev->ctx();
@@ -126,7 +166,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
       const std::size_t NW = static_cast<std::size_t>(n) * W, MW = static_cast<std::size_t>(m) * W;
       std::vector<double> C(NW), L(NW), U(NW), RL(MW), RU(MW), CM(NW), CLO(NW), CUO(NW), RLO(MW), RUO(MW);
       for (int t = 0; t < W; ++t) {
-        const ScenarioData& d = D[sid[t]];
+        const ScenarioVectors& d = D[sid[t]];
         if (eo.scaling.bound_objective_rescaling) {
           double b2 = 0, c2 = 0;
           for (int i = 0; i < m; ++i) {
@@ -165,7 +205,7 @@ std::vector<Solution> solve_batch(const Model& base, const std::vector<Scenario>
       // initial iterate: x = proj(0), y = 0 (reference batch.cpp:238-242), or the warm point
       std::vector<double> X(NW), Y(MW, 0.0);
       for (int t = 0; t < W; ++t) {
-        const ScenarioData& d = D[sid[t]];
+        const ScenarioVectors& d = D[sid[t]];
         for (int j = 0; j < n; ++j) {
           const std::size_t q = static_cast<std::size_t>(j) * W + t;
           if (!warm_x.empty()) {

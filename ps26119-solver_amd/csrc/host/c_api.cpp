@@ -493,14 +493,55 @@ int ps26119_amd_presolve_info(int num_rows, int num_cols, int sense, double obj_
     if (!out || make_model(num_rows, num_cols, sense, obj_offset, c, col_lower, col_upper, row_lower, row_upper, col_start,
                            row_index, value, m, err))
       return PS26119_INVALID_ARGUMENT;
-    const amd::PresolveResult r = amd::presolve(m);
-    out[0] = r.outcome == amd::PresolveResult::Outcome::Reduced ? 0 : (r.outcome == amd::PresolveResult::Outcome::Infeasible ? 1 : 2);
-    out[1] = r.removed_rows;
-    out[2] = r.removed_cols;
+    const amd::Presolved r = amd::presolve(m);
+    using O = amd::Presolved::Outcome;
+    out[0] = r.outcome == O::Reduced ? 0 : (r.outcome == O::ProvedInfeasible ? 1 : 2);
+    out[1] = r.rows_dropped;
+    out[2] = r.cols_dropped;
     out[3] = r.reduced.num_rows;
     out[4] = r.reduced.num_cols;
     out[5] = static_cast<double>(r.reduced.nnz());
     out[6] = r.reduced.obj_offset;
+    return PS26119_OPTIMAL;
+  } catch (...) {
+    return PS26119_NUMERICAL_ERROR;
+  }
+}
+
+int ps26119_amd_presolve_model(int num_rows, int num_cols, int sense, double obj_offset, const double* c,
+                               const double* col_lower, const double* col_upper, const double* row_lower,
+                               const double* row_upper, const int* col_start, const int* row_index, const double* value,
+                               double* info, int* kept_rows, int* kept_cols, double* rl, double* ru, double* cl,
+                               double* cu, int* r_col_start, int* r_row_index, double* r_value) {
+  try {
+    Model m;
+    std::string err;
+    if (!info || make_model(num_rows, num_cols, sense, obj_offset, c, col_lower, col_upper, row_lower, row_upper, col_start,
+                            row_index, value, m, err))
+      return PS26119_INVALID_ARGUMENT;
+    const amd::Presolved r = amd::presolve(m);
+    using O = amd::Presolved::Outcome;
+    info[0] = r.outcome == O::Reduced ? 0 : (r.outcome == O::ProvedInfeasible ? 1 : 2);
+    info[1] = r.rows_dropped;
+    info[2] = r.cols_dropped;
+    info[3] = r.reduced.num_rows;
+    info[4] = r.reduced.num_cols;
+    info[5] = static_cast<double>(r.reduced.nnz());
+    info[6] = r.reduced.obj_offset;
+    if (r.outcome != O::Reduced) return PS26119_OPTIMAL;
+    const Model& R = r.reduced;
+    auto put = [](auto* dst, const auto& src) {
+      if (dst && !src.empty()) std::copy(src.begin(), src.end(), dst);
+    };
+    put(kept_rows, r.kept_rows);
+    put(kept_cols, r.kept_cols);
+    put(rl, R.row_lower);
+    put(ru, R.row_upper);
+    put(cl, R.col_lower);
+    put(cu, R.col_upper);
+    put(r_col_start, R.col_start);
+    put(r_row_index, R.row_index);
+    put(r_value, R.value);
     return PS26119_OPTIMAL;
   } catch (...) {
     return PS26119_NUMERICAL_ERROR;

@@ -6,7 +6,6 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <sstream>
 
 #include "contract.h"
 
@@ -71,73 +70,93 @@ std::string Model::fingerprint_hex() const {
   return b;
 }
 
+// The first defect of the model contract found, in the order: sizes and sense, then column by
+// column (pointers, objective, entries), then the bound vectors; "" when the model is usable.
 std::string Model::validate() const {
-  const auto n = static_cast<std::size_t>(num_cols), m = static_cast<std::size_t>(num_rows);
-  if (num_rows < 0 || num_cols < 0) return "negative dimension";
-  if (sense != 1 && sense != -1) return "sense must be +1 or -1";
-  if (!std::isfinite(obj_offset)) return "obj_offset is not finite";
-  if (obj.size() != n) return "obj has wrong size";
-  if (col_lower.size() != n || col_upper.size() != n) return "column bounds have wrong size";
-  if (row_lower.size() != m || row_upper.size() != m) return "row bounds have wrong size";
-  if (col_start.size() != n + 1) return "col_start must have num_cols+1 entries";
-  if (col_start[0] != 0) return "col_start[0] must be 0";
-  if (row_index.size() != value.size()) return "row_index and value differ in size";
-  if (static_cast<std::size_t>(col_start[n]) != value.size()) return "col_start[n] != nnz";
-  if (!is_integer.empty() && is_integer.size() != n) return "is_integer has wrong size";
-  std::vector<int> seen(m, -1);
-  std::ostringstream err;
+  const std::size_t n = num_cols < 0 ? 0 : static_cast<std::size_t>(num_cols);
+  const std::size_t m = num_rows < 0 ? 0 : static_cast<std::size_t>(num_rows);
+  const struct {
+    bool bad;
+    const char* what;
+  } shape[] = {
+      {num_rows < 0 || num_cols < 0, "a dimension is negative"},
+      {sense != 1 && sense != -1, "sense is neither +1 (min) nor -1 (max)"},
+      {!std::isfinite(obj_offset), "the objective offset is not finite"},
+      {obj.size() != n, "the objective does not have num_cols entries"},
+      {col_lower.size() != n || col_upper.size() != n, "a column bound vector does not have num_cols entries"},
+      {row_lower.size() != m || row_upper.size() != m, "a row bound vector does not have num_rows entries"},
+      {col_start.size() != n + 1, "col_start does not have num_cols + 1 entries"},
+      {!col_start.empty() && col_start[0] != 0, "col_start does not begin at 0"},
+      {row_index.size() != value.size(), "row_index and value differ in length"},
+      {col_start.size() == n + 1 && static_cast<std::size_t>(col_start[n]) != value.size(),
+       "col_start does not end at the number of entries"},
+      {!is_integer.empty() && is_integer.size() != n, "is_integer does not have num_cols entries"},
+  };
+  for (const auto& c : shape)
+    if (c.bad) return c.what;
+  char buf[160];
+  std::vector<int> last_col(m, -1);  // the column that last used each row: duplicate detection
   for (std::size_t j = 0; j < n; ++j) {
+    const int col = static_cast<int>(j);
     if (col_start[j + 1] < col_start[j]) {
-      err << "col_start not monotone at column " << j;
-      return err.str();
+      std::snprintf(buf, sizeof buf, "col_start decreases at column %d", col);
+      return buf;
     }
     if (!std::isfinite(obj[j])) {
-      err << "objective coefficient of column " << j << " is not finite";
-      return err.str();
+      std::snprintf(buf, sizeof buf, "column %d has a non-finite objective coefficient", col);
+      return buf;
     }
     for (int k = col_start[j]; k < col_start[j + 1]; ++k) {
       const int i = row_index[k];
-      if (i < 0 || static_cast<std::size_t>(i) >= m) {
-        err << "row index out of range in column " << j;
-        return err.str();
+      if (i < 0 || static_cast<std::size_t>(i) >= m)
+        std::snprintf(buf, sizeof buf, "column %d names row %d, outside [0, %d)", col, i, num_rows);
+      else if (last_col[i] == col)
+        std::snprintf(buf, sizeof buf, "column %d has two entries in row %d", col, i);
+      else if (!std::isfinite(value[k]))
+        std::snprintf(buf, sizeof buf, "entry (%d, %d) is not finite", i, col);
+      else {
+        last_col[i] = col;
+        continue;
       }
-      if (seen[i] == static_cast<int>(j)) {
-        err << "duplicate entry (" << i << "," << j << ")";
-        return err.str();
-      }
-      seen[i] = static_cast<int>(j);
-      if (!std::isfinite(value[k])) {
-        err << "matrix entry (" << i << "," << j << ") is not finite";
-        return err.str();
-      }
+      return buf;
     }
   }
-  auto bounds = [](const std::vector<double>& lo, const std::vector<double>& up, const char* what) -> std::string {
-    for (std::size_t k = 0; k < lo.size(); ++k) {
-      std::ostringstream e;
-      if (std::isnan(lo[k]) || std::isnan(up[k])) e << what << " bound " << k << " is NaN";
-      else if (lo[k] == kInf) e << what << " lower bound " << k << " is +inf";
-      else if (up[k] == -kInf) e << what << " upper bound " << k << " is -inf";
-      if (!e.str().empty()) return e.str();
+  const struct {
+    const std::vector<double>& lo;
+    const std::vector<double>& up;
+    const char* kind;
+  } ranges[] = {{col_lower, col_upper, "column"}, {row_lower, row_upper, "row"}};
+  for (const auto& r : ranges)
+    for (std::size_t k = 0; k < r.lo.size(); ++k) {
+      const char* flaw = std::isnan(r.lo[k]) || std::isnan(r.up[k]) ? "a NaN bound"
+                         : r.lo[k] == kInf                          ? "a lower bound of +inf"
+                         : r.up[k] == -kInf                         ? "an upper bound of -inf"
+                                                                    : nullptr;
+      if (flaw) {
+        std::snprintf(buf, sizeof buf, "%s %zu has %s", r.kind, k, flaw);
+        return buf;
+      }
     }
-    return {};
-  };
-  if (auto s = bounds(col_lower, col_upper, "column"); !s.empty()) return s;
-  return bounds(row_lower, row_upper, "row");
+  return {};
 }
 
+// The first column, then row, whose lower bound exceeds its upper bound; "" when none does.
 std::string Model::crossed_bounds() const {
-  auto first = [](const std::vector<double>& lo, const std::vector<double>& up, const char* what) -> std::string {
-    for (std::size_t k = 0; k < lo.size() && k < up.size(); ++k)
-      if (lo[k] > up[k]) {
-        std::ostringstream e;
-        e << what << " " << k << " has lower bound " << lo[k] << " > upper bound " << up[k];
-        return e.str();
+  const struct {
+    const std::vector<double>& lo;
+    const std::vector<double>& up;
+    const char* kind;
+  } ranges[] = {{col_lower, col_upper, "column"}, {row_lower, row_upper, "row"}};
+  for (const auto& r : ranges) {
+    const std::size_t len = std::min(r.lo.size(), r.up.size());
+    for (std::size_t k = 0; k < len; ++k)
+      if (r.lo[k] > r.up[k]) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "%s %zu needs at least %.17g and at most %.17g", r.kind, k, r.lo[k], r.up[k]);
+        return buf;
       }
-    return {};
-  };
-  if (auto s = first(col_lower, col_upper, "column"); !s.empty()) return s;
-  return first(row_lower, row_upper, "row");
+  }
+  return {};
 }
 
 double Model::objective_value(const std::vector<double>& x) const {

@@ -1,5 +1,7 @@
-// presolve.cpp — see presolve.h. Row access comes from a CSR built once; the reduction
-// passes themselves are sequential because each reduction changes what the next one sees.
+// presolve.cpp — see presolve.h. A Reducer keeps working copies of the bounds and, per row and
+// per column, the number of entries still alive; each round collects the rows (then the
+// columns) a reduction applies to, in index order, and applies them one after another, because
+// a bound set by one row is what the next row on the same column compares against.
 #include "presolve.h"
 
 #include <algorithm>
@@ -11,173 +13,232 @@
 namespace ps26119::amd {
 
 namespace {
-constexpr double kFeasTol = 1e-9;
-bool within(double v, double lo, double up) {
-  return v >= lo - kFeasTol * (1 + std::fabs(lo)) && v <= up + kFeasTol * (1 + std::fabs(up));
-}
-}  // namespace
 
-PresolveResult presolve(const Model& M) {
-  PresolveResult r;
-  const int m = M.num_rows, n = M.num_cols;
-  const Csr A = transpose(csc_as_transpose(M));
-  std::vector<double> rl = M.row_lower, ru = M.row_upper, cl = M.col_lower, cu = M.col_upper;
-  std::vector<char> row_alive(m, 1), col_alive(n, 1);
-  std::vector<int> row_cnt(m), col_cnt(n);
-  for (int i = 0; i < m; ++i) row_cnt[i] = A.ptr[i + 1] - A.ptr[i];
-  for (int j = 0; j < n; ++j) col_cnt[j] = M.col_start[j + 1] - M.col_start[j];
-  r.fixed_value.assign(n, kNaN);
-  double offset = M.obj_offset;
-  auto infeasible = [&](const std::string& why) {
-    r.outcome = PresolveResult::Outcome::Infeasible;
-    r.message = why;
-    return r;
-  };
-  bool changed = true;
-  while (changed) {
-    changed = false;
-    for (int i = 0; i < m; ++i) {  // R1 and R4
-      if (!row_alive[i]) continue;
-      if (row_cnt[i] == 0) {
-        if (!within(0.0, rl[i], ru[i])) return infeasible("empty row " + std::to_string(i) + " excludes 0");
-        row_alive[i] = 0;
-        ++r.removed_rows;
-        changed = true;
-      } else if (row_cnt[i] == 1) {
-        int j = -1;
-        double a = 0;
-        for (int p = A.ptr[i]; p < A.ptr[i + 1]; ++p)
-          if (col_alive[A.col[p]]) j = A.col[p], a = A.val[p];
-        if (j < 0 || a == 0.0) continue;
-        const double lo = a > 0 ? rl[i] / a : ru[i] / a;
-        const double hi = a > 0 ? ru[i] / a : rl[i] / a;
-        const bool set_lower = lo > cl[j], set_upper = hi < cu[j];
-        double nl = set_lower ? lo : cl[j], nu = set_upper ? hi : cu[j];
-        if (nl > nu) {
-          if (nl - nu > kFeasTol * (1 + std::fabs(nl)))
-            return infeasible("singleton row " + std::to_string(i) + " conflicts with the bounds of its column");
-          nl = nu = 0.5 * (nl + nu);
-        }
-        cl[j] = nl;
-        cu[j] = nu;
-        r.singletons.push_back({i, j, a, set_lower, set_upper});
-        r.tightened_bounds += set_lower + set_upper;
-        row_alive[i] = 0;
-        --col_cnt[j];
-        ++r.removed_rows;
-        changed = true;
-      }
+constexpr double kPresolveTol = 1e-9;  // relative slack of presolve's own feasibility tests
+
+double slack(double bound) { return kPresolveTol * (1.0 + std::fabs(bound)); }
+
+class Reducer {
+ public:
+  explicit Reducer(const Model& M)
+      : M_(M), rows_(transpose(csc_as_transpose(M))), rl_(M.row_lower), ru_(M.row_upper), cl_(M.col_lower),
+        cu_(M.col_upper), row_live_(M.num_rows), col_live_(M.num_cols), row_gone_(M.num_rows, 0),
+        col_gone_(M.num_cols, 0), constant_(M.obj_offset) {
+    for (int i = 0; i < M.num_rows; ++i) row_live_[i] = rows_.ptr[i + 1] - rows_.ptr[i];
+    for (int j = 0; j < M.num_cols; ++j) col_live_[j] = M.col_start[j + 1] - M.col_start[j];
+  }
+
+  Presolved run() {
+    for (;;) {
+      bool rows_changed = false, cols_changed = false;
+      if (!row_pass(rows_changed)) return std::move(out_);
+      cols_changed = column_pass();
+      if (!rows_changed && !cols_changed) break;
     }
-    for (int j = 0; j < n; ++j) {  // R2 and R3
-      if (!col_alive[j]) continue;
-      double v;
-      if (cl[j] == cu[j]) {
-        v = cl[j];
-      } else if (col_cnt[j] == 0) {
-        const double c = M.sense * M.obj[j];
-        v = c > 0 ? cl[j] : (c < 0 ? cu[j] : std::min(std::max(0.0, cl[j]), cu[j]));
-        if (!std::isfinite(v)) continue;
-      } else {
+    if (out_.rows_dropped == 0 && out_.cols_dropped == 0) {
+      out_.outcome = Presolved::Outcome::NothingToDo;
+      return std::move(out_);
+    }
+    assemble();
+    out_.outcome = Presolved::Outcome::Reduced;
+    return std::move(out_);
+  }
+
+ private:
+  // Rows with at most one live entry, in index order. Returns false when a row proves the model
+  // infeasible (out_ then says which).
+  bool row_pass(bool& changed) {
+    std::vector<int> candidates;
+    for (int i = 0; i < M_.num_rows; ++i)
+      if (!row_gone_[i] && row_live_[i] <= 1) candidates.push_back(i);
+    for (int i : candidates) {
+      if (row_live_[i] == 0) {
+        if (!(0.0 >= rl_[i] - slack(rl_[i]) && 0.0 <= ru_[i] + slack(ru_[i])))
+          return prove_infeasible("row " + std::to_string(i) + " has no entries and a range that excludes 0");
+        row_gone_[i] = 1;
+        ++out_.rows_dropped;
+        out_.log.push_back({ReductionStep::Kind::DropEmptyRow, i, -1, 0.0, 0.0, false, false});
+        changed = true;
         continue;
       }
-      for (int p = M.col_start[j]; p < M.col_start[j + 1]; ++p) {
-        const int i = M.row_index[p];
-        if (!row_alive[i]) continue;
-        const double t = M.value[p] * v;
-        if (std::isfinite(rl[i])) rl[i] -= t;
-        if (std::isfinite(ru[i])) ru[i] -= t;
-        --row_cnt[i];
+      int col = -1;
+      double a = 0.0;
+      for (int p = rows_.ptr[i]; p < rows_.ptr[i + 1]; ++p)
+        if (!col_gone_[rows_.col[p]]) col = rows_.col[p], a = rows_.val[p];
+      if (col < 0 || a == 0.0) continue;
+      // the row's range divided by its coefficient (infinite ends stay infinite)
+      const double from = a > 0 ? rl_[i] / a : ru_[i] / a;
+      const double to = a > 0 ? ru_[i] / a : rl_[i] / a;
+      const bool lower_moves = from > cl_[col], upper_moves = to < cu_[col];
+      double lower = lower_moves ? from : cl_[col];
+      double upper = upper_moves ? to : cu_[col];
+      if (lower > upper) {
+        if (lower - upper > slack(lower))
+          return prove_infeasible("row " + std::to_string(i) + " bounds column " + std::to_string(col) +
+                                  " outside the column's own range");
+        lower = upper = 0.5 * (lower + upper);  // within rounding: collapse to the midpoint
       }
-      offset += M.obj[j] * v;
-      r.fixed_value[j] = v;
-      col_alive[j] = 0;
-      ++r.removed_cols;
+      cl_[col] = lower;
+      cu_[col] = upper;
+      out_.log.push_back({ReductionStep::Kind::RowToBound, i, col, a, 0.0, lower_moves, upper_moves});
+      out_.bounds_tightened += (lower_moves ? 1 : 0) + (upper_moves ? 1 : 0);
+      row_gone_[i] = 1;
+      --col_live_[col];
+      ++out_.rows_dropped;
       changed = true;
     }
+    return true;
   }
-  if (r.removed_rows == 0 && r.removed_cols == 0) {
-    r.outcome = PresolveResult::Outcome::Unchanged;
-    return r;
-  }
-  Model& R = r.reduced;
-  std::vector<int> new_row(m, -1);
-  for (int i = 0; i < m; ++i)
-    if (row_alive[i]) new_row[i] = static_cast<int>(r.row_map.size()), r.row_map.push_back(i);
-  for (int j = 0; j < n; ++j)
-    if (col_alive[j]) r.col_map.push_back(j);
-  R.name = M.name;
-  R.sense = M.sense;
-  R.obj_offset = offset;
-  R.num_rows = static_cast<int>(r.row_map.size());
-  R.num_cols = static_cast<int>(r.col_map.size());
-  R.col_start.reserve(R.num_cols + 1);
-  R.col_start.push_back(0);
-  for (int j : r.col_map) {
-    R.obj.push_back(M.obj[j]);
-    R.col_lower.push_back(cl[j]);
-    R.col_upper.push_back(cu[j]);
-    for (int p = M.col_start[j]; p < M.col_start[j + 1]; ++p) {
-      const int i = M.row_index[p];
-      if (new_row[i] >= 0) R.row_index.push_back(new_row[i]), R.value.push_back(M.value[p]);
+
+  // Fixed columns and columns with no live entry, in index order.
+  bool column_pass() {
+    std::vector<std::pair<int, double>> fixes;
+    for (int j = 0; j < M_.num_cols; ++j) {
+      if (col_gone_[j]) continue;
+      if (cl_[j] == cu_[j]) {
+        fixes.emplace_back(j, cl_[j]);
+      } else if (col_live_[j] == 0) {
+        const double v = best_bound(j);
+        if (std::isfinite(v)) fixes.emplace_back(j, v);  // an unbounded direction is the engine's to find
+      }
     }
-    R.col_start.push_back(static_cast<int>(R.value.size()));
-    if (!M.is_integer.empty()) R.is_integer.push_back(M.is_integer[j]);
+    for (const auto& [j, v] : fixes) substitute(j, v);
+    return !fixes.empty();
   }
-  for (int i : r.row_map) {
-    R.row_lower.push_back(rl[i]);
-    R.row_upper.push_back(ru[i]);
+
+  // The value an entry-free column takes at an optimum: its bound in the improving direction.
+  double best_bound(int j) const {
+    const double c = M_.sense * M_.obj[j];
+    if (c > 0) return cl_[j];
+    if (c < 0) return cu_[j];
+    return std::min(std::max(0.0, cl_[j]), cu_[j]);
   }
-  r.outcome = PresolveResult::Outcome::Reduced;
-  return r;
+
+  void substitute(int j, double v) {
+    for (int p = M_.col_start[j]; p < M_.col_start[j + 1]; ++p) {
+      const int i = M_.row_index[p];
+      if (row_gone_[i]) continue;
+      const double moved = M_.value[p] * v;
+      if (std::isfinite(rl_[i])) rl_[i] -= moved;
+      if (std::isfinite(ru_[i])) ru_[i] -= moved;
+      --row_live_[i];
+    }
+    constant_ += M_.obj[j] * v;
+    out_.log.push_back({ReductionStep::Kind::SubstituteColumn, -1, j, 0.0, v, false, false});
+    col_gone_[j] = 1;
+    ++out_.cols_dropped;
+  }
+
+  bool prove_infeasible(std::string why) {
+    out_.outcome = Presolved::Outcome::ProvedInfeasible;
+    out_.reason = std::move(why);
+    return false;
+  }
+
+  // The reduced model: live rows and columns in their original order.
+  void assemble() {
+    std::vector<int> renumber(M_.num_rows, -1);
+    for (int i = 0; i < M_.num_rows; ++i)
+      if (!row_gone_[i]) {
+        renumber[i] = static_cast<int>(out_.kept_rows.size());
+        out_.kept_rows.push_back(i);
+      }
+    for (int j = 0; j < M_.num_cols; ++j)
+      if (!col_gone_[j]) out_.kept_cols.push_back(j);
+    Model& R = out_.reduced;
+    R.name = M_.name;
+    R.sense = M_.sense;
+    R.obj_offset = constant_;
+    R.num_rows = static_cast<int>(out_.kept_rows.size());
+    R.num_cols = static_cast<int>(out_.kept_cols.size());
+    R.col_start.assign(1, 0);
+    for (int j : out_.kept_cols) {
+      R.obj.push_back(M_.obj[j]);
+      R.col_lower.push_back(cl_[j]);
+      R.col_upper.push_back(cu_[j]);
+      if (!M_.is_integer.empty()) R.is_integer.push_back(M_.is_integer[j]);
+      for (int p = M_.col_start[j]; p < M_.col_start[j + 1]; ++p)
+        if (renumber[M_.row_index[p]] >= 0) {
+          R.row_index.push_back(renumber[M_.row_index[p]]);
+          R.value.push_back(M_.value[p]);
+        }
+      R.col_start.push_back(static_cast<int>(R.value.size()));
+    }
+    for (int i : out_.kept_rows) {
+      R.row_lower.push_back(rl_[i]);
+      R.row_upper.push_back(ru_[i]);
+    }
+  }
+
+  const Model& M_;
+  const Csr rows_;  // A by rows, to find a singleton row's column
+  std::vector<double> rl_, ru_, cl_, cu_;
+  std::vector<int> row_live_, col_live_;  // entries in live columns / live rows
+  std::vector<unsigned char> row_gone_, col_gone_;
+  double constant_;
+  Presolved out_;
+};
+
+// Σ over column j of a_ij·v_i subtracted from `start`, in the column's storage order.
+double minus_column_dot(const Model& M, int j, const std::vector<double>& v, double start) {
+  for (int p = M.col_start[j]; p < M.col_start[j + 1]; ++p) start -= M.value[p] * v[M.row_index[p]];
+  return start;
 }
 
-Solution postsolve(const Model& M, const PresolveResult& r, const Solution& red) {
+}  // namespace
+
+Presolved presolve(const Model& model) { return Reducer(model).run(); }
+
+Solution postsolve(const Model& M, const Presolved& p, const Solution& red) {
   Solution s = red;
   const int m = M.num_rows, n = M.num_cols;
+  const bool have_x = red.x.size() == p.kept_cols.size();
+  const bool have_y = red.y.size() == p.kept_rows.size();
   s.x.assign(n, 0.0);
   s.y.assign(m, 0.0);
-  const bool have_x = red.x.size() == r.col_map.size();
-  const bool have_y = red.y.size() == r.row_map.size();
-  for (int j = 0; j < n; ++j)
-    if (!std::isnan(r.fixed_value[j])) s.x[j] = r.fixed_value[j];
   if (have_x)
-    for (std::size_t c = 0; c < r.col_map.size(); ++c) s.x[r.col_map[c]] = red.x[c];
+    for (std::size_t k = 0; k < p.kept_cols.size(); ++k) s.x[p.kept_cols[k]] = red.x[k];
   if (have_y)
-    for (std::size_t q = 0; q < r.row_map.size(); ++q) s.y[r.row_map[q]] = red.y[q];
-  // a singleton row takes the reduced cost of its column when the column sits on the bound
-  // that row created (newest first)
-  for (auto it = r.singletons.rbegin(); it != r.singletons.rend(); ++it) {
-    const int j = it->col;
-    double z = M.obj[j];
-    for (int p = M.col_start[j]; p < M.col_start[j + 1]; ++p) z -= M.value[p] * s.y[M.row_index[p]];
-    const double zm = M.sense * z;
-    if ((zm > 0 && it->set_lower) || (zm < 0 && it->set_upper)) s.y[it->row] = z / it->a;
+    for (std::size_t k = 0; k < p.kept_rows.size(); ++k) s.y[p.kept_rows[k]] = red.y[k];
+  // Undo in reverse: a substituted column gets its value back; a row that became a column bound
+  // takes over the column's reduced cost when the column rests on a bound that row set (the
+  // newest such row first owns it).
+  for (auto it = p.log.rbegin(); it != p.log.rend(); ++it) {
+    if (it->kind == ReductionStep::Kind::SubstituteColumn) {
+      s.x[it->col] = it->value;
+    } else if (it->kind == ReductionStep::Kind::RowToBound) {
+      const double z = minus_column_dot(M, it->col, s.y, M.obj[it->col]);
+      const double zmin = M.sense * z;
+      if ((zmin > 0 && it->sets_lower) || (zmin < 0 && it->sets_upper)) s.y[it->row] = z / it->coef;
+    }
   }
   s.z = M.obj;
   parallel_for(n, [&](std::int64_t b, std::int64_t e) {
-    for (std::int64_t j = b; j < e; ++j)
-      for (int p = M.col_start[j]; p < M.col_start[j + 1]; ++p) s.z[j] -= M.value[p] * s.y[M.row_index[p]];
+    for (std::int64_t j = b; j < e; ++j) s.z[j] = minus_column_dot(M, static_cast<int>(j), s.y, M.obj[j]);
   });
   s.row_activity = activity(M, s.x);
   s.objective = M.objective_value(s.x);
-  if (!have_x) {
+  if (!have_x) {  // no point came back: no objective either
     s.x.clear();
     s.row_activity.clear();
     s.objective = kNaN;
   }
-  if (!have_y) s.y.clear(), s.z.clear();
+  if (!have_y) {
+    s.y.clear();
+    s.z.clear();
+  }
   return s;
 }
 
-std::vector<double> postsolve_farkas(const Model& M, const PresolveResult& r, const std::vector<double>& red) {
+std::vector<double> lift_dual_ray(const Model& M, const Presolved& p, const std::vector<double>& red) {
+  if (red.size() != p.kept_rows.size()) return {};
   std::vector<double> ray(M.num_rows, 0.0);
-  if (red.size() != r.row_map.size()) return {};
-  for (std::size_t q = 0; q < r.row_map.size(); ++q) ray[r.row_map[q]] = red[q];
-  for (auto it = r.singletons.rbegin(); it != r.singletons.rend(); ++it) {
-    const int j = it->col;
-    double lam = 0.0;
-    for (int p = M.col_start[j]; p < M.col_start[j + 1]; ++p) lam -= M.value[p] * ray[M.row_index[p]];
-    if ((lam > 0 && it->set_lower) || (lam < 0 && it->set_upper)) ray[it->row] = lam / it->a;
+  for (std::size_t k = 0; k < p.kept_rows.size(); ++k) ray[p.kept_rows[k]] = red[k];
+  // a Farkas vector is a dual with c = 0: the same recovery with λ_j = −(Aᵀr)_j
+  for (auto it = p.log.rbegin(); it != p.log.rend(); ++it) {
+    if (it->kind != ReductionStep::Kind::RowToBound) continue;
+    const double lam = minus_column_dot(M, it->col, ray, 0.0);
+    if ((lam > 0 && it->sets_lower) || (lam < 0 && it->sets_upper)) ray[it->row] = lam / it->coef;
   }
   return ray;
 }

@@ -1,8 +1,10 @@
-// solve.cpp — solve(): validation, presolve wrapper, dispatch to the MI355X r²HPDHG engine,
-// the verification gate and the certified bound, with the reference's semantics
-// (reference src/core/solve.cpp:119-609, LP part). What this library serves is the GPU
-// first-order path: a request for a CPU engine, or for another algorithm, is answered with
-// NotSolved and a reason instead of being run on the CPU.
+// solve.cpp — solve(): the path from a caller's model to a verified answer on the MI355X.
+// A Pipeline runs the stages in order — admission, presolve, the engine on the reduced model,
+// lifting the answer back and re-checking it on the original model, the certified bound, the
+// gate — with one budget: every follow-up solve gets only what is left of the caller's time and
+// iterations. The decisions and their order are the reference's (src/core/solve.cpp:119-185,
+// 266-467, LP part); requests for a CPU engine or another algorithm are answered NotSolved with
+// a reason instead of being run here.
 #include "solve.h"
 
 #include <algorithm>
@@ -20,40 +22,61 @@ namespace ps26119::amd {
 
 namespace {
 
-bool within_bound_budget(double elapsed, double time_limit) { return elapsed < 1.1 * time_limit + 1.0; }
+using Clock = std::chrono::steady_clock;
 
-bool has_integers(const Model& m) {
-  for (auto v : m.is_integer)
-    if (v) return true;
-  return false;
+double seconds_since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+void append(Solution& s, const std::string& t) { s.message += std::string(s.message.empty() ? "" : "; ") + t; }
+
+bool all_finite(const std::vector<double>& v) {
+  return std::all_of(v.begin(), v.end(), [](double a) { return std::isfinite(a); });
 }
 
-KktStats original_kkt(const Model& M, const Solution& s) {
-  ScaledProblem sp;  // only the original matrix is used by kkt_on_original
+bool warm_start_sized(const Model& model, const Options& o) {
+  return (o.warm_x.empty() || static_cast<int>(o.warm_x.size()) == model.num_cols) &&
+         (o.warm_y.empty() || static_cast<int>(o.warm_y.size()) == model.num_rows);
+}
+
+// The certified bound may also use implied bounds while the call is still within its time.
+bool may_use_implied_bounds(double elapsed, double time_limit) { return elapsed < 1.1 * time_limit + 1.0; }
+
+// Relative KKT error of a point on the original model (the reduced solve's yardstick too).
+KktStats kkt_of_point(const Model& M, const Solution& s) {
+  ScaledProblem sp;  // kkt_on_original reads only the original matrix and model
   sp.m = M.num_rows;
   sp.n = M.num_cols;
   sp.original = &M;
   sp.A = transpose(csc_as_transpose(M));
   sp.a_orig = sp.A.val;
-  std::vector<double> ymin(s.y);
-  for (double& v : ymin) v *= M.sense;
-  return kkt_on_original(sp, s.x, ymin);
+  std::vector<double> y_min(s.y);
+  for (double& v : y_min) v *= M.sense;
+  return kkt_on_original(sp, s.x, y_min);
 }
 
-Solution trivially_infeasible(const Model& model, const Options& options, const std::string& why) {
+Solution infeasible_by_bounds(const Model& model, const Options& options, const std::string& which) {
   Solution s;
   s.status = Status::Infeasible;
   s.engine = to_string(options.algorithm);
   s.precision = to_string(options.precision);
   s.model_fingerprint = model.fingerprint_hex();
-  s.message = "bounds cross: " + why;
+  s.message = "bounds cross: " + which;
   return s;
 }
 
-void note(Solution& s, const std::string& t) { s.message += std::string(s.message.empty() ? "" : "; ") + t; }
+Solution refuse(const Model& model, const char* engine, const std::string& why, Status st = Status::NotSolved) {
+  Solution s;
+  s.status = st;
+  s.engine = engine;
+  s.message = why;
+  s.model_fingerprint = model.fingerprint_hex();
+  return s;
+}
 
-Solution solve_direct(const Model& model, const Options& options, bool certify) {
-  const auto t0 = std::chrono::steady_clock::now();
+// The engine on one model, exactly as given (no presolve): argument checks, r²HPDHG or PDLP on
+// the device, and for an Unbounded verdict whose point is not primal-feasible a zero-objective
+// solve for a point the ray can start from. `certify`: also the certified dual bound.
+Solution engine_solve(const Model& model, const Options& options, bool certify) {
+  const auto t0 = Clock::now();
   Solution sol;
   sol.model_fingerprint = model.fingerprint_hex();
   sol.precision = to_string(options.precision);
@@ -63,20 +86,18 @@ Solution solve_direct(const Model& model, const Options& options, bool certify) 
     sol.message = "invalid model: " + err;
     return sol;
   }
-  if ((!options.warm_x.empty() && static_cast<int>(options.warm_x.size()) != model.num_cols) ||
-      (!options.warm_y.empty() && static_cast<int>(options.warm_y.size()) != model.num_rows)) {
+  if (!warm_start_sized(model, options)) {
     sol.status = Status::NotSolved;
-    sol.message = "warm start vectors have the wrong size";
+    sol.message = "a warm start vector does not match the model's size";
     return sol;
   }
-  auto finite = [](const std::vector<double>& v) { return std::all_of(v.begin(), v.end(), [](double a) { return std::isfinite(a); }); };
-  if (!finite(options.warm_x) || !finite(options.warm_y) ||
+  if (!all_finite(options.warm_x) || !all_finite(options.warm_y) ||
       !(options.warm_primal_weight >= 0 && std::isfinite(options.warm_primal_weight))) {
     sol.status = Status::NotSolved;
-    sol.message = "warm start contains non-finite values";
+    sol.message = "a warm start holds a non-finite value";
     return sol;
   }
-  if (auto why = model.crossed_bounds(); !why.empty()) return trivially_infeasible(model, options, why);
+  if (auto which = model.crossed_bounds(); !which.empty()) return infeasible_by_bounds(model, options, which);
   try {
     EngineOptions eo = engine_options_from(options);
     if (auto bad = apply_engine_params(options, eo); !bad.empty()) {
@@ -86,32 +107,33 @@ Solution solve_direct(const Model& model, const Options& options, bool certify) 
     }
     sol = options.algorithm == Algorithm::Pdlp ? solve_pdlp(model, eo) : solve_r2hpdhg(model, eo);
     if (sol.status == Status::Unbounded && !check_unboundedness_certificate(model, sol.x, sol.primal_ray).passed) {
-      // a valid ray from a point that is not primal-feasible: find a feasible point with a
-      // zero-objective solve within the remaining budget (reference solve.cpp:552-582)
-      Model feas = model;
-      std::fill(feas.obj.begin(), feas.obj.end(), 0.0);
+      // The ray is only a certificate from a feasible point: look for one with c = 0 in what is
+      // left of the budget (reference solve.cpp:552-582).
+      Model feasibility = model;
+      std::fill(feasibility.obj.begin(), feasibility.obj.end(), 0.0);
       Options fo = options;
       fo.presolve = false;
       fo.warm_x.clear();
       fo.warm_y.clear();
-      fo.time_limit = std::max(0.0, options.time_limit - std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+      fo.time_limit = std::max(0.0, options.time_limit - seconds_since(t0));
       fo.iteration_limit = std::max<std::int64_t>(0, options.iteration_limit - sol.iterations);
-      Solution f = solve_direct(feas, fo, true);
-      const std::int64_t total = sol.iterations + f.iterations;
+      Solution f = engine_solve(feasibility, fo, true);
+      const std::int64_t both = sol.iterations + f.iterations;
       if (f.status == Status::Optimal) {
         sol.x = f.x;
         sol.row_activity = activity(model, sol.x);
-        note(sol, "feasible point from a zero-objective solve (" + std::to_string(f.iterations) + " iterations)");
+        append(sol, "a zero-objective solve supplied the feasible point (" + std::to_string(f.iterations) +
+                        " iterations)");
       } else if (f.status == Status::Infeasible) {
-        f.message = "unbounded direction found, but the model is infeasible: " + f.message;
+        f.message = "a direction of unboundedness exists but no feasible point does: " + f.message;
         f.primal_ray.clear();
         sol = f;
       } else {
         sol.status = f.status;
-        note(sol, std::string("no feasible point found for the ray (") + to_string(f.status) + ")");
+        append(sol, std::string("the search for a feasible point ended ") + to_string(f.status));
         sol.primal_ray.clear();
       }
-      sol.iterations = total;
+      sol.iterations = both;
     }
   } catch (const std::bad_alloc&) {
     sol = Solution{};
@@ -125,164 +147,207 @@ Solution solve_direct(const Model& model, const Options& options, bool certify) 
   sol.model_fingerprint = model.fingerprint_hex();
   if (certify && static_cast<int>(sol.y.size()) == model.num_rows && model.num_rows + model.num_cols > 0)
     sol.certified_bound =
-        certified_dual_bound(model, sol.y,
-                             within_bound_budget(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
-                                                 options.time_limit))
-            .bound;
-  sol.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        certified_dual_bound(model, sol.y, may_use_implied_bounds(seconds_since(t0), options.time_limit)).bound;
+  sol.seconds = seconds_since(t0);
   return sol;
 }
 
-Solution solve_impl(const Model& model, const Options& options) {
-  const bool bad_warm = (!options.warm_x.empty() && static_cast<int>(options.warm_x.size()) != model.num_cols) ||
-                        (!options.warm_y.empty() && static_cast<int>(options.warm_y.size()) != model.num_rows);
-  auto finite = [](const std::vector<double>& v) { return std::all_of(v.begin(), v.end(), [](double a) { return std::isfinite(a); }); };
-  const bool bad_values = !finite(options.warm_x) || !finite(options.warm_y);
-  if (!options.presolve || bad_warm || bad_values || !model.validate().empty()) return solve_direct(model, options, true);
-  if (auto why = model.crossed_bounds(); !why.empty()) return trivially_infeasible(model, options, why);
-  const auto t0 = std::chrono::steady_clock::now();
-  PresolveResult pr = presolve(model);
-  if (options.verbosity >= 1)
-    std::fprintf(stderr, "presolve: %.3fs\n", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
-  if (pr.outcome == PresolveResult::Outcome::Unchanged) return solve_direct(model, options, true);
-  Options inner = options;
-  inner.presolve = false;
-  {  // assess the reduced solve with the ORIGINAL model's norms (reference solve.cpp:282-294)
-    double b2 = 0, c2 = 0;
-    for (int i = 0; i < model.num_rows; ++i) {
-      const double lo = std::isfinite(model.row_lower[i]) ? std::fabs(model.row_lower[i]) : 0.0;
-      const double up = std::isfinite(model.row_upper[i]) ? std::fabs(model.row_upper[i]) : 0.0;
-      b2 += std::max(lo, up) * std::max(lo, up);
+class Pipeline {
+ public:
+  Pipeline(const Model& model, const Options& options) : M_(model), opt_(options), t0_(Clock::now()) {}
+
+  Solution run() {
+    const bool usable = options_usable();
+    if (!opt_.presolve || !usable || !M_.validate().empty()) return engine_solve(M_, opt_, true);
+    if (auto which = M_.crossed_bounds(); !which.empty()) return infeasible_by_bounds(M_, opt_, which);
+    const auto tp = Clock::now();
+    P_ = presolve(M_);
+    if (opt_.verbosity >= 1) std::fprintf(stderr, "presolve: %.3fs\n", seconds_since(tp));
+    switch (P_.outcome) {
+      case Presolved::Outcome::NothingToDo:
+        return engine_solve(M_, opt_, true);
+      case Presolved::Outcome::ProvedInfeasible:
+        return solve_original(0, "it proved the model infeasible (" + P_.reason + ")");
+      case Presolved::Outcome::Reduced:
+        break;
     }
-    for (double c : model.obj) c2 += c * c;
-    inner.kkt_b_norm = std::sqrt(b2);
-    inner.kkt_c_norm = std::sqrt(c2);
-    inner.kkt_obj_shift = model.sense * (pr.reduced.obj_offset - model.obj_offset);
+    summary_ = "presolve: " + std::to_string(P_.rows_dropped) + " rows and " + std::to_string(P_.cols_dropped) +
+               " columns eliminated";
+    const Solution red = solve_reduced();
+    if (Solution lifted; lift_certificate(red, lifted)) return lifted;
+    if (red.status == Status::Infeasible || red.status == Status::Unbounded)
+      return solve_original(red.iterations, std::string("the reduced model came back ") + to_string(red.status));
+    return finish_optimal_path(red);
   }
-  auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
-  auto budget = [&](Options o, std::int64_t used) {
-    o.time_limit = std::max(0.0, options.time_limit - elapsed());
-    o.iteration_limit = std::max<std::int64_t>(0, options.iteration_limit - used);
+
+ private:
+  bool options_usable() const {
+    return warm_start_sized(M_, opt_) && all_finite(opt_.warm_x) && all_finite(opt_.warm_y);
+  }
+
+  // The caller's options with what is left of its time and iterations.
+  Options remaining(Options o, std::int64_t spent_iterations) const {
+    o.time_limit = std::max(0.0, opt_.time_limit - seconds_since(t0_));
+    o.iteration_limit = std::max<std::int64_t>(0, opt_.iteration_limit - spent_iterations);
     return o;
-  };
-  auto on_original = [&](const Solution& why, const std::string& what) {
-    Options direct = budget(options, why.iterations);
-    direct.presolve = false;
-    Solution d = solve_direct(model, direct, true);
-    d.iterations += why.iterations;
-    d.seconds = elapsed();
-    note(d, "presolve: " + what + "; re-solved the original model for an original-space certificate");
-    return d;
-  };
-  if (pr.outcome == PresolveResult::Outcome::Infeasible) return on_original(Solution{}, "reduction found infeasibility (" + pr.message + ")");
-  if (!options.warm_x.empty()) {
-    inner.warm_x.clear();
-    for (int j : pr.col_map) inner.warm_x.push_back(options.warm_x[j]);
   }
-  if (!options.warm_y.empty()) {
-    inner.warm_y.clear();
-    for (int i : pr.row_map) inner.warm_y.push_back(options.warm_y[i]);
+
+  // Options for solves of the reduced model: no presolve, and the original model's ‖b‖, ‖c‖ and
+  // objective shift in the relative KKT terms, so that a tolerance means the same as on the
+  // original.
+  Options reduced_options() const {
+    Options o = opt_;
+    o.presolve = false;
+    double b2 = 0.0, c2 = 0.0;
+    for (int i = 0; i < M_.num_rows; ++i) {
+      const double lo = std::isfinite(M_.row_lower[i]) ? std::fabs(M_.row_lower[i]) : 0.0;
+      const double up = std::isfinite(M_.row_upper[i]) ? std::fabs(M_.row_upper[i]) : 0.0;
+      const double big = std::max(lo, up);
+      b2 += big * big;
+    }
+    for (double c : M_.obj) c2 += c * c;
+    o.kkt_b_norm = std::sqrt(b2);
+    o.kkt_c_norm = std::sqrt(c2);
+    o.kkt_obj_shift = M_.sense * (P_.reduced.obj_offset - M_.obj_offset);
+    return o;
   }
-  Solution red;
-  if (pr.reduced.num_cols == 0 && pr.reduced.num_rows == 0) {
-    red.status = Status::Optimal;
-    red.engine = "presolve";
-  } else {
-    red = solve_direct(pr.reduced, budget(inner, 0), false);
-  }
-  const std::string removed =
-      "presolve removed " + std::to_string(pr.removed_rows) + " rows, " + std::to_string(pr.removed_cols) + " cols";
-  if (red.status == Status::Infeasible && static_cast<int>(red.dual_ray.size()) == pr.reduced.num_rows) {
-    std::vector<double> r = postsolve_farkas(model, pr, red.dual_ray);
-    if (!r.empty() && check_infeasibility_certificate(model, r).passed) {
-      Solution s = red;
-      s.x.clear();
-      s.y.clear();
-      s.z.clear();
-      s.row_activity.clear();
-      s.dual_ray = std::move(r);
-      note(s, removed + "; reduced-model Farkas certificate postsolved to the original rows");
-      s.model_fingerprint = model.fingerprint_hex();
-      s.seconds = elapsed();
+
+  Solution solve_reduced() {
+    Options o = reduced_options();
+    if (!opt_.warm_x.empty()) {
+      o.warm_x.clear();
+      for (int j : P_.kept_cols) o.warm_x.push_back(opt_.warm_x[j]);
+    }
+    if (!opt_.warm_y.empty()) {
+      o.warm_y.clear();
+      for (int i : P_.kept_rows) o.warm_y.push_back(opt_.warm_y[i]);
+    }
+    if (P_.reduced.num_cols == 0 && P_.reduced.num_rows == 0) {  // presolve decided everything
+      Solution s;
+      s.status = Status::Optimal;
+      s.engine = "presolve";
       return s;
     }
+    return engine_solve(P_.reduced, remaining(o, 0), false);
   }
-  if (red.status == Status::Unbounded && static_cast<int>(red.primal_ray.size()) == pr.reduced.num_cols &&
-      static_cast<int>(red.x.size()) == pr.reduced.num_cols) {
-    std::vector<double> d(model.num_cols, 0.0);
-    for (int j = 0; j < pr.reduced.num_cols; ++j) d[pr.col_map[j]] = red.primal_ray[j];
-    Solution s = postsolve(model, pr, red);
-    if (check_unboundedness_certificate(model, s.x, d).passed) {
-      s.status = Status::Unbounded;
-      s.primal_ray = std::move(d);
-      s.dual_ray.clear();
-      s.y.clear();
-      s.z.clear();
-      s.objective = s.dual_objective = kNaN;
-      note(s, removed + "; reduced-model ray and point postsolved to the original model");
-      s.model_fingerprint = model.fingerprint_hex();
-      s.seconds = elapsed();
-      return s;
-    }
-  }
-  if (red.status == Status::Infeasible || red.status == Status::Unbounded)
-    return on_original(red, std::string("reduced model ") + to_string(red.status));
-  Solution post = postsolve(model, pr, red);
-  post.message = post.message.empty() ? removed : post.message + "; " + removed;
-  post.model_fingerprint = model.fingerprint_hex();
-  auto passes = [&](const Solution& s) {
-    return static_cast<int>(s.x.size()) == model.num_cols && static_cast<int>(s.y.size()) == model.num_rows &&
-           original_kkt(model, s).converged(options.tolerance);
-  };
-  if (post.status == Status::Optimal && !passes(post)) {
-    // tighten the reduced solve 100× warm-started, else solve the original cold
-    Options tight = inner;
-    tight.tolerance = std::max(options.tolerance * 1e-2, 1e-13);
-    tight.warm_x = red.x;
-    tight.warm_y = red.y;
-    const Solution red2 = solve_direct(pr.reduced, budget(tight, red.iterations), true);
-    Solution post2 = postsolve(model, pr, red2);
-    post2.iterations += red.iterations;
-    if (post2.status == Status::Optimal && passes(post2)) {
-      char buf[64];
-      std::snprintf(buf, sizeof buf, "%.0e", tight.tolerance);
-      post2.message = removed + "; reduced model re-solved to " + buf + " to pass on the original";
-      post = post2;
+
+  // An Infeasible / Unbounded verdict of the reduced model whose certificate, lifted to the
+  // original model, holds there: true with the answer in `out`.
+  bool lift_certificate(const Solution& red, Solution& out) const {
+    if (red.status == Status::Infeasible && static_cast<int>(red.dual_ray.size()) == P_.reduced.num_rows) {
+      std::vector<double> ray = lift_dual_ray(M_, P_, red.dual_ray);
+      if (ray.empty() || !check_infeasibility_certificate(M_, ray).passed) return false;
+      out = red;
+      out.x.clear();
+      out.y.clear();
+      out.z.clear();
+      out.row_activity.clear();
+      out.dual_ray = std::move(ray);
+      append(out, summary_ + "; the reduced model's dual ray, lifted to the original rows, holds there");
+    } else if (red.status == Status::Unbounded && static_cast<int>(red.primal_ray.size()) == P_.reduced.num_cols &&
+               static_cast<int>(red.x.size()) == P_.reduced.num_cols) {
+      // removed columns move by 0 along the ray; a bound a removed row created is a reduced
+      // column bound, which the ray already respects
+      std::vector<double> dir(M_.num_cols, 0.0);
+      for (int k = 0; k < P_.reduced.num_cols; ++k) dir[P_.kept_cols[k]] = red.primal_ray[k];
+      Solution s = postsolve(M_, P_, red);
+      if (!check_unboundedness_certificate(M_, s.x, dir).passed) return false;
+      out = std::move(s);
+      out.status = Status::Unbounded;
+      out.primal_ray = std::move(dir);
+      out.dual_ray.clear();
+      out.y.clear();
+      out.z.clear();
+      out.objective = out.dual_objective = kNaN;
+      append(out, summary_ + "; the reduced model's ray and point, lifted, hold on the original model");
     } else {
-      Solution cold = solve_direct(model, budget(inner, red.iterations + red2.iterations), true);
-      cold.iterations += red.iterations + red2.iterations;
-      note(cold, removed + "; postsolved point missed the tolerance, fell back to solving the original");
-      post = cold;
+      return false;
     }
-  } else if (post.status == Status::Optimal && static_cast<int>(post.x.size()) != model.num_cols) {
-    post.status = Status::NumericalError;
+    out.model_fingerprint = M_.fingerprint_hex();
+    out.seconds = seconds_since(t0_);
+    return true;
   }
-  if (static_cast<int>(post.y.size()) == model.num_rows && model.num_rows + model.num_cols > 0)
-    post.certified_bound = certified_dual_bound(model, post.y, within_bound_budget(elapsed(), options.time_limit)).bound;
-  post.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return post;
-}
+
+  // The original model solved directly, for a verdict with a certificate in its own space.
+  Solution solve_original(std::int64_t spent, const std::string& why) const {
+    Options o = remaining(opt_, spent);
+    o.presolve = false;
+    Solution s = engine_solve(M_, o, true);
+    s.iterations += spent;
+    s.seconds = seconds_since(t0_);
+    append(s, "presolve: " + why + "; the original model was solved directly");
+    return s;
+  }
+
+  bool meets_tolerance(const Solution& s) const {
+    return static_cast<int>(s.x.size()) == M_.num_cols && static_cast<int>(s.y.size()) == M_.num_rows &&
+           kkt_of_point(M_, s).converged(opt_.tolerance);
+  }
+
+  Solution finish_optimal_path(const Solution& red) const {
+    Solution post = postsolve(M_, P_, red);
+    append(post, summary_);
+    post.model_fingerprint = M_.fingerprint_hex();
+    if (post.status == Status::Optimal && !meets_tolerance(post)) {
+      // The reduced model's KKT terms differ from the original's: first the reduced model again,
+      // warm-started from its answer at 100× the tolerance, then the original from scratch.
+      Options tighter = reduced_options();
+      tighter.tolerance = std::max(opt_.tolerance * 1e-2, 1e-13);
+      tighter.warm_x = red.x;
+      tighter.warm_y = red.y;
+      const Solution red2 = engine_solve(P_.reduced, remaining(tighter, red.iterations), true);
+      Solution post2 = postsolve(M_, P_, red2);
+      post2.iterations += red.iterations;
+      if (post2.status == Status::Optimal && meets_tolerance(post2)) {
+        char tol[32];
+        std::snprintf(tol, sizeof tol, "%.0e", tighter.tolerance);
+        post2.message = summary_ + "; the reduced model was solved again to " + tol + " to hold on the original";
+        post = std::move(post2);
+      } else {
+        const std::int64_t spent = red.iterations + red2.iterations;
+        Solution direct = engine_solve(M_, remaining(reduced_options(), spent), true);
+        direct.iterations += spent;
+        append(direct, summary_ + "; the lifted point missed the tolerance on the original model, which was solved "
+                                   "directly");
+        post = std::move(direct);
+      }
+    } else if (post.status == Status::Optimal && static_cast<int>(post.x.size()) != M_.num_cols) {
+      post.status = Status::NumericalError;
+    }
+    if (static_cast<int>(post.y.size()) == M_.num_rows && M_.num_rows + M_.num_cols > 0)
+      post.certified_bound =
+          certified_dual_bound(M_, post.y, may_use_implied_bounds(seconds_since(t0_), opt_.time_limit)).bound;
+    post.seconds = seconds_since(t0_);
+    return post;
+  }
+
+  const Model& M_;
+  const Options& opt_;
+  const Clock::time_point t0_;
+  Presolved P_;
+  std::string summary_;
+};
 
 }  // namespace
 
-void gate(const Model& model, const Options& options, Solution& s) {  // reference solve.cpp:119-185 (LP)
+void gate(const Model& model, const Options& options, Solution& s) {
   if (s.status == Status::Infeasible || s.status == Status::Unbounded) {
     s.objective = s.dual_objective = kNaN;
     if (s.status == Status::Infeasible && !model.crossed_bounds().empty()) {
-      s.check = "PASS";
+      s.check = "PASS";  // crossed bounds certify themselves
       return;
     }
     const CertificateCheck c = s.status == Status::Infeasible ? check_infeasibility_certificate(model, s.dual_ray)
                                                               : check_unboundedness_certificate(model, s.x, s.primal_ray);
     if (!c.present) {
       s.check.clear();
-      note(s, "not certified (the engine returned no certificate)");
+      append(s, "no certificate came back from the engine");
     } else if (c.passed) {
       s.check = "PASS";
-      note(s, c.detail);
+      append(s, c.detail);
     } else {
       s.check = "FAIL";
-      note(s, std::string(to_string(s.status)) + " withdrawn, " + c.detail);
+      append(s, std::string(to_string(s.status)) + " revoked: " + c.detail);
       s.status = Status::NumericalError;
     }
     return;
@@ -290,7 +355,7 @@ void gate(const Model& model, const Options& options, Solution& s) {  // referen
   if (s.status != Status::Optimal) return;
   if (static_cast<int>(s.x.size()) != model.num_cols || static_cast<int>(s.y.size()) != model.num_rows) {
     s.status = Status::NumericalError;
-    note(s, "engine reported Optimal without primal and dual vectors");
+    append(s, "Optimal came back without its primal and dual vectors");
     return;
   }
   const CheckReport rep = check_solution(model, s.x, s.y);
@@ -303,68 +368,48 @@ void gate(const Model& model, const Options& options, Solution& s) {  // referen
   if (std::isnan(s.dual_residual)) s.dual_residual = rep.max_dual_violation;
   if (std::isnan(s.gap)) s.gap = rep.relative_gap;
   if (rep.passed()) return;
-  const std::string what = "in-process check on the original model: " + rep.summary();
-  if (options.tolerance < tol::kVerifyPrimal) {  // first-order engine that promised verifier grade
+  const std::string finding = "the check on the original model found " + rep.summary();
+  if (options.tolerance < tol::kVerifyPrimal) {  // a tolerance that asked for verifier-grade accuracy
     s.status = Status::NumericalError;
-    note(s, "Optimal withdrawn, " + what);
+    append(s, "Optimal revoked: " + finding);
   } else {
-    note(s, what + " (requested tolerance is looser than the verifier's)");
+    append(s, finding + " (the requested tolerance is looser than the checker's)");
   }
 }
 
 Solution solve(const Model& model, const Options& options) {
   try {
-    Solution s;
     if (options.algorithm != Algorithm::Auto && options.algorithm != Algorithm::R2hpdhg &&
-        options.algorithm != Algorithm::Pdlp) {
-      s.status = Status::NotSolved;
-      s.engine = to_string(options.algorithm);
-      s.message = std::string("the MI355X backend runs the first-order engines (r2hpdhg, pdlp); algorithm '") +
-                  to_string(options.algorithm) + "' is served by the reference CPU library";
-      s.model_fingerprint = model.fingerprint_hex();
-      return s;
-    }
-    if (!options.use_gpu) {
-      s.status = Status::NotSolved;
-      s.engine = "r2hpdhg";
-      s.message = "the MI355X backend serves GPU solves: set use_gpu = 1 (CPU engines are in the reference library)";
-      s.model_fingerprint = model.fingerprint_hex();
-      return s;
-    }
-    if (has_integers(model) && !options.relax_integrality) {
-      s.status = Status::NotSolved;
-      s.engine = "r2hpdhg";
-      s.message = "integer columns: branch-and-bound is not part of the MI355X backend (solve the LP relaxation with "
-                  "relax_integrality)";
-      s.model_fingerprint = model.fingerprint_hex();
-      return s;
-    }
+        options.algorithm != Algorithm::Pdlp)
+      return refuse(model, to_string(options.algorithm),
+                    std::string("the MI355X backend runs the first-order engines (r2hpdhg, pdlp); algorithm '") +
+                        to_string(options.algorithm) + "' is served by the reference CPU library");
+    if (!options.use_gpu)
+      return refuse(model, "r2hpdhg",
+                    "the MI355X backend serves GPU solves: set use_gpu = 1 (CPU engines are in the reference library)");
+    const bool integers = std::any_of(model.is_integer.begin(), model.is_integer.end(), [](auto v) { return v != 0; });
+    if (integers && !options.relax_integrality)
+      return refuse(model, "r2hpdhg",
+                    "integer columns: branch-and-bound is not part of the MI355X backend (solve the LP relaxation "
+                    "with relax_integrality)");
     Options resolved = options;
-    const bool was_auto = options.algorithm == Algorithm::Auto;
-    if (was_auto) resolved.algorithm = Algorithm::R2hpdhg;  // a GPU request defaults to r2hpdhg
+    const bool chose = options.algorithm == Algorithm::Auto;
+    if (chose) resolved.algorithm = Algorithm::R2hpdhg;  // a GPU request defaults to r2hpdhg
     set_host_threads(options.threads);
-    s = solve_impl(model, resolved);
+    Solution s = Pipeline(model, resolved).run();
     gate(model, resolved, s);
-    if (was_auto) {
-      char buf[96];
+    if (chose) {
+      char head[96];
       const double work = static_cast<double>(std::max(model.num_rows, 1)) * static_cast<double>(model.nnz());
-      std::snprintf(buf, sizeof buf, "auto: r2hpdhg (rows*nnz = %.2g)", work);
-      s.message = s.message.empty() ? buf : std::string(buf) + "; " + s.message;
+      std::snprintf(head, sizeof head, "auto: r2hpdhg (rows*nnz = %.2g)", work);
+      s.message = s.message.empty() ? head : std::string(head) + "; " + s.message;
     }
     if (s.engine.empty()) s.engine = resolved.algorithm == Algorithm::Pdlp ? "pdlp" : "r2hpdhg";
     return s;
   } catch (const std::bad_alloc&) {
-    Solution s;
-    s.status = Status::NotSolved;
-    s.message = "out of memory";
-    s.model_fingerprint = model.fingerprint_hex();
-    return s;
+    return refuse(model, "", "out of memory");
   } catch (const std::exception& e) {
-    Solution s;
-    s.status = Status::NumericalError;
-    s.message = std::string("internal error: ") + e.what();
-    s.model_fingerprint = model.fingerprint_hex();
-    return s;
+    return refuse(model, "", std::string("internal error: ") + e.what(), Status::NumericalError);
   }
 }
 

@@ -132,6 +132,9 @@ def _load():
     lib.ps26119_amd_check_solution.restype = ctypes.c_int
     lib.ps26119_amd_presolve_info.argtypes = model + [D]
     lib.ps26119_amd_presolve_info.restype = ctypes.c_int
+    IP = ctypes.POINTER(ctypes.c_int)
+    lib.ps26119_amd_presolve_model.argtypes = model + [D, IP, IP, D, D, D, D, IP, IP, D]
+    lib.ps26119_amd_presolve_model.restype = ctypes.c_int
     return lib
 
 
@@ -323,7 +326,7 @@ def solve_batch(c, A, row_lower, row_upper, col_lower=None, col_upper=None, scen
             a = sc.get(key)
             if a is not None:
                 a = np.ascontiguousarray(a, dtype=np.float64).ravel()
-                if a.size != length:  # the library's "wrong sizes or non-finite data" verdict
+                if a.size != length:  # NaNs: the library refuses a scenario with a non-finite cost or NaN bound
                     a = np.full(length, np.nan)
                 keep.append(a)
             ptrs.append(_ptr(a))

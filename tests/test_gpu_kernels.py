@@ -151,6 +151,29 @@ def test_phased_products_equal_row_tiles(gpu_available, oracle, name):
     b.close()
 
 
+def test_chunk_graphs_follow_the_reflection(gpu_available, oracle):
+    """ρ is baked into a captured chunk's launches: chunks with ρ = 1, 0.5, 0.5, 1 on one context
+    (graph replay) give the bits of the same chunks launched eagerly on another."""
+    from ps26119_amd import psh
+    lp = model("rand300")
+    K, tau, sigma = 7, 0.3, 0.4
+    a = context(oracle, lp, 0)
+    b = context(oracle, lp, 0)
+    b.graphs(0)
+    for ctx in (a, b):
+        ctx.upload(psh.X, np.sin(1.0 + np.arange(lp.n)))
+        ctx.upload(psh.Y, np.cos(2.0 + np.arange(lp.m)))
+        ctx.copy(psh.X0, psh.X)
+        ctx.copy(psh.Y0, psh.Y)
+    for k, rho in enumerate((1.0, 0.5, 0.5, 1.0)):
+        for ctx in (a, b):
+            ctx.chunk(K, K * k, tau, sigma, rho)
+        for slot in (psh.X, psh.XB, psh.Y):
+            assert np.array_equal(a.download(slot), b.download(slot)), (rho, slot)
+    a.close()
+    b.close()
+
+
 _big = {}
 
 

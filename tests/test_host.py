@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, hand_models, netlib
+from conftest import NETLIB, ROOT, hand_models, netlib
 
 HEADERS = [os.path.join(ROOT, "include", "ps26119", "ps26119.h"), os.path.join(ROOT, "include", "ps26119", "ps26119_amd.h"),
            os.path.join(ROOT, "include", "ps26119_hip.h")]
@@ -132,6 +132,56 @@ def test_host_presolve_counts(oracle):
     empty_cols = int(np.sum(np.diff(lp.col_start) == 0))
     assert out[0] == 0 and out[1] == 0 and out[2] == empty_cols  # random LP: presolve drops the empty columns
     assert out[4] == lp.n - empty_cols and out[5] == lp.nnz
+
+
+def _presolve_cases():
+    from ps26119_amd import lpgen
+    inf = np.inf
+    cases = {name: netlib(name) for name in NETLIB}
+    cases.update(hand_models())
+    cases["random300_s3"] = lpgen.random_lp(300, seed=3)
+    cases["random2000_s4"] = lpgen.random_lp(2000, seed=4)
+    cases["refinery_T6_s3"] = lpgen.refinery_lp(T=6, seed=3)
+    # a singleton row that tightens both bounds, a fixed column whose substitution leaves a row
+    # empty (dropped) and another a singleton (next round), an entry-free column at its best bound
+    cases["rounds"] = lpgen.from_dense([1, -2, 3, 1, 0.5], [[2, 0, 0, 0, 0], [1, 1, 0, 0, 0], [0, 1, 1, 1, 0],
+                                                            [0, 0, 0, 3, 0]],
+                                       [1, -inf, 2, 6], [4, 5, 8, 6], [0, 0, 0, 2, -1], [10, 10, 10, 2, 4])
+    cases["singleton_conflict"] = lpgen.from_dense([1, 1], [[1, 0], [1, 1]], [20, 0], [30, 5], [0, 0], [10, 10])
+    cases["empty_row_excludes_0"] = lpgen.from_dense([1, 1], [[0, 0], [1, 1]], [1, 0], [2, 5], [0, 0], [10, 10])
+    return cases
+
+
+@pytest.mark.parametrize("name", sorted(_presolve_cases()))
+def test_host_presolve_is_bit_identical_to_oracle(oracle, name):
+    """The reduced model of the product's presolve equals the oracle's (whose presolve = 1 solves
+    are pinned to the reference's fixtures) bit for bit: outcome, counts, offset, kept rows and
+    columns, reduced bounds and the reduced matrix."""
+    import ctypes
+    import ps26119
+    lp = _presolve_cases()[name]
+    a = _arrays(lp)
+    info = np.zeros(7)
+    I = ctypes.c_int
+    got = {"kept_rows": np.zeros(lp.m, np.int32), "kept_cols": np.zeros(lp.n, np.int32), "rl": np.zeros(lp.m),
+           "ru": np.zeros(lp.m), "cl": np.zeros(lp.n), "cu": np.zeros(lp.n), "col_start": np.zeros(lp.n + 1, np.int32),
+           "row_index": np.zeros(max(1, lp.nnz), np.int32), "value": np.zeros(max(1, lp.nnz))}
+    rc = ps26119.lib().ps26119_amd_presolve_model(
+        *_model_args(lp, a), _p(info), _p(got["kept_rows"], I), _p(got["kept_cols"], I), _p(got["rl"]), _p(got["ru"]),
+        _p(got["cl"]), _p(got["cu"]), _p(got["col_start"], I), _p(got["row_index"], I), _p(got["value"]))
+    assert rc == 0
+    einfo, exp = oracle.presolve(lp)
+    print(f"{name}: outcome {int(info[0])}, {int(info[1])} rows and {int(info[2])} columns removed", flush=True)
+    assert [float(v).hex() for v in info] == [float(v).hex() for v in einfo]
+    import orc
+    got = orc._trim(info, got)
+    assert got.keys() == exp.keys()
+    for k in exp:
+        assert np.array_equal(got[k], exp[k]) and got[k].tobytes() == exp[k].tobytes(), k
+    if name == "rounds":
+        assert info[0] == 0 and info[1] >= 2 and info[2] >= 2
+    if name in ("singleton_conflict", "empty_row_excludes_0"):
+        assert info[0] == 1
 
 
 def _solve_raw(lp, opt=None, result=True, **over):
