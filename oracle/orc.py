@@ -2,6 +2,7 @@
 
     lib = orc.load("oracle")      # oracle/liboracle.so   — our restatement (orc_* symbols)
     lib = orc.load("ref")         # oracle/_ref/libps26119_ref.so — reference sources (ref_*)
+    lib = orc.load("ref_hip")     # the same with the MI355X backend behind its make_backend(true)
     res, vecs = lib.solve(lp, presolve=False)
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
@@ -14,8 +15,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-PATHS = {"oracle": os.path.join(HERE, "liboracle.so"), "ref": os.path.join(HERE, "_ref", "libps26119_ref.so")}
-PREFIX = {"oracle": "orc_", "ref": "ref_"}
+PATHS = {"oracle": os.path.join(HERE, "liboracle.so"), "ref": os.path.join(HERE, "_ref", "libps26119_ref.so"),
+         "ref_hip": os.path.join(HERE, "_ref", "libps26119_ref_hip.so")}
+PREFIX = {"oracle": "orc_", "ref": "ref_", "ref_hip": "ref_"}
 
 D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
@@ -32,7 +34,7 @@ class OrcOpts(ctypes.Structure):
                 ("threads", ctypes.c_int), ("check_every", ctypes.c_int), ("tolerance", ctypes.c_double),
                 ("time_limit", ctypes.c_double), ("iteration_limit", ctypes.c_longlong), ("warm_x", D),
                 ("warm_y", D), ("warm_primal_weight", ctypes.c_double), ("n_params", ctypes.c_int),
-                ("param_names", ctypes.POINTER(ctypes.c_char_p)), ("param_values", D)]
+                ("param_names", ctypes.POINTER(ctypes.c_char_p)), ("param_values", D), ("use_gpu", ctypes.c_int)]
 
 
 class OrcResult(ctypes.Structure):
@@ -104,8 +106,10 @@ class Checker:
                           if hasattr(self.lib, PREFIX[kind] + "presolve") else None)
 
     def solve(self, lp, precision=0, presolve=True, tolerance=1e-8, iteration_limit=0, time_limit=0.0,
-              threads=1, check_every=0, warm_x=None, warm_y=None, warm_primal_weight=0.0, params=None, algorithm=3):
-        """algorithm: 3 = r²HPDHG, 2 = PDLP (the reference build only)."""
+              threads=1, check_every=0, warm_x=None, warm_y=None, warm_primal_weight=0.0, params=None, algorithm=3,
+              use_gpu=0):
+        """algorithm: 3 = r²HPDHG, 2 = PDLP (the reference build only). use_gpu = 1 (the ref_hip build
+        only) runs the reference's drivers on the MI355X backend."""
         h = LpHandle(lp)
         names = [k.encode() for k in (params or {})]
         vals = np.array(list((params or {}).values()), dtype=np.float64)
@@ -113,7 +117,8 @@ class Checker:
         wx = None if warm_x is None else np.ascontiguousarray(warm_x, dtype=np.float64)
         wy = None if warm_y is None else np.ascontiguousarray(warm_y, dtype=np.float64)
         o = OrcOpts(algorithm, precision, int(presolve), threads, check_every, tolerance, time_limit, iteration_limit,
-                    _p(wx), _p(wy), warm_primal_weight, len(names), name_arr, _p(vals) if len(names) else None)
+                    _p(wx), _p(wy), warm_primal_weight, len(names), name_arr, _p(vals) if len(names) else None,
+                    int(use_gpu))
         r = OrcResult()
         x, y, z = np.full(lp.n, np.nan), np.full(lp.m, np.nan), np.full(lp.n, np.nan)
         dr, pr = np.full(lp.m, np.nan), np.full(lp.n, np.nan)

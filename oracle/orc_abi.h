@@ -1,8 +1,11 @@
-/* orc_abi.h — the extern "C" surface shared by the two CPU checkers of this repository:
+/* orc_abi.h — the extern "C" surface shared by the CPU checkers of this repository:
  *   oracle/liboracle.so            our CPU restatement of the reference r²HPDHG path (prefix orc_)
  *   oracle/_ref/libps26119_ref.so  the reference's own sources + oracle/ref_driver.cpp (prefix ref_)
- * Both are TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
- * leg load them, and only as the checker. The product never links or calls either.
+ *   oracle/_ref/libps26119_ref_hip.so  the same with the MI355X backend answering the reference's
+ *                                  GPU hook (oracle/ref_hip_hook.cpp), for use_gpu = 1 (prefix ref_)
+ * All are TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg load them, and only as the checker (ref_hip: as the caller of the backend under test). The
+ * product never links or calls any of them.
  *
  * Model layout is the reference contract (include/ps26119/model.h of the reference): CSC A,
  * infinity = HUGE_VAL, sense +1 min / -1 max, objective value cᵀx + offset.
@@ -35,6 +38,7 @@ typedef struct {
   int n_params;
   const char* const* param_names;
   const double* param_values;
+  int use_gpu;    /* reference build with the MI355X backend hooked in (_ref/libps26119_ref_hip.so) only */
 } orc_opts;
 
 typedef struct {

@@ -27,7 +27,15 @@ SRCS := src/core/model.cpp src/core/status.cpp src/core/solve.cpp src/core/c_api
         src/mip/branch_and_bound.cpp
 OBJS := $(patsubst src/%.cpp,$(OUT)/obj/%.o,$(SRCS))
 
-all: $(OUT)/libps26119_ref.so
+# The same core with the reference's GPU hook switched on (only src/pdhg/cpu_backend.cpp reads
+# PS26119_HAVE_CUDA) and answered by the MI355X backend adapter: the reference's drivers over
+# the psh_* kernels, for the GPU tests. Needs libps26119_hip.so (make -C ps26119-solver_amd/csrc).
+PKG      := ps26119-solver_amd/csrc
+HIP_LIB  := $(PKG)/hip/libps26119_hip.so
+HIP_OBJS := $(filter-out $(OUT)/obj/pdhg/cpu_backend.o,$(OBJS)) $(OUT)/obj_hip/cpu_backend.o \
+            $(OUT)/obj_hip/hip_backend.o $(OUT)/obj_hip/ref_hip_hook.o $(OUT)/obj/ref_driver.o
+
+all: $(OUT)/libps26119_ref.so $(OUT)/libps26119_ref_hip.so
 
 $(OUT)/obj/%.o: $(REF)/src/%.cpp
 	@mkdir -p $(dir $@)
@@ -39,6 +47,22 @@ $(OUT)/obj/ref_driver.o: oracle/ref_driver.cpp
 
 $(OUT)/libps26119_ref.so: $(OBJS) $(OUT)/obj/ref_driver.o
 	$(CXX) -shared -o $@ $^ -lpthread
+
+$(OUT)/obj_hip/cpu_backend.o: $(REF)/src/pdhg/cpu_backend.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -DPS26119_HAVE_CUDA -c $< -o $@
+
+$(OUT)/obj_hip/hip_backend.o: $(PKG)/adapter/hip_backend.cpp include/ps26119_hip.h
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -DPS26119_HAVE_CUDA -Iinclude -Wall -Wextra -c $< -o $@
+
+$(OUT)/obj_hip/ref_hip_hook.o: oracle/ref_hip_hook.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -DPS26119_HAVE_CUDA -c $< -o $@
+
+$(OUT)/libps26119_ref_hip.so: $(HIP_OBJS) $(HIP_LIB)
+	$(CXX) -shared -o $@ $(HIP_OBJS) -L$(PKG)/hip -lps26119_hip \
+	    -Wl,-rpath,'$$ORIGIN/../../$(PKG)/hip' -lpthread
 
 clean:
 	rm -rf $(OUT)

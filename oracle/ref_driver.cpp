@@ -58,6 +58,7 @@ int ref_solve(const orc_lp* lp, const orc_opts* o, orc_result* r, double* x, dou
   if (o->warm_x) opt.warm_x.assign(o->warm_x, o->warm_x + lp->n);
   if (o->warm_y) opt.warm_y.assign(o->warm_y, o->warm_y + lp->m);
   opt.warm_primal_weight = o->warm_primal_weight;
+  opt.use_gpu = o->use_gpu != 0;
   for (int k = 0; k < o->n_params; ++k) opt.engine_params.emplace_back(o->param_names[k], o->param_values[k]);
   const Solution s = solve(m, opt);
   std::memset(r, 0, sizeof *r);

@@ -232,6 +232,24 @@ def test_no_device_is_not_solved_with_reason():
     assert r.status == ps26119.NOT_SOLVED and "HIP backend unavailable" in r.message
 
 
+def test_reference_gpu_hook_reaches_the_hip_backend():
+    """The reference core built with its GPU hook answered by the pdhg::Backend adapter
+    (oracle/_ref/libps26119_ref_hip.so) runs its CPU backend unchanged for use_gpu = 0 and reaches
+    the adapter for use_gpu = 1, which without a device reports why (make_backend's NotSolved)."""
+    import orc
+    import ps26119
+    if not orc.available("ref_hip"):
+        pytest.skip("oracle/_ref/libps26119_ref_hip.so not built (needs the reference sources)")
+    if ps26119.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    L = orc.load("ref_hip")
+    lp = hand_models()["wyndor"]
+    cpu, v = L.solve(lp, use_gpu=0)
+    assert cpu["status"] == 0 and abs(cpu["objective"] - 36) <= 1e-6
+    gpu, _ = L.solve(lp, use_gpu=1, presolve=False)
+    assert gpu["status"] == 6 and "no usable MI355X" in gpu["message"], gpu["message"]
+
+
 def test_engine_parameters_are_checked_before_the_device():
     """Every engine knob the extension header documents is accepted by the parser, which runs
     before the device is opened (so this holds without a GPU); an unknown name is refused with
