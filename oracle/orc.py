@@ -70,6 +70,18 @@ class LpHandle:
                        _p(cs, ctypes.c_int), _p(ri, ctypes.c_int), _p(v))
 
 
+class OrcSolutionFile(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_char_p) for k in ("status", "engine", "precision", "message", "check")] + \
+               [(k, ctypes.c_double) for k in ("objective", "dual_objective", "certified_bound", "primal_residual",
+                                               "dual_residual", "gap")] + \
+               [("iterations", ctypes.c_longlong), ("seconds", ctypes.c_double), ("setup_seconds", ctypes.c_double),
+                ("primal_weight", ctypes.c_double), ("iterations_to_fast", ctypes.c_longlong)] + \
+               [(k, ctypes.c_double) for k in ("seconds_to_fast", "check_primal", "check_dual", "check_gap")] + \
+               [(k, D) for k in ("x", "z", "activity", "y")] + \
+               [("dual_ray_len", ctypes.c_longlong), ("primal_ray_len", ctypes.c_longlong), ("dual_ray", D),
+                ("primal_ray", D)]
+
+
 class OrcScenario(ctypes.Structure):
     _fields_ = [("obj", D), ("col_lower", D), ("col_upper", D), ("row_lower", D), ("row_upper", D)]
 
@@ -101,6 +113,8 @@ class Checker:
         self._batch = fn("solve_batch", [L, ctypes.c_int, ctypes.POINTER(OrcScenario), ctypes.POINTER(OrcOpts),
                                          ctypes.POINTER(OrcResult), D, D])
         self._check = fn("check_solution", [L, D, D, D])
+        self._write_sol = (fn("write_solution", [ctypes.c_char_p, ctypes.POINTER(OrcSolutionFile), ctypes.c_char_p])
+                           if hasattr(self.lib, PREFIX[kind] + "write_solution") else None)
         I = ctypes.POINTER(ctypes.c_int)
         self._presolve = (fn("presolve", [L, D, I, I, D, D, D, D, I, I, D])
                           if hasattr(self.lib, PREFIX[kind] + "presolve") else None)
@@ -204,6 +218,25 @@ class Checker:
         h = LpHandle(lp)
         y = np.ascontiguousarray(y, dtype=np.float64)
         return self._bound(ctypes.byref(h.s), _p(y), int(use_implied))
+
+    def write_solution(self, lpm_path, fields, out_path):
+        """The reference's solution writer (the reference build only) on the model its .lpm reader
+        reads from lpm_path and `fields` (the keys of OrcSolutionFile; vectors as sequences, absent
+        keys = no such block). Returns the writer's code (0 = written)."""
+        keep = {k: np.ascontiguousarray(fields[k], dtype=np.float64)
+                for k in ("x", "z", "activity", "y", "dual_ray", "primal_ray") if fields.get(k) is not None}
+        f = OrcSolutionFile()
+        for k, t in OrcSolutionFile._fields_:
+            if k in keep:
+                setattr(f, k, _p(keep[k]))
+            elif k.endswith("_ray_len"):
+                ray = keep.get(k[:-4])
+                setattr(f, k, 0 if ray is None else ray.size)
+            elif t is ctypes.c_char_p:
+                setattr(f, k, fields.get(k, "").encode())
+            elif k in fields:
+                setattr(f, k, fields[k])
+        return self._write_sol(lpm_path.encode(), ctypes.byref(f), out_path.encode())
 
     def check_solution(self, lp, x, y):
         h = LpHandle(lp)

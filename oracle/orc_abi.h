@@ -57,6 +57,22 @@ typedef struct {
   const double *obj, *col_lower, *col_upper, *row_lower, *row_upper;
 } orc_scenario;
 
+/* A solution file's contents (the reference format, src/io/solution_writer.h of the reference),
+ * to be written again by the reference's own writer (ref_write_solution). Strings are
+ * NUL-terminated; a vector pointer is NULL when the file has no such block. */
+typedef struct {
+  const char *status, *engine, *precision, *message, *check;
+  double objective, dual_objective, certified_bound, primal_residual, dual_residual, gap;
+  long long iterations;
+  double seconds, setup_seconds, primal_weight;
+  long long iterations_to_fast;
+  double seconds_to_fast, check_primal, check_dual, check_gap;
+  const double *x, *z;           /* COLUMNS (n) */
+  const double *activity, *y;    /* ROWS (m) */
+  long long dual_ray_len, primal_ray_len;
+  const double *dual_ray, *primal_ray;
+} orc_solution_file;
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,8 @@
 #include <vector>
 
 #include "core/safe_bound.h"
+#include "io/lpm_reader.h"
+#include "io/solution_writer.h"
 #include "core/solution_checker.h"
 #include "la/csr.h"
 #include "pdhg/backend.h"
@@ -255,6 +257,44 @@ int ref_check_solution(const orc_lp* lp, const double* x, const double* y, doubl
   out[4] = r.dual_objective;
   out[5] = r.passed() ? 1.0 : 0.0;
   return 0;
+}
+
+// The reference's writer on the model read by the reference's .lpm reader and the given
+// contents (the fingerprint is left for the writer to compute from the model). 0 = written,
+// -1 = the model does not read, -2 = unknown status, -3 = write failed.
+int ref_write_solution(const char* lpm_path, const orc_solution_file* f, const char* out_path) {
+  Model m;
+  if (!io::read_lpm(lpm_path, m).ok) return -1;
+  Solution s;
+  if (!status_from_string(f->status, s.status)) return -2;
+  s.engine = f->engine;
+  s.precision = f->precision;
+  s.message = f->message;
+  s.check = f->check;
+  s.objective = f->objective;
+  s.dual_objective = f->dual_objective;
+  s.certified_bound = f->certified_bound;
+  s.primal_residual = f->primal_residual;
+  s.dual_residual = f->dual_residual;
+  s.gap = f->gap;
+  s.iterations = f->iterations;
+  s.seconds = f->seconds;
+  s.setup_seconds = f->setup_seconds;
+  s.primal_weight = f->primal_weight;
+  s.iterations_to_fast = f->iterations_to_fast;
+  s.seconds_to_fast = f->seconds_to_fast;
+  s.check_primal = f->check_primal;
+  s.check_dual = f->check_dual;
+  s.check_gap = f->check_gap;
+  const int n = m.num_cols, mm = m.num_rows;
+  if (f->x) s.x.assign(f->x, f->x + n);
+  if (f->z) s.z.assign(f->z, f->z + n);
+  if (f->activity) s.row_activity.assign(f->activity, f->activity + mm);
+  if (f->y) s.y.assign(f->y, f->y + mm);
+  if (f->dual_ray) s.dual_ray.assign(f->dual_ray, f->dual_ray + f->dual_ray_len);
+  if (f->primal_ray) s.primal_ray.assign(f->primal_ray, f->primal_ray + f->primal_ray_len);
+  std::string err;
+  return io::write_solution(out_path, m, s, err) ? 0 : -3;
 }
 
 }  // extern "C"

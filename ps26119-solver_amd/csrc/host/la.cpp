@@ -1,8 +1,13 @@
 // la.cpp — see la.h.
 #include "la.h"
 
+#include <unistd.h>
+
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
+#include <thread>
 
 namespace ps26119::amd {
 
@@ -18,18 +23,78 @@ int host_threads() {
 
 void set_host_threads(int t) { g_threads.store(t < 0 ? 0 : t); }
 
+namespace {
+
+// Parked workers shared by every host loop of the process, started on first use and never
+// stopped (the pool outlives static destruction on purpose). One loop owns the pool at a time;
+// a loop that finds it owned (by another thread, or by its own caller when nested) or that runs
+// in a forked child (whose copy of the pool has no threads) does its pieces on the calling
+// thread, which gives the same results because every index is still visited exactly once.
+class WorkerPool {
+ public:
+  static WorkerPool& get() {
+    static WorkerPool* pool = new WorkerPool();
+    return *pool;
+  }
+
+  bool try_run(int pieces, std::int64_t n, const std::function<void(std::int64_t, std::int64_t)>& f) {
+    if (::getpid() != pid_) return false;
+    std::unique_lock<std::mutex> own(owner_, std::try_to_lock);
+    if (!own.owns_lock()) return false;
+    {
+      std::lock_guard<std::mutex> g(m_);
+      while (started_ < pieces - 1) {
+        std::thread([this] { work(); }).detach();
+        ++started_;
+      }
+      job_ = &f;
+      n_ = n;
+      pieces_ = pieces;
+      next_ = 1;  // piece 0 is the caller's
+      left_ = pieces - 1;
+    }
+    wake_.notify_all();
+    f(0, n / pieces);
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [this] { return left_ == 0; });
+    job_ = nullptr;
+    return true;
+  }
+
+ private:
+  WorkerPool() : pid_(::getpid()) {}
+
+  void work() {
+    std::unique_lock<std::mutex> g(m_);
+    for (;;) {
+      wake_.wait(g, [this] { return job_ != nullptr && next_ < pieces_; });
+      const int k = next_++;
+      const auto* f = job_;
+      const std::int64_t n = n_;
+      const int p = pieces_;
+      g.unlock();
+      (*f)(n * k / p, n * (k + 1) / p);
+      g.lock();
+      if (--left_ == 0) done_.notify_one();
+    }
+  }
+
+  const pid_t pid_;
+  std::mutex owner_;  // held by the loop that uses the pool
+  std::mutex m_;      // guards everything below
+  std::condition_variable wake_, done_;
+  int started_ = 0;
+  const std::function<void(std::int64_t, std::int64_t)>* job_ = nullptr;
+  std::int64_t n_ = 0;
+  int pieces_ = 0, next_ = 0, left_ = 0;
+};
+
+}  // namespace
+
 void parallel_for(std::int64_t n, const std::function<void(std::int64_t, std::int64_t)>& f, std::int64_t min_piece) {
   if (n <= 0) return;
   const int t = static_cast<int>(std::min<std::int64_t>(host_threads(), (n + min_piece - 1) / min_piece));
-  if (t <= 1) {
-    f(0, n);
-    return;
-  }
-  std::vector<std::thread> pool;
-  pool.reserve(t - 1);
-  for (int k = 1; k < t; ++k) pool.emplace_back([&, k] { f(n * k / t, n * (k + 1) / t); });
-  f(0, n / t);
-  for (auto& th : pool) th.join();
+  if (t <= 1 || !WorkerPool::get().try_run(t, n, f)) f(0, n);
 }
 
 Csr csc_as_transpose(const Model& m) {

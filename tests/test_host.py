@@ -90,19 +90,38 @@ def _model_args(lp, a, offset=True):
                    _p(a["ri"], ctypes.c_int), _p(a["v"])]
 
 
-@pytest.mark.parametrize("name", ["afiro", "adlittle", "random"])
-def test_host_scaling_is_bit_identical_to_oracle(oracle, name):
+def _host_scale(lp):
     import ps26119
-    from ps26119_amd import lpgen
-    lp = lpgen.random_lp(3000, seed=8) if name == "random" else netlib(name)
     a = _arrays(lp)
     rs, cs, sc = np.zeros(lp.m), np.zeros(lp.n), np.zeros(4)
     av, atv = np.zeros(lp.nnz), np.zeros(lp.nnz)
     assert ps26119.lib().ps26119_amd_scale(*_model_args(lp, a, offset=False), _p(rs), _p(cs), _p(sc), _p(av), _p(atv)) == 0
+    return rs, cs, sc, av, atv
+
+
+@pytest.mark.parametrize("name", ["afiro", "adlittle", "random", "random200k"])
+def test_host_scaling_is_bit_identical_to_oracle(oracle, name):
+    """random200k runs every host sweep on several pieces of the worker pool."""
+    from ps26119_amd import lpgen
+    lp = {"random": lambda: lpgen.random_lp(3000, seed=8), "random200k": lambda: lpgen.random_lp(200_000, seed=8)}.get(
+        name, lambda: netlib(name))()
+    rs, cs, sc, av, atv = _host_scale(lp)
     e = oracle.scale(lp)
     assert np.array_equal(rs, e["row_scale"]) and np.array_equal(cs, e["col_scale"])
     assert np.array_equal(av, e["a_vals"]) and np.array_equal(atv, e["at_vals"])
     assert sc[0] == e["bound_scale"] and sc[1] == e["obj_scale"] and bool(sc[3]) == e["geometric_mean_applied"]
+
+
+def test_host_loops_from_several_threads_give_the_same_bits():
+    """Host loops called from several threads at once (the serving finish workers do this): one
+    owns the worker pool, the others run on their own thread, and every result is the same."""
+    from concurrent.futures import ThreadPoolExecutor
+    from ps26119_amd import lpgen
+    lp = lpgen.random_lp(150_000, seed=2)
+    first = _host_scale(lp)
+    with ThreadPoolExecutor(4) as ex:
+        for got in ex.map(lambda _: _host_scale(lp), range(8)):
+            assert all(np.array_equal(a, b) for a, b in zip(got, first))
 
 
 @pytest.mark.parametrize("name", ["stocfor1", "wyndor", "random"])
