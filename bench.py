@@ -30,8 +30,9 @@ Other workloads (each prints one JSON line of the same shape):
       against sequential plain fp64 solves of the same requests.
 
   copy       a device copy of known size (torch), the calibration kernel of the counter summaries.
+  stream     1 GiB copy / read / write rates of torch's kernels: the practical HBM ceiling.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lp1m|refinery|scenarios|serving|copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lp1m|refinery|scenarios|serving|copy|stream]
                     [--m M] [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
                     [--epoch-cap N] [--compare-seconds S]
                     [--iteration-limit N]   (profiling: same per-launch work, N iterations)
@@ -253,6 +254,37 @@ def bench_copy(args):
             "roofline": None, "cpu_baseline": None}
 
 
+def bench_stream(args):
+    """The streaming rates torch's own kernels reach on a 1 GiB fp64 buffer — a copy, a read (a
+    sum) and a write (a fill), 20 launches each timed with HIP events: the practical ceiling of
+    this GPU's HBM beside its 8 TB/s peak."""
+    import torch
+    n = (1 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    rates = {}
+    for name, fn, nbytes in (("copy", lambda: y.copy_(x), 16 * n), ("read", lambda: x.sum(), 8 * n),
+                             ("write", lambda: y.fill_(1.0), 8 * n)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(20):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / 20
+        rates[name] = {"us": round(us, 1), "GB/s": round(nbytes / us / 1e3, 1)}
+    return {"metric": "HBM streaming rate", "value": rates["copy"]["GB/s"], "unit": "GB/s", "n_gpus": 1, "steps": 20,
+            "warmup": 3, "ms_per_step": rates["copy"]["us"] / 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "1 GiB device copy / sum / fill (torch)"}, "rates": rates,
+            "roofline": {"bound": "hbm", "achieved": rates["copy"]["GB/s"], "peak": 8000.0, "unit": "GB/s",
+                         "frac": rates["copy"]["GB/s"] / 8000.0, "traffic": 16 * n},
+            "cpu_baseline": None}
+
+
 # ------------------------------------------------------------------ refinery
 def bench_refinery(args, ps26119, lpgen):
     """Refinery planning LP, mixed precision in increment form vs plain fp64.
@@ -318,10 +350,11 @@ def bench_scenarios(args, ps26119, lpgen):
         use cost scenarios (c scaled by lognormal(0, 0.05) factors), which all stay live: scaling b,
         c and u together (below) makes most scenarios infeasible, and those leave the batch after a
         few checks with a certificate, which would change the width being timed;
-    (b) end to end at 10,000 × 20,000 with b, c and u scaled by lognormal(0, 0.05) factors,
-        warm-started from the base solution, to 1e-6: fp64 batch,
-        mixed batch and sequential single solves (4 timed, scaled to 64), with the verdicts of
-        each (Optimal, or Infeasible with a checked dual-ray certificate);
+    (b) end to end at --e2e-m rows (10,000 by default; 100,000 is the full size) with b, c and u
+        scaled by lognormal(0, 0.05) factors, warm-started from the base solution, to 1e-6 and to
+        1e-8: fp64 batch and mixed batch (each capped at 3M iterations or 150 s), and sequential
+        single solves to 1e-6 (--seq-sample timed, scaled to 64), with the verdicts of each
+        (Optimal, or Infeasible with a checked dual-ray certificate);
     (c) at 10,000 × 20,000 the 64 cost scenarios warm-started to 1e-6 and to 1e-8, fp64 batch
         against mixed batch (capped at 6M iterations or 150 s), both with the --epoch-cap knobs."""
     m = args.m if args.m != 1_000_000 else 100_000
@@ -361,21 +394,26 @@ def bench_scenarios(args, ps26119, lpgen):
         line.update({"mixed_scenario_iterations_per_s": mixed_rate, "loop_GBs": loop_gbs, "roofline": None,
                      "cpu_baseline": None})
         return line
-    # (b) end to end at 10k rows, b, c and u scaled
+    # (b) end to end at --e2e-m rows, b, c and u scaled, to 1e-6 and to 1e-8
     small = lpgen.random_lp(10_000, seed=args.seed)
-    sscen = lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)
-    b = ps26119.solve_model(small, tolerance=1e-6)
+    e2e_lp = small if args.e2e_m == small.m else lpgen.random_lp(args.e2e_m, seed=args.seed)
+    sscen = lpgen.scenario_set(e2e_lp, args.scenarios, sigma=0.05, seed=7)
     cap = {"iteration_limit": 3_000_000, "time_limit": 150.0}
-    runs = {}
-    for prec in ("fp64", "mixed"):
-        t = time.time()
-        res = ps26119.solve_model_batch(small, sscen, tolerance=1e-6, warm_x=b.x, warm_y=b.y, verbosity=1,
-                                        precision=prec, **cap)
-        runs[prec] = (time.time() - t, res)
+    runs, warm = {}, {}
+    for tol in (1e-6, 1e-8):
+        base_sol = ps26119.solve_model(e2e_lp, tolerance=tol)
+        warm[tol] = base_sol
+        for prec in ("fp64", "mixed"):
+            t = time.time()
+            res = ps26119.solve_model_batch(e2e_lp, sscen, tolerance=tol, warm_x=base_sol.x, warm_y=base_sol.y,
+                                            verbosity=1, precision=prec, **cap)
+            runs[(prec, tol)] = (time.time() - t, res)
+            log(f"{e2e_lp.m} rows, b/c/u scenarios to {tol:.0e}, {prec} batch: {runs[(prec, tol)][0]:.2f}s")
+    b = warm[1e-6]
     k_seq = min(args.seq_sample, len(sscen))
     t = time.time()
-    seq = [ps26119.solve_model(lpgen.with_scenario(small, sc), tolerance=1e-6, warm_x=b.x, warm_y=b.y, presolve=False,
-                               **cap) for sc in sscen[:k_seq]]
+    seq = [ps26119.solve_model(lpgen.with_scenario(e2e_lp, sc), tolerance=1e-6, warm_x=b.x, warm_y=b.y,
+                               presolve=False, **cap) for sc in sscen[:k_seq]]
     seq_s = (time.time() - t) / k_seq * len(sscen)
 
     def verdicts(res):
@@ -384,11 +422,13 @@ def bench_scenarios(args, ps26119, lpgen):
             out[r.status_name] = out.get(r.status_name, 0) + 1
         return out
 
-    agree = sum(r.status == q.status for r, q in zip(runs["fp64"][1], seq))
+    agree = sum(r.status == q.status for r, q in zip(runs[("fp64", 1e-6)][1], seq))
     # (c) feasible scenarios at 10k rows (costs scaled only), fp64 batch vs mixed batch to 1e-6
-    cscen = [{"obj": sc["obj"]} for sc in sscen]
     cost = {}
-    for tol in (1e-6, 1e-8):
+    if not args.skip_cost:
+        b = ps26119.solve_model(small, tolerance=1e-6)
+        cscen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)]
+    for tol in ((1e-6, 1e-8) if not args.skip_cost else ()):
         for prec in ("fp64", "mixed"):
             t = time.time()
             res = ps26119.solve_model_batch(small, cscen, tolerance=tol, warm_x=b.x, warm_y=b.y, verbosity=1,
@@ -400,8 +440,9 @@ def bench_scenarios(args, ps26119, lpgen):
         fp, mx = cost[f"fp64_{tol:.0e}"], cost[f"mixed_{tol:.0e}"]
         if fp["verdicts"].get("Optimal") == mx["verdicts"].get("Optimal") == len(cscen):
             cost[f"mixed_speedup_{tol:.0e}"] = fp["seconds"] / mx["seconds"]
-    log(f"10k end to end: fp64 batch {runs['fp64'][0]:.2f}s {verdicts(runs['fp64'][1])}, mixed batch "
-        f"{runs['mixed'][0]:.2f}s {verdicts(runs['mixed'][1])}, sequential estimate {seq_s:.2f}s "
+    f6, m6 = runs[("fp64", 1e-6)], runs[("mixed", 1e-6)]
+    log(f"{e2e_lp.m} rows end to end to 1e-6: fp64 batch {f6[0]:.2f}s {verdicts(f6[1])}, mixed batch "
+        f"{m6[0]:.2f}s {verdicts(m6[1])}, sequential estimate {seq_s:.2f}s "
         f"(sample {verdicts(seq)}, {agree}/{k_seq} verdicts equal to the batch's)")
     line = base_line(args, batch_rate, "scenario-iterations/s", True, "f64",
                      {"workload": f"{W} scenarios sharing a random {base.m}x{base.n} matrix (6 per row), c scaled by "
@@ -418,14 +459,16 @@ def bench_scenarios(args, ps26119, lpgen):
         "loop_GBs": loop_gbs, "single_lp_iterations_per_s": single_rate,
         "speedup_vs_single_lp_iterations": batch_rate / single_rate,
         "mixed_scenario_iterations_per_s": mixed_rate,
-        "end_to_end_10k": {"scenarios": len(sscen), "tolerance": 1e-6, "warm_start": "base solution",
-                           "perturbation": "b, c and u scaled by lognormal(0, 0.05) factors",
-                           "fp64_batch_seconds": runs["fp64"][0], "fp64_verdicts": verdicts(runs["fp64"][1]),
-                           "mixed_batch_seconds": runs["mixed"][0], "mixed_verdicts": verdicts(runs["mixed"][1]),
-                           "sequential_seconds_estimate": seq_s, "sequential_sample": k_seq,
-                           "sequential_sample_verdicts": verdicts(seq),
-                           "speedup_fp64_batch": seq_s / runs["fp64"][0] if runs["fp64"][0] > 0 else None},
-        "cost_scenarios_10k": {"perturbation": "c scaled by lognormal(0, 0.05) factors", "tolerances": [1e-6, 1e-8],
+        "end_to_end": {"m": e2e_lp.m, "n": e2e_lp.n, "scenarios": len(sscen), "warm_start": "base solution",
+                       "perturbation": "b, c and u scaled by lognormal(0, 0.05) factors", "caps": "3M iterations, 150 s",
+                       **{f"{prec}_batch_{tol:.0e}": {"seconds": runs[(prec, tol)][0],
+                                                       "verdicts": verdicts(runs[(prec, tol)][1]),
+                                                       "iterations_max": max(r.iterations for r in runs[(prec, tol)][1])}
+                          for prec in ("fp64", "mixed") for tol in (1e-6, 1e-8)},
+                       "sequential_seconds_estimate_1e-06": seq_s, "sequential_sample": k_seq,
+                       "sequential_sample_verdicts": verdicts(seq),
+                       "speedup_fp64_batch_1e-06": seq_s / f6[0] if f6[0] > 0 else None},
+        "cost_scenarios_10k": None if args.skip_cost else {"perturbation": "c scaled by lognormal(0, 0.05) factors", "tolerances": [1e-6, 1e-8],
                                "warm_start": "base solution", "caps": "6M iterations, 150 s",
                                "epoch_cap": args.epoch_cap, **cost},
     })
@@ -518,7 +561,8 @@ def main():
                     help="refinery / scenarios: iterations of the rate measurements; lp1m runs to 1e-8 and "
                          "reports the iterations it ran as steps")
     ap.add_argument("--warmup", type=int, default=1, help="untimed warm-up iterations (whole 64-iteration blocks)")
-    ap.add_argument("--workload", default="lp1m", choices=["lp1m", "refinery", "scenarios", "serving", "copy"])
+    ap.add_argument("--workload", default="lp1m", choices=["lp1m", "refinery", "scenarios", "serving", "copy",
+                                                          "stream"])
     ap.add_argument("--m", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--time-limit", type=float, default=450.0)
@@ -538,6 +582,8 @@ def main():
     ap.add_argument("--refinery-rate-only", action="store_true", help="refinery: only the rates at --T (profiling runs)")
     ap.add_argument("--scenarios", type=int, default=64, help="scenarios")
     ap.add_argument("--seq-sample", type=int, default=4, help="scenarios: sequential solves timed for the comparison")
+    ap.add_argument("--e2e-m", type=int, default=10_000, help="scenarios: rows of the end-to-end b/c/u scenario set")
+    ap.add_argument("--skip-cost", action="store_true", help="scenarios: leave out the 10k cost-scenario comparison")
     ap.add_argument("--scenario-iterations", type=int, default=16384, help="scenarios: iterations of the rate measurement")
     ap.add_argument("--scenario-rate-only", action="store_true", help="scenarios: only the batch rates (profiling runs)")
     ap.add_argument("--requests", type=int, default=48, help="serving")
@@ -558,6 +604,9 @@ def main():
 
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)), flush=True)
+        return
+    if args.workload == "stream":
+        print(json.dumps(bench_stream(args)), flush=True)
         return
     import ps26119
     from ps26119_amd import lpgen

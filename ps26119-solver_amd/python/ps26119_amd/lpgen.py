@@ -202,19 +202,26 @@ def read_lpm(path: str) -> Lp:
 
 
 def write_lpm(lp: Lp, path: str) -> None:
-    """Writes `lp` in the `.lpm` text format, version 1 (doubles as repr, so they round-trip)."""
-    def nums(v):
-        return " ".join(repr(float(x)) if np.isfinite(x) else ("inf" if x > 0 else "-inf") for x in v)
+    """Writes `lp` in the `.lpm` text format, version 1, as the reference's tools/lpm.py lays it
+    out (tools/lpm.py:186-222 of the reference): each section tag on its own line, then its numbers
+    eight to a line, doubles as repr (so they round-trip) and infinities as inf / -inf."""
+    def block(f, tag, values, ints=False):
+        f.write(tag + "\n")
+        text = [str(int(x)) if ints else (repr(float(x)) if np.isfinite(x) else ("inf" if x > 0 else "-inf"))
+                for x in values]
+        for k in range(0, len(text), 8):
+            f.write(" ".join(text[k:k + 8]) + "\n")
 
     with open(path, "w") as f:
         f.write(f"LPM 1\nNAME {lp.name or 'model'}\nSENSE {'MAX' if lp.sense < 0 else 'MIN'}\n")
         f.write(f"ROWS {lp.m}\nCOLS {lp.n}\nNNZ {lp.nnz}\nOFFSET {float(lp.offset)!r}\n")
-        for key, v in (("OBJ", lp.c), ("COL_LOWER", lp.col_lower), ("COL_UPPER", lp.col_upper),
+        for tag, v in (("OBJ", lp.c), ("COL_LOWER", lp.col_lower), ("COL_UPPER", lp.col_upper),
                        ("ROW_LOWER", lp.row_lower), ("ROW_UPPER", lp.row_upper)):
-            f.write(f"{key} {nums(v)}\n")
-        f.write("COL_START " + " ".join(str(int(x)) for x in lp.col_start) + "\n")
-        f.write("ROW_INDEX " + " ".join(str(int(x)) for x in lp.row_index) + "\n")
-        f.write(f"VALUE {nums(lp.value)}\nEND\n")
+            block(f, tag, v)
+        block(f, "COL_START", lp.col_start, ints=True)
+        block(f, "ROW_INDEX", lp.row_index, ints=True)
+        block(f, "VALUE", lp.value)
+        f.write("END\n")
 
 
 def from_dense(c, A, rl, ru, cl, cu, sense: int = 1, name: str = "") -> Lp:

@@ -85,6 +85,25 @@ def assert_reference_writer_agrees(lpm, ours, tmp_path):
     assert open(ours, "rb").read() == open(theirs, "rb").read()
 
 
+def test_lpm_files_written_here_read_in_the_reference_reader(tmp_path):
+    """lpgen.write_lpm lays files out as the reference's tools/lpm.py does: the reference's own
+    .lpm reader (src/io/lpm_reader.cpp of the reference, built in oracle/_ref) accepts them, and
+    reading them back here gives the same arrays."""
+    import orc
+    from conftest import hand_models
+    from ps26119_amd import lpgen
+    if not orc.available("ref"):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    ref = orc.load("ref")
+    for name, lp in list(hand_models().items()) + [("random300", lpgen.random_lp(300, seed=3))]:
+        path = str(tmp_path / f"{name}.lpm")
+        lpgen.write_lpm(lp, path)
+        assert ref.write_solution(path, {"status": "NotSolved"}, str(tmp_path / "x.sol")) == 0, name
+        back = lpgen.read_lpm(path)
+        for key in ("c", "col_lower", "col_upper", "row_lower", "row_upper", "col_start", "row_index", "value"):
+            assert np.array_equal(getattr(lp, key), getattr(back, key)), (name, key)
+
+
 def test_version_and_usage():
     assert run("--version").returncode == 0
     assert run().returncode == 2

@@ -305,5 +305,5 @@ def test_bench_refuses_more_than_one_gpu_and_lists_its_workloads():
     h = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert h.returncode == 0
-    for name in ("lp1m", "refinery", "scenarios", "serving", "copy", "--epoch-cap", "--compare-seconds"):
+    for name in ("lp1m", "refinery", "scenarios", "serving", "copy", "stream", "--epoch-cap", "--compare-seconds"):
         assert name in h.stdout, name
