@@ -13,7 +13,10 @@ schedules also run for `--compare-seconds` and report the gap they reach. Per-ke
 come from HIP events on the solve's stream in a second, short run; `roofline` reports the fused
 Ãᵀy + primal-update kernel against the 8 TB/s HBM peak, with the HBM bytes per launch from the
 committed rocprofv3 counter summary. `cpu_baseline` times the reference CPU solver (oracle/_ref,
-all host cores) for 256 iterations of the same model, in the same unit.
+all host cores) for 256 iterations of the same model, in the same unit. `reference_instances`
+solves the reference's own published benchmark instances (rand-1e6-s1 and refinery-T8760-s1, made
+by its generators restated in lpgen) with default options in fp64 and mixed, timed as the
+reference times them, beside its published RTX 4050 and 12-thread CPU times.
 
 Other workloads (each prints one JSON line of the same shape):
   refinery   planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
@@ -224,6 +227,8 @@ def bench_lp1m(args, ps26119, lpgen):
                         "relative_primal": r.primal_residual, "seconds": r.seconds}
             log(f"{tag} for {args.compare_seconds:.0f}s: {r.status_name} it={r.iterations} gap={r.gap:.2e}")
         line["equal_time_comparison"] = cmp
+    if not profiling and not args.skip_reference_instances:
+        line["reference_instances"] = reference_instances(ps26119, lpgen)
     cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP")
     if cb:
         cpu_gbs = (pb + db) / (cb["ms_per_iteration"] * 1e-3) / 1e9
@@ -235,6 +240,40 @@ def bench_lp1m(args, ps26119, lpgen):
               "time_to_1e-8_s_extrapolated": cb["value"], "speedup_per_iteration": cb["ms_per_iteration"] / (per_it * 1e3)}
     line["cpu_baseline"] = cb
     return line
+
+
+# The reference's published time-to-1e-8 runs on its own instances (whole solve() as there,
+# bench/scale.py:74-83 of the reference; bench/results/scale-rtx4050-laptop-183c59c.csv lines):
+# name -> {precision: (iterations, GPU seconds (RTX 4050 Laptop), CPU 12-thread seconds)}
+REFERENCE_PUBLISHED = {
+    "rand-1000000-s1": {"fp64": (17600, 115.7293, 489.3651), "mixed": (16576, 105.8289, 435.7844)},
+    "refinery-T8760-s1": {"fp64": (2880, 6.1839, 18.7071), "mixed": (3136, 5.7836, 14.0595)},
+}
+
+
+def reference_instances(ps26119, lpgen):
+    """The reference's own benchmark instances (its generators restated in lpgen, instance and
+    planted optimum bit for bit) solved here with default options — presolve, 1e-8, the
+    reference's restart schedule, the reference's reductions for decisions — in fp64, in mixed
+    with the reference's precision policy, and in mixed as the increment form, each timed as the
+    reference times it (Solution.seconds) beside its published GPU and 12-thread CPU times."""
+    out = {}
+    for name, make in (("rand-1000000-s1", lambda: lpgen.reference_bench_lp(1_000_000, seed=1)),
+                       ("refinery-T8760-s1", lambda: lpgen.reference_refinery_lp(8760, seed=1))):
+        lp = make()
+        rows = {}
+        for tag, prec, params in (("fp64", "fp64", None), ("mixed", "mixed", {"hip_refine": 0}),
+                                  ("mixed_increment_form", "mixed", None)):
+            r = ps26119.solve_model(lp, precision=prec, params=params)
+            pub = REFERENCE_PUBLISHED[name]["fp64" if prec == "fp64" else "mixed"]
+            rows[tag] = {"status": r.status_name, "check": r.check, "iterations": r.iterations,
+                         "seconds": r.seconds, "objective_error_vs_planted": abs(r.objective - lp.optimum) / (1 + abs(lp.optimum)),
+                         "published_iterations": pub[0], "published_gpu_seconds": pub[1],
+                         "published_cpu12_seconds": pub[2], "speedup_vs_published_gpu": pub[1] / r.seconds}
+            log(f"{name} {tag}: {r.status_name} it={r.iterations} (published {pub[0]}) {r.seconds:.2f}s "
+                f"(published GPU {pub[1]:.2f}s) check={r.check}")
+        out[name] = {"m": lp.m, "n": lp.n, "nnz": lp.nnz, **rows}
+    return out
 
 
 # ------------------------------------------------------------------ copy (counter calibration)
@@ -572,6 +611,8 @@ def main():
     ap.add_argument("--iteration-limit", type=int, default=0,
                     help="stop each timed solve after this many iterations (profiling runs; 0 = to 1e-8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-reference-instances", action="store_true",
+                    help="lp1m: leave out the reference's own published instances (rand-1e6, refinery-T8760)")
     ap.add_argument("--epoch-cap", type=int, default=8192,
                     help="lp1m and the cost scenarios of scenarios: hip_max_epoch with hip_max_epoch_gap_only "
                          "(0 = the reference's restart schedule alone)")

@@ -16,6 +16,9 @@ refinery_lp          T periods of a 49 x 60 block with yield/balance coefficient
                      and t+1 (block staircase); capacities U(10, 1000), demands U(1, 100)
                      times a daily sinusoid, costs U(-5, 20) (see the function docstring).
 scenario_set         perturbs b, c and upper bounds of a base LP by lognormal(0, sigma).
+reference_bench_lp   the reference's own benchmark family rand-<m>-s<seed> (its
+                     bench/generate_lp.py), restated draw for draw so that the same seed gives
+                     the same instance as the reference's published scale runs.
 """
 from __future__ import annotations
 
@@ -107,6 +110,250 @@ def random_lp(m: int, n: int | None = None, per_row: int = 6, seed: int = 1) -> 
     return Lp(m=m, n=n, c=c, col_lower=np.zeros(n), col_upper=np.full(n, 10.0), row_lower=rl, row_upper=ru,
               col_start=col_start, row_index=row_index, value=value, optimum=opt,
               name=f"random-{m}x{n}-k{per_row}-s{seed}", x_star=x, y_star=y)
+
+
+def _plant_duals(rng, m, n, col_start, row_index, value, x, status, eq, ge, le, ranged, active, slack):
+    """Row bounds, duals and costs around a primal point x (the KKT construction of the
+    reference's bench/lpgen.py:67-118, draw for draw): an equality row's bounds are its activity
+    (snapped to 0 below 1e-12) with a dual ±U(0.5, 5); an active ≥ / ≤ row is tight with dual
+    +/−U(0.5, 5); an inactive one, and a ranged row, keeps a slack U(slack)·(1 + |activity|);
+    columns at a bound get reduced costs ±U(0.5, 5) (sign by the bound), others 0, and
+    c = Aᵀy + z. Returns c, rl, ru, y and the optimum cᵀx summed exactly."""
+    import math
+    cidx = np.repeat(np.arange(n), np.diff(col_start))
+    ax = np.bincount(row_index, weights=value * x[cidx], minlength=m)
+    slack_lo = rng.uniform(slack[0], slack[1], m) * (1 + np.abs(ax))
+    slack_up = rng.uniform(slack[0], slack[1], m) * (1 + np.abs(ax))
+    dual_mag = rng.uniform(0.5, 5.0, m)
+    y = np.zeros(m)
+    rl = np.full(m, -INF)
+    ru = np.full(m, INF)
+    snapped = np.where(np.abs(ax) < 1e-12, 0.0, ax)
+    rl[eq] = snapped[eq]
+    ru[eq] = snapped[eq]
+    y[eq] = dual_mag[eq] * rng.choice([-1.0, 1.0], int(eq.sum()))
+    rl[ge & active] = ax[ge & active]
+    y[ge & active] = dual_mag[ge & active]
+    rl[ge & ~active] = (ax - slack_lo)[ge & ~active]
+    ru[le & active] = ax[le & active]
+    y[le & active] = -dual_mag[le & active]
+    ru[le & ~active] = (ax + slack_up)[le & ~active]
+    rl[ranged] = (ax - slack_lo)[ranged]
+    ru[ranged] = (ax + slack_up)[ranged]
+    z_mag = rng.uniform(0.5, 5.0, n)
+    z = np.where(status < 0, z_mag, np.where(status > 0, -z_mag, 0.0))
+    c = np.bincount(cidx, weights=value * y[row_index], minlength=n) + z
+    return c, rl, ru, y, math.fsum((c * x).tolist())
+
+
+def _summed_csc(m, n, r_idx, c_idx, vals):
+    """CSC of triplets with duplicates summed in input order and exact zeros dropped (the
+    reference's bench/lpgen.py:37-54)."""
+    key = c_idx.astype(np.int64) * m + r_idx.astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    key, vals = key[order], vals[order]
+    uniq, first = np.unique(key, return_index=True)
+    summed = np.add.reduceat(vals, first) if len(vals) else vals
+    nz = summed != 0.0
+    uniq, summed = uniq[nz], summed[nz]
+    col_start = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(col_start, uniq // m + 1, 1)
+    return np.cumsum(col_start), uniq % m, summed
+
+
+def reference_bench_lp(m: int, seed: int = 1, cols_per_row: float = 1.5, nnz_per_col: int = 4) -> Lp:
+    """The reference's benchmark LP rand-<m>-s<seed> (reference bench/generate_lp.py:25-57 with
+    the KKT construction of bench/lpgen.py:37-118), restated here: the random draws must come in
+    the reference's order from numpy's default_rng(seed) for the instance — and its planted
+    optimum — to be the reference's. n = round(1.5 m) columns, each with 4 random rows (duplicates
+    summed), then one entry for every row left empty; values ±U(0.5, 2). Columns: 40% at the lower
+    bound 0, 10% at an upper bound, 50% between (a tenth of those free); rows: 20% equalities,
+    40% active one-sided (half ≥, half ≤), 40% inactive (ranged or one-sided). Duals of active
+    rows are ±U(0.5, 5), reduced costs of bound columns ±U(0.5, 5), and c = Aᵀy + z, so the
+    planted point is optimal; its objective is summed exactly (math.fsum), as there."""
+    rng = np.random.default_rng(seed)
+    n = int(round(cols_per_row * m))
+    k = min(nnz_per_col, m)
+    # the matrix: k draws per column, then one random column for each row no draw hit
+    r_idx = rng.integers(0, m, size=(n, k)).ravel()
+    c_idx = np.repeat(np.arange(n), k)
+    empty = np.setdiff1d(np.arange(m), r_idx)
+    r_idx = np.concatenate([r_idx, empty])
+    c_idx = np.concatenate([c_idx, rng.integers(0, n, len(empty))])
+    magnitude = rng.uniform(0.5, 2.0, len(r_idx))
+    vals = magnitude * rng.choice([-1.0, 1.0], len(r_idx))
+    col_start, row_index, value = _summed_csc(m, n, r_idx, c_idx, vals)
+    # column statuses and bounds around x*
+    draw = rng.random(n)
+    status = np.where(draw < 0.4, -1, np.where(draw < 0.5, 1, 0))  # -1 at lower, +1 at upper
+    free = (status == 0) & (rng.random(n) < 0.1)
+    lower = np.where(free, -INF, 0.0)
+    x = np.where(status < 0, 0.0, rng.uniform(0.5, 5.0, n))
+    finite_up = rng.random(n) < 0.4
+    upper = np.where(status > 0, x, np.where(finite_up & ~free, x + rng.uniform(0.5, 5.0, n), INF))
+    upper = np.where(status < 0, np.where(finite_up, rng.uniform(0.5, 5.0, n), INF), upper)
+    x = np.where(free, rng.uniform(-5.0, 5.0, n), x)
+    upper = np.where(free, INF, upper)
+    # row types: 20% eq, 20% ≥ and 20% ≤ active, 20% ranged, 10% ≥ and 10% ≤ inactive
+    t = rng.random(m)
+    eq, ge, le, ranged = t < 0.2, ((t >= 0.2) & (t < 0.4)) | ((t >= 0.8) & (t < 0.9)), \
+        ((t >= 0.4) & (t < 0.6)) | (t >= 0.9), (t >= 0.6) & (t < 0.8)
+    active = (t >= 0.2) & (t < 0.6)
+    c, rl, ru, y, optimum = _plant_duals(rng, m, n, col_start, row_index, value, x, status, eq, ge, le, ranged, active,
+                                         slack=(0.1, 2.0))
+    return Lp(m=m, n=n, c=c, col_lower=lower, col_upper=upper, row_lower=rl, row_upper=ru,
+              col_start=col_start.astype(np.int32), row_index=row_index.astype(np.int32), value=value,
+              optimum=optimum, name=f"rand-{m}-s{seed}", x_star=x, y_star=y)
+
+
+# The reference's refinery model (bench/generate_refinery_lp.py:45-54 there): six crudes bought,
+# run through the crude unit and stored; six cuts; an FCC unit with four products; ten blend
+# components routed over fifteen arcs into five products with six quality specs; product tanks
+# and demands. Each period has 59 columns and 49 rows in this order.
+_RF_CUTS, _RF_FCC, _RF_PRODS = 6, 4, 5
+_RF_COMPS = ("lpg", "nap", "kero", "dsl", "vgo", "res", "fcc_lpg", "fcc_gas", "lco", "slurry")
+_RF_ARCS = ((0, 0), (6, 0), (1, 1), (7, 1), (0, 1), (2, 2), (2, 3), (3, 3), (8, 3), (4, 4), (5, 4), (9, 4), (8, 4),
+            (3, 4), (1, 4))  # (component, product)
+_RF_SPECS = ((1, 0, "le"), (1, 1, "ge"), (2, 0, "le"), (3, 0, "le"), (4, 0, "le"), (4, 1, "ge"))  # (product, quality, sense)
+_RF_FY = np.array([0.15, 0.50, 0.25, 0.10])  # FCC yields
+
+
+def _rf_layout():
+    """Offsets of the column and row groups of one period."""
+    cols, rows, at = {}, {}, 0
+    for name, size in (("buy", 6), ("cdu", 6), ("cinv", 6), ("prod", _RF_CUTS), ("fcc", 1), ("fout", _RF_FCC),
+                       ("blend", len(_RF_ARCS)), ("out", _RF_PRODS), ("pinv", _RF_PRODS), ("sell", _RF_PRODS)):
+        cols[name], at = np.arange(at, at + size), at + size
+    ncols, at = at, 0
+    for name, size in (("cbal", 6), ("cdu", 1), ("yield", _RF_CUTS), ("fcccap", 1), ("fy", _RF_FCC),
+                       ("comp", len(_RF_COMPS)), ("prod", _RF_PRODS), ("spec", len(_RF_SPECS)), ("pbal", _RF_PRODS),
+                       ("dem", _RF_PRODS)):
+        rows[name], at = np.arange(at, at + size), at + size
+    return cols, rows, ncols, at
+
+
+def reference_refinery_lp(T: int, seed: int = 1) -> Lp:
+    """The reference's refinery benchmark LP refinery-T<T>-s<seed> (reference
+    bench/generate_refinery_lp.py:88-276 with the KKT construction of bench/lpgen.py), restated
+    draw for draw so that a seed gives the reference's instance and planted optimum: a physically
+    consistent flow x* simulated period by period (crude unit at capacity in 30% of periods, FCC
+    cap binding in 30%, some crude purchases at their availability limit, tanks at capacity when
+    full), then duals and prices planted around it with slacks U(0.05, 0.5); a MAX problem."""
+    rng = np.random.default_rng(seed)
+    C, R, ncols, nrows = _rf_layout()
+    K, J, P = 6, _RF_CUTS, _RF_PRODS
+    yields = rng.dirichlet(np.ones(J) * 3, K)  # crude k → cut j, rows sum to 1
+    quality = np.stack([rng.uniform(0.1, 3.0, len(_RF_COMPS)), rng.uniform(60, 100, len(_RF_COMPS))], 1)
+    cdu_cap, crude_tank, prod_tank = 1000.0, 3000.0, 2000.0
+    # which period columns feed each component's balance row
+    supply = [[(C["prod"][0], 1.0)], [(C["prod"][1], 1.0)], [(C["prod"][2], 1.0)], [(C["prod"][3], 1.0)],
+              [(C["prod"][4], 1.0), (C["fcc"][0], -1.0)], [(C["prod"][5], 1.0)], [(C["fout"][0], 1.0)],
+              [(C["fout"][1], 1.0)], [(C["fout"][2], 1.0)], [(C["fout"][3], 1.0)]]
+    # the period template: (row, column, value, period offset of the column)
+    tpl = []
+    for k in range(K):
+        tpl += [(R["cbal"][k], C["cinv"][k], 1.0, 0), (R["cbal"][k], C["cinv"][k], -1.0, -1),
+                (R["cbal"][k], C["buy"][k], -1.0, 0), (R["cbal"][k], C["cdu"][k], 1.0, 0), (R["cdu"][0], C["cdu"][k], 1.0, 0)]
+    for j in range(J):
+        tpl.append((R["yield"][j], C["prod"][j], 1.0, 0))
+        tpl += [(R["yield"][j], C["cdu"][k], -yields[k, j], 0) for k in range(K)]
+    tpl.append((R["fcccap"][0], C["fcc"][0], 1.0, 0))
+    for o in range(_RF_FCC):
+        tpl += [(R["fy"][o], C["fout"][o], 1.0, 0), (R["fy"][o], C["fcc"][0], -_RF_FY[o], 0)]
+    for ci, terms in enumerate(supply):
+        tpl += [(R["comp"][ci], col, coef, 0) for col, coef in terms]
+    for a, (ci, pi) in enumerate(_RF_ARCS):
+        tpl += [(R["comp"][ci], C["blend"][a], -1.0, 0), (R["prod"][pi], C["blend"][a], -1.0, 0)]
+    tpl += [(R["prod"][p], C["out"][p], 1.0, 0) for p in range(P)]
+    for s_, (pi, qi, _) in enumerate(_RF_SPECS):
+        arcs = [a for a, (_, pp) in enumerate(_RF_ARCS) if pp == pi]
+        limit = float(np.mean(quality[[_RF_ARCS[a][0] for a in arcs], qi]))
+        tpl += [(R["spec"][s_], C["blend"][a], quality[_RF_ARCS[a][0], qi] - limit, 0) for a in arcs]
+    for p in range(P):
+        tpl += [(R["pbal"][p], C["pinv"][p], 1.0, 0), (R["pbal"][p], C["pinv"][p], -1.0, -1),
+                (R["pbal"][p], C["out"][p], -1.0, 0), (R["pbal"][p], C["sell"][p], 1.0, 0), (R["dem"][p], C["sell"][p], 1.0, 0)]
+    tr, tc, tv, ts = (np.asarray(v) for v in zip(*tpl))
+    period = np.repeat(np.arange(T), len(tr))
+    col_period = period + np.tile(ts, T)
+    kept = col_period >= 0  # period 0 has no previous inventory column
+    m, n = T * nrows, T * ncols
+    col_start, row_index, value = _summed_csc(m, n, (np.tile(tr, T) + period * nrows)[kept],
+                                              (np.tile(tc, T) + col_period * ncols)[kept], np.tile(tv, T)[kept])
+    # the flows x*, period by period
+    x = np.zeros((T, ncols))
+    share = rng.dirichlet(np.ones(K), T)
+    load = rng.uniform(0.6, 1.0, T)
+    cdu_full = rng.random(T) < 0.3
+    load[cdu_full] = 1.0
+    x[:, C["cdu"]] = share * (cdu_cap * load)[:, None]
+    x[:, C["prod"]] = x[:, C["cdu"]] @ yields
+    x[:, C["fcc"][0]] = rng.uniform(0.5, 0.9, T) * x[:, C["prod"][4]]
+    fcc_full = rng.random(T) < 0.3
+    x[:, C["fout"]] = x[:, [C["fcc"][0]]] * _RF_FY[None, :]
+    avail = np.zeros((T, len(_RF_COMPS)))
+    for ci, terms in enumerate(supply):
+        for col, coef in terms:
+            avail[:, ci] += coef * x[:, col]
+    for ci in range(len(_RF_COMPS)):  # each component split over its arcs, the last arc takes the rest
+        arcs = [a for a, (cc, _) in enumerate(_RF_ARCS) if cc == ci]
+        split = rng.dirichlet(np.ones(len(arcs)), T) * avail[:, [ci]]
+        split[:, -1] = avail[:, ci] - split[:, :-1].sum(1)
+        x[:, C["blend"][arcs]] = split
+    for p in range(P):
+        x[:, C["out"][p]] = x[:, C["blend"][[a for a, (_, pp) in enumerate(_RF_ARCS) if pp == p]]].sum(1)
+    crude, stock = np.full(K, 500.0), np.full(P, 300.0)
+    for t in range(T):  # purchases around consumption and sales around output, inventories within tanks
+        buy = np.clip(x[t, C["cdu"]] * rng.uniform(0.7, 1.3, K), 0, None)
+        buy = np.minimum(buy, crude_tank - crude + x[t, C["cdu"]])
+        crude = crude + buy - x[t, C["cdu"]]
+        short = crude < 0
+        buy[short] -= crude[short]
+        crude[short] = 0.0
+        x[t, C["buy"]], x[t, C["cinv"]] = buy, crude
+        sell = x[t, C["out"]] * rng.uniform(0.7, 1.3, P)
+        nxt = stock + x[t, C["out"]] - sell
+        under = nxt < 0
+        sell[under] += nxt[under]
+        nxt[under] = 0.0
+        over = nxt > prod_tank
+        sell[over] += nxt[over] - prod_tank
+        nxt[over] = prod_tank
+        stock = nxt
+        x[t, C["sell"]], x[t, C["pinv"]] = sell, stock
+    xs = np.maximum(x, 0.0).ravel()
+    # bounds and the columns that rest on one
+    lower, upper = np.zeros(n), np.full(n, INF)
+    status = np.zeros(n, dtype=np.int8)
+    status[xs == 0.0] = -1
+    base = np.arange(T)[:, None] * ncols
+    buys = (base + C["buy"][None, :]).ravel()
+    at_limit = (rng.random(buys.size) < 0.2) & (xs[buys] > 0)
+    upper[buys] = np.where(at_limit, xs[buys], xs[buys] * 1.5 + 50.0)
+    status[buys[at_limit]] = 1
+    cinvs, pinvs = (base + C["cinv"][None, :]).ravel(), (base + C["pinv"][None, :]).ravel()
+    upper[cinvs], upper[pinvs] = crude_tank, prod_tank
+    status[pinvs[xs[pinvs] >= prod_tank]] = 1
+    status[cinvs[xs[cinvs] >= crude_tank]] = 1
+    # row senses and which inequalities are tight at x*
+    eq, ge, le, active = (np.zeros(m, dtype=bool) for _ in range(4))
+    rbase = np.arange(T)[:, None] * nrows
+    for g in ("cbal", "yield", "fy", "comp", "prod", "pbal"):
+        eq[(rbase + R[g][None, :]).ravel()] = True
+    cdu_rows, fcc_rows = np.arange(T) * nrows + R["cdu"][0], np.arange(T) * nrows + R["fcccap"][0]
+    le[cdu_rows], le[fcc_rows] = True, True
+    active[cdu_rows], active[fcc_rows] = cdu_full, fcc_full
+    dem_rows = (rbase + R["dem"][None, :]).ravel()
+    le[dem_rows] = True
+    active[dem_rows] = rng.random(dem_rows.size) < 0.6
+    spec_rows = (rbase + R["spec"][None, :]).ravel()
+    spec_ge = np.tile([sense == "ge" for _, _, sense in _RF_SPECS], T)
+    ge[spec_rows[spec_ge]], le[spec_rows[~spec_ge]] = True, True
+    active[spec_rows] = rng.random(spec_rows.size) < 0.5
+    c, rl, ru, y, optimum = _plant_duals(rng, m, n, col_start, row_index, value, xs, status, eq, ge, le,
+                                         np.zeros(m, dtype=bool), active, slack=(0.05, 0.5))
+    return Lp(m=m, n=n, c=-c, col_lower=lower, col_upper=upper, row_lower=rl, row_upper=ru,
+              col_start=col_start.astype(np.int32), row_index=row_index.astype(np.int32), value=value, sense=-1,
+              optimum=-optimum, name=f"refinery-T{T}-s{seed}", x_star=xs, y_star=y)
 
 
 def scenario_set(base: Lp, k: int, sigma: float = 0.05, seed: int = 7):
