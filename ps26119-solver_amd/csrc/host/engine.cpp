@@ -571,7 +571,9 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
     dev_->check(psh_download(ctx(), ys_slot, ys.data()), "download");
     sp_.unscale_primal(xs, sol.x);
     sp_.unscale_dual(ys, y_min);
+    const double t_download = elapsed();
     const KktStats k = adjust(kkt_on_original(sp_, sol.x, y_min));
+    const double t_kkt = elapsed();
     if (!k.finite()) status = Status::NumericalError;
     if (status == Status::Optimal && !k.converged(opt_.tolerance)) status = Status::NumericalError;
     const double sense = model_.sense;
@@ -594,6 +596,9 @@ class Context {  // reference EngineContext (engine.cpp:195-307)
       multiply(A, sol.x.data(), sol.row_activity.data());
     }
     sol.objective = model_.objective_value(sol.x);
+    if (opt_.verbosity >= 1)
+      std::fprintf(stderr, "finish: download and unscale %.3fs, KKT on the original %.3fs, z and activities %.3fs\n",
+                   t_download - loop_end, t_kkt - t_download, elapsed() - t_kkt);
     sol.dual_objective = sense * k.dual_obj + model_.obj_offset;
     sol.primal_residual = k.rel_primal();
     sol.dual_residual = k.rel_dual();

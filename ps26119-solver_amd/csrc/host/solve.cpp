@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <memory>
 #include <new>
 
 #include "engine.h"
@@ -74,8 +75,9 @@ Solution refuse(const Model& model, const char* engine, const std::string& why, 
 
 // The engine on one model, exactly as given (no presolve): argument checks, r²HPDHG or PDLP on
 // the device, and for an Unbounded verdict whose point is not primal-feasible a zero-objective
-// solve for a point the ray can start from. `certify`: also the certified dual bound.
-Solution engine_solve(const Model& model, const Options& options, bool certify) {
+// solve for a point the ray can start from. `certify`: also the certified dual bound (with the
+// implied bounds of `implied` when it was started for this model).
+Solution engine_solve(const Model& model, const Options& options, bool certify, ImpliedBoundsJob* implied = nullptr) {
   const auto t0 = Clock::now();
   Solution sol;
   sol.model_fingerprint = model.fingerprint_hex();
@@ -147,25 +149,26 @@ Solution engine_solve(const Model& model, const Options& options, bool certify) 
   sol.model_fingerprint = model.fingerprint_hex();
   if (certify && static_cast<int>(sol.y.size()) == model.num_rows && model.num_rows + model.num_cols > 0)
     sol.certified_bound =
-        certified_dual_bound(model, sol.y, may_use_implied_bounds(seconds_since(t0), options.time_limit)).bound;
+        certified_dual_bound(model, sol.y, may_use_implied_bounds(seconds_since(t0), options.time_limit), implied).bound;
   sol.seconds = seconds_since(t0);
   return sol;
 }
 
 class Pipeline {
  public:
-  Pipeline(const Model& model, const Options& options) : M_(model), opt_(options), t0_(Clock::now()) {}
+  Pipeline(const Model& model, const Options& options, ImpliedBoundsJob* implied)
+      : M_(model), opt_(options), t0_(Clock::now()), implied_(implied) {}
 
   Solution run() {
     const bool usable = options_usable();
-    if (!opt_.presolve || !usable || !M_.validate().empty()) return engine_solve(M_, opt_, true);
+    if (!opt_.presolve || !usable || !M_.validate().empty()) return engine_solve(M_, opt_, true, implied_);
     if (auto which = M_.crossed_bounds(); !which.empty()) return infeasible_by_bounds(M_, opt_, which);
     const auto tp = Clock::now();
     P_ = presolve(M_);
     if (opt_.verbosity >= 1) std::fprintf(stderr, "presolve: %.3fs\n", seconds_since(tp));
     switch (P_.outcome) {
       case Presolved::Outcome::NothingToDo:
-        return engine_solve(M_, opt_, true);
+        return engine_solve(M_, opt_, true, implied_);
       case Presolved::Outcome::ProvedInfeasible:
         return solve_original(0, "it proved the model infeasible (" + P_.reason + ")");
       case Presolved::Outcome::Reduced:
@@ -272,7 +275,7 @@ class Pipeline {
   Solution solve_original(std::int64_t spent, const std::string& why) const {
     Options o = remaining(opt_, spent);
     o.presolve = false;
-    Solution s = engine_solve(M_, o, true);
+    Solution s = engine_solve(M_, o, true, implied_);
     s.iterations += spent;
     s.seconds = seconds_since(t0_);
     append(s, "presolve: " + why + "; the original model was solved directly");
@@ -285,10 +288,14 @@ class Pipeline {
   }
 
   Solution finish_optimal_path(const Solution& red) const {
+    const auto tf = Clock::now();
     Solution post = postsolve(M_, P_, red);
     append(post, summary_);
     post.model_fingerprint = M_.fingerprint_hex();
-    if (post.status == Status::Optimal && !meets_tolerance(post)) {
+    const double t_post = seconds_since(tf);
+    const bool met = post.status != Status::Optimal || meets_tolerance(post);
+    const double t_met = seconds_since(tf);
+    if (post.status == Status::Optimal && !met) {
       // The reduced model's KKT terms differ from the original's: first the reduced model again,
       // warm-started from its answer at 100× the tolerance, then the original from scratch.
       Options tighter = reduced_options();
@@ -305,7 +312,7 @@ class Pipeline {
         post = std::move(post2);
       } else {
         const std::int64_t spent = red.iterations + red2.iterations;
-        Solution direct = engine_solve(M_, remaining(reduced_options(), spent), true);
+        Solution direct = engine_solve(M_, remaining(reduced_options(), spent), true, implied_);
         direct.iterations += spent;
         append(direct, summary_ + "; the lifted point missed the tolerance on the original model, which was solved "
                                    "directly");
@@ -314,9 +321,13 @@ class Pipeline {
     } else if (post.status == Status::Optimal && static_cast<int>(post.x.size()) != M_.num_cols) {
       post.status = Status::NumericalError;
     }
+    const double t_before_bound = seconds_since(tf);
     if (static_cast<int>(post.y.size()) == M_.num_rows && M_.num_rows + M_.num_cols > 0)
       post.certified_bound =
-          certified_dual_bound(M_, post.y, may_use_implied_bounds(seconds_since(t0_), opt_.time_limit)).bound;
+          certified_dual_bound(M_, post.y, may_use_implied_bounds(seconds_since(t0_), opt_.time_limit), implied_).bound;
+    if (opt_.verbosity >= 1)
+      std::fprintf(stderr, "postsolve %.3fs, KKT of the lifted point %.3fs, certified bound %.3fs\n", t_post,
+                   t_met - t_post, seconds_since(tf) - t_before_bound);
     post.seconds = seconds_since(t0_);
     return post;
   }
@@ -324,6 +335,7 @@ class Pipeline {
   const Model& M_;
   const Options& opt_;
   const Clock::time_point t0_;
+  ImpliedBoundsJob* implied_;  // the original model's implied bounds, computed while the engine runs
   Presolved P_;
   std::string summary_;
 };
@@ -396,7 +408,15 @@ Solution solve(const Model& model, const Options& options) {
     const bool chose = options.algorithm == Algorithm::Auto;
     if (chose) resolved.algorithm = Algorithm::R2hpdhg;  // a GPU request defaults to r2hpdhg
     set_host_threads(options.threads);
-    Solution s = Pipeline(model, resolved).run();
+    // The certified bound needs the implied column bounds only when a column has an infinite
+    // bound; they depend on the model alone, so they are computed while the engine runs.
+    std::unique_ptr<ImpliedBoundsJob> implied;
+    auto open = [](const std::vector<double>& b) {
+      return std::any_of(b.begin(), b.end(), [](double v) { return !std::isfinite(v); });
+    };
+    if ((open(model.col_lower) || open(model.col_upper)) && model.num_rows > 0 && model.validate().empty())
+      implied = std::make_unique<ImpliedBoundsJob>(model);
+    Solution s = Pipeline(model, resolved, implied.get()).run();
     gate(model, resolved, s);
     if (chose) {
       char head[96];

@@ -222,7 +222,9 @@ SafeBound bound_with(const Model& M, const std::vector<double>& y, const std::ve
 
 // Bounds implied by the rows (feasibility-based tightening), loosened by 1e-9·scale and rounded
 // outward; rows are revisited only when one of their columns changed.
-void implied_bounds(const Model& M, std::vector<double>& lo, std::vector<double>& up, int max_passes = 20) {
+// Returns false when `cancel` stopped it (lo and up are then incomplete).
+bool implied_bounds(const Model& M, std::vector<double>& lo, std::vector<double>& up,
+                    const std::atomic<bool>* cancel = nullptr, int max_passes = 20) {
   lo = M.col_lower;
   up = M.col_upper;
   const Csr A = transpose(csc_as_transpose(M));
@@ -237,6 +239,7 @@ void implied_bounds(const Model& M, std::vector<double>& lo, std::vector<double>
     bool changed = false;
     for (int i = 0; i < M.num_rows; ++i) {
       if (!dirty[i]) continue;
+      if (cancel && (i & 4095) == 0 && cancel->load(std::memory_order_relaxed)) return false;
       dirty[i] = 0;
       const bool has_up = std::isfinite(M.row_upper[i]), has_lo = std::isfinite(M.row_lower[i]);
       if (!has_up && !has_lo) continue;
@@ -294,6 +297,7 @@ void implied_bounds(const Model& M, std::vector<double>& lo, std::vector<double>
     }
     if (!changed) break;
   }
+  return true;
 }
 
 }  // namespace
@@ -387,7 +391,30 @@ CheckReport check_solution(const Model& lp, const std::vector<double>& x, const 
   return rep;
 }
 
-SafeBound certified_dual_bound(const Model& M, const std::vector<double>& y_in, bool use_implied_bounds) {
+ImpliedBoundsJob::ImpliedBoundsJob(const Model& model) : model_(model) {
+  thread_ = std::thread([this] {
+    try {
+      ok_ = implied_bounds(model_, lo_, up_, &cancel_);
+    } catch (...) {
+      ok_ = false;
+    }
+  });
+}
+
+ImpliedBoundsJob::~ImpliedBoundsJob() {
+  cancel_.store(true, std::memory_order_relaxed);
+  if (thread_.joinable()) thread_.join();
+}
+
+bool ImpliedBoundsJob::wait(const std::vector<double>** lo, const std::vector<double>** up) {
+  if (thread_.joinable()) thread_.join();
+  *lo = &lo_;
+  *up = &up_;
+  return ok_;
+}
+
+SafeBound certified_dual_bound(const Model& M, const std::vector<double>& y_in, bool use_implied_bounds,
+                               ImpliedBoundsJob* implied) {
   if (static_cast<int>(y_in.size()) != M.num_rows) return SafeBound{};
   // Any multipliers give a valid bound: zero those pointing at an infinite row bound.
   std::vector<double> y = y_in;
@@ -397,9 +424,14 @@ SafeBound certified_dual_bound(const Model& M, const std::vector<double>& y_in, 
   }
   SafeBound r = bound_with(M, y, M.col_lower, M.col_upper);
   if (use_implied_bounds && r.unbounded_col_terms > 0 && r.unbounded_row_terms == 0) {
-    std::vector<double> lo, up;
-    implied_bounds(M, lo, up);
-    r = bound_with(M, y, lo, up);
+    const std::vector<double>*lo = nullptr, *up = nullptr;
+    if (implied && &implied->model() == &M && implied->wait(&lo, &up)) {
+      r = bound_with(M, y, *lo, *up);
+    } else {
+      std::vector<double> lo_own, up_own;
+      implied_bounds(M, lo_own, up_own);
+      r = bound_with(M, y, lo_own, up_own);
+    }
   }
   return r;
 }

@@ -9,7 +9,9 @@
 // is written by one task and every sequential sum keeps the reference's order.
 #pragma once
 
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "contract.h"
@@ -31,7 +33,32 @@ struct SafeBound {
   double bound = 0.0;
   int unbounded_row_terms = 0, unbounded_col_terms = 0;
 };
-SafeBound certified_dual_bound(const Model& model, const std::vector<double>& y, bool use_implied_bounds = true);
+
+// The column bounds the rows imply (a function of the model alone, sequential by definition),
+// computed on a thread of its own from the moment a solve starts, so that the certified bound
+// after the solve finds them ready instead of adding their time to the solve's. The model must
+// outlive the job; destroying the job stops the thread early if it is still running.
+class ImpliedBoundsJob {
+ public:
+  explicit ImpliedBoundsJob(const Model& model);
+  ~ImpliedBoundsJob();
+  ImpliedBoundsJob(const ImpliedBoundsJob&) = delete;
+  ImpliedBoundsJob& operator=(const ImpliedBoundsJob&) = delete;
+  const Model& model() const { return model_; }
+  // Waits for the bounds; false when they could not be computed (the caller then computes them).
+  bool wait(const std::vector<double>** lo, const std::vector<double>** up);
+
+ private:
+  const Model& model_;
+  std::vector<double> lo_, up_;
+  std::atomic<bool> cancel_{false};
+  bool ok_ = false;
+  std::thread thread_;
+};
+
+// `implied`, when given, must belong to `model`: its bounds are used instead of computing them.
+SafeBound certified_dual_bound(const Model& model, const std::vector<double>& y, bool use_implied_bounds = true,
+                               ImpliedBoundsJob* implied = nullptr);
 
 struct CertificateCheck {
   bool present = false, passed = false, rigorous = false;
