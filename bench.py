@@ -8,15 +8,16 @@ one r²HPDHG iteration of the solver's loop (two fused launches, the check every
 restarts), `steps` is the number the solve ran; `value` is the HBM GB/s of the iteration loop
 (algorithmic bytes per iteration × iterations ÷ the loop's wall time, inputs resident in HBM).
 The reference's restart schedule alone stalls near a relative gap of 3e-7 on this instance, so
-epochs are capped at `--epoch-cap 8192` iterations while the gap leads (said in `config`); both
-schedules also run for `--compare-seconds` and report the gap they reach. Per-kernel durations
-come from HIP events on the solve's stream in a second, short run; `roofline` reports the fused
-Ãᵀy + primal-update kernel against the 8 TB/s HBM peak, with the HBM bytes per launch from the
-committed rocprofv3 counter summary. `cpu_baseline` times the reference CPU solver (oracle/_ref,
-all host cores) for 256 iterations of the same model, in the same unit. `reference_instances`
-solves the reference's own published benchmark instances (rand-1e6-s1 and refinery-T8760-s1, made
-by its generators restated in lpgen) with default options in fp64 and mixed, timed as the
-reference times them, beside its published RTX 4050 and 12-thread CPU times.
+epochs are capped at `--epoch-cap 8192` iterations while the gap leads (said in `config`);
+`--compare-seconds S` also runs both schedules for S seconds and reports the gap each reaches.
+Per-kernel durations come from HIP events on the solve's stream in a second, short run; `roofline`
+reports the fused Ãᵀy + primal-update kernel against the 8 TB/s HBM peak, with the HBM bytes per
+launch from the committed rocprofv3 counter summary. `cpu_baseline` times the reference CPU solver
+(oracle/_ref, all host cores) for 256 iterations of the same model, in the same unit (one thread for
+32 beside it). `reference_instances` solves the reference's own published benchmark instances
+(rand-1e6-s1 and refinery-T8760-s1, made by its generators restated in lpgen) with default options
+in fp64 and mixed, timed as the reference times them, beside its published RTX 4050 and 12-thread
+CPU times.
 
 Other workloads (each prints one JSON line of the same shape):
   refinery   planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
@@ -99,8 +100,10 @@ def host_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(lp, gpu_iterations, gpu_solve, what):
-    """Reference CPU solver on all host cores, bounded sample: setup plus 256 iterations."""
+def cpu_baseline(lp, gpu_iterations, gpu_solve, what, single_thread_iterations=0):
+    """Reference CPU solver on all host cores, bounded sample: setup plus 256 iterations; with
+    single_thread_iterations > 0 also a one-thread sample of that many iterations (the
+    reference's own CPU rows pair its fastest thread count with one thread)."""
     import orc
     if not orc.available("ref"):
         return None
@@ -111,12 +114,17 @@ def cpu_baseline(lp, gpu_iterations, gpu_solve, what):
     wall = time.time() - t
     per_it = (r["seconds"] - r["setup_seconds"]) / max(1, r["iterations"])
     est = r["setup_seconds"] + per_it * gpu_iterations + max(0.0, wall - r["seconds"])
-    return {"value": est, "unit": "s", "cores": host_cores(), "kind": "reference",
-            "sample": f"reference r2HPDHG (oracle/_ref, threads=all) on the same {what}, {r['iterations']} iterations "
-                      f"in {r['seconds']:.1f} s (setup {r['setup_seconds']:.1f} s, {per_it * 1e3:.2f} ms/iteration), "
-                      f"extrapolated to the GPU run's {gpu_iterations} iterations",
-            "ms_per_iteration": per_it * 1e3, "setup_seconds": r["setup_seconds"],
-            "speedup_vs_cpu": est / gpu_solve if gpu_solve > 0 else None}
+    out = {"value": est, "unit": "s", "cores": host_cores(), "kind": "reference",
+           "sample": f"reference r2HPDHG (oracle/_ref, threads=all) on the same {what}, {r['iterations']} iterations "
+                     f"in {r['seconds']:.1f} s (setup {r['setup_seconds']:.1f} s, {per_it * 1e3:.2f} ms/iteration), "
+                     f"extrapolated to the GPU run's {gpu_iterations} iterations",
+           "ms_per_iteration": per_it * 1e3, "setup_seconds": r["setup_seconds"],
+           "speedup_vs_cpu": est / gpu_solve if gpu_solve > 0 else None}
+    if single_thread_iterations > 0:
+        r1, _ = ref.solve(lp, presolve=True, iteration_limit=single_thread_iterations, threads=1)
+        out["single_thread"] = {"iterations": r1["iterations"], "setup_seconds": r1["setup_seconds"],
+                                "ms_per_iteration": (r1["seconds"] - r1["setup_seconds"]) / max(1, r1["iterations"]) * 1e3}
+    return out
 
 
 def base_line(args, value, unit, higher, dtype, config):
@@ -162,8 +170,8 @@ def bench_lp1m(args, ps26119, lpgen):
     (inputs resident in HBM; setup and finish excluded). The reference's restart schedule alone
     plateaus near a relative gap of 3e-7 on this instance, so the solve ends an epoch at
     --epoch-cap iterations while the gap is the largest KKT term (engine knobs hip_max_epoch,
-    hip_max_epoch_gap_only; in `config`); beside it both schedules run for --compare-seconds and
-    report the gap they reach. --iteration-limit N instead stops the timed solve after N
+    hip_max_epoch_gap_only; in `config`); --compare-seconds S > 0 also runs both schedules for S
+    seconds and reports the gap each reaches. --iteration-limit N instead stops the timed solve after N
     iterations (profiling runs)."""
     t = time.time()
     lp = lpgen.random_lp(args.m, seed=args.seed)
@@ -229,8 +237,10 @@ def bench_lp1m(args, ps26119, lpgen):
         line["equal_time_comparison"] = cmp
     if not profiling and not args.skip_reference_instances:
         line["reference_instances"] = reference_instances(ps26119, lpgen)
-    cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP")
+    cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP",
+                                                         single_thread_iterations=32)
     if cb:
+        one = cb.get("single_thread")
         cpu_gbs = (pb + db) / (cb["ms_per_iteration"] * 1e-3) / 1e9
         sample = (f"reference r2HPDHG (oracle/_ref, threads=all) on the same 1M-row LP: 256 iterations after its setup, "
                   f"{cb['ms_per_iteration']:.2f} ms/iteration; value = the same algorithmic bytes per iteration ÷ that time; "
@@ -238,6 +248,9 @@ def bench_lp1m(args, ps26119, lpgen):
         cb = {"value": cpu_gbs, "unit": "GB/s", "cores": cb["cores"], "kind": "reference", "sample": sample,
               "cpu_model": cpu_model(), "ms_per_iteration": cb["ms_per_iteration"], "setup_seconds": cb["setup_seconds"],
               "time_to_1e-8_s_extrapolated": cb["value"], "speedup_per_iteration": cb["ms_per_iteration"] / (per_it * 1e3)}
+        if one:
+            cb["single_thread"] = {**one, "GB/s": (pb + db) / (one["ms_per_iteration"] * 1e-3) / 1e9,
+                                   "sample": f"the same solver with threads=1: {one['iterations']} iterations after its setup"}
     line["cpu_baseline"] = cb
     return line
 
@@ -605,8 +618,8 @@ def main():
     ap.add_argument("--m", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--time-limit", type=float, default=450.0)
-    ap.add_argument("--compare-seconds", type=float, default=30.0,
-                    help="lp1m: both restart schedules run this long and report the gap they reach (0 = skip)")
+    ap.add_argument("--compare-seconds", type=float, default=0.0,
+                    help="lp1m: both restart schedules also run this long and report the gap they reach (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=64, help="copy: buffer size of the calibration copy")
     ap.add_argument("--iteration-limit", type=int, default=0,
                     help="stop each timed solve after this many iterations (profiling runs; 0 = to 1e-8)")

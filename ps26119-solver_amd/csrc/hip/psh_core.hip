@@ -187,22 +187,50 @@ __global__ void __launch_bounds__(64) k_ordered_final(const RedJob* __restrict__
   results[first + blockIdx.x] = s;
 }
 
-static int launch_reduce(psh_ctx* c, const RedJob* dev, const RedJob* host, int njobs, int first) {
+// parts: 1 = tree jobs, 2 = ordered jobs, 3 = both, on stream s
+static int launch_reduce(psh_ctx* c, const RedJob* dev, const RedJob* host, int njobs, int first, int parts = 3,
+                         hipStream_t s = nullptr) {
+  if (!s) s = c->stream;
   bool tree = false, ordered = false;
   for (int k = 0; k < njobs; ++k) {
     if (host[k].op == 1 || host[k].mode == 0) tree = true;
     else ordered = true;
   }
-  if (tree) {
-    hipLaunchKernelGGL(k_tree1, dim3(kTreeBlocks, njobs), dim3(kBlock), 0, c->stream, dev, c->partials);
-    hipLaunchKernelGGL(k_tree2, dim3(njobs), dim3(kBlock), 0, c->stream, dev, c->partials, c->results, first);
+  if (tree && (parts & 1)) {
+    hipLaunchKernelGGL(k_tree1, dim3(kTreeBlocks, njobs), dim3(kBlock), 0, s, dev, c->partials);
+    hipLaunchKernelGGL(k_tree2, dim3(njobs), dim3(kBlock), 0, s, dev, c->partials, c->results, first);
   }
   // ordered jobs use their own rows of `partials` (a job is either a tree job or an ordered one)
-  if (ordered) {
-    hipLaunchKernelGGL(k_ordered, dim3(64, njobs), dim3(64), 0, c->stream, dev, c->partials);
-    hipLaunchKernelGGL(k_ordered_final, dim3(njobs), dim3(64), 0, c->stream, dev, c->partials, c->results, first);
+  if (ordered && (parts & 2)) {
+    hipLaunchKernelGGL(k_ordered, dim3(64, njobs), dim3(64), 0, s, dev, c->partials);
+    hipLaunchKernelGGL(k_ordered_final, dim3(njobs), dim3(64), 0, s, dev, c->partials, c->results, first);
   }
   PSH_LAUNCHED(c, "reduction kernels");
+  return PSH_OK;
+}
+
+int reduce_table_part(psh_ctx* c, const JobTable& t, int first, int part, hipStream_t s) {
+  if (t.host.empty()) return PSH_OK;
+  return launch_reduce(c, t.dev, t.host.data(), static_cast<int>(t.host.size()), first, part, s);
+}
+
+int ensure_aux(psh_ctx* c) {
+  if (c->aux) return PSH_OK;
+  PSH_TRY(c, hipSetDevice(c->device));
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
+  hipStream_t s = nullptr;
+  hipEvent_t f = nullptr, j = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (f) (void)hipEventDestroy(f);
+    if (s) (void)hipStreamDestroy(s);
+    return fail(c, PSH_E_HIP, "check pass: second stream");
+  }
+  c->aux = s;
+  c->fork = f;
+  c->join = j;
   return PSH_OK;
 }
 
@@ -704,6 +732,7 @@ void psh_destroy(psh_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->aux) (void)hipStreamSynchronize(c->aux);
   psh::destroy_batch(c);
   for (auto& kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
   {
@@ -727,9 +756,12 @@ void psh_destroy(psh_ctx* c) {
   }
   for (auto& p : c->pending) c->ev_pool.push_back(p.a), c->ev_pool.push_back(p.b);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-  if (c->stream) {
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->stream || c->aux) {
     std::lock_guard<std::recursive_mutex> lock(api_mutex());
-    (void)hipStreamDestroy(c->stream);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
   }
   delete c;
 }
@@ -779,6 +811,8 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
       (rc = up(&c->clo, p->cl_o, n)) || (rc = up(&c->cuo, p->cu_o, n)) || (rc = up(&c->rlo, p->rl_o, m)) ||
       (rc = up(&c->ruo, p->ru_o, m)) || (rc = up(&c->cmin, p->c_min, n)))
     return rc;
+  c->boxed = true;
+  for (int j = 0; j < n && c->boxed; ++j) c->boxed = std::isfinite(p->cl_o[j]) && std::isfinite(p->cu_o[j]);
   const std::vector<int> ta = build_tiles(p->a_ptr, m), tt = build_tiles(p->at_ptr, n);
   c->A.ntiles = static_cast<int>(ta.size()) - 1;
   c->At.ntiles = static_cast<int>(tt.size()) - 1;

@@ -331,6 +331,7 @@ template <class T>
 struct CheckArgs {
   unsigned flags;
   int m, n;
+  int pray;         // form A·d for the primal rays (0 when every column is boxed: d is then 0)
   // primal side
   const T *xh, *xb, *x0;
   T* dx;            // F_RES: Δx = x̄ − x̂ is written here; otherwise the ray direction is read here
@@ -493,14 +494,17 @@ __global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a, const int
       term(a, T_RD2ROW)[i] = rd;
     }
   }
-  if (a.flags & F_RAY) {  // A·d against the finite row bounds (engine.cpp:140-145)
+  // A·d against the finite row bounds (engine.cpp:140-145). With every column boxed each entry
+  // of d is clamped to 0 from both sides (or is NaN, which the maximum |d| skips), so the ray's
+  // maximum is 0, the test is skipped (engine.cpp:136) and the product is not needed.
+  if ((a.flags & F_RAY) && a.pray) {
     tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv, false, s, sa);
     if (in) {
       if (isfinite(lo)) pviol = dmaxd(pviol, -s);
       if (isfinite(up)) pviol = dmaxd(pviol, s);
     }
   }
-  if (a.flags & F_DRIFT) {
+  if ((a.flags & F_DRIFT) && a.pray) {
     tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv2, false, s, sa);
     if (in) {
       if (isfinite(lo)) pviol2 = dmaxd(pviol2, -s);
@@ -618,6 +622,7 @@ CheckArgs<T> make_args(psh_ctx* c, unsigned flags, int xh, int xb, int x0, int d
   a.flags = flags;
   a.m = c->m;
   a.n = c->n;
+  a.pray = c->boxed ? 0 : 1;
   a.xh = sp<T>(c, xh);
   a.xb = sp<T>(c, xb);
   a.x0 = sp<T>(c, x0);
@@ -659,15 +664,23 @@ CheckArgs<T> make_args(psh_ctx* c, unsigned flags, int xh, int xb, int x0, int d
 
 // Launches the four check kernels for `flags`; the dual-side pass runs before the row pass
 // because the row pass reads Δy, ŷ (unscaled) and the dual ray.
+// `parts`: 1 = the elementwise passes and the row pass (everything the residual and PID sums
+// read), 2 = the column pass and the maxima, 3 = both.
 template <class T>
-int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool maxes) {
+int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool maxes, int parts = 3) {
   const unsigned nf = a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT | F_PID);
   const unsigned mf = nf;
-  if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
-  if (mf && c->m > 0) hipLaunchKernelGGL((k_check_m<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
-  if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->A.ntiles > 0)
-    hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, a, c->A.tiles,
-                       c->A.ntiles);
+  if (parts & 1) {
+    if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
+    if (mf && c->m > 0) hipLaunchKernelGGL((k_check_m<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
+    if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->A.ntiles > 0)
+      hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, a,
+                         c->A.tiles, c->A.ntiles);
+  }
+  if (!(parts & 2)) {
+    PSH_LAUNCHED(c, "check kernels");
+    return PSH_OK;
+  }
   if ((a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->At.ntiles > 0)
     hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream, a,
                        c->At.tiles, c->At.ntiles);
@@ -978,20 +991,40 @@ int psh_check(psh_ctx* c, const psh_check_args* g, psh_stats* out) {
   if (g->want_ray) flags |= F_RAY;
   if (g->want_drift) flags |= F_DRIFT;
   if (g->want_pid) flags |= F_PID;
-  rc = with_t(c, [&](auto* tag) -> int {
-    using T = std::remove_pointer_t<decltype(tag)>;
-    const CheckArgs<T> a = make_args<T>(c, flags, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
-    hipEvent_t ev;
-    time_begin(c, 2, &ev);
-    int r = launch_check<T>(c, a, true);
-    time_end(c, 2, ev);
-    return r;
-  });
-  if (rc) return rc;
   int err = PSH_OK;
   const JobTable* t = check_table(c, g->mode, flags, &err);
   if (!t) return err;
-  if ((rc = reduce_table(c, *t, 0))) return rc;
+  // In decision mode the 64-chunk sums (residual, PID distances) read only what the elementwise
+  // passes and the row pass write; they are latency-bound (one sequential chain per chunk), so
+  // they run on a second stream while the column pass and the tree sums run on the context's.
+  // Per-kernel timing keeps one stream.
+  const bool overlap = g->mode == PSH_RED_DECISION && !c->timing;
+  if (overlap && (rc = ensure_aux(c))) return rc;
+  rc = with_t(c, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    const CheckArgs<T> a = make_args<T>(c, flags, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
+    if (!overlap) {
+      hipEvent_t ev;
+      time_begin(c, 2, &ev);
+      int r = launch_check<T>(c, a, true);
+      time_end(c, 2, ev);
+      return r;
+    }
+    int r = launch_check<T>(c, a, true, 1);
+    if (r) return r;
+    PSH_TRY(c, hipEventRecord(c->fork, c->stream));
+    PSH_TRY(c, hipStreamWaitEvent(c->aux, c->fork, 0));
+    if ((r = reduce_table_part(c, *t, 0, 2, c->aux))) return r;
+    PSH_TRY(c, hipEventRecord(c->join, c->aux));
+    return launch_check<T>(c, a, true, 2);
+  });
+  if (rc) return rc;
+  if (overlap) {
+    if ((rc = reduce_table_part(c, *t, 0, 1, c->stream))) return rc;
+    PSH_TRY(c, hipStreamWaitEvent(c->stream, c->join, 0));
+  } else if ((rc = reduce_table(c, *t, 0))) {
+    return rc;
+  }
   double r[kResultCount];
   if ((rc = fetch_results(c, kResultCount, r))) return rc;
   out->px = r[R_PX];

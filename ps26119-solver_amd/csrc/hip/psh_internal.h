@@ -190,6 +190,9 @@ struct psh_ctx {
   double *mpart_m = nullptr, *mpart_n = nullptr; // per-workgroup maxima of the check passes
   int mpart_rows_m = 0, mpart_rows_n = 0;        // workgroup slots in each
   long long term_len = 0;
+  bool boxed = false;                           // every column has two finite bounds: the primal ray is 0
+  hipStream_t aux = nullptr;                    // second stream of the check pass (created on first use)
+  hipEvent_t fork = nullptr, join = nullptr;
   double* partials = nullptr;                   // tree first level
   double* results = nullptr;                    // device results
   double* host_results = nullptr;               // pinned
@@ -269,6 +272,11 @@ long long slot_len(const psh_ctx* c, int s);
 int reduce(psh_ctx* c, const RedJob* jobs, int njobs, int first);
 // Same, with a table already on the device (no upload, no synchronisation).
 int reduce_table(psh_ctx* c, const JobTable& t, int first);
+// Only the tree jobs (part 1) or only the 64-chunk / sequential sum jobs (part 2) of a table, on
+// stream `s`. The two parts write disjoint rows of `partials` and slots of `results`.
+int reduce_table_part(psh_ctx* c, const JobTable& t, int first, int part, hipStream_t s);
+// The check pass's second stream and its fork / join events (created on first use).
+int ensure_aux(psh_ctx* c);
 // The table for a check pass with `flags` in reduction `mode` (created on first use).
 const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err);
 // Copies `count` device results to the host (synchronises the stream).
