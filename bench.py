@@ -17,7 +17,9 @@ launch from the committed rocprofv3 counter summary. `cpu_baseline` times the re
 32 beside it). `reference_instances` solves the reference's own published benchmark instances
 (rand-1e6-s1 and refinery-T8760-s1, made by its generators restated in lpgen) with default options
 in fp64 and mixed, timed as the reference times them, beside its published RTX 4050 and 12-thread
-CPU times.
+CPU times. `serving` is the metric's other half: a short Poisson stream (`--line-requests`, m
+log-uniform in [1e3, 2e4]) served with continuous batching, solves/s and p50 / p99 latency beside
+the same requests solved one after another in plain fp64.
 
 Other workloads (each prints one JSON line of the same shape):
   refinery   planning LP (T periods of 49 rows × 60 columns), mixed precision in increment form
@@ -237,6 +239,9 @@ def bench_lp1m(args, ps26119, lpgen):
         line["equal_time_comparison"] = cmp
     if not profiling and not args.skip_reference_instances:
         line["reference_instances"] = reference_instances(ps26119, lpgen)
+    if not profiling and args.line_requests > 0:  # the metric's serving half: solves/s and p99 of a short stream
+        line["serving"] = serving_stream(ps26119, lpgen, args.line_requests, args.m_lo, args.m_hi, args.rate,
+                                         args.budget_gb, args.serving_precision, args.seed, 60.0)
     cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP",
                                                          single_thread_iterations=32)
     if cb:
@@ -529,19 +534,21 @@ def bench_scenarios(args, ps26119, lpgen):
 
 
 # ------------------------------------------------------------------ serving
-def bench_serving(args, ps26119, lpgen):
-    rng = np.random.default_rng(args.seed)
-    N = args.requests
-    ms = np.exp(rng.uniform(np.log(args.m_lo), np.log(args.m_hi), N)).astype(int)
+def serving_stream(ps26119, lpgen, N, m_lo, m_hi, rate, budget_gb, precision, seed, time_limit, eager=False,
+                   sequential=True):
+    """A Poisson stream of N random LPs (6 per row, n = 2m, m log-uniform in [m_lo, m_hi]; half at
+    tolerance 1e-6, half at 1e-8) served with continuous batching under an HBM byte budget, then
+    (sequential=True) the same requests as back-to-back plain fp64 solves. HIP events around
+    every served chunk give the device time of the served iterations (the roofline)."""
+    rng = np.random.default_rng(seed)
+    ms = np.exp(rng.uniform(np.log(m_lo), np.log(m_hi), N)).astype(int)
     tols = np.where(rng.random(N) < 0.5, 1e-6, 1e-8)
-    gaps = rng.exponential(1.0 / args.rate, N)
+    gaps = rng.exponential(1.0 / rate, N)
     arrivals = np.cumsum(gaps) - gaps[0]
     lps = [lpgen.random_lp(int(m), seed=1000 + k) for k, m in enumerate(ms)]
-    log(f"serving: {N} requests, m in [{ms.min()}, {ms.max()}], rate {args.rate}/s, budget {args.budget_gb} GB")
-    # HIP events around every chunk give the device time of the served iterations (the roofline
-    # below); --serving-eager launches the kernels directly instead of replaying graphs (profiling)
-    srv = ps26119.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_instances=256, setup_threads=8,
-                         eager=args.serving_eager, timing=True)
+    log(f"serving: {N} requests, m in [{ms.min()}, {ms.max()}], rate {rate}/s, budget {budget_gb} GB")
+    srv = ps26119.Server(budget_bytes=int(budget_gb * (1 << 30)), max_instances=256, setup_threads=8, eager=eager,
+                         timing=True)
     ids = [None] * N
 
     def feed():
@@ -550,8 +557,7 @@ def bench_serving(args, ps26119, lpgen):
             d = arrivals[k] - (time.time() - t0)
             if d > 0:
                 time.sleep(d)
-            ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=args.time_limit,
-                                precision=args.serving_precision)
+            ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=time_limit, precision=precision)
 
     t = time.time()
     th = threading.Thread(target=feed)
@@ -573,35 +579,39 @@ def bench_serving(args, ps26119, lpgen):
     lat = np.array([r.stats["latency"] for r in res])
     ok = sum(r.status == ps26119.OPTIMAL for r in res)
     log(f"served: {ok}/{N} optimal in {served:.2f}s, p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+    out = {"requests": N, "m_range": [int(ms.min()), int(ms.max())], "precision": precision,
+           "solves_per_s": N / served, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
+           "p99_latency_s": float(np.percentile(lat, 99)), "served_seconds": served, "roofline": roof}
+    if sequential:  # the same requests as plain fp64 solves, back to back
+        t = time.time()
+        for lp, tol in zip(lps, tols):
+            ps26119.solve_model(lp, tolerance=float(tol), time_limit=time_limit)
+        seq = time.time() - t
+        log(f"sequential: {seq:.2f}s")
+        out["sequential"] = {"seconds": seq, "solves_per_s": N / seq}
+        out["speedup_vs_sequential"] = seq / served
+    return out
+
+
+def bench_serving(args, ps26119, lpgen):
+    r = serving_stream(ps26119, lpgen, args.requests, args.m_lo, args.m_hi, args.rate, args.budget_gb,
+                       args.serving_precision, args.seed, args.time_limit, eager=args.serving_eager,
+                       sequential=not args.serving_only)
+    N = args.requests
+    dtype = "f32+f64" if args.serving_precision == "mixed" else "f64"
     if args.serving_only:
-        line = base_line(args, N / served, "solves/s", True, "f32+f64" if args.serving_precision == "mixed" else "f64",
+        line = base_line(args, r["solves_per_s"], "solves/s", True, dtype,
                          {"workload": f"serving: {N} requests, served part only", "requests": N})
-        line.update({"roofline": roof, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
-                     "p99_latency_s": float(np.percentile(lat, 99)), "cpu_baseline": None})
-        return line
-    # sequential plain fp64 solves of the same requests, back to back
-    t = time.time()
-    seq_lat = []
-    for lp, tol in zip(lps, tols):
-        r = ps26119.solve_model(lp, tolerance=float(tol), time_limit=args.time_limit)
-        seq_lat.append(r.seconds)
-    sequential = time.time() - t
-    log(f"sequential: {sequential:.2f}s")
-    value = N / served
-    mixed = args.serving_precision == "mixed"
-    line = base_line(args, value, "solves/s", True, "f32+f64" if mixed else "f64",
-                     {"workload": f"serving: {N} Poisson arrivals at {args.rate}/s of random LPs (6 per row, n=2m), m "
-                                  f"log-uniform in [{args.m_lo}, {args.m_hi}], half at tolerance 1e-6 and half at 1e-8, "
-                                  f"HBM budget {args.budget_gb} GB, continuous batching, "
-                                  + ("mixed precision (fp32 increments on fp64 anchors)" if mixed else "fp64")
-                                  + "; baseline: the same requests as sequential plain fp64 solves",
-                      "requests": N})
-    line.update({
-        "roofline": roof, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
-        "p99_latency_s": float(np.percentile(lat, 99)), "served_seconds": served,
-        "sequential": {"seconds": sequential, "solves_per_s": N / sequential},
-        "speedup_vs_sequential": sequential / served,
-    })
+    else:
+        line = base_line(args, r["solves_per_s"], "solves/s", True, dtype,
+                         {"workload": f"serving: {N} Poisson arrivals at {args.rate}/s of random LPs (6 per row, n=2m), m "
+                                      f"log-uniform in [{args.m_lo}, {args.m_hi}], half at tolerance 1e-6 and half at 1e-8, "
+                                      f"HBM budget {args.budget_gb} GB, continuous batching, "
+                                      + ("mixed precision (fp32 increments on fp64 anchors)"
+                                         if args.serving_precision == "mixed" else "fp64")
+                                      + "; baseline: the same requests as sequential plain fp64 solves",
+                          "requests": N})
+    line.update({k: v for k, v in r.items() if k not in ("requests", "solves_per_s")})
     line["cpu_baseline"] = None
     return line
 
@@ -641,6 +651,8 @@ def main():
     ap.add_argument("--scenario-iterations", type=int, default=16384, help="scenarios: iterations of the rate measurement")
     ap.add_argument("--scenario-rate-only", action="store_true", help="scenarios: only the batch rates (profiling runs)")
     ap.add_argument("--requests", type=int, default=48, help="serving")
+    ap.add_argument("--line-requests", type=int, default=24,
+                    help="lp1m: requests of the short serving stream reported in the line's `serving` (0 = skip)")
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
     ap.add_argument("--m-lo", type=int, default=1000, help="serving")
     ap.add_argument("--m-hi", type=int, default=20000,
