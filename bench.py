@@ -563,7 +563,11 @@ def serving_stream(ps26119, lpgen, N, m_lo, m_hi, rate, budget_gb, precision, se
     th = threading.Thread(target=feed)
     th.start()
     th.join()
-    res = [srv.wait(i) for i in ids]
+    res = []
+    for k, i in enumerate(ids):  # one line per finished request: progress for a long stream
+        res.append(srv.wait(i))
+        log(f"request {k}: m={ms[k]} tol={tols[k]:.0e} {res[-1].status_name} it={res[-1].iterations} "
+            f"latency {res[-1].stats['latency']:.3f}s")
     served = time.time() - t
     kt = srv.kernel_time()
     srv.close()
@@ -584,8 +588,9 @@ def serving_stream(ps26119, lpgen, N, m_lo, m_hi, rate, budget_gb, precision, se
            "p99_latency_s": float(np.percentile(lat, 99)), "served_seconds": served, "roofline": roof}
     if sequential:  # the same requests as plain fp64 solves, back to back
         t = time.time()
-        for lp, tol in zip(lps, tols):
-            ps26119.solve_model(lp, tolerance=float(tol), time_limit=time_limit)
+        for k, (lp, tol) in enumerate(zip(lps, tols)):
+            r = ps26119.solve_model(lp, tolerance=float(tol), time_limit=time_limit)
+            log(f"sequential {k}: {r.status_name} it={r.iterations} {r.seconds:.3f}s")
         seq = time.time() - t
         log(f"sequential: {seq:.2f}s")
         out["sequential"] = {"seconds": seq, "solves_per_s": N / seq}
