@@ -14,7 +14,7 @@ Per-kernel durations come from HIP events on the solve's stream in a second, sho
 reports the fused Ãᵀy + primal-update kernel against the 8 TB/s HBM peak, with the HBM bytes per
 launch from the committed rocprofv3 counter summary. `cpu_baseline` times the reference CPU solver
 (oracle/_ref, all host cores) for 256 iterations of the same model, in the same unit (one thread for
-32 beside it). `reference_instances` solves the reference's own published benchmark instances
+64 beside it). `reference_instances` solves the reference's own published benchmark instances
 (rand-1e6-s1 and refinery-T8760-s1, made by its generators restated in lpgen) with default options
 in fp64 and mixed, timed as the reference times them, beside its published RTX 4050 and 12-thread
 CPU times. `serving` is the metric's other half: a short Poisson stream (`--line-requests`, m
@@ -243,7 +243,7 @@ def bench_lp1m(args, ps26119, lpgen):
         line["serving"] = serving_stream(ps26119, lpgen, args.line_requests, args.m_lo, args.m_hi, args.rate,
                                          args.budget_gb, args.serving_precision, args.seed, 60.0)
     cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP",
-                                                         single_thread_iterations=32)
+                                                         single_thread_iterations=64)
     if cb:
         one = cb.get("single_thread")
         cpu_gbs = (pb + db) / (cb["ms_per_iteration"] * 1e-3) / 1e9
