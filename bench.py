@@ -8,7 +8,7 @@ one r²HPDHG iteration of the solver's loop (two fused launches, the check every
 restarts), `steps` is the number the solve ran; `value` is the HBM GB/s of the iteration loop
 (algorithmic bytes per iteration × iterations ÷ the loop's wall time, inputs resident in HBM).
 The reference's restart schedule alone stalls near a relative gap of 3e-7 on this instance, so
-epochs are capped at `--epoch-cap 8192` iterations while the gap leads (said in `config`);
+epochs are capped at `--epoch-cap 4096` iterations while the gap leads (said in `config`);
 `--compare-seconds S` also runs both schedules for S seconds and reports the gap each reaches.
 Per-kernel durations come from HIP events on the solve's stream in a second, short run; `roofline`
 reports the fused Ãᵀy + primal-update kernel against the 8 TB/s HBM peak, with the HBM bytes per
@@ -641,7 +641,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-reference-instances", action="store_true",
                     help="lp1m: leave out the reference's own published instances (rand-1e6, refinery-T8760)")
-    ap.add_argument("--epoch-cap", type=int, default=8192,
+    ap.add_argument("--epoch-cap", type=int, default=4096,
                     help="lp1m and the cost scenarios of scenarios: hip_max_epoch with hip_max_epoch_gap_only "
                          "(0 = the reference's restart schedule alone)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
