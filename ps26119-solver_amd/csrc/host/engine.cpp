@@ -783,6 +783,7 @@ Solution solve_r2hpdhg(const Model& model, const EngineOptions& opt) {
     }
     const bool promoted = policy.on_check(k.rel_kkt(), it);
     if (promoted) msg += std::string(msg.empty() ? "" : "; ") + "fp32 -> fp64 at iteration " + std::to_string(it);
+    if (promoted) gap_best = kInf;  // fp64 starts its own record of the gap's progress
     if (k.rel_gap() < 0.9 * gap_best) gap_best = k.rel_gap(), gap_best_it = it;
     const bool cap_ok = opt.max_epoch_gap_only != 2 || it - gap_best_it >= opt.max_epoch;
     const bool restart = promoted || restart_due(opt, r, r0, r_last, inner, it) ||

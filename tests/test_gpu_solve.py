@@ -315,6 +315,24 @@ def test_epoch_cap_reaches_the_planted_optimum(gpu_available, golden, prec):
     assert r.iterations < exp["iterations"]
 
 
+def test_epoch_cap_waiting_for_a_gap_stall(gpu_available, golden):
+    """hip_max_epoch_gap_only = 2 forces a restart only once the relative gap has gained less than
+    10% for hip_max_epoch iterations. With a cap longer than the whole solve it never fires and
+    the trajectory is the reference's (iteration count of the fixture); with 4,096 it does fire
+    (another iteration count) and the solve still ends at the planted optimum."""
+    import ps26119
+    lp = _model("random10000_s1")
+    exp = golden["solves"]["random10000_s1|presolve=1|fp64"]
+    never = ps26119.solve_model(lp, params={"hip_max_epoch": 1 << 40, "hip_max_epoch_gap_only": 2})
+    assert never.status == ps26119.OPTIMAL and never.iterations == exp["iterations"], never.iterations
+    r = ps26119.solve_model(lp, params={"hip_max_epoch": 4096, "hip_max_epoch_gap_only": 2})
+    print(f"random 10k, cap 4096 after a gap stall: it={r.iterations} (reference schedule {exp['iterations']})",
+          flush=True)
+    assert r.status == ps26119.OPTIMAL and r.check == "PASS", r.message
+    assert r.iterations != exp["iterations"]
+    assert abs(r.objective - lp.optimum) <= 1e-9 * (1 + abs(lp.optimum))
+
+
 PDLP_CASES = ["afiro", "sc50a", "sc105", "stocfor1", "wyndor", "random300_s3"]
 
 
