@@ -143,10 +143,18 @@ def roofline_of(st, pb, name):
             "traffic": None, "kernel": name, "bytes_per_launch": pb, "avg_launch_us": avg_us[0]}, avg_us
 
 
-def epoch_knobs(args):
-    """Engine knobs of --epoch-cap: end an epoch at that length while the relative gap is the
-    largest KKT term (not in the reference; {} = its restart schedule alone)."""
-    return {"hip_max_epoch": args.epoch_cap, "hip_max_epoch_gap_only": 1} if args.epoch_cap > 0 else {}
+def epoch_knobs(args, cap=None):
+    """Engine knobs of --epoch-cap (or `cap`): end an epoch at that length while the relative gap
+    is the largest KKT term (not in the reference; {} = its restart schedule alone)."""
+    n = args.epoch_cap if cap is None else cap
+    return {"hip_max_epoch": n, "hip_max_epoch_gap_only": 1} if n > 0 else {}
+
+
+def serving_knobs(args):
+    """Engine knobs of the served requests: --serving-epoch-cap N caps their epochs at N iterations
+    while the gap leads ({} = the reference's restart schedule)."""
+    n = args.serving_epoch_cap
+    return {"hip_max_epoch": n, "hip_max_epoch_gap_only": 1} if n > 0 else {}
 
 
 # ------------------------------------------------------------------ lp1m
@@ -241,7 +249,8 @@ def bench_lp1m(args, ps26119, lpgen):
         line["reference_instances"] = reference_instances(ps26119, lpgen)
     if not profiling and args.line_requests > 0:  # the metric's serving half: solves/s and p99 of a short stream
         line["serving"] = serving_stream(ps26119, lpgen, args.line_requests, args.m_lo, args.m_hi, args.rate,
-                                         args.budget_gb, args.serving_precision, args.seed, 60.0)
+                                         args.budget_gb, args.serving_precision, args.seed, 60.0,
+                                         params=serving_knobs(args))
     cb = None if args.no_cpu_baseline else cpu_baseline(lp, res.iterations, res.seconds, "1M-row LP",
                                                          single_thread_iterations=64)
     if cb:
@@ -413,7 +422,7 @@ def bench_scenarios(args, ps26119, lpgen):
         single solves to 1e-6 (--seq-sample timed, scaled to 64), with the verdicts of each
         (Optimal, or Infeasible with a checked dual-ray certificate);
     (c) at 10,000 × 20,000 the 64 cost scenarios warm-started to 1e-6 and to 1e-8, fp64 batch
-        against mixed batch (capped at 6M iterations or 150 s), both with the --epoch-cap knobs."""
+        against mixed batch (capped at 6M iterations or 150 s), both with the --scenario-epoch-cap knobs."""
     m = args.m if args.m != 1_000_000 else 100_000
     base = lpgen.random_lp(m, seed=args.seed)
     scen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)]
@@ -490,7 +499,7 @@ def bench_scenarios(args, ps26119, lpgen):
             t = time.time()
             res = ps26119.solve_model_batch(small, cscen, tolerance=tol, warm_x=b.x, warm_y=b.y, verbosity=1,
                                             precision=prec, iteration_limit=6_000_000, time_limit=150.0,
-                                            params=epoch_knobs(args))
+                                            params=epoch_knobs(args, args.scenario_epoch_cap))
             cost[f"{prec}_{tol:.0e}"] = {"seconds": time.time() - t, "verdicts": verdicts(res),
                                          "iterations_max": max(r.iterations for r in res)}
             log(f"10k cost scenarios to {tol:.0e}, {prec} batch: {cost[f'{prec}_{tol:.0e}']}")
@@ -527,7 +536,7 @@ def bench_scenarios(args, ps26119, lpgen):
                        "speedup_fp64_batch_1e-06": seq_s / f6[0] if f6[0] > 0 else None},
         "cost_scenarios_10k": None if args.skip_cost else {"perturbation": "c scaled by lognormal(0, 0.05) factors", "tolerances": [1e-6, 1e-8],
                                "warm_start": "base solution", "caps": "6M iterations, 150 s",
-                               "epoch_cap": args.epoch_cap, **cost},
+                               "epoch_cap": args.scenario_epoch_cap, **cost},
     })
     line["cpu_baseline"] = None
     return line
@@ -535,11 +544,12 @@ def bench_scenarios(args, ps26119, lpgen):
 
 # ------------------------------------------------------------------ serving
 def serving_stream(ps26119, lpgen, N, m_lo, m_hi, rate, budget_gb, precision, seed, time_limit, eager=False,
-                   sequential=True):
+                   sequential=True, params=None):
     """A Poisson stream of N random LPs (6 per row, n = 2m, m log-uniform in [m_lo, m_hi]; half at
     tolerance 1e-6, half at 1e-8) served with continuous batching under an HBM byte budget, then
-    (sequential=True) the same requests as back-to-back plain fp64 solves. HIP events around
-    every served chunk give the device time of the served iterations (the roofline)."""
+    (sequential=True) the same requests as back-to-back plain fp64 solves, with the same engine
+    knobs (`params`). HIP events around every served chunk give the device time of the served
+    iterations (the roofline)."""
     rng = np.random.default_rng(seed)
     ms = np.exp(rng.uniform(np.log(m_lo), np.log(m_hi), N)).astype(int)
     tols = np.where(rng.random(N) < 0.5, 1e-6, 1e-8)
@@ -557,7 +567,8 @@ def serving_stream(ps26119, lpgen, N, m_lo, m_hi, rate, budget_gb, precision, se
             d = arrivals[k] - (time.time() - t0)
             if d > 0:
                 time.sleep(d)
-            ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=time_limit, precision=precision)
+            ids[k] = srv.submit(lps[k], tolerance=float(tols[k]), time_limit=time_limit, precision=precision,
+                                params=params)
 
     t = time.time()
     th = threading.Thread(target=feed)
@@ -583,13 +594,13 @@ def serving_stream(ps26119, lpgen, N, m_lo, m_hi, rate, budget_gb, precision, se
     lat = np.array([r.stats["latency"] for r in res])
     ok = sum(r.status == ps26119.OPTIMAL for r in res)
     log(f"served: {ok}/{N} optimal in {served:.2f}s, p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
-    out = {"requests": N, "m_range": [int(ms.min()), int(ms.max())], "precision": precision,
+    out = {"requests": N, "m_range": [int(ms.min()), int(ms.max())], "precision": precision, "engine_knobs": params or {},
            "solves_per_s": N / served, "optimal": ok, "p50_latency_s": float(np.percentile(lat, 50)),
            "p99_latency_s": float(np.percentile(lat, 99)), "served_seconds": served, "roofline": roof}
     if sequential:  # the same requests as plain fp64 solves, back to back
         t = time.time()
         for k, (lp, tol) in enumerate(zip(lps, tols)):
-            r = ps26119.solve_model(lp, tolerance=float(tol), time_limit=time_limit)
+            r = ps26119.solve_model(lp, tolerance=float(tol), time_limit=time_limit, params=params)
             log(f"sequential {k}: {r.status_name} it={r.iterations} {r.seconds:.3f}s")
         seq = time.time() - t
         log(f"sequential: {seq:.2f}s")
@@ -601,7 +612,7 @@ def serving_stream(ps26119, lpgen, N, m_lo, m_hi, rate, budget_gb, precision, se
 def bench_serving(args, ps26119, lpgen):
     r = serving_stream(ps26119, lpgen, args.requests, args.m_lo, args.m_hi, args.rate, args.budget_gb,
                        args.serving_precision, args.seed, args.time_limit, eager=args.serving_eager,
-                       sequential=not args.serving_only)
+                       sequential=not args.serving_only, params=serving_knobs(args))
     N = args.requests
     dtype = "f32+f64" if args.serving_precision == "mixed" else "f64"
     if args.serving_only:
@@ -642,8 +653,9 @@ def main():
     ap.add_argument("--skip-reference-instances", action="store_true",
                     help="lp1m: leave out the reference's own published instances (rand-1e6, refinery-T8760)")
     ap.add_argument("--epoch-cap", type=int, default=4096,
-                    help="lp1m and the cost scenarios of scenarios: hip_max_epoch with hip_max_epoch_gap_only "
-                         "(0 = the reference's restart schedule alone)")
+                    help="lp1m: hip_max_epoch with hip_max_epoch_gap_only (0 = the reference's restart schedule alone)")
+    ap.add_argument("--scenario-epoch-cap", type=int, default=8192,
+                    help="scenarios: the same for the cost-scenario batches (0 = the reference's schedule)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="lp1m: mixed = fp32 increments on fp64 anchors (the line's dtype says which)")
     ap.add_argument("--T", type=int, default=8760, help="refinery: planning periods of the rate measurement")
@@ -665,6 +677,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=16.0, help="serving: HBM byte budget of the server")
     ap.add_argument("--serving-precision", default="mixed", choices=["fp64", "mixed"], help="serving: precision of the served requests")
     ap.add_argument("--serving-only", action="store_true", help="serving: skip the sequential comparison (profiling runs)")
+    ap.add_argument("--serving-epoch-cap", type=int, default=0,
+                    help="serving: hip_max_epoch with hip_max_epoch_gap_only for the served requests (0 = off)")
     ap.add_argument("--serving-eager", action="store_true",
                     help="serving: launch the kernels directly and time every primal launch (the line's roofline)")
     args = ap.parse_args()
