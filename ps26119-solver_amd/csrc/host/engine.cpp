@@ -95,16 +95,12 @@ KktStats kkt_on_original(const ScaledProblem& sp, const std::vector<double>& x, 
   const Model& M = *sp.original;
   const int m = sp.m, n = sp.n;
   const double sense = M.sense;
-  std::vector<double> ax(m), aty(n);
-  Csr A;  // the original A in Ã's layout
-  A.rows = m;
-  A.cols = n;
-  A.ptr = sp.A.ptr;
-  A.col = sp.A.col;
-  A.val = sp.a_orig;
-  multiply(A, x.data(), ax.data());
-  const Csr At = csc_as_transpose(M);
-  multiply(At, y.data(), aty.data());
+  // A·x by the model's columns: every row's products are added in increasing column order, the
+  // order of a row-major product over Ã's layout (same bits, no copy of the matrix); Aᵀy by
+  // columns in storage order
+  const std::vector<double> ax = M.row_activity(x);
+  std::vector<double> aty(n);
+  column_products(M, y.data(), aty.data());
   double rp2 = 0, rd2 = 0, b2 = 0, c2 = 0, p = 0, d = 0, cinf = 0, pmax = 0, dmax = 0;
   auto elem_viol = [](double v, double lo, double up) {
     if (v < lo) return (lo - v) / (1.0 + std::fabs(lo));

@@ -127,6 +127,16 @@ Csr transpose(const Csr& a) {
   return t;
 }
 
+void column_products(const Model& m, const double* y, double* out) {
+  parallel_for(m.num_cols, [&](std::int64_t b, std::int64_t e) {
+    for (std::int64_t j = b; j < e; ++j) {
+      double s = 0;
+      for (int k = m.col_start[j]; k < m.col_start[j + 1]; ++k) s += m.value[k] * y[m.row_index[k]];
+      out[j] = s;
+    }
+  });
+}
+
 void multiply(const Csr& a, const double* x, double* out) {
   parallel_for(a.rows, [&](std::int64_t b, std::int64_t e) {
     for (std::int64_t i = b; i < e; ++i) {

@@ -40,5 +40,8 @@ Csr csc_as_transpose(const Model& m);
 Csr transpose(const Csr& a);
 // out = a·x, each row summed sequentially in CSR order with an fp64 accumulator.
 void multiply(const Csr& a, const double* x, double* out);
+// out = Aᵀy over the model's CSC arrays: each column summed sequentially in storage order (the
+// bits of multiply(csc_as_transpose(m), y, out), without copying the matrix).
+void column_products(const Model& m, const double* y, double* out);
 
 }  // namespace ps26119::amd

@@ -43,12 +43,10 @@ bool may_use_implied_bounds(double elapsed, double time_limit) { return elapsed 
 
 // Relative KKT error of a point on the original model (the reduced solve's yardstick too).
 KktStats kkt_of_point(const Model& M, const Solution& s) {
-  ScaledProblem sp;  // kkt_on_original reads only the original matrix and model
+  ScaledProblem sp;  // kkt_on_original reads only the original model
   sp.m = M.num_rows;
   sp.n = M.num_cols;
   sp.original = &M;
-  sp.A = transpose(csc_as_transpose(M));
-  sp.a_orig = sp.A.val;
   std::vector<double> y_min(s.y);
   for (double& v : y_min) v *= M.sense;
   return kkt_on_original(sp, s.x, y_min);
