@@ -40,8 +40,10 @@ Other workloads (each prints one JSON line of the same shape):
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lp1m|refinery|scenarios|serving|copy|stream]
                     [--m M] [--precision fp64|mixed] [--time-limit S] [--no-cpu-baseline]
-                    [--epoch-cap N] [--compare-seconds S]
+                    [--epoch-cap N] [--compare-seconds S] [--line-requests N]
+                    [--scenario-epoch-cap N] [--serving-epoch-cap N] [--m-hi M] [--requests N]
                     [--iteration-limit N]   (profiling: same per-launch work, N iterations)
+    (python bench.py --help lists every option)
 """
 import argparse
 import json
