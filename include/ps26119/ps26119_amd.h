@@ -30,6 +30,9 @@ extern "C" {
  * iterations, single solves only, experimental), "hip_bf16_until" (> 0: the increment form starts with bf16 matrix values
  * and moves to fp32 values at this relative KKT error or on a stall; default 0 = off),
  * "hip_device_scaling" (1 = default: the scaling sweeps run on the device, same bits as the host).
+ * "warm_primal_weight" is not an engine knob but the reference's Options::warm_primal_weight
+ * (options.h there), which the C struct lacks: the first primal weight of a warm-started solve or
+ * batch (0 = the default rule; a re-solve passes the earlier solve's stats->primal_weight).
  * presolve: 1 (default) / 0. */
 typedef struct {
   int presolve;

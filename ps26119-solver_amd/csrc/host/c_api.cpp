@@ -106,6 +106,23 @@ const char* make_model(int m, int n, int sense, double offset, const double* c, 
   return nullptr;
 }
 
+// The extension's named parameters into Options: "warm_primal_weight" is the reference's
+// Options::warm_primal_weight (the first primal weight of a warm-started solve or batch; the C
+// struct has no field for it), every other name goes to the engine, which checks it.
+const char* take_params(const ps26119_amd_options* ext, Options& o) {
+  for (int k = 0; k < ext->n_params; ++k) {
+    if (!ext->param_names || !ext->param_values || !ext->param_names[k]) return "engine parameter list is NULL";
+    const double v = ext->param_values[k];
+    if (std::strcmp(ext->param_names[k], "warm_primal_weight") == 0) {
+      if (!(v >= 0 && std::isfinite(v))) return "warm_primal_weight must be finite and non-negative";
+      o.warm_primal_weight = v;
+    } else {
+      o.engine_params.emplace_back(ext->param_names[k], v);
+    }
+  }
+  return nullptr;
+}
+
 template <class F>
 int guarded(ps26119_result* result, F&& f) {
   try {
@@ -145,10 +162,7 @@ int solve_common(int num_rows, int num_cols, int sense, double obj_offset, const
     if (ext) {
       o.presolve = ext->presolve != 0;
       if (ext->check_every > 0) o.termination_check_every = ext->check_every;
-      for (int k = 0; k < ext->n_params; ++k) {
-        if (!ext->param_names || !ext->param_values || !ext->param_names[k]) return invalid("engine parameter list is NULL");
-        o.engine_params.emplace_back(ext->param_names[k], ext->param_values[k]);
-      }
+      if (const char* bad = take_params(ext, o)) return invalid(bad);
     }
     const Solution s = amd::solve(m, o);
     fill(s, result);
@@ -268,10 +282,7 @@ int ps26119_amd_solve_batch(int num_rows, int num_cols, int sense, double obj_of
     if (const char* bad = to_options(opt, num_rows, num_cols, o)) return invalid_all(bad);
     if (ext) {
       if (ext->check_every > 0) o.termination_check_every = ext->check_every;
-      for (int k = 0; k < ext->n_params; ++k) {
-        if (!ext->param_names || !ext->param_values || !ext->param_names[k]) return invalid_all("engine parameter list is NULL");
-        o.engine_params.emplace_back(ext->param_names[k], ext->param_values[k]);
-      }
+      if (const char* bad = take_params(ext, o)) return invalid_all(bad);
     }
     std::vector<Scenario> S(num_scenarios);
     for (int k = 0; k < num_scenarios; ++k) {
@@ -354,10 +365,7 @@ long long ps26119_amd_server_submit(ps26119_amd_server* s, int num_rows, int num
     o.use_gpu = true;
     if (ext) {
       if (ext->check_every > 0) o.termination_check_every = ext->check_every;
-      for (int k = 0; k < ext->n_params; ++k) {
-        if (!ext->param_names || !ext->param_values || !ext->param_names[k]) return -1;
-        o.engine_params.emplace_back(ext->param_names[k], ext->param_values[k]);
-      }
+      if (take_params(ext, o)) return -1;
     }
     const long long id = s->srv->submit(std::move(m), o);
     std::lock_guard<std::mutex> g(s->mu);

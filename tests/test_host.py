@@ -233,6 +233,19 @@ def test_invalid_arguments_never_crash():
         assert len(res.message) > 0
 
 
+def test_warm_primal_weight_is_checked():
+    """The named parameter "warm_primal_weight" sets the reference's Options::warm_primal_weight
+    (the C struct has no field for it); a negative or non-finite value is an invalid argument,
+    in the single solve and in the batch."""
+    import ps26119
+    lp = hand_models()["wyndor"]
+    for bad in (-1.0, float("nan"), float("inf")):
+        r = ps26119.solve_model(lp, params={"warm_primal_weight": bad})
+        assert r.status == ps26119.INVALID_ARGUMENT and "warm_primal_weight" in r.message, (bad, r.message)
+        res = ps26119.solve_model_batch(lp, [{}], params={"warm_primal_weight": bad})
+        assert res[0].status == ps26119.INVALID_ARGUMENT and "warm_primal_weight" in res[0].message
+
+
 def test_cpu_requests_and_other_engines_are_not_solved():
     import ps26119
     lp = hand_models()["wyndor"]
