@@ -135,16 +135,17 @@ def test_warm_start_from_base_solution(gpu_available):
 
 def test_warm_primal_weight_is_the_first_weight(gpu_available):
     """The named parameter "warm_primal_weight" (the reference's Options::warm_primal_weight)
-    is the first primal weight: a solve stopped at its first check, before any restart, reports
-    it unchanged; a batch warm-started from the base solution and its final weight reaches the
-    same answers as a cold batch."""
+    is the first primal weight (reference engine.cpp:208 takes it whenever it is set): a solve
+    stopped at its first check, before any restart, reports it unchanged; a batch warm-started
+    from the base solution and its final weight reaches the same answers as a cold batch."""
     import ps26119
     from ps26119_amd import lpgen
     base = lpgen.random_lp(300, seed=3)
     b = ps26119.solve_model(base, stats=True)
-    r = ps26119.solve_model(base, warm_x=b.x, warm_y=b.y, params={"warm_primal_weight": 1e3}, iteration_limit=64,
-                            stats=True)
-    assert r.status == ps26119.ITERATION_LIMIT and r.stats["primal_weight"] == 1e3, r.stats["primal_weight"]
+    r = ps26119.solve_model(base, params={"warm_primal_weight": 1e3}, iteration_limit=64, stats=True)
+    assert r.status == ps26119.ITERATION_LIMIT and r.stats["primal_weight"] == 1e3, (r.status_name, r.stats)
+    d = ps26119.solve_model(base, iteration_limit=64, stats=True)
+    assert d.stats["primal_weight"] != 1e3
     scen = lpgen.scenario_set(base, 8, sigma=0.01)
     cold = ps26119.solve_model_batch(base, scen)
     warm = ps26119.solve_model_batch(base, scen, warm_x=b.x, warm_y=b.y,
