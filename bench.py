@@ -423,9 +423,9 @@ def bench_scenarios(args, ps26119, lpgen):
         1e-8: fp64 batch and mixed batch (each capped at 3M iterations or 150 s), and sequential
         single solves to 1e-6 (--seq-sample timed, scaled to 64), with the verdicts of each
         (Optimal, or Infeasible with a checked dual-ray certificate);
-    (c) at 10,000 × 20,000 the 64 cost scenarios warm-started (x, y and the base solution's primal
-        weight) to 1e-6 and to 1e-8, fp64 batch against mixed batch (capped at 6M iterations or
-        150 s), both with the --scenario-epoch-cap knobs."""
+    (c) at 10,000 × 20,000 the 64 cost scenarios warm-started to 1e-6 and to 1e-8, fp64 batch
+        against mixed batch (capped at 6M iterations or 150 s), both with the --scenario-epoch-cap
+        knobs."""
     m = args.m if args.m != 1_000_000 else 100_000
     base = lpgen.random_lp(m, seed=args.seed)
     scen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(base, args.scenarios, sigma=0.05, seed=7)]
@@ -495,16 +495,14 @@ def bench_scenarios(args, ps26119, lpgen):
     # (c) feasible scenarios at 10k rows (costs scaled only), fp64 batch vs mixed batch to 1e-6
     cost = {}
     if not args.skip_cost:
-        b = ps26119.solve_model(small, tolerance=1e-6, stats=True)
+        b = ps26119.solve_model(small, tolerance=1e-6)
         cscen = [{"obj": sc["obj"]} for sc in lpgen.scenario_set(small, args.scenarios, sigma=0.05, seed=7)]
-        # the warm start is the base's whole state: x, y and its last primal weight
-        warm_w = {"warm_primal_weight": b.stats["primal_weight"]}
     for tol in ((1e-6, 1e-8) if not args.skip_cost else ()):
         for prec in ("fp64", "mixed"):
             t = time.time()
             res = ps26119.solve_model_batch(small, cscen, tolerance=tol, warm_x=b.x, warm_y=b.y, verbosity=1,
                                             precision=prec, iteration_limit=6_000_000, time_limit=150.0,
-                                            params={**epoch_knobs(args, args.scenario_epoch_cap), **warm_w})
+                                            params=epoch_knobs(args, args.scenario_epoch_cap))
             cost[f"{prec}_{tol:.0e}"] = {"seconds": time.time() - t, "verdicts": verdicts(res),
                                          "iterations_max": max(r.iterations for r in res)}
             log(f"10k cost scenarios to {tol:.0e}, {prec} batch: {cost[f'{prec}_{tol:.0e}']}")
@@ -540,7 +538,7 @@ def bench_scenarios(args, ps26119, lpgen):
                        "sequential_sample_verdicts": verdicts(seq),
                        "speedup_fp64_batch_1e-06": seq_s / f6[0] if f6[0] > 0 else None},
         "cost_scenarios_10k": None if args.skip_cost else {"perturbation": "c scaled by lognormal(0, 0.05) factors", "tolerances": [1e-6, 1e-8],
-                               "warm_start": "base solution and its primal weight", "caps": "6M iterations, 150 s",
+                               "warm_start": "base solution", "caps": "6M iterations, 150 s",
                                "epoch_cap": args.scenario_epoch_cap, **cost},
     })
     line["cpu_baseline"] = None
