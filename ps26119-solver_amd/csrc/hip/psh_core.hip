@@ -151,23 +151,7 @@ __global__ void __launch_bounds__(64) k_ordered(const RedJob* __restrict__ jobs,
   const RedJob J = jobs[blockIdx.y];
   if (J.op != 0 || J.mode == 0) return;
   __shared__ double stage[kSeqBatch];
-  double* out = partials + static_cast<size_t>(blockIdx.y) * kTreeBlocks + blockIdx.x;
-  auto ga = [&](long long i) { return J.a[i]; };
-  auto gb = [&](long long i) { return J.b[i]; };
-  if (J.mode == 2) {
-    if (blockIdx.x != 0) return;
-    double s = wave_seq_sum(ga, 0, J.na, 0.0, stage);
-    s = wave_seq_sum(gb, 0, J.nb, s, stage);
-    if (threadIdx.x == 0) *out = s;
-    return;
-  }
-  const long long n = J.na;
-  const int chunks = static_cast<int>(n < 1 ? 1 : (n > 64 ? 64 : n));
-  const int c = static_cast<int>(blockIdx.x);
-  if (c >= chunks) return;
-  const long long b = n * c / chunks, e = n * (c + 1) / chunks;
-  const double s = wave_seq_sum(ga, b, e, 0.0, stage);
-  if (threadIdx.x == 0) *out = s;
+  ordered_chunk<false>(J, partials + static_cast<size_t>(blockIdx.y) * kTreeBlocks, static_cast<int>(blockIdx.x), stage);
 }
 
 __global__ void __launch_bounds__(64) k_ordered_final(const RedJob* __restrict__ jobs,
@@ -187,51 +171,48 @@ __global__ void __launch_bounds__(64) k_ordered_final(const RedJob* __restrict__
   results[first + blockIdx.x] = s;
 }
 
-// parts: 1 = tree jobs, 2 = ordered jobs, 3 = both, on stream s
-static int launch_reduce(psh_ctx* c, const RedJob* dev, const RedJob* host, int njobs, int first, int parts = 3,
-                         hipStream_t s = nullptr) {
-  if (!s) s = c->stream;
+// parts: 1 = tree jobs, 2 = ordered jobs (chunk sums and their combine), 4 = only the combine
+static int launch_reduce(psh_ctx* c, const RedJob* dev, const RedJob* host, int njobs, int first, int parts = 3) {
   bool tree = false, ordered = false;
   for (int k = 0; k < njobs; ++k) {
     if (host[k].op == 1 || host[k].mode == 0) tree = true;
     else ordered = true;
   }
   if (tree && (parts & 1)) {
-    hipLaunchKernelGGL(k_tree1, dim3(kTreeBlocks, njobs), dim3(kBlock), 0, s, dev, c->partials);
-    hipLaunchKernelGGL(k_tree2, dim3(njobs), dim3(kBlock), 0, s, dev, c->partials, c->results, first);
+    hipLaunchKernelGGL(k_tree1, dim3(kTreeBlocks, njobs), dim3(kBlock), 0, c->stream, dev, c->partials);
+    hipLaunchKernelGGL(k_tree2, dim3(njobs), dim3(kBlock), 0, c->stream, dev, c->partials, c->results, first);
   }
   // ordered jobs use their own rows of `partials` (a job is either a tree job or an ordered one)
-  if (ordered && (parts & 2)) {
-    hipLaunchKernelGGL(k_ordered, dim3(64, njobs), dim3(64), 0, s, dev, c->partials);
-    hipLaunchKernelGGL(k_ordered_final, dim3(njobs), dim3(64), 0, s, dev, c->partials, c->results, first);
-  }
+  if (ordered && (parts & 2))
+    hipLaunchKernelGGL(k_ordered, dim3(64, njobs), dim3(64), 0, c->stream, dev, c->partials);
+  if (ordered && (parts & 6))
+    hipLaunchKernelGGL(k_ordered_final, dim3(njobs), dim3(64), 0, c->stream, dev, c->partials, c->results, first);
   PSH_LAUNCHED(c, "reduction kernels");
   return PSH_OK;
 }
 
-int reduce_table_part(psh_ctx* c, const JobTable& t, int first, int part, hipStream_t s) {
+int reduce_table_part(psh_ctx* c, const JobTable& t, int first, int parts) {
   if (t.host.empty()) return PSH_OK;
-  return launch_reduce(c, t.dev, t.host.data(), static_cast<int>(t.host.size()), first, part, s);
+  return launch_reduce(c, t.dev, t.host.data(), static_cast<int>(t.host.size()), first, parts);
 }
 
-int ensure_aux(psh_ctx* c) {
-  if (c->aux) return PSH_OK;
-  PSH_TRY(c, hipSetDevice(c->device));
-  std::lock_guard<std::recursive_mutex> lock(api_mutex());
-  hipStream_t s = nullptr;
-  hipEvent_t f = nullptr, j = nullptr;
-  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
-    (void)hipGetLastError();
-    if (f) (void)hipEventDestroy(f);
-    if (s) (void)hipStreamDestroy(s);
-    return fail(c, PSH_E_HIP, "check pass: second stream");
+ChunkSums chunk_sums_of(const psh_ctx* c, const JobTable& t) {
+  ChunkSums cs;
+  cs.jobs = t.dev;
+  cs.partials = c->partials;
+  int count = 0;
+  for (std::size_t k = 0; k < t.host.size(); ++k) {
+    const RedJob& J = t.host[k];
+    if (J.op != 0 || J.mode == 0) continue;
+    if (J.mode != 1 || k >= 32) {
+      cs.blocks = -1;
+      return cs;
+    }
+    cs.mask |= 1u << k;
+    ++count;
   }
-  c->aux = s;
-  c->fork = f;
-  c->join = j;
-  return PSH_OK;
+  cs.blocks = count * 64 / (kBlock / 64);
+  return cs;
 }
 
 int reduce(psh_ctx* c, const RedJob* jobs, int njobs, int first) {
@@ -732,7 +713,6 @@ void psh_destroy(psh_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->aux) (void)hipStreamSynchronize(c->aux);
   psh::destroy_batch(c);
   for (auto& kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
   {
@@ -756,12 +736,9 @@ void psh_destroy(psh_ctx* c) {
   }
   for (auto& p : c->pending) c->ev_pool.push_back(p.a), c->ev_pool.push_back(p.b);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-  if (c->fork) (void)hipEventDestroy(c->fork);
-  if (c->join) (void)hipEventDestroy(c->join);
-  if (c->stream || c->aux) {
+  if (c->stream) {
     std::lock_guard<std::recursive_mutex> lock(api_mutex());
-    if (c->aux) (void)hipStreamDestroy(c->aux);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
   }
   delete c;
 }

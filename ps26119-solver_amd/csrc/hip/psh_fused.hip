@@ -302,9 +302,10 @@ __device__ inline double wave_max(double v) {
 }
 
 // Maximum of non-negative values over the workgroup (NaN ignored, as std::max(acc, NaN) does),
-// stored as this workgroup's partial for quantity q; k_max_final combines the partials. Every
-// thread of the workgroup must reach this call (no atomics: one store per workgroup).
-__device__ inline void block_max_into(double* part, int q, double v) {
+// stored as partial row `row` (the workgroup's index among the pass's workgroups) for quantity
+// q; k_max_final combines the partials. Every thread of the workgroup must reach this call (no
+// atomics: one store per workgroup).
+__device__ inline void block_max_into(double* part, int q, double v, int row) {
   __shared__ double wm[kBlock / 64];
   v = v >= 0.0 ? v : 0.0;
   v = wave_max(v);
@@ -313,7 +314,7 @@ __device__ inline void block_max_into(double* part, int q, double v) {
   if (threadIdx.x == 0) {
     double mx = wm[0];
     for (int k = 1; k < kBlock / 64; ++k) mx = fmax(mx, wm[k]);
-    part[static_cast<size_t>(blockIdx.x) * kMaxCount + q] = mx;
+    part[static_cast<size_t>(row) * kMaxCount + q] = mx;
   }
   __syncthreads();
 }
@@ -409,8 +410,8 @@ __global__ void __launch_bounds__(kBlock) k_check_n(CheckArgs<T> a) {
       term(a, T_PIDX)[j] = t * t;
     }
   }
-  if (a.flags & F_RAY) block_max_into(a.mpart_n, M_DMAXRAY, dmax_ray);
-  if (a.flags & F_DRIFT) block_max_into(a.mpart_n, M_DMAXRAY2, dmax_ray2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_n, M_DMAXRAY, dmax_ray, blockIdx.x);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_n, M_DMAXRAY2, dmax_ray2, blockIdx.x);
 }
 
 template <class T>
@@ -451,8 +452,8 @@ __global__ void __launch_bounds__(kBlock) k_check_m(CheckArgs<T> a) {
       term(a, T_PIDY)[i] = t * t;
     }
   }
-  if (a.flags & F_RAY) block_max_into(a.mpart_m, M_RMAX, rmax);
-  if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_RMAX2, rmax2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_m, M_RMAX, rmax, blockIdx.x);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_RMAX2, rmax2, blockIdx.x);
 }
 
 // Tiles of A (Ã and the original A share the structure): residual, KKT row part and ray tests.
@@ -512,11 +513,11 @@ __global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a, const int
     }
   }
   if (a.flags & F_KKT) {
-    block_max_into(a.mpart_m, M_PMAX, pmax);
-    block_max_into(a.mpart_m, M_DMAX, dmax);
+    block_max_into(a.mpart_m, M_PMAX, pmax, blockIdx.x);
+    block_max_into(a.mpart_m, M_DMAX, dmax, blockIdx.x);
   }
-  if (a.flags & F_RAY) block_max_into(a.mpart_m, M_PVIOL, pviol);
-  if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_PVIOL2, pviol2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_m, M_PVIOL, pviol, blockIdx.x);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_PVIOL2, pviol2, blockIdx.x);
 }
 
 // Dual-ray column terms of λ = −Aᵀr (engine.cpp:98-119).
@@ -533,10 +534,37 @@ __device__ inline void ray_col_terms(double g, double gabs, double lo, double up
   }
 }
 
-// Tiles of the original Aᵀ (the model's CSC columns; Ãᵀ's tiles).
+// Tiles of the original Aᵀ (the model's CSC columns; Ãᵀ's tiles), behind cs.blocks workgroups
+// that sum the chunks of `cs` (one chain per wavefront, each in its own quarter of the LDS
+// buffer the column tiles stage products in). cs.blocks is a multiple of kXcd, so the tiles keep
+// their XCD assignment. The chains' unrolled sums take the kernel from 8 to 6 wavefronts per
+// SIMD; holding it at 8 slows the chains more than it speeds the column tiles (measured on the
+// 1M-row LP: 1.5% more time per iteration).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles) {
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles,
+                                                       ChunkSums cs) {
+  static_assert((kBlock / 64) * kSeqBatch <= kTileNnz, "a stage per wavefront in the tile buffer");
+  __shared__ double lds[kTileNnz];
+  if (static_cast<int>(blockIdx.x) < cs.blocks) {
+    const int w = static_cast<int>(threadIdx.x) >> 6;
+    const int chain = static_cast<int>(blockIdx.x) * (kBlock / 64) + w;
+    int job = -1;
+    for (int b = 0, k = chain >> 6; b < 32; ++b) {
+      if (!((cs.mask >> b) & 1u)) continue;
+      if (k == 0) {
+        job = b;
+        break;
+      }
+      --k;
+    }
+    if (job >= 0) {
+      const RedJob J = cs.jobs[job];
+      ordered_chunk<true>(J, cs.partials + static_cast<size_t>(job) * kTreeBlocks, chain & 63, lds + w * kSeqBatch);
+    }
+    return;
+  }
+  const int prow = static_cast<int>(blockIdx.x) - cs.blocks;  // this workgroup's row of the max partials
+  const int tile = tile_of_block(prow, static_cast<int>(gridDim.x) - cs.blocks);
   const bool valid = tile < ntiles;
   const int r0 = valid ? tiles[tile] : 0, r1 = valid ? tiles[tile + 1] : 0;
   const int j = r0 + static_cast<int>(threadIdx.x);
@@ -544,7 +572,7 @@ __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int
   double pmax = 0, dmax = 0, cinf = 0, dviol = 0, dviol2 = 0, s, sa;
   const double lo = in ? a.clo[j] : 0.0, up = in ? a.cuo[j] : 0.0;
   if (a.flags & F_KKT) {  // column part of kkt_general (cpu_backend.cpp:271-282)
-    tile_sums<double, double>(r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.yo, false, s, sa);
+    tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.yo, false, s, sa);
     if (in) {
       const double cj = a.cmin[j], xj = a.xo[j];
       term(a, T_C2)[j] = cj * cj;
@@ -563,7 +591,7 @@ __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int
     }
   }
   if (a.flags & F_RAY) {
-    tile_sums<double, double>(r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv, true, s, sa);
+    tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv, true, s, sa);
     if (in) {
       double obj, mag;
       ray_col_terms(s, sa, lo, up, obj, mag, dviol);
@@ -572,7 +600,7 @@ __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int
     }
   }
   if (a.flags & F_DRIFT) {
-    tile_sums<double, double>(r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv2, true, s, sa);
+    tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv2, true, s, sa);
     if (in) {
       double obj, mag;
       ray_col_terms(s, sa, lo, up, obj, mag, dviol2);
@@ -581,12 +609,12 @@ __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int
     }
   }
   if (a.flags & F_KKT) {
-    block_max_into(a.mpart_n, M_PMAX, pmax);
-    block_max_into(a.mpart_n, M_DMAX, dmax);
-    block_max_into(a.mpart_n, M_CINF, cinf);
+    block_max_into(a.mpart_n, M_PMAX, pmax, prow);
+    block_max_into(a.mpart_n, M_DMAX, dmax, prow);
+    block_max_into(a.mpart_n, M_CINF, cinf, prow);
   }
-  if (a.flags & F_RAY) block_max_into(a.mpart_n, M_DVIOL, dviol);
-  if (a.flags & F_DRIFT) block_max_into(a.mpart_n, M_DVIOL2, dviol2);
+  if (a.flags & F_RAY) block_max_into(a.mpart_n, M_DVIOL, dviol, prow);
+  if (a.flags & F_DRIFT) block_max_into(a.mpart_n, M_DVIOL2, dviol2, prow);
 }
 
 // Maxima of the per-workgroup partials of both sides into out[0 .. kMaxCount).
@@ -663,27 +691,23 @@ CheckArgs<T> make_args(psh_ctx* c, unsigned flags, int xh, int xb, int x0, int d
 }
 
 // Launches the four check kernels for `flags`; the dual-side pass runs before the row pass
-// because the row pass reads Δy, ŷ (unscaled) and the dual ray.
-// `parts`: 1 = the elementwise passes and the row pass (everything the residual and PID sums
-// read), 2 = the column pass and the maxima, 3 = both.
+// because the row pass reads Δy, ŷ (unscaled) and the dual ray. The column pass carries the
+// chunk sums of `cs` when cs.blocks > 0 (they read only what the passes before it write).
 template <class T>
-int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool maxes, int parts = 3) {
+int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool maxes, const ChunkSums& cs = ChunkSums{}) {
   const unsigned nf = a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT | F_PID);
   const unsigned mf = nf;
-  if (parts & 1) {
-    if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
-    if (mf && c->m > 0) hipLaunchKernelGGL((k_check_m<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
-    if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->A.ntiles > 0)
-      hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, a,
-                         c->A.tiles, c->A.ntiles);
-  }
-  if (!(parts & 2)) {
-    PSH_LAUNCHED(c, "check kernels");
-    return PSH_OK;
-  }
-  if ((a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->At.ntiles > 0)
-    hipLaunchKernelGGL((k_check_cols<T>), dim3(grid_for_tiles(c->At.ntiles)), dim3(kBlock), 0, c->stream, a,
-                       c->At.tiles, c->At.ntiles);
+  if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
+  if (mf && c->m > 0) hipLaunchKernelGGL((k_check_m<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
+  if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->A.ntiles > 0)
+    hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, a, c->A.tiles,
+                       c->A.ntiles);
+  const bool cols = (a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->At.ntiles > 0;
+  ChunkSums carry = cs;
+  if (carry.blocks < 0) carry.blocks = 0;
+  if (cols || carry.blocks > 0)
+    hipLaunchKernelGGL((k_check_cols<T>), dim3(carry.blocks + (cols ? grid_for_tiles(c->At.ntiles) : 0)), dim3(kBlock),
+                       0, c->stream, a, c->At.tiles, cols ? c->At.ntiles : 0, carry);
   if (maxes)
     hipLaunchKernelGGL(k_max_final, dim3(kMaxCount), dim3(kBlock), 0, c->stream, c->mpart_m, c->mpart_rows_m,
                        c->mpart_n, c->mpart_rows_n, c->results + R_MAX0);
@@ -996,35 +1020,21 @@ int psh_check(psh_ctx* c, const psh_check_args* g, psh_stats* out) {
   if (!t) return err;
   // In decision mode the 64-chunk sums (residual, PID distances) read only what the elementwise
   // passes and the row pass write; they are latency-bound (one sequential chain per chunk), so
-  // they run on a second stream while the column pass and the tree sums run on the context's.
-  // Per-kernel timing keeps one stream.
-  const bool overlap = g->mode == PSH_RED_DECISION && !c->timing;
-  if (overlap && (rc = ensure_aux(c))) return rc;
+  // the column pass carries them and only their in-order combine runs after it.
+  ChunkSums cs;
+  if (g->mode == PSH_RED_DECISION) cs = chunk_sums_of(c, *t);
+  const bool carried = cs.blocks > 0;
   rc = with_t(c, [&](auto* tag) -> int {
     using T = std::remove_pointer_t<decltype(tag)>;
     const CheckArgs<T> a = make_args<T>(c, flags, PSH_XH, PSH_XB, PSH_X0, PSH_DX, PSH_YH, PSH_YB, PSH_Y0, PSH_DY);
-    if (!overlap) {
-      hipEvent_t ev;
-      time_begin(c, 2, &ev);
-      int r = launch_check<T>(c, a, true);
-      time_end(c, 2, ev);
-      return r;
-    }
-    int r = launch_check<T>(c, a, true, 1);
-    if (r) return r;
-    PSH_TRY(c, hipEventRecord(c->fork, c->stream));
-    PSH_TRY(c, hipStreamWaitEvent(c->aux, c->fork, 0));
-    if ((r = reduce_table_part(c, *t, 0, 2, c->aux))) return r;
-    PSH_TRY(c, hipEventRecord(c->join, c->aux));
-    return launch_check<T>(c, a, true, 2);
+    hipEvent_t ev;
+    time_begin(c, 2, &ev);
+    int r = launch_check<T>(c, a, true, carried ? cs : ChunkSums{});
+    time_end(c, 2, ev);
+    return r;
   });
   if (rc) return rc;
-  if (overlap) {
-    if ((rc = reduce_table_part(c, *t, 0, 1, c->stream))) return rc;
-    PSH_TRY(c, hipStreamWaitEvent(c->stream, c->join, 0));
-  } else if ((rc = reduce_table(c, *t, 0))) {
-    return rc;
-  }
+  if ((rc = carried ? reduce_table_part(c, *t, 0, 1 | 4) : reduce_table(c, *t, 0))) return rc;
   double r[kResultCount];
   if ((rc = fetch_results(c, kResultCount, r))) return rc;
   out->px = r[R_PX];

@@ -87,6 +87,17 @@ struct JobTable {
   std::vector<RedJob> host;
 };
 
+// The 64-chunk sums of a job table that a column pass carries: one wavefront per (job, chunk),
+// in workgroups placed ahead of the column tiles of the same launch. The chunk sums are
+// latency-bound sequential chains that read only what the passes before the column pass wrote,
+// so they run beside the column tiles without a second stream.
+struct ChunkSums {
+  const RedJob* jobs = nullptr;  // the table on the device
+  double* partials = nullptr;    // chunk k of job j goes to partials[j * kTreeBlocks + k]
+  unsigned mask = 0;             // bit j: job j of the table is a 64-chunk sum
+  int blocks = 0;                // workgroups ahead of the column tiles (16 per job: 4 chains each)
+};
+
 // Per-element term arrays of one check pass (each c->term_len doubles in c->terms).
 enum Term {
   T_DX2, T_DY2, T_CROSS,                               // fixed-point residual
@@ -191,8 +202,6 @@ struct psh_ctx {
   int mpart_rows_m = 0, mpart_rows_n = 0;        // workgroup slots in each
   long long term_len = 0;
   bool boxed = false;                           // every column has two finite bounds: the primal ray is 0
-  hipStream_t aux = nullptr;                    // second stream of the check pass (created on first use)
-  hipEvent_t fork = nullptr, join = nullptr;
   double* partials = nullptr;                   // tree first level
   double* results = nullptr;                    // device results
   double* host_results = nullptr;               // pinned
@@ -272,11 +281,13 @@ long long slot_len(const psh_ctx* c, int s);
 int reduce(psh_ctx* c, const RedJob* jobs, int njobs, int first);
 // Same, with a table already on the device (no upload, no synchronisation).
 int reduce_table(psh_ctx* c, const JobTable& t, int first);
-// Only the tree jobs (part 1) or only the 64-chunk / sequential sum jobs (part 2) of a table, on
-// stream `s`. The two parts write disjoint rows of `partials` and slots of `results`.
-int reduce_table_part(psh_ctx* c, const JobTable& t, int first, int part, hipStream_t s);
-// The check pass's second stream and its fork / join events (created on first use).
-int ensure_aux(psh_ctx* c);
+// Parts of a table's reductions: 1 = the tree jobs, 2 = the 64-chunk / sequential sum jobs,
+// 4 = only the final in-order combine of the sum jobs' chunk partials (the chunks having been
+// summed by a column pass that carried them, see ChunkSums).
+int reduce_table_part(psh_ctx* c, const JobTable& t, int first, int parts);
+// The 64-chunk sums (mode 1) of a table, carried by a column pass (blocks = -1 when the table
+// has a job that cannot be carried: a single-loop sum or an index beyond the mask).
+ChunkSums chunk_sums_of(const psh_ctx* c, const JobTable& t);
 // The table for a check pass with `flags` in reduction `mode` (created on first use).
 const JobTable* check_table(psh_ctx* c, int mode, unsigned flags, int* err);
 // Copies `count` device results to the host (synchronises the stream).
@@ -356,11 +367,11 @@ __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, co
 
 // Row sums of one tile with the products staged in LDS (CSR order, as tile_rows): s = Σ a·x and,
 // with `abs_too`, sa = Σ |a·x|. The whole workgroup calls it.
+// tile_sums_in stages the products in `prod` (kTileNnz doubles of LDS that the caller owns).
 template <class V, class X>
-__device__ inline void tile_sums(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
-                                 const V* __restrict__ val, const X* __restrict__ x, bool abs_too, double& s,
-                                 double& sa) {
-  __shared__ double prod[kTileNnz];
+__device__ inline void tile_sums_in(double* __restrict__ prod, int r0, int r1, const int* __restrict__ ptr,
+                                    const int* __restrict__ col, const V* __restrict__ val, const X* __restrict__ x,
+                                    bool abs_too, double& s, double& sa) {
   const int p0 = ptr[r0], p1 = ptr[r1];
   const int row = r0 + static_cast<int>(threadIdx.x);
   const bool has = row < r1;
@@ -380,6 +391,14 @@ __device__ inline void tile_sums(int r0, int r1, const int* __restrict__ ptr, co
     }
     __syncthreads();
   }
+}
+
+template <class V, class X>
+__device__ inline void tile_sums(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
+                                 const V* __restrict__ val, const X* __restrict__ x, bool abs_too, double& s,
+                                 double& sa) {
+  __shared__ double prod[kTileNnz];
+  tile_sums_in(prod, r0, r1, ptr, col, val, x, abs_too, s, sa);
 }
 
 // ---- reduction shapes shared by the context reductions and the allocation-free entry point
@@ -406,16 +425,31 @@ __device__ inline double block_tree_max(double acc) {
   return red[0];
 }
 
-// s + t(b) + t(b+1) + ... + t(e−1), strictly left to right, by one wavefront (64 threads): the
-// lanes form 512 consecutive terms (coalesced loads) into an LDS stage, issue the loads of the
-// next 512, and meanwhile every lane adds the staged terms in order (a uniform dependent chain,
-// so the lanes agree). The last batch is padded with +0.0: the running sum is never −0.0 (it
-// starts at +0.0 or at a sum that started there, and x + (−x) rounds to +0.0), so adding +0.0
-// leaves it unchanged bit for bit. `stage` holds kSeqBatch doubles of LDS.
+// Orders the LDS accesses of one wavefront around this point: a wavefront's LDS operations are
+// performed in the order it issues them, so keeping the compiler from moving loads and stores
+// across it is enough for the lanes of one wavefront to see each other's stores.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// s + t(b) + t(b+1) + ... + t(e−1), strictly left to right, by one wavefront: the lanes form 512
+// consecutive terms (coalesced loads) into an LDS stage, issue the loads of the next 512, and
+// meanwhile every lane adds the staged terms in order (a uniform dependent chain, so the lanes
+// agree). The last batch is padded with +0.0: the running sum is never −0.0 (it starts at +0.0
+// or at a sum that started there, and x + (−x) rounds to +0.0), so adding +0.0 leaves it
+// unchanged bit for bit. `stage` holds kSeqBatch doubles of LDS. WaveOnly: the stage belongs to
+// this wavefront alone (one of several in a workgroup, each with its own chain), so only the
+// wavefront synchronises; otherwise the workgroup is this one wavefront and uses its barrier.
 constexpr int kSeqBatch = 512;
-template <class Get>
+template <bool WaveOnly = false, class Get>
 __device__ inline double wave_seq_sum(Get get, long long b, long long e, double s, double* __restrict__ stage) {
-  const int lane = static_cast<int>(threadIdx.x);
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  auto sync = [] {
+    if constexpr (WaveOnly) wave_lds_sync();
+    else __syncthreads();
+  };
   double v[kSeqBatch / 64];
 #pragma unroll
   for (int u = 0; u < kSeqBatch / 64; ++u) {
@@ -425,7 +459,7 @@ __device__ inline double wave_seq_sum(Get get, long long b, long long e, double 
   for (long long base = b; base < e; base += kSeqBatch) {
 #pragma unroll
     for (int u = 0; u < kSeqBatch / 64; ++u) stage[u * 64 + lane] = v[u];
-    __syncthreads();
+    sync();
     const long long next = base + kSeqBatch;
 #pragma unroll
     for (int u = 0; u < kSeqBatch / 64; ++u) {
@@ -434,9 +468,31 @@ __device__ inline double wave_seq_sum(Get get, long long b, long long e, double 
     }
 #pragma unroll 16
     for (int k = 0; k < kSeqBatch; ++k) s += stage[k];
-    __syncthreads();
+    sync();
   }
   return s;
+}
+
+// Chunk `k` of a CPU-order sum job by one wavefront (la/parallel.h:169-188): mode 1 splits a into
+// min(64, max(1, na)) chunks [na·k/chunks, na·(k+1)/chunks) and writes the chunk's sequential
+// sum to row[k]; mode 2 is one sequential loop over a then b (cpu_backend.cpp:240-274), run by
+// chunk 0 into row[0]. The in-order combine of the chunk sums is k_ordered_final's.
+template <bool WaveOnly>
+__device__ inline void ordered_chunk(const RedJob& J, double* __restrict__ row, int k, double* __restrict__ stage) {
+  const bool lead = (threadIdx.x & 63) == 0;
+  auto ga = [&](long long i) { return J.a[i]; };
+  if (J.mode == 2) {
+    if (k != 0) return;
+    double s = wave_seq_sum<WaveOnly>(ga, 0, J.na, 0.0, stage);
+    s = wave_seq_sum<WaveOnly>([&](long long i) { return J.b[i]; }, 0, J.nb, s, stage);
+    if (lead) row[0] = s;
+    return;
+  }
+  const long long n = J.na;
+  const int chunks = static_cast<int>(n < 1 ? 1 : (n > 64 ? 64 : n));
+  if (k >= chunks) return;
+  const double s = wave_seq_sum<WaveOnly>(ga, n * k / chunks, n * (k + 1) / chunks, 0.0, stage);
+  if (lead) row[k] = s;
 }
 
 // The phased product (struct Phased) with the epilogue of the row-tiled kernels. Thread t
