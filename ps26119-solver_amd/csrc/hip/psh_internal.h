@@ -477,14 +477,20 @@ __device__ inline double wave_seq_sum(Get get, long long b, long long e, double 
 // min(64, max(1, na)) chunks [na·k/chunks, na·(k+1)/chunks) and writes the chunk's sequential
 // sum to row[k]; mode 2 is one sequential loop over a then b (cpu_backend.cpp:240-274), run by
 // chunk 0 into row[0]. The in-order combine of the chunk sums is k_ordered_final's.
+// The terms are read through global-address-space pointers: the table's pointers are plain
+// (generic) ones, whose loads count on the LDS counter too, so every wait for the staged terms
+// would also wait for the next batch's loads from memory.
+using GlobalDouble = const __attribute__((address_space(1))) double;
 template <bool WaveOnly>
 __device__ inline void ordered_chunk(const RedJob& J, double* __restrict__ row, int k, double* __restrict__ stage) {
   const bool lead = (threadIdx.x & 63) == 0;
-  auto ga = [&](long long i) { return J.a[i]; };
+  GlobalDouble* const a = (GlobalDouble*)J.a;
+  auto ga = [a](long long i) { return a[i]; };
   if (J.mode == 2) {
     if (k != 0) return;
+    GlobalDouble* const b = (GlobalDouble*)J.b;
     double s = wave_seq_sum<WaveOnly>(ga, 0, J.na, 0.0, stage);
-    s = wave_seq_sum<WaveOnly>([&](long long i) { return J.b[i]; }, 0, J.nb, s, stage);
+    s = wave_seq_sum<WaveOnly>([b](long long i) { return b[i]; }, 0, J.nb, s, stage);
     if (lead) row[0] = s;
     return;
   }
