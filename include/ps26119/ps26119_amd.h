@@ -18,8 +18,8 @@ extern "C" {
  * (1 = every sum a device tree: a deterministic trajectory of its own), "hip_timing" (1 = per-kernel
  * HIP event timing into ps26119_amd_stats), "hip_graphs" (1 = default: replay captured HIP
  * graphs of the iteration chunks; timing runs launch eagerly), "hip_device", "hip_sliced"
- * (1 = default: phased products for large fp64 solves, the same bits as the row tiles; 0 = row
- * tiles only; 2 = phased at any size), "hip_refine"
+ * (1 = default: phased products for large fp64 solves with scattered gathers, the same bits as
+ * the row tiles; 0 = row tiles only; 2 = phased at any size), "hip_refine"
  * (mixed precision as fp32 increments on fp64 anchors; default 1, 0 in CPU-order mode: the
  * reference's fp32-then-fp64 policy), "hip_refine_patience" (stalled checks before fp64 takes
  * over, default 40), "hip_refine_max_epoch" (longest epoch of the increment form, default

@@ -165,9 +165,13 @@ int psh_graphs(psh_ctx* ctx, int on); /* 1 (default): replay captured chunk grap
 /* Phased products for fp64 chunks: persistent workgroups sweep column slices of Ã (Ãᵀ) in order,
  * so the workgroups of an XCD gather from one L2-sized slice of x̄ (y) at a time while every row's
  * running sum keeps the CSR order: the same bits as the row-tiled kernels, in every reduction
- * mode. 1 (default): built at psh_load for a matrix whose gathered vector exceeds 4 MiB and used
- * by fp64 chunks; 0: row tiles only; 2: built at any size (tests). Call before psh_load for 0 and
- * 2 to govern the build; afterwards 0 / nonzero only switches the use of what was built. */
+ * mode. The fp64 check pass sums the original matrix over the same layouts (its values copied
+ * into their order), again with the row tiles' bits. 1 (default): built at psh_load for a matrix
+ * whose gathered vector exceeds 4 MiB and whose row tiles gather from scattered cache lines (at
+ * least half a distinct 128-byte line per nonzero; banded or staircase matrices reuse lines and
+ * keep the tiles), used by fp64 chunks and checks; 0: row tiles only; 2: built at any size
+ * (tests). Call before psh_load for 0 and 2 to govern the build; afterwards 0 / nonzero only
+ * switches the use of what was built. */
 int psh_sliced(psh_ctx* ctx, int on);
 /* Mixed precision in increment form (no counterpart in the reference; its mixed mode is
  * psh_set_fp32): the iterate is the fp64 anchor in X0 / Y0 plus an fp32 increment, the matrix

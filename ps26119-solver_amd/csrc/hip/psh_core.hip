@@ -211,7 +211,7 @@ ChunkSums chunk_sums_of(const psh_ctx* c, const JobTable& t) {
     cs.mask |= 1u << k;
     ++count;
   }
-  cs.blocks = count * 64 / (kBlock / 64);
+  cs.blocks = count * 64 / kChainsPerBlock;
   return cs;
 }
 
@@ -573,17 +573,38 @@ int spmv_impl(psh_ctx* c, const DevMat& M, int xs, int out) {
 
 namespace {
 
+// Distinct 128-byte lines of the gathered vector per entry, sampled over every 8th block of 256
+// rows (the row tiles' size): ≈ 1 when a tile's gathers fall at random over a large vector, a few
+// hundredths for banded or staircase structure (multi-period planning models), whose tiles keep
+// reusing the same lines from L2.
+double gather_lines_per_entry(const int* ptr, const int* col, int rows) {
+  long long lines = 0, entries = 0;
+  std::vector<int> v;
+  for (int b0 = 0; b0 < rows; b0 += 8 * 256) {
+    const int b1 = std::min(rows, b0 + 256);
+    v.assign(col + ptr[b0], col + ptr[b1]);
+    for (int& x : v) x >>= 4;
+    std::sort(v.begin(), v.end());
+    lines += std::unique(v.begin(), v.end()) - v.begin();
+    entries += ptr[b1] - ptr[b0];
+  }
+  return entries > 0 ? static_cast<double>(lines) / static_cast<double>(entries) : 0.0;
+}
+
 // The phased layout of a CSR (struct Phased), built when the gathered vector (cols fp64 values)
-// is larger than an XCD's L2 share (kPhMinBytes) or when `force` is set: S phases (slices of
+// is larger than an XCD's L2 share (kPhMinBytes) and the gathers are scattered (at least
+// kPhMinLinesPerEntry distinct lines per entry), or when `force` is set: S phases (slices of
 // about 2 MiB, at least 8), K rows per thread so that the workgroups stay resident together
 // (about 1024 of 512 threads), J product slots per thread for the largest (workgroup, phase)
 // block. Left unbuilt (the row tiles serve) when a block or a row's entries in one phase do not
-// fit (more than kPhMaxJ·kPhBlock entries, or 255 in one row).
-int build_phased(psh_ctx* c, Phased& P, const int* ptr, const int* col, const double* vals_dev, int rows, int cols,
-                 bool force) {
+// fit (more than kPhMaxJ·kPhBlock entries, or 255 in one row). `orig_dev` (may be null): the
+// original values in the same CSR order, copied into the phased order for the check pass.
+int build_phased(psh_ctx* c, Phased& P, const int* ptr, const int* col, const double* vals_dev,
+                 const double* orig_dev, int rows, int cols, bool force) {
   const long long nz = rows > 0 ? ptr[rows] : 0;
   const long long bytes = static_cast<long long>(cols) * 8;
   if (nz == 0 || rows == 0 || cols == 0 || (!force && bytes <= kPhMinBytes)) return PSH_OK;
+  if (!force && gather_lines_per_entry(ptr, col, rows) < kPhMinLinesPerEntry) return PSH_OK;
   for (int r = 0; r < rows; ++r)
     for (int q = ptr[r] + 1; q < ptr[r + 1]; ++q)
       if (col[q] < col[q - 1]) return PSH_OK;  // a row with unsorted columns: row tiles only
@@ -631,9 +652,12 @@ int build_phased(psh_ctx* c, Phased& P, const int* ptr, const int* col, const do
   };
   if ((rc = put(&P.blk, blk.data(), static_cast<long long>(blk.size()))) ||
       (rc = put(&P.cnt, cnt.data(), static_cast<long long>(cnt.size()))) || (rc = put(&P.col, pcol.data(), nz)) ||
-      (rc = put(&P.src, psrc.data(), nz)) || (rc = alloc_n(c, &P.val, nz)))
+      (rc = put(&P.src, psrc.data(), nz)) || (rc = alloc_n(c, &P.val, nz)) ||
+      (orig_dev && (rc = alloc_n(c, &P.valo, nz))))
     return rc;
   hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(nz)), dim3(kBlock), 0, c->stream, vals_dev, P.src, P.val, nz);
+  if (orig_dev)
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(nz)), dim3(kBlock), 0, c->stream, orig_dev, P.src, P.valo, nz);
   PSH_LAUNCHED(c, "phased values");
   PSH_TRY(c, hipStreamSynchronize(c->stream));  // the staging vectors go out of scope
   P.S = S;
@@ -725,7 +749,8 @@ void psh_destroy(psh_ctx* c) {
                   c->ru, c->cf, c->lf, c->uf, c->rlf, c->ruf, c->rs, c->cs, c->clo, c->cuo, c->rlo, c->ruo, c->cmin,
                   c->xo, c->yo, c->rv, c->dv, c->rv2, c->dv2, c->terms, c->partials, c->results, c->jobs_dev, c->lv,
                   c->lvp, c->lw, c->lt, c->r0_dev, c->prm, c->mpart_m, c->mpart_n, c->Pa.blk, c->Pa.cnt,
-                  c->Pa.col, c->Pa.src, c->Pa.val, c->Pat.blk, c->Pat.cnt, c->Pat.col, c->Pat.src, c->Pat.val};
+                  c->Pa.col, c->Pa.src, c->Pa.val, c->Pa.valo, c->Pat.blk, c->Pat.cnt, c->Pat.col, c->Pat.src,
+                  c->Pat.val, c->Pat.valo};
   for (void* p : ptrs) dev_free(p);
   for (int s = 0; s < PSH_NSLOT; ++s) dev_free(c->vd[s]), dev_free(c->vf[s]);
   for (auto& kv : c->tables) dev_free(kv.second.dev);
@@ -796,8 +821,12 @@ int psh_load(psh_ctx* c, const psh_problem* p) {
   if ((rc = up(&c->A.tiles, ta.data(), static_cast<long long>(ta.size()))) ||
       (rc = up(&c->At.tiles, tt.data(), static_cast<long long>(tt.size()))))
     return rc;
-  if (c->sliced && ((rc = build_phased(c, c->Pa, p->a_ptr, p->a_col, c->A.val, m, n, c->sliced == 2)) ||
-                    (rc = build_phased(c, c->Pat, p->at_ptr, p->at_col, c->At.val, n, m, c->sliced == 2))))
+  // The check pass sums the original Aᵀ in the model's CSC order, which is Ãᵀ's only when the
+  // column indices agree; otherwise its column pass keeps the row tiles.
+  if (c->sliced &&
+      ((rc = build_phased(c, c->Pa, p->a_ptr, p->a_col, c->A.val, c->a_orig, m, n, c->sliced == 2)) ||
+       (rc = build_phased(c, c->Pat, p->at_ptr, p->at_col, c->At.val, c->ato_alias ? c->ato_val : nullptr, n, m,
+                          c->sliced == 2))))
     return rc;
   for (int s = 0; s < PSH_NSLOT; ++s)
     if ((rc = alloc_n(c, &c->vd[s], slot_len(c, s)))) return rc;

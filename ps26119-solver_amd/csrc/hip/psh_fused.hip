@@ -303,21 +303,31 @@ __device__ inline double wave_max(double v) {
 
 // Maximum of non-negative values over the workgroup (NaN ignored, as std::max(acc, NaN) does),
 // stored as partial row `row` (the workgroup's index among the pass's workgroups) for quantity
-// q; k_max_final combines the partials. Every thread of the workgroup must reach this call (no
-// atomics: one store per workgroup).
+// q; k_max_final combines the partials. Every thread of the NT-thread workgroup must reach this
+// call (no atomics: one store per workgroup).
+template <int NT = kBlock>
 __device__ inline void block_max_into(double* part, int q, double v, int row) {
-  __shared__ double wm[kBlock / 64];
+  __shared__ double wm[NT / 64];
   v = v >= 0.0 ? v : 0.0;
   v = wave_max(v);
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double mx = wm[0];
-    for (int k = 1; k < kBlock / 64; ++k) mx = fmax(mx, wm[k]);
+    for (int k = 1; k < NT / 64; ++k) mx = fmax(mx, wm[k]);
     part[static_cast<size_t>(row) * kMaxCount + q] = mx;
   }
   __syncthreads();
 }
+
+// A pass of `used` workgroups: workgroup g writes zeros into the partial rows g + used, g + 2·used,
+// … below `rows` of its quantity q, so that maxima an earlier check on the same context left there
+// with a launch of more workgroups (the row tiles) do not count.
+__device__ inline void clear_max_rows(double* part, int q, int g, int used, int rows) {
+  if (threadIdx.x != 0) return;
+  for (int r = used + g; r < rows; r += used) part[static_cast<size_t>(r) * kMaxCount + q] = 0.0;
+}
+__device__ inline double nonneg(double v) { return v >= 0.0 ? v : 0.0; }  // NaN → 0, as block_max_into
 
 __device__ inline double dmaxd(double a, double b) { return a < b ? b : a; }  // std::max
 __device__ inline double dmind(double a, double b) { return b < a ? b : a; }  // std::min
@@ -456,70 +466,6 @@ __global__ void __launch_bounds__(kBlock) k_check_m(CheckArgs<T> a) {
   if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_RMAX2, rmax2, blockIdx.x);
 }
 
-// Tiles of A (Ã and the original A share the structure): residual, KKT row part and ray tests.
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles) {
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
-  const bool valid = tile < ntiles;  // padding workgroups still write their (zero) maxima
-  const int r0 = valid ? tiles[tile] : 0, r1 = valid ? tiles[tile + 1] : 0;
-  const int i = r0 + static_cast<int>(threadIdx.x);
-  const bool in = i < r1;
-  double pmax = 0, dmax = 0, pviol = 0, pviol2 = 0, s, sa;
-  if (a.flags & F_RES) {  // Ã·Δx (fp64 accumulate, stored in T), cross term Δyᵀ(ÃΔx)
-    tile_sums<T, T>(r0, r1, a.a_ptr, a.a_col, a.a_val, a.dx, false, s, sa);
-    if (in) {
-      const T ax = static_cast<T>(s);
-      a.ax[i] = ax;
-      term(a, T_CROSS)[i] = static_cast<double>(a.dy[i]) * static_cast<double>(ax);
-    }
-  }
-  const double lo = in ? a.rlo[i] : 0.0, up = in ? a.ruo[i] : 0.0;
-  if (a.flags & F_KKT) {  // row part of kkt_general (cpu_backend.cpp:254-270)
-    tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.xo, false, s, sa);
-    if (in) {
-      const double viol = s < lo ? lo - s : (s > up ? s - up : 0.0);
-      term(a, T_RP2)[i] = viol * viol;
-      pmax = elem_viol(s, lo, up);
-      const double bl = isfinite(lo) ? fabs(lo) : 0.0, bu = isfinite(up) ? fabs(up) : 0.0;
-      term(a, T_B2)[i] = dmaxd(bl, bu) * dmaxd(bl, bu);
-      const double y = a.yo[i];
-      double dt = 0.0, rd = 0.0;
-      if (y > 0) {
-        if (isfinite(lo)) dt = y * lo;
-        else rd = y * y, dmax = y;
-      } else if (y < 0) {
-        if (isfinite(up)) dt = y * up;
-        else rd = y * y, dmax = -y;
-      }
-      term(a, T_DROW)[i] = dt;
-      term(a, T_RD2ROW)[i] = rd;
-    }
-  }
-  // A·d against the finite row bounds (engine.cpp:140-145). With every column boxed each entry
-  // of d is clamped to 0 from both sides (or is NaN, which the maximum |d| skips), so the ray's
-  // maximum is 0, the test is skipped (engine.cpp:136) and the product is not needed.
-  if ((a.flags & F_RAY) && a.pray) {
-    tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv, false, s, sa);
-    if (in) {
-      if (isfinite(lo)) pviol = dmaxd(pviol, -s);
-      if (isfinite(up)) pviol = dmaxd(pviol, s);
-    }
-  }
-  if ((a.flags & F_DRIFT) && a.pray) {
-    tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv2, false, s, sa);
-    if (in) {
-      if (isfinite(lo)) pviol2 = dmaxd(pviol2, -s);
-      if (isfinite(up)) pviol2 = dmaxd(pviol2, s);
-    }
-  }
-  if (a.flags & F_KKT) {
-    block_max_into(a.mpart_m, M_PMAX, pmax, blockIdx.x);
-    block_max_into(a.mpart_m, M_DMAX, dmax, blockIdx.x);
-  }
-  if (a.flags & F_RAY) block_max_into(a.mpart_m, M_PVIOL, pviol, blockIdx.x);
-  if (a.flags & F_DRIFT) block_max_into(a.mpart_m, M_PVIOL2, pviol2, blockIdx.x);
-}
-
 // Dual-ray column terms of λ = −Aᵀr (engine.cpp:98-119).
 __device__ inline void ray_col_terms(double g, double gabs, double lo, double up, double& obj, double& mag, double& viol) {
   const double lam = -g;
@@ -534,33 +480,212 @@ __device__ inline void ray_col_terms(double g, double gabs, double lo, double up
   }
 }
 
+// Row i of the row pass, one part at a time from that part's row sum s: F_RES (Ã·Δx)_i, the
+// residual's cross term; F_KKT (A·x)_i, the row part of kkt_general (cpu_backend.cpp:254-270);
+// F_RAY / F_DRIFT (A·d)_i, the primal-ray test against the finite row bounds
+// (engine.cpp:140-145). lo, up: the row's original bounds (loaded by the caller ahead of the
+// sums). The row's maxima are folded into mx.
+struct RowMax {
+  double pmax = 0.0, dmax = 0.0, pviol = 0.0, pviol2 = 0.0;
+};
+template <class T>
+__device__ inline void check_row(const CheckArgs<T>& a, unsigned part, int i, double lo, double up, double s,
+                                 RowMax& mx) {
+  if (part == F_RES) {  // Ã·Δx (fp64 accumulate, stored in T), cross term Δyᵀ(ÃΔx)
+    const T ax = static_cast<T>(s);
+    a.ax[i] = ax;
+    term(a, T_CROSS)[i] = static_cast<double>(a.dy[i]) * static_cast<double>(ax);
+    return;
+  }
+  if (part == F_KKT) {
+    const double viol = s < lo ? lo - s : (s > up ? s - up : 0.0);
+    term(a, T_RP2)[i] = viol * viol;
+    mx.pmax = fmax(mx.pmax, nonneg(elem_viol(s, lo, up)));
+    const double bl = isfinite(lo) ? fabs(lo) : 0.0, bu = isfinite(up) ? fabs(up) : 0.0;
+    term(a, T_B2)[i] = dmaxd(bl, bu) * dmaxd(bl, bu);
+    const double y = a.yo[i];
+    double dt = 0.0, rd = 0.0, dm = 0.0;
+    if (y > 0) {
+      if (isfinite(lo)) dt = y * lo;
+      else rd = y * y, dm = y;
+    } else if (y < 0) {
+      if (isfinite(up)) dt = y * up;
+      else rd = y * y, dm = -y;
+    }
+    mx.dmax = fmax(mx.dmax, nonneg(dm));
+    term(a, T_DROW)[i] = dt;
+    term(a, T_RD2ROW)[i] = rd;
+    return;
+  }
+  double v = 0.0;  // F_RAY or F_DRIFT
+  if (isfinite(lo)) v = dmaxd(v, -s);
+  if (isfinite(up)) v = dmaxd(v, s);
+  if (part == F_RAY) mx.pviol = fmax(mx.pviol, nonneg(v));
+  else mx.pviol2 = fmax(mx.pviol2, nonneg(v));
+}
+
+// The row pass's maxima of an NT-thread workgroup into partial row `row`; with used < rows the
+// rows past the launch's workgroups are cleared (clear_max_rows).
+template <int NT, class T>
+__device__ inline void row_maxima(const CheckArgs<T>& a, const RowMax& mx, int row, int used, int rows) {
+  auto put = [&](int q, double v) {
+    block_max_into<NT>(a.mpart_m, q, v, row);
+    clear_max_rows(a.mpart_m, q, row, used, rows);
+  };
+  if (a.flags & F_KKT) {
+    put(M_PMAX, mx.pmax);
+    put(M_DMAX, mx.dmax);
+  }
+  if (a.flags & F_RAY) put(M_PVIOL, mx.pviol);
+  if (a.flags & F_DRIFT) put(M_PVIOL2, mx.pviol2);
+}
+
+// Column j of the column pass, one part at a time from that part's sums: F_KKT s = (Aᵀy)_j, the
+// column part of kkt_general (cpu_backend.cpp:271-282); F_RAY / F_DRIFT s = (Aᵀr)_j and
+// sa = Σ_i |a_ij r_i|, the dual-ray column terms. lo, up: the column's original bounds (loaded
+// by the caller ahead of the sums). The column's maxima are folded into mx.
+struct ColMax {
+  double pmax = 0.0, dmax = 0.0, cinf = 0.0, dviol = 0.0, dviol2 = 0.0;
+};
+template <class T>
+__device__ inline void check_col(const CheckArgs<T>& a, unsigned part, int j, double lo, double up, double s,
+                                 double sa, ColMax& mx) {
+  if (part == F_KKT) {
+    const double cj = a.cmin[j], xj = a.xo[j];
+    term(a, T_C2)[j] = cj * cj;
+    mx.cinf = fmax(mx.cinf, nonneg(fabs(cj)));
+    term(a, T_P)[j] = cj * xj;
+    mx.pmax = fmax(mx.pmax, nonneg(elem_viol(xj, lo, up)));
+    const double lam = cj - s;
+    double z;
+    if (isfinite(lo) && isfinite(up)) z = lam;
+    else if (isfinite(lo)) z = dmaxd(lam, 0.0);
+    else if (isfinite(up)) z = dmind(lam, 0.0);
+    else z = 0.0;
+    term(a, T_RD2COL)[j] = (lam - z) * (lam - z);
+    mx.dmax = fmax(mx.dmax, nonneg(fabs(lam - z)));
+    term(a, T_DCOL)[j] = z > 0 ? z * lo : (z < 0 ? z * up : 0.0);
+    return;
+  }
+  double obj, mag, v = 0.0;
+  ray_col_terms(s, sa, lo, up, obj, mag, v);
+  if (part == F_RAY) {
+    term(a, T_ROBJ_COL)[j] = obj;
+    term(a, T_RMAG_COL)[j] = mag;
+    mx.dviol = fmax(mx.dviol, nonneg(v));
+  } else {
+    term(a, T_ROBJ_COL2)[j] = obj;
+    term(a, T_RMAG_COL2)[j] = mag;
+    mx.dviol2 = fmax(mx.dviol2, nonneg(v));
+  }
+}
+
+template <int NT, class T>
+__device__ inline void col_maxima(const CheckArgs<T>& a, const ColMax& mx, int row, int used, int rows) {
+  auto put = [&](int q, double v) {
+    block_max_into<NT>(a.mpart_n, q, v, row);
+    clear_max_rows(a.mpart_n, q, row, used, rows);
+  };
+  if (a.flags & F_KKT) {
+    put(M_PMAX, mx.pmax);
+    put(M_DMAX, mx.dmax);
+    put(M_CINF, mx.cinf);
+  }
+  if (a.flags & F_RAY) put(M_DVIOL, mx.dviol);
+  if (a.flags & F_DRIFT) put(M_DVIOL2, mx.dviol2);
+}
+
+// The chains of `cs` carried by workgroup blockIdx.x < cs.blocks of a column pass: wavefront
+// w < kChainsPerBlock sums one (job, chunk) in its own kSeqBatch doubles of `lds`.
+__device__ inline void carry_chains(const ChunkSums& cs, double* lds) {
+  const int w = static_cast<int>(threadIdx.x) >> 6;
+  if (w >= kChainsPerBlock) return;
+  const int chain = static_cast<int>(blockIdx.x) * kChainsPerBlock + w;
+  int job = -1;
+  for (int b = 0, k = chain >> 6; b < 32; ++b) {
+    if (!((cs.mask >> b) & 1u)) continue;
+    if (k == 0) {
+      job = b;
+      break;
+    }
+    --k;
+  }
+  if (job < 0) return;
+  const RedJob J = cs.jobs[job];
+  ordered_chunk<true>(J, cs.partials + static_cast<size_t>(job) * kTreeBlocks, chain & 63, lds + w * kSeqBatch);
+}
+
+// Tiles of A (Ã and the original A share the structure): residual, KKT row part and ray tests.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_check_rows(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles) {
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const bool valid = tile < ntiles;  // padding workgroups still write their (zero) maxima
+  const int r0 = valid ? tiles[tile] : 0, r1 = valid ? tiles[tile + 1] : 0;
+  const int i = r0 + static_cast<int>(threadIdx.x);
+  const bool in = i < r1;
+  const double lo = in ? a.rlo[i] : 0.0, up = in ? a.ruo[i] : 0.0;
+  double s_res = 0.0, s_kkt = 0.0, s_ray = 0.0, s_dr = 0.0, sa;
+  if (a.flags & F_RES) tile_sums<T, T>(r0, r1, a.a_ptr, a.a_col, a.a_val, a.dx, false, s_res, sa);
+  if (a.flags & F_KKT) tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.xo, false, s_kkt, sa);
+  if ((a.flags & F_RAY) && a.pray) tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv, false, s_ray, sa);
+  if ((a.flags & F_DRIFT) && a.pray)
+    tile_sums<double, double>(r0, r1, a.a_ptr, a.a_col, a.a_orig, a.dv2, false, s_dr, sa);
+  RowMax mx;
+  if (in) {
+    if (a.flags & F_RES) check_row(a, F_RES, i, lo, up, s_res, mx);
+    if (a.flags & F_KKT) check_row(a, F_KKT, i, lo, up, s_kkt, mx);
+    if ((a.flags & F_RAY) && a.pray) check_row(a, F_RAY, i, lo, up, s_ray, mx);
+    if ((a.flags & F_DRIFT) && a.pray) check_row(a, F_DRIFT, i, lo, up, s_dr, mx);
+  }
+  row_maxima<kBlock>(a, mx, blockIdx.x, gridDim.x, gridDim.x);
+}
+
+// The row pass over the phased layout of Ã (P.val: Ã, P.valo: A, in the layout's order): the
+// tiles' sums in the same order, one sweep per product, then the rows k·kPhBlock + t of each
+// workgroup. `rows_max`: the partial rows of the row-side maxima.
+template <int K, int J>
+__global__ void __launch_bounds__(kPhBlock) k_check_rows_ph(CheckArgs<double> a, Phased P, int rows_max) {
+  __shared__ double prod[J * kPhBlock];
+  __shared__ double part[K * kPhBlock];
+  __shared__ int wtot[kPhBlock / 64];
+  const int g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
+  double sum[K], acc[K], unused[K], lo[K], up[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {  // the rows' bounds, loaded ahead of the sweeps
+    const int i = g * P.R + k * kPhBlock + t;
+    lo[k] = i < a.m ? a.rlo[i] : 0.0;
+    up[k] = i < a.m ? a.ruo[i] : 0.0;
+  }
+  RowMax mx;
+  // one sweep per part, each followed by that part's row work (only one part's sums stay live)
+  auto sweep = [&](unsigned p, const double* val, const double* x) {
+    phased_accumulate<K, J, false>(P, g, val, x, prod, wtot, acc, unused);
+    phased_transpose<K>(acc, part, sum);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = g * P.R + k * kPhBlock + t;
+      if (i < a.m) check_row(a, p, i, lo[k], up[k], sum[k], mx);
+    }
+  };
+  if (a.flags & F_RES) sweep(F_RES, P.val, a.dx);
+  if (a.flags & F_KKT) sweep(F_KKT, P.valo, a.xo);
+  if ((a.flags & F_RAY) && a.pray) sweep(F_RAY, P.valo, a.dv);
+  if ((a.flags & F_DRIFT) && a.pray) sweep(F_DRIFT, P.valo, a.dv2);
+  row_maxima<kPhBlock>(a, mx, g, P.G, rows_max);
+}
+
 // Tiles of the original Aᵀ (the model's CSC columns; Ãᵀ's tiles), behind cs.blocks workgroups
-// that sum the chunks of `cs` (one chain per wavefront, each in its own quarter of the LDS
-// buffer the column tiles stage products in). cs.blocks is a multiple of kXcd, so the tiles keep
-// their XCD assignment. The chains' unrolled sums take the kernel from 8 to 6 wavefronts per
-// SIMD; holding it at 8 slows the chains more than it speeds the column tiles (measured on the
-// 1M-row LP: 1.5% more time per iteration).
+// that carry the chunk sums of `cs` in the LDS buffer the tiles stage products in. cs.blocks is a
+// multiple of kXcd, so the tiles keep their XCD assignment. The chains' unrolled sums take the
+// kernel from 8 to 6 wavefronts per SIMD; holding it at 8 slows the chains more than it speeds
+// the column tiles (measured on the 1M-row LP: 1.5% more time per iteration).
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles,
                                                        ChunkSums cs) {
-  static_assert((kBlock / 64) * kSeqBatch <= kTileNnz, "a stage per wavefront in the tile buffer");
+  static_assert(kChainsPerBlock * kSeqBatch <= kTileNnz && kChainsPerBlock <= kBlock / 64, "chain stages");
   __shared__ double lds[kTileNnz];
   if (static_cast<int>(blockIdx.x) < cs.blocks) {
-    const int w = static_cast<int>(threadIdx.x) >> 6;
-    const int chain = static_cast<int>(blockIdx.x) * (kBlock / 64) + w;
-    int job = -1;
-    for (int b = 0, k = chain >> 6; b < 32; ++b) {
-      if (!((cs.mask >> b) & 1u)) continue;
-      if (k == 0) {
-        job = b;
-        break;
-      }
-      --k;
-    }
-    if (job >= 0) {
-      const RedJob J = cs.jobs[job];
-      ordered_chunk<true>(J, cs.partials + static_cast<size_t>(job) * kTreeBlocks, chain & 63, lds + w * kSeqBatch);
-    }
+    carry_chains(cs, lds);
     return;
   }
   const int prow = static_cast<int>(blockIdx.x) - cs.blocks;  // this workgroup's row of the max partials
@@ -569,52 +694,56 @@ __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int
   const int r0 = valid ? tiles[tile] : 0, r1 = valid ? tiles[tile + 1] : 0;
   const int j = r0 + static_cast<int>(threadIdx.x);
   const bool in = j < r1;
-  double pmax = 0, dmax = 0, cinf = 0, dviol = 0, dviol2 = 0, s, sa;
   const double lo = in ? a.clo[j] : 0.0, up = in ? a.cuo[j] : 0.0;
-  if (a.flags & F_KKT) {  // column part of kkt_general (cpu_backend.cpp:271-282)
-    tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.yo, false, s, sa);
-    if (in) {
-      const double cj = a.cmin[j], xj = a.xo[j];
-      term(a, T_C2)[j] = cj * cj;
-      cinf = fabs(cj);
-      term(a, T_P)[j] = cj * xj;
-      pmax = elem_viol(xj, lo, up);
-      const double lam = cj - s;
-      double z;
-      if (isfinite(lo) && isfinite(up)) z = lam;
-      else if (isfinite(lo)) z = dmaxd(lam, 0.0);
-      else if (isfinite(up)) z = dmind(lam, 0.0);
-      else z = 0.0;
-      term(a, T_RD2COL)[j] = (lam - z) * (lam - z);
-      dmax = fabs(lam - z);
-      term(a, T_DCOL)[j] = z > 0 ? z * lo : (z < 0 ? z * up : 0.0);
+  double s_kkt = 0.0, s_ray = 0.0, a_ray = 0.0, s_dr = 0.0, a_dr = 0.0, sa;
+  if (a.flags & F_KKT) tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.yo, false, s_kkt, sa);
+  if (a.flags & F_RAY) tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv, true, s_ray, a_ray);
+  if (a.flags & F_DRIFT)
+    tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv2, true, s_dr, a_dr);
+  ColMax mx;
+  if (in) {
+    if (a.flags & F_KKT) check_col(a, F_KKT, j, lo, up, s_kkt, 0.0, mx);
+    if (a.flags & F_RAY) check_col(a, F_RAY, j, lo, up, s_ray, a_ray, mx);
+    if (a.flags & F_DRIFT) check_col(a, F_DRIFT, j, lo, up, s_dr, a_dr, mx);
+  }
+  const int used = static_cast<int>(gridDim.x) - cs.blocks;
+  col_maxima<kBlock>(a, mx, prow, used, used);
+}
+
+// The column pass over the phased layout of Ãᵀ (P.valo: the model's CSC values, in the layout's
+// order; built only when the CSC's row order is Ãᵀ's), behind cs.blocks workgroups that carry the
+// chunk sums of `cs` (4 of the 8 wavefronts each). `rows_max`: the partial rows of the
+// column-side maxima.
+template <int K, int J>
+__global__ void __launch_bounds__(kPhBlock) k_check_cols_ph(CheckArgs<double> a, Phased P, int rows_max, ChunkSums cs) {
+  constexpr int kProd = (J + K) * kPhBlock, kStage = kChainsPerBlock * kSeqBatch;
+  __shared__ double lds[kProd > kStage ? kProd : kStage];
+  __shared__ int wtot[kPhBlock / 64];
+  if (static_cast<int>(blockIdx.x) < cs.blocks) {
+    carry_chains(cs, lds);
+    return;
+  }
+  double* const prod = lds;
+  double* const part = lds + J * kPhBlock;
+  const int g = static_cast<int>(blockIdx.x) - cs.blocks, t = static_cast<int>(threadIdx.x);
+  double sum[K], asum[K], acc[K], acca[K];
+  ColMax mx;
+  // one sweep per part, each followed by that part's column work
+  auto sweep = [&](unsigned p, const double* x, auto abs_tag) {
+    constexpr bool kAbs = decltype(abs_tag)::value;
+    phased_accumulate<K, J, kAbs>(P, g, P.valo, x, prod, wtot, acc, acca);
+    phased_transpose<K>(acc, part, sum);
+    if constexpr (kAbs) phased_transpose<K>(acca, part, asum);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = g * P.R + k * kPhBlock + t;
+      if (j < a.n) check_col(a, p, j, a.clo[j], a.cuo[j], sum[k], kAbs ? asum[k] : 0.0, mx);
     }
-  }
-  if (a.flags & F_RAY) {
-    tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv, true, s, sa);
-    if (in) {
-      double obj, mag;
-      ray_col_terms(s, sa, lo, up, obj, mag, dviol);
-      term(a, T_ROBJ_COL)[j] = obj;
-      term(a, T_RMAG_COL)[j] = mag;
-    }
-  }
-  if (a.flags & F_DRIFT) {
-    tile_sums_in<double, double>(lds, r0, r1, a.at_ptr, a.ato_col, a.ato_val, a.rv2, true, s, sa);
-    if (in) {
-      double obj, mag;
-      ray_col_terms(s, sa, lo, up, obj, mag, dviol2);
-      term(a, T_ROBJ_COL2)[j] = obj;
-      term(a, T_RMAG_COL2)[j] = mag;
-    }
-  }
-  if (a.flags & F_KKT) {
-    block_max_into(a.mpart_n, M_PMAX, pmax, prow);
-    block_max_into(a.mpart_n, M_DMAX, dmax, prow);
-    block_max_into(a.mpart_n, M_CINF, cinf, prow);
-  }
-  if (a.flags & F_RAY) block_max_into(a.mpart_n, M_DVIOL, dviol, prow);
-  if (a.flags & F_DRIFT) block_max_into(a.mpart_n, M_DVIOL2, dviol2, prow);
+  };
+  if (a.flags & F_KKT) sweep(F_KKT, a.yo, std::false_type{});
+  if (a.flags & F_RAY) sweep(F_RAY, a.rv, std::true_type{});
+  if (a.flags & F_DRIFT) sweep(F_DRIFT, a.rv2, std::true_type{});
+  col_maxima<kPhBlock>(a, mx, g, P.G, rows_max);
 }
 
 // Maxima of the per-workgroup partials of both sides into out[0 .. kMaxCount).
@@ -699,13 +828,35 @@ int launch_check(psh_ctx* c, const CheckArgs<T>& a, bool maxes, const ChunkSums&
   const unsigned mf = nf;
   if (nf && c->n > 0) hipLaunchKernelGGL((k_check_n<T>), dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, a);
   if (mf && c->m > 0) hipLaunchKernelGGL((k_check_m<T>), dim3(grid_for(c->m)), dim3(kBlock), 0, c->stream, a);
-  if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->A.ntiles > 0)
-    hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, a, c->A.tiles,
-                       c->A.ntiles);
+  // fp64 checks use the phased layouts where they are built (the original values in P.valo)
+  bool ph_rows = false, ph_cols = false;
+  if constexpr (sizeof(T) == 8) {
+    ph_rows = c->sliced && c->Pa.S > 0 && c->Pa.valo;
+    ph_cols = c->sliced && c->Pat.S > 0 && c->Pat.valo;
+  }
+  if ((a.flags & (F_RES | F_KKT | F_RAY | F_DRIFT)) && c->A.ntiles > 0) {
+    if constexpr (sizeof(T) == 8) {
+      if (ph_rows)
+        phased_dispatch(c->Pa, [&](auto kc, auto jc) {
+          hipLaunchKernelGGL((k_check_rows_ph<decltype(kc)::value, decltype(jc)::value>), dim3(c->Pa.G), dim3(kPhBlock),
+                             0, c->stream, a, c->Pa, c->mpart_rows_m);
+        });
+    }
+    if (!ph_rows)
+      hipLaunchKernelGGL((k_check_rows<T>), dim3(grid_for_tiles(c->A.ntiles)), dim3(kBlock), 0, c->stream, a,
+                         c->A.tiles, c->A.ntiles);
+  }
   const bool cols = (a.flags & (F_KKT | F_RAY | F_DRIFT)) && c->At.ntiles > 0;
   ChunkSums carry = cs;
   if (carry.blocks < 0) carry.blocks = 0;
-  if (cols || carry.blocks > 0)
+  if constexpr (sizeof(T) == 8) {
+    if (cols && ph_cols)
+      phased_dispatch(c->Pat, [&](auto kc, auto jc) {
+        hipLaunchKernelGGL((k_check_cols_ph<decltype(kc)::value, decltype(jc)::value>), dim3(carry.blocks + c->Pat.G),
+                           dim3(kPhBlock), 0, c->stream, a, c->Pat, c->mpart_rows_n, carry);
+      });
+  }
+  if ((cols && !ph_cols) || (!cols && carry.blocks > 0))
     hipLaunchKernelGGL((k_check_cols<T>), dim3(carry.blocks + (cols ? grid_for_tiles(c->At.ntiles) : 0)), dim3(kBlock),
                        0, c->stream, a, c->At.tiles, cols ? c->At.ntiles : 0, carry);
   if (maxes)

@@ -68,9 +68,15 @@ struct Phased {
   int* col = nullptr;             // nnz
   int* src = nullptr;             // nnz: index of each entry in the CSR
   double* val = nullptr;          // nnz
+  double* valo = nullptr;         // nnz: the original (unscaled) values, for the check pass
 };
 constexpr int kPhBlock = 512;                    // threads per phased workgroup
 constexpr long long kPhMinBytes = 4LL << 20;     // gathered vectors up to this size stay in L2 anyway
+// Below this many distinct 128-byte lines of the gathered vector per entry the row tiles reuse
+// lines from L2 and beat the phased layout (a staircase planning LP at 4.2 MB gathered: 0.025
+// lines per entry, 41k iterations/s on tiles against 28k phased; a random LP: 0.99 lines per
+// entry, 7.75k phased against 5.29k on tiles).
+constexpr double kPhMinLinesPerEntry = 0.5;
 constexpr int kPhMaxJ = 8;                       // ≤ kPhMaxJ·kPhBlock entries per (workgroup, phase)
 
 // A reduction job: Σ or max over t[0..na) followed by u[0..nb) (u optional).
@@ -97,6 +103,7 @@ struct ChunkSums {
   unsigned mask = 0;             // bit j: job j of the table is a 64-chunk sum
   int blocks = 0;                // workgroups ahead of the column tiles (16 per job: 4 chains each)
 };
+constexpr int kChainsPerBlock = 4;  // chains per workgroup of a column pass that carries ChunkSums
 
 // Per-element term arrays of one check pass (each c->term_len doubles in c->terms).
 enum Term {
@@ -501,20 +508,18 @@ __device__ inline void ordered_chunk(const RedJob& J, double* __restrict__ row, 
   if (lead) row[k] = s;
 }
 
-// The phased product (struct Phased) with the epilogue of the row-tiled kernels. Thread t
-// accumulates the rows t·K .. t·K + K − 1 of its workgroup in registers across the phases
-// (products staged in LDS, each row's products added in order), then the sums go through LDS
-// and the epilogue runs on rows k·kPhBlock + t (coalesced). The whole workgroup calls it.
-template <int K, int J, class V, class X, class Epi>
-__device__ inline void phased_rows(const Phased& P, int rows, const V* __restrict__ val, const X* __restrict__ x,
-                                   Epi& epi) {
-  __shared__ double prod[J * kPhBlock];
-  __shared__ double part[K * kPhBlock];
-  __shared__ int wtot[kPhBlock / 64];
-  const int t = static_cast<int>(threadIdx.x), g = static_cast<int>(blockIdx.x), lane = t & 63, w = t >> 6;
-  double acc[K];
+// One sweep of the phased product (struct Phased) by workgroup g: thread t accumulates the rows
+// t·K .. t·K + K − 1 of the workgroup in registers across the phases (products staged in `prod`,
+// J·kPhBlock doubles of LDS, each row's products added in order), and with ABS the sums of the
+// products' magnitudes in the same order. `wtot` is kPhBlock / 64 ints of LDS. The whole
+// workgroup calls it.
+template <int K, int J, bool ABS, class V, class X>
+__device__ inline void phased_accumulate(const Phased& P, int g, const V* __restrict__ val, const X* __restrict__ x,
+                                         double* __restrict__ prod, int* __restrict__ wtot, double (&acc)[K],
+                                         double (&acca)[K]) {
+  const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
 #pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int k = 0; k < K; ++k) acc[k] = 0.0, acca[k] = 0.0;
   for (int p = 0; p < P.S; ++p) {
     const int bp = g * P.S + p;
     const int b0 = P.blk[bp], ne = P.blk[bp + 1] - b0;
@@ -552,9 +557,40 @@ __device__ inline void phased_rows(const Phased& P, int rows, const V* __restric
     for (int k = 0; k < w; ++k) off += wtot[k];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      for (unsigned c = 0; c < cn[k]; ++c) acc[k] += prod[off++];
+      for (unsigned c = 0; c < cn[k]; ++c) {
+        const double v = prod[off++];
+        acc[k] += v;
+        if constexpr (ABS) acca[k] += fabs(v);
+      }
     __syncthreads();
   }
+}
+
+// A sweep's sums in the epilogue's row order: out[k] is the sum of the workgroup's row
+// k·kPhBlock + t (so the epilogue's accesses are coalesced), through `part` (K·kPhBlock doubles
+// of LDS). The whole workgroup calls it.
+template <int K>
+__device__ inline void phased_transpose(const double (&acc)[K], double* __restrict__ part, double (&out)[K]) {
+  const int t = static_cast<int>(threadIdx.x);
+#pragma unroll
+  for (int k = 0; k < K; ++k) part[t * K + k] = acc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = part[k * kPhBlock + t];
+  __syncthreads();
+}
+
+// The phased product with the epilogue of the row-tiled kernels: one sweep (phased_accumulate),
+// then the epilogue runs on rows k·kPhBlock + t. The whole workgroup calls it.
+template <int K, int J, class V, class X, class Epi>
+__device__ inline void phased_rows(const Phased& P, int rows, const V* __restrict__ val, const X* __restrict__ x,
+                                   Epi& epi) {
+  __shared__ double prod[J * kPhBlock];
+  __shared__ double part[K * kPhBlock];
+  __shared__ int wtot[kPhBlock / 64];
+  const int t = static_cast<int>(threadIdx.x), g = static_cast<int>(blockIdx.x);
+  double acc[K], unused[K];
+  phased_accumulate<K, J, false>(P, g, val, x, prod, wtot, acc, unused);
 #pragma unroll
   for (int k = 0; k < K; ++k) part[t * K + k] = acc[k];
   __syncthreads();

@@ -151,6 +151,43 @@ def test_phased_products_equal_row_tiles(gpu_available, oracle, name):
     b.close()
 
 
+def _stats_bits(s):
+    """Every field of a psh_stats, floats as their exact hex strings."""
+    vals = [s.px, s.py, s.cross, s.r0_px, s.r0_py, s.r0_cross, s.have_r0, s.dx_norm, s.dy_norm, s.finite]
+    vals += s.kkt.as_list()
+    vals += [getattr(r, f) for r in (s.ray, s.drift) for f, _ in r._fields_]
+    return [v.hex() if isinstance(v, float) else v for v in vals]
+
+
+@pytest.mark.parametrize("name", MODELS)
+def test_phased_check_equals_row_tiles(gpu_available, oracle, name):
+    """The check pass over the phased layouts (the original values copied into the layouts' order;
+    hip_sliced = 2 builds them at any size) gives the row tiles' bits: every scalar of the check
+    in every reduction mode, with and without the ray and drift tests (the netlib models have
+    unboxed columns, so the primal-ray products run too), the epoch residual, the stored Ã·Δx, and
+    the KKT of named slots."""
+    from ps26119_amd import psh
+    lp = model(name)
+    K, tau, sigma = 7, 0.3, 0.4
+    sc = psh.scaled_problem(oracle.scale(lp), lp)
+    a = psh.Context(sc, sliced=2)
+    b = psh.Context(sc, sliced=0)
+    for mode in (psh.RED_TREE, psh.RED_CPU, psh.RED_DECISION):
+        for ctx in (a, b):
+            ctx.upload(psh.X, np.sin(1.0 + np.arange(lp.n)))
+            ctx.upload(psh.Y, np.cos(2.0 + np.arange(lp.m)))
+            ctx.copy(psh.X0, psh.X)
+            ctx.copy(psh.Y0, psh.Y)
+            ctx.chunk(K, 0, tau, sigma, mode=mode)
+        for flags in ((1, 1, 1), (0, 0, 1), (1, 0, 0)):
+            sa, sb = a.check(1.3, 0.7, *flags, mode=mode), b.check(1.3, 0.7, *flags, mode=mode)
+            assert _stats_bits(sa) == _stats_bits(sb), (mode, flags)
+        assert np.array_equal(a.download(psh.AX), b.download(psh.AX)), mode
+        assert a.kkt(psh.XH, psh.YH, mode) == b.kkt(psh.XH, psh.YH, mode), mode
+    a.close()
+    b.close()
+
+
 def test_chunk_graphs_follow_the_reflection(gpu_available, oracle):
     """ρ is baked into a captured chunk's launches: chunks with ρ = 1, 0.5, 0.5, 1 on one context
     (graph replay) give the bits of the same chunks launched eagerly on another."""
