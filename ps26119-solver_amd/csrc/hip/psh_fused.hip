@@ -678,7 +678,8 @@ __global__ void __launch_bounds__(kPhBlock) k_check_rows_ph(CheckArgs<double> a,
 // that carry the chunk sums of `cs` in the LDS buffer the tiles stage products in. cs.blocks is a
 // multiple of kXcd, so the tiles keep their XCD assignment. The chains' unrolled sums take the
 // kernel from 8 to 6 wavefronts per SIMD; holding it at 8 slows the chains more than it speeds
-// the column tiles (measured on the 1M-row LP: 1.5% more time per iteration).
+// the column tiles (measured when the 1M-row LP's column pass ran on these tiles: 1.5% more time
+// per iteration).
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_check_cols(CheckArgs<T> a, const int* __restrict__ tiles, int ntiles,
                                                        ChunkSums cs) {
