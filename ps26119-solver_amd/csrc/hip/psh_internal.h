@@ -342,16 +342,57 @@ __device__ inline Bf16 ld_stream<Bf16>(const Bf16* p) {
   return Bf16{__builtin_nontemporal_load(&p->bits)};
 }
 
+// Device data (matrices and vectors) lives in global memory. A pointer that was itself read from
+// memory (a table of instances or jobs) reaches the kernel as a generic one, whose loads also
+// count on the LDS counter and may alias LDS stores; as_global tells the compiler the address
+// space, so such loads can be issued ahead of LDS stores and waited for on their own counter.
+template <class T>
+__device__ inline const T* as_global(const T* p) {
+  return (const T*)(const __attribute__((address_space(1))) T*)p;
+}
+
 // An epilogue may load its row's operands ahead of the product loop: prefetch(row, has).
 template <class E, class = void>
 struct has_prefetch : std::false_type {};
 template <class E>
 struct has_prefetch<E, std::void_t<decltype(std::declval<E&>().prefetch(0, true))>> : std::true_type {};
 
+// A thread's products of one LDS pass over entries [cb, ce) of a tile (ce − cb ≤ kTileNnz): entry
+// cb + threadIdx.x + j·kBlock gives v[j] = val·x[col], for the slots j of [J0, J0 + NJ). Every
+// load is issued unconditionally, at entry cb for a slot past ce (a valid entry whose product is
+// not stored), so that the loads and gathers of all NJ slots are in flight together instead of
+// one slot's after another's.
+template <int J0, int NJ, class V, class X>
+__device__ inline void tile_products(int cb, int ce, const int* __restrict__ col, const V* __restrict__ val,
+                                     const X* __restrict__ x, double (&v)[kTileNnz / kBlock]) {
+  int c[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int p = cb + static_cast<int>(threadIdx.x) + (J0 + j) * kBlock;
+    c[j] = ld_stream(col + (p < ce ? p : cb));
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int p = cb + static_cast<int>(threadIdx.x) + (J0 + j) * kBlock;
+    v[J0 + j] = static_cast<double>(ld_stream(val + (p < ce ? p : cb))) * static_cast<double>(x[c[j]]);
+  }
+}
+
+// All of a pass's slots in two batches of half the slots: the second only when the pass has entries
+// there (a uniform test), so that passes of short rows do not issue loads for empty slots.
+template <class V, class X>
+__device__ inline void tile_products(int cb, int ce, const int* __restrict__ col, const V* __restrict__ val,
+                                     const X* __restrict__ x, double (&v)[kTileNnz / kBlock]) {
+  constexpr int kHalf = kTileNnz / kBlock / 2;
+  tile_products<0, kHalf>(cb, ce, col, val, x, v);
+  if (cb + kHalf * kBlock < ce) tile_products<kHalf, kHalf>(cb, ce, col, val, x, v);
+}
+
 template <class V, class X, class Epi>
 __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, const int* __restrict__ col,
                                  const V* __restrict__ val, const X* __restrict__ x, Epi& epi) {
   __shared__ double prod[kTileNnz];
+  ptr = as_global(ptr), col = as_global(col), val = as_global(val), x = as_global(x);
   const int p0 = ptr[r0], p1 = ptr[r1];
   const int row = r0 + static_cast<int>(threadIdx.x);
   const bool has = row < r1;
@@ -360,8 +401,13 @@ __device__ inline void tile_rows(int r0, int r1, const int* __restrict__ ptr, co
   double s = 0.0;
   for (int cb = p0; cb < p1; cb += kTileNnz) {
     const int ce = p1 - cb > kTileNnz ? cb + kTileNnz : p1;
-    for (int p = cb + static_cast<int>(threadIdx.x); p < ce; p += kBlock)
-      prod[p - cb] = static_cast<double>(ld_stream(val + p)) * static_cast<double>(x[ld_stream(col + p)]);
+    double v[kTileNnz / kBlock];
+    tile_products(cb, ce, col, val, x, v);
+#pragma unroll
+    for (int j = 0; j < kTileNnz / kBlock; ++j) {
+      const int p = cb + static_cast<int>(threadIdx.x) + j * kBlock;
+      if (p < ce) prod[p - cb] = v[j];
+    }
     __syncthreads();
     if (has) {
       const int b = rb > cb ? rb : cb, e = re < ce ? re : ce;
@@ -379,6 +425,7 @@ template <class V, class X>
 __device__ inline void tile_sums_in(double* __restrict__ prod, int r0, int r1, const int* __restrict__ ptr,
                                     const int* __restrict__ col, const V* __restrict__ val, const X* __restrict__ x,
                                     bool abs_too, double& s, double& sa) {
+  ptr = as_global(ptr), col = as_global(col), val = as_global(val), x = as_global(x);
   const int p0 = ptr[r0], p1 = ptr[r1];
   const int row = r0 + static_cast<int>(threadIdx.x);
   const bool has = row < r1;
@@ -387,8 +434,13 @@ __device__ inline void tile_sums_in(double* __restrict__ prod, int r0, int r1, c
   sa = 0.0;
   for (int cb = p0; cb < p1; cb += kTileNnz) {
     const int ce = p1 - cb > kTileNnz ? cb + kTileNnz : p1;
-    for (int p = cb + static_cast<int>(threadIdx.x); p < ce; p += kBlock)
-      prod[p - cb] = static_cast<double>(val[p]) * static_cast<double>(x[col[p]]);
+    double v[kTileNnz / kBlock];
+    tile_products(cb, ce, col, val, x, v);
+#pragma unroll
+    for (int j = 0; j < kTileNnz / kBlock; ++j) {
+      const int p = cb + static_cast<int>(threadIdx.x) + j * kBlock;
+      if (p < ce) prod[p - cb] = v[j];
+    }
     __syncthreads();
     if (has) {
       const int b = rb > cb ? rb : cb, e = re < ce ? re : ce;
